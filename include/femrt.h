@@ -1,0 +1,206 @@
+/*
+ * femrt.h -- C ABI of the MI355X curved-hex FEM ray tracer.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No C++
+ * or torch types cross this boundary.  Every call returns FEMRT_OK (0) or a
+ * negative FEMRT_E* code; femrt_last_error() returns the message of the last
+ * failure on a context.  A context is externally synchronised: one host
+ * thread per context.  Device work is queued on the context's HIP stream;
+ * calls that hand results back to host memory synchronise that stream.
+ *
+ * Reference interfaces replaced (paths relative to FEM-AABB/ of the
+ * reference tree):
+ *   femrt_mesh_upload_hex   <- Scene::add_hex + the host pointer patching and
+ *                              cudaMemcpy of HexElementData / coefficient pools
+ *                              (raytracer.h:118-166, main.cu:336-511)
+ *   femrt_mesh_build_bvh    <- Scene::build_bvh -> BVH::build
+ *                              (raytracer.h:338-344, BVH.cu:180-247)
+ *   femrt_eval_field        <- EvaluateModalField / EvaluateModalGradientWorld
+ *                              (fem/HexElement.cu:274-330)
+ *   femrt_invert_points     <- WorldToReferenceNewton (fem/HexElement.cu:227-272)
+ *   femrt_trace             <- render_kernel -> ray_color -> bvh_intersect_device
+ *                              hex branch (raytracer.h:313-331,
+ *                              raytracer.cu:102-254, 410-476)
+ *   femrt_gen_structured_hex, femrt_load_xml_fld
+ *                           <- load_xml_fld (xml_fld_loader.h:14-16) and the
+ *                              geometry fit it calls (BuildHexGeometry,
+ *                              xml_fld_geometry_builder.cpp:68-381)
+ */
+#ifndef FEMRT_H
+#define FEMRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEMRT_API __attribute__((visibility("default")))
+
+enum {
+  FEMRT_OK = 0,
+  FEMRT_EINVAL = -1,   /* bad argument (null pointer, size, mode count)   */
+  FEMRT_EHIP = -2,     /* a HIP runtime call failed                        */
+  FEMRT_ENOMEM = -3,   /* device or host allocation failed                 */
+  FEMRT_EUNSUP = -4,   /* mode counts / options this build has no kernel for */
+  FEMRT_EIO = -5,      /* file missing or malformed                        */
+  FEMRT_ESTATE = -6    /* call out of order (e.g. trace before BVH build)  */
+};
+
+typedef struct femrt_ctx femrt_ctx;
+typedef struct femrt_mesh femrt_mesh;
+typedef struct femrt_frame femrt_frame;
+
+/* ---------------------------------------------------------------- context */
+FEMRT_API int femrt_version(void);
+FEMRT_API int femrt_ctx_create(int device, femrt_ctx** out);
+FEMRT_API int femrt_ctx_destroy(femrt_ctx* ctx);
+FEMRT_API const char* femrt_last_error(const femrt_ctx* ctx);
+FEMRT_API int femrt_sync(femrt_ctx* ctx);
+/* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
+ * on the context stream and its duration accumulated per kernel name. */
+FEMRT_API int femrt_timing_enable(femrt_ctx* ctx, int on);
+FEMRT_API int femrt_timing_reset(femrt_ctx* ctx);
+/* Copies up to cap entries: names (NUL-separated, 48 bytes each), total ms,
+ * launch counts.  Returns the number of distinct kernels (>=0). */
+FEMRT_API int femrt_timing_read(femrt_ctx* ctx, int cap, char* names48, double* total_ms,
+                                long long* launches);
+
+/* ------------------------------------------------------------------ mesh */
+/* Uniform-order hex mesh in SoA host arrays (the reference stores per-element
+ * mode counts; every mesh of the benchmark configurations is uniform, and a
+ * mixed-order mesh is uploaded as several meshes).
+ *   vertices  n_elem x 8 x 3   corner vertices, XML/VTK order (HexElement.cuh:10)
+ *   geom      n_elem x 3 x ng^3 modal geometry coefficients, x block, y block,
+ *             z block; mode (i,j,k) at (k*ng+j)*ng+i (StdHexExp.cpp:516-532)
+ *   field     n_elem x nf^3     modal field coefficients, same ordering
+ *   adjacency n_elem x 6 x 2    (neighbour element, neighbour local face) per
+ *             local face 0..5 = t-,s-,r+,s+,r-,t+ (high_order.h:172-184), -1 on
+ *             the boundary; may be NULL (element walk then re-queries the BVH)
+ * The mesh is copied to device memory; the host arrays may be freed after the
+ * call returns. */
+FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* ctx, int n_elem, int ng, int nf,
+                                    const float* vertices, const float* geom, const float* field,
+                                    const int* adjacency, femrt_mesh** out);
+FEMRT_API int femrt_mesh_destroy(femrt_mesh* mesh);
+/* Conservative element and face boxes: exact Bernstein hull of the modal
+ * geometry (replaces the sampled +15% box of ComputeHexAabb,
+ * high_order.h:51-113).  out: n_elem x 7 x 6 floats (element box, then the 6
+ * face boxes; each lo.xyz, hi.xyz). */
+FEMRT_API int femrt_mesh_boxes(const femrt_mesh* mesh, float* out);
+/* builder 0: binned SAH on the host, 16 bins, leaves of <= 4 elements
+ * (BVH.cu:35-177).  Returns node count via *n_nodes (may be NULL). */
+FEMRT_API int femrt_mesh_build_bvh(femrt_mesh* mesh, int builder, int* n_nodes);
+/* Host copy of the BVH: nodes x 8 floats (lo.xyz, a as int bits, hi.xyz, b
+ * as int bits).  Internal node: a, b = child node indices.  Leaf: b < 0,
+ * a = first slot in the element index list, -b = element count.  prims:
+ * n_elem element indices in leaf order. */
+FEMRT_API int femrt_mesh_get_bvh(const femrt_mesh* mesh, float* nodes8, int* prims);
+
+/* ---------------------------------------------------------- point kernels */
+/* Field value and world gradient at (elem, rst) points: out is 4 x n SoA
+ * (u, du/dx, du/dy, du/dz).  rst is 3 x n SoA.  precision 0 = FP32, 1 = FP64
+ * arithmetic (results stored as float / double respectively: out points to
+ * float or double). Host buffers; synchronous. */
+FEMRT_API int femrt_eval_field(femrt_mesh* mesh, int n, const int* elem, const float* rst,
+                               int precision, void* out);
+/* Point inversion x -> rst (Newton from rst = 0, as WorldToReferenceNewton):
+ * x is 3 x n SoA; rst_out 3 x n SoA doubles after an FP64 polish; iters_out
+ * (may be NULL) the FP32 iteration count, -1 when the iteration failed. */
+FEMRT_API int femrt_invert_points(femrt_mesh* mesh, int n, const int* elem, const float* x,
+                                  double* rst_out, int* iters_out);
+
+/* ------------------------------------------------------------------ trace */
+enum {
+  FEMRT_GUESS_PLAIN = 0,   /* reference initial guesses: face seeds {-0.8,0,0.8}^2,
+                              point inversion from rst = 0 */
+  FEMRT_GUESS_WARM = 1,    /* engineered: warm starts along the ray + bilinear seeds */
+  FEMRT_GUESS_LEARNED = 2  /* MLP initial guesses (femrt_mlp_*), plain fallback */
+};
+
+#define FEMRT_MAX_ISO 8
+
+typedef struct {
+  int n_iso;                     /* isovalues searched (0..FEMRT_MAX_ISO)         */
+  int iso_enable;                /* 0 disables the iso stage (0.0 is a valid iso) */
+  float iso[FEMRT_MAX_ISO];
+  int samples;                   /* uniform sub-intervals per element segment (16) */
+  int guess;                     /* FEMRT_GUESS_*                                 */
+  float t_min;                   /* 0.001 in the reference (raytracer.cu:221)     */
+  float t_max;
+  int max_march;                 /* element-walk step cap per ray                  */
+  int flags;                     /* bit0: write RGBA                              */
+} femrt_trace_params;
+
+FEMRT_API void femrt_trace_params_default(femrt_trace_params* p);
+
+/* A frame owns device buffers for n_rays rays and their results.
+ * Per-ray results (SoA, device):
+ *   hit_elem int32 (-1 miss), hit_face int32, hit_t f64, hit_rst 3 x f64,
+ *   hit_u f64 (field at the hit), and per isovalue k: iso_elem int32,
+ *   iso_t f64, iso_rst 3 x f64, iso_u f64; rgba uint32. */
+FEMRT_API int femrt_frame_create(femrt_ctx* ctx, int n_rays, femrt_frame** out);
+FEMRT_API int femrt_frame_destroy(femrt_frame* f);
+/* rays: 6 x n SoA floats (ox, oy, oz, dx, dy, dz), host memory */
+FEMRT_API int femrt_frame_set_rays(femrt_frame* f, const float* rays);
+/* pinhole camera (raytracer.h:31-48), pixel-centre rays, y = 0 bottom row,
+ * generated on the device.  cam: lookfrom.xyz, lookat.xyz, vup.xyz, vfov_deg */
+FEMRT_API int femrt_frame_camera_rays(femrt_frame* f, const float* cam10, int width, int height);
+FEMRT_API int femrt_frame_get_rays(femrt_frame* f, float* rays);
+/* Queue the trace of every ray of the frame on the context stream (async). */
+FEMRT_API int femrt_trace(femrt_mesh* mesh, femrt_frame* f, const femrt_trace_params* p);
+/* Copy results to host (synchronises).  Any pointer may be NULL.  iso arrays
+ * are n_iso x n_rays. */
+FEMRT_API int femrt_frame_get_hits(femrt_frame* f, int* hit_elem, int* hit_face, double* hit_t,
+                                   double* hit_rst, double* hit_u);
+FEMRT_API int femrt_frame_get_iso(femrt_frame* f, int n_iso, int* iso_elem, double* iso_t,
+                                  double* iso_rst, double* iso_u);
+FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
+/* Work counters of the last trace (validation runs): [0] rays, [1] hit rays,
+ * [2] face-Newton iterations FP32, [3] point-inversion iterations FP32,
+ * [4] field evaluations FP32, [5] FP64 geometry iterations, [6] FP64 field
+ * evaluations, [7] slab tests, [8] MLP inferences, [9] elements marched. */
+#define FEMRT_NCOUNTERS 10
+FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
+
+/* ------------------------------------------------ host-only utilities */
+/* The same box and BVH builders the mesh upload uses, on host arrays (no
+ * device needed).  boxes: n_elem x 7 x 6 floats.  nodes8 must hold
+ * 2 * n_elem nodes; *n_nodes receives the count. */
+FEMRT_API int femrt_host_boxes(int n_elem, int ng, const float* geom, float* boxes);
+FEMRT_API int femrt_host_build_bvh(int n_elem, const float* boxes, int* n_nodes, float* nodes8,
+                                   int* prims);
+
+/* ------------------------------------------------------- synthetic meshes */
+typedef struct {
+  int n;          /* elements per axis on the unit cube                     */
+  int q;          /* geometry order (ng = q + 1)                            */
+  int p;          /* field order (nf = p + 1)                               */
+  int pert;       /* 0 none, 1 A sin(2 pi a) sin(2 pi b), 2 seeded gaussian
+                     A*h*N(0,1) clipped to 3 sigma                          */
+  double amp;
+  uint64_t seed;
+} femrt_gen_params;
+
+/* Structured n^3 hex mesh of [0,1]^3: straight edges, interior face nodes
+ * (evenly spaced, as PolyEvenlySpaced CURVED faces) displaced along the face
+ * normal, geometry fitted on GLL nodes exactly as BuildHexGeometry does;
+ * field c_ijk = N(0,1)/(1+i+j+k)^2 from a counter-based generator keyed by
+ * (seed, element, mode).  Output arrays sized as for femrt_mesh_upload_hex. */
+FEMRT_API int femrt_gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
+                                       float* field, int* adjacency);
+
+/* Drop-in loader: Nektar++ XML mesh + FLD field (xml_fld_loader.h:14-16).
+ * Two-phase: call with NULL arrays to get sizes, then with arrays.
+ * flags bit0: replicate the reference's vertex-0 parse defect
+ * (xml_fld_geometry_parser.cpp:54-77); bit1: replicate geomModes inflation
+ * to max(field, geometry) modes (xml_fld_loader.cpp:252-254). */
+FEMRT_API int femrt_load_xml_fld(const char* xml_path, const char* fld_path, int flags,
+                                 int* n_elem, int* ng, int* nf, float* vertices, float* geom,
+                                 float* field, int* adjacency);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

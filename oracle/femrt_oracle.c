@@ -1,0 +1,607 @@
+/*
+ * femrt_oracle.c -- FP64 CPU restatement of the curved-hex ray-tracing method.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle: it is
+ * compiled into oracle/_build/liboracle.so and loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
+ * checker and never as the thing measured or shipped.  The product path
+ * (test-_amd/csrc) never links or calls it.
+ *
+ * Every function restates the reference algorithm in double precision from the
+ * mathematics, citing the reference file:line it follows (paths relative to
+ * FEM-AABB/ of the reference tree).  Where the reference is ill-defined the
+ * oracle fixes a semantics (documented at each function):
+ *   - face intersection returns the NEAREST in-bounds root reached from the
+ *     reference's 3x3 seed grid (the reference returns the first seed that
+ *     converges, high_order.h:213-250);
+ *   - the iso search runs over the ray's true segment inside each element
+ *     (entry/exit face roots), not over the padded AABB, with the reference's
+ *     16 uniform sub-intervals and a bisection endgame to 1e-14;
+ *   - isovalue 0.0 is a valid isovalue (the reference gates on iso != 0,
+ *     raytracer.cu:207);
+ *   - the geometry first hit and the first hit of each isovalue are separate
+ *     outputs.
+ * Coefficients arrive as float (the product stores FP32 coefficients, as the
+ * reference does, raytracer.h:73-90) and are widened exactly to double, so the
+ * oracle solves the same problem as the GPU path in exact-ish arithmetic.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+
+/* Acceptance slack on the free face coordinates (u,v): a root is on the face
+ * when max(|u|,|v|) <= 1 + ORC_FACE_ACC.  The reference uses 1.001
+ * (high_order.h:243-245). */
+#define ORC_FACE_ACC 1e-4
+#define ORC_MAXN 12
+
+/* ------------------------------------------------------------------------ */
+/* A1  Modified_A 1-D basis: fem/ModalBasis.cuh:16-70                        */
+/* ------------------------------------------------------------------------ */
+
+/* Jacobi polynomial P_n^{(a,b)}(x) by the standard three-term recurrence
+ * (Karniadakis & Sherwin, Spectral/hp Element Methods, App. A). */
+static double jacobi(int n, double a, double b, double x)
+{
+  if (n == 0) return 1.0;
+  double p0 = 1.0;
+  double p1 = 0.5 * (a - b + (a + b + 2.0) * x);
+  for (int k = 2; k <= n; ++k) {
+    const double c = 2.0 * k + a + b;
+    const double a1 = 2.0 * k * (k + a + b) * (c - 2.0);
+    const double a2 = (c - 1.0) * (a * a - b * b);
+    const double a3 = (c - 2.0) * (c - 1.0) * c;
+    const double a4 = 2.0 * (k + a - 1.0) * (k + b - 1.0) * c;
+    const double p2 = ((a2 + a3 * x) * p1 - a4 * p0) / a1;
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+/* d/dx P_n^{(a,b)} = (n+a+b+1)/2 * P_{n-1}^{(a+1,b+1)}  (ModalBasis.cuh:43-51) */
+static double jacobi_d(int n, double a, double b, double x)
+{
+  if (n == 0) return 0.0;
+  return 0.5 * (n + a + b + 1.0) * jacobi(n - 1, a + 1.0, b + 1.0, x);
+}
+
+/* phi_0=(1-x)/2, phi_1=(1+x)/2, phi_p=(1-x)/2 (1+x)/2 P_{p-2}^{(1,1)}(x)
+ * (ModalBasis.cuh:53-60) */
+ORC_EXPORT double orc_modA(int i, double x)
+{
+  if (i == 0) return 0.5 * (1.0 - x);
+  if (i == 1) return 0.5 * (1.0 + x);
+  return 0.25 * (1.0 - x) * (1.0 + x) * jacobi(i - 2, 1.0, 1.0, x);
+}
+
+/* derivative of phi_i (ModalBasis.cuh:62-70) */
+ORC_EXPORT double orc_modA_d(int i, double x)
+{
+  if (i == 0) return -0.5;
+  if (i == 1) return 0.5;
+  return -0.5 * x * jacobi(i - 2, 1.0, 1.0, x) +
+         0.25 * (1.0 - x) * (1.0 + x) * jacobi_d(i - 2, 1.0, 1.0, x);
+}
+
+/* ------------------------------------------------------------------------ */
+/* A2/A6  Modal tensor sums, brute force: fem/HexElement.cu:8-94, 280-306;  */
+/* coefficient order p (r) fastest: idx = (k*n + j)*n + i                    */
+/* (StdHexExp.cpp:516-532).  Deliberately NOT sum-factorised: the GPU path   */
+/* is, so this is an independent check of the factorisation.                */
+/* ------------------------------------------------------------------------ */
+ORC_EXPORT void orc_eval_modal(const float* c, int n, const double* rst, double* out4)
+{
+  double v = 0.0, dr = 0.0, ds = 0.0, dt = 0.0;
+  int idx = 0;
+  for (int k = 0; k < n; ++k) {
+    const double tk = orc_modA(k, rst[2]);
+    const double tkd = orc_modA_d(k, rst[2]);
+    for (int j = 0; j < n; ++j) {
+      const double sj = orc_modA(j, rst[1]);
+      const double sjd = orc_modA_d(j, rst[1]);
+      for (int i = 0; i < n; ++i) {
+        const double ri = orc_modA(i, rst[0]);
+        const double rid = orc_modA_d(i, rst[0]);
+        const double cc = (double)c[idx++];
+        v += cc * ri * sj * tk;
+        dr += cc * rid * sj * tk;
+        ds += cc * ri * sjd * tk;
+        dt += cc * ri * sj * tkd;
+      }
+    }
+  }
+  out4[0] = v;
+  out4[1] = dr;
+  out4[2] = ds;
+  out4[3] = dt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mesh as the oracle sees it: uniform mode counts, FP32 coefficients.      */
+/* geom: n_elem x 3 x ng^3 (x block, y block, z block per element)          */
+/* field: n_elem x nf^3                                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int n_elem;
+  int ng;
+  int nf;
+  int pad_;
+  const float* geom;
+  const float* field;
+} orc_mesh_t;
+
+static inline const float* geom_ptr(const orc_mesh_t* m, int e, int comp)
+{
+  const size_t n3 = (size_t)m->ng * m->ng * m->ng;
+  return m->geom + ((size_t)e * 3 + comp) * n3;
+}
+
+/* A3/A4  x(rst) and J = dx/drst, row-major (rows x,y,z; cols r,s,t):
+ * HexElement.cu:122-139 (ReferenceToWorldHighOrder), :194-206 (JacobianHighOrder) */
+ORC_EXPORT void orc_map(const orc_mesh_t* m, int e, const double* rst, double* x3, double* J9)
+{
+  for (int c = 0; c < 3; ++c) {
+    double o[4];
+    orc_eval_modal(geom_ptr(m, e, c), m->ng, rst, o);
+    x3[c] = o[0];
+    if (J9) {
+      J9[3 * c + 0] = o[1];
+      J9[3 * c + 1] = o[2];
+      J9[3 * c + 2] = o[3];
+    }
+  }
+}
+
+static double det3(const double* J)
+{
+  return J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+         J[2] * (J[3] * J[7] - J[4] * J[6]);
+}
+
+/* cofactor inverse (HexElement.cu:208-225); returns 0 on singular J */
+static int inv3(const double* J, double* inv)
+{
+  const double det = det3(J);
+  if (!(fabs(det) > 1e-300)) return 0;
+  const double id = 1.0 / det;
+  inv[0] = (J[4] * J[8] - J[5] * J[7]) * id;
+  inv[1] = (J[2] * J[7] - J[1] * J[8]) * id;
+  inv[2] = (J[1] * J[5] - J[2] * J[4]) * id;
+  inv[3] = (J[5] * J[6] - J[3] * J[8]) * id;
+  inv[4] = (J[0] * J[8] - J[2] * J[6]) * id;
+  inv[5] = (J[2] * J[3] - J[0] * J[5]) * id;
+  inv[6] = (J[3] * J[7] - J[4] * J[6]) * id;
+  inv[7] = (J[1] * J[6] - J[0] * J[7]) * id;
+  inv[8] = (J[0] * J[4] - J[1] * J[3]) * id;
+  return 1;
+}
+
+/* A5  point inversion x -> rst: HexElement.cu:227-272.  Same iteration
+ * (start at rst=0, undamped Newton), converged in FP64 to |drst|_inf < 1e-14;
+ * if the undamped iteration fails to converge in 60 steps a damped restart
+ * (step capped at 0.5 in inf-norm) is tried.  Returns the iteration count or
+ * -1. */
+ORC_EXPORT int orc_invert(const orc_mesh_t* m, int e, const double* x, double* rst)
+{
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    rst[0] = rst[1] = rst[2] = 0.0;
+    for (int it = 0; it < 80; ++it) {
+      double X[3], J[9], iJ[9];
+      orc_map(m, e, rst, X, J);
+      if (!inv3(J, iJ)) break;
+      const double dx = X[0] - x[0], dy = X[1] - x[1], dz = X[2] - x[2];
+      double d0 = iJ[0] * dx + iJ[1] * dy + iJ[2] * dz;
+      double d1 = iJ[3] * dx + iJ[4] * dy + iJ[5] * dz;
+      double d2 = iJ[6] * dx + iJ[7] * dy + iJ[8] * dz;
+      if (attempt == 1) {
+        const double mx = fmax(fabs(d0), fmax(fabs(d1), fabs(d2)));
+        if (mx > 0.5) {
+          const double s = 0.5 / mx;
+          d0 *= s; d1 *= s; d2 *= s;
+        }
+      }
+      rst[0] -= d0;
+      rst[1] -= d1;
+      rst[2] -= d2;
+      if (!(fabs(rst[0]) < 1e6 && fabs(rst[1]) < 1e6 && fabs(rst[2]) < 1e6)) break;
+      if (fabs(d0) < 1e-14 && fabs(d1) < 1e-14 && fabs(d2) < 1e-14) return it + 1;
+    }
+  }
+  return -1;
+}
+
+/* A6  field value, reference gradient and world gradient J^{-T} grad_ref:
+ * HexElement.cu:274-330 */
+ORC_EXPORT void orc_field(const orc_mesh_t* m, int e, const double* rst, double* u,
+                          double* gref, double* gworld)
+{
+  const size_t n3 = (size_t)m->nf * m->nf * m->nf;
+  double o[4];
+  orc_eval_modal(m->field + (size_t)e * n3, m->nf, rst, o);
+  *u = o[0];
+  if (gref) { gref[0] = o[1]; gref[1] = o[2]; gref[2] = o[3]; }
+  if (gworld) {
+    double X[3], J[9], iJ[9];
+    orc_map(m, e, rst, X, J);
+    if (!inv3(J, iJ)) { gworld[0] = gworld[1] = gworld[2] = NAN; return; }
+    gworld[0] = iJ[0] * o[1] + iJ[3] * o[2] + iJ[6] * o[3];
+    gworld[1] = iJ[1] * o[1] + iJ[4] * o[2] + iJ[7] * o[3];
+    gworld[2] = iJ[2] * o[1] + iJ[5] * o[2] + iJ[8] * o[3];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* A9  face-constrained Newton: high_order.h:164-304                        */
+/* Face order (Nektar): 0:t=-1 1:s=-1 2:r=+1 3:s=+1 4:r=-1 5:t=+1           */
+/* ------------------------------------------------------------------------ */
+static const int FACE_FIX[6] = {2, 1, 0, 1, 0, 2};
+static const double FACE_VAL[6] = {-1, -1, 1, 1, -1, 1};
+static const int FACE_U[6] = {0, 0, 1, 0, 1, 0};
+static const int FACE_V[6] = {1, 2, 2, 2, 2, 1};
+
+ORC_EXPORT void orc_face_axes(int face, int* fix, double* val, int* ua, int* va)
+{
+  *fix = FACE_FIX[face];
+  *val = FACE_VAL[face];
+  *ua = FACE_U[face];
+  *va = FACE_V[face];
+}
+
+/* Nearest in-bounds root of X(face(u,v)) = o + tau d over the reference's 9
+ * seeds {-0.8,0,0.8}^2 (high_order.h:206-211).  Each seed follows the
+ * reference iteration: tau0 = projection of X(seed) onto the ray clamped to
+ * [tmin,tmax] (:221-225); Newton on (u,v,tau) with the step scaled to
+ * |.|_inf <= 0.25 (:288-293), tau clamped to [tmin,tmax] (:294), u,v clamped
+ * to [-1.5,1.5] (:297-298), divergence break when iter>5 and the residual
+ * grew 1.5x (:253).  Converged when |R|_inf < 1e-12 (the reference stops at
+ * 1e-4, :197,:240); then two unconstrained polishing steps.  A converged root
+ * is accepted when max(|u|,|v|) <= 1+ORC_FACE_ACC and tmin <= tau <= tmax.
+ * Returns 1 and the nearest accepted root; *margin receives
+ * 1+ORC_FACE_ACC-max(|u|,|v|) for the returned root, and *nearmiss the
+ * smallest |1+ORC_FACE_ACC-max(|u|,|v|)| over converged but rejected roots
+ * (both used to classify tie pixels). */
+ORC_EXPORT int orc_face_hit(const orc_mesh_t* m, int e, int face, const double* o,
+                            const double* d, double tmin, double tmax, double* t_out,
+                            double* rst_out, double* margin, double* nearmiss)
+{
+  const int fa = FACE_FIX[face], ua = FACE_U[face], va = FACE_V[face];
+  const double fv = FACE_VAL[face];
+  const double sv[3] = {-0.8, 0.0, 0.8};
+  int found = 0;
+  double best_t = tmax;
+  double nm = 1e30;
+  const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  for (int sj = 0; sj < 3; ++sj)
+    for (int si = 0; si < 3; ++si) {
+      double rst[3] = {0, 0, 0};
+      rst[fa] = fv;
+      rst[ua] = sv[si];
+      rst[va] = sv[sj];
+      double X[3], J[9];
+      orc_map(m, e, rst, X, NULL);
+      double tau = dd > 0 ? ((X[0] - o[0]) * d[0] + (X[1] - o[1]) * d[1] + (X[2] - o[2]) * d[2]) / dd : 0.0;
+      tau = fmin(fmax(tau, tmin), tmax);
+      double prev = DBL_MAX;
+      int conv = 0;
+      for (int it = 0; it < 200; ++it) {
+        orc_map(m, e, rst, X, J);
+        const double R0 = X[0] - (o[0] + tau * d[0]);
+        const double R1 = X[1] - (o[1] + tau * d[1]);
+        const double R2 = X[2] - (o[2] + tau * d[2]);
+        const double res = fmax(fabs(R0), fmax(fabs(R1), fabs(R2)));
+        if (res < 1e-12) { conv = 1; break; }
+        if (it > 5 && res > prev * 1.5) break;
+        prev = res;
+        double A[9] = {J[0 + ua], J[0 + va], -d[0], J[3 + ua], J[3 + va], -d[1],
+                       J[6 + ua], J[6 + va], -d[2]};
+        double iA[9];
+        if (!inv3(A, iA)) break;
+        double du = -(iA[0] * R0 + iA[1] * R1 + iA[2] * R2);
+        double dv = -(iA[3] * R0 + iA[4] * R1 + iA[5] * R2);
+        double dt = -(iA[6] * R0 + iA[7] * R1 + iA[8] * R2);
+        const double mx = fmax(fabs(du), fmax(fabs(dv), fabs(dt)));
+        const double sc = mx > 0.25 ? 0.25 / mx : 1.0;
+        rst[ua] += du * sc;
+        rst[va] += dv * sc;
+        tau += dt * sc;
+        tau = fmin(fmax(tau, tmin), tmax);
+        rst[ua] = fmin(fmax(rst[ua], -1.5), 1.5);
+        rst[va] = fmin(fmax(rst[va], -1.5), 1.5);
+      }
+      if (!conv) continue;
+      /* unconstrained polish */
+      for (int it = 0; it < 2; ++it) {
+        orc_map(m, e, rst, X, J);
+        const double R0 = X[0] - (o[0] + tau * d[0]);
+        const double R1 = X[1] - (o[1] + tau * d[1]);
+        const double R2 = X[2] - (o[2] + tau * d[2]);
+        double A[9] = {J[0 + ua], J[0 + va], -d[0], J[3 + ua], J[3 + va], -d[1],
+                       J[6 + ua], J[6 + va], -d[2]};
+        double iA[9];
+        if (!inv3(A, iA)) break;
+        rst[ua] -= iA[0] * R0 + iA[1] * R1 + iA[2] * R2;
+        rst[va] -= iA[3] * R0 + iA[4] * R1 + iA[5] * R2;
+        tau -= iA[6] * R0 + iA[7] * R1 + iA[8] * R2;
+      }
+      const double mu = fmax(fabs(rst[ua]), fabs(rst[va]));
+      const double mg = 1.0 + ORC_FACE_ACC - mu;
+      if (mg >= 0.0 && tau >= tmin && tau <= tmax) {
+        if (!found || tau < best_t) {
+          found = 1;
+          best_t = tau;
+          rst_out[0] = rst[0];
+          rst_out[1] = rst[1];
+          rst_out[2] = rst[2];
+          if (margin) *margin = mg;
+        }
+      } else if (tau >= tmin && tau <= tmax) {
+        if (fabs(mg) < nm) nm = fabs(mg);
+      }
+    }
+  if (nearmiss) *nearmiss = nm;
+  if (found) *t_out = best_t;
+  return found;
+}
+
+/* ------------------------------------------------------------------------ */
+/* A7/A8  conservative element box by dense sampling (independent of the    */
+/* product's Bernstein bound) and the slab test (high_order.h:115-152).     */
+/* ------------------------------------------------------------------------ */
+ORC_EXPORT void orc_elem_box(const orc_mesh_t* m, int e, int samples, double pad_frac, double* lo,
+                             double* hi)
+{
+  lo[0] = lo[1] = lo[2] = DBL_MAX;
+  hi[0] = hi[1] = hi[2] = -DBL_MAX;
+  for (int k = 0; k < samples; ++k)
+    for (int j = 0; j < samples; ++j)
+      for (int i = 0; i < samples; ++i) {
+        const double rst[3] = {-1.0 + 2.0 * i / (samples - 1), -1.0 + 2.0 * j / (samples - 1),
+                               -1.0 + 2.0 * k / (samples - 1)};
+        double X[3];
+        orc_map(m, e, rst, X, NULL);
+        for (int c = 0; c < 3; ++c) {
+          lo[c] = fmin(lo[c], X[c]);
+          hi[c] = fmax(hi[c], X[c]);
+        }
+      }
+  double ext = 0.0;
+  for (int c = 0; c < 3; ++c) ext = fmax(ext, hi[c] - lo[c]);
+  for (int c = 0; c < 3; ++c) {
+    lo[c] -= pad_frac * ext + 1e-9;
+    hi[c] += pad_frac * ext + 1e-9;
+  }
+}
+
+ORC_EXPORT int orc_slab(const double* o, const double* d, const double* lo, const double* hi,
+                        double tmin, double tmax, double* t0, double* t1)
+{
+  double a = tmin, b = tmax;
+  for (int c = 0; c < 3; ++c) {
+    if (fabs(d[c]) < 1e-300) {
+      if (o[c] < lo[c] || o[c] > hi[c]) return 0;
+      continue;
+    }
+    const double inv = 1.0 / d[c];
+    double ta = (lo[c] - o[c]) * inv, tb = (hi[c] - o[c]) * inv;
+    if (ta > tb) { const double s = ta; ta = tb; tb = s; }
+    a = fmax(a, ta);
+    b = fmin(b, tb);
+    if (a > b) return 0;
+  }
+  *t0 = a;
+  *t1 = b;
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* A11  isosurface search in one element segment: fem/RayIsosurface.cu:44-188*/
+/* g(t) = u(x^{-1}(o+t d)) - iso sampled at M+1 uniform points over the      */
+/* segment [ta,tb] (reference: 16 sub-intervals, :78-86); the first          */
+/* sub-interval with g(prev)*g(cur) <= 0 (:108) is refined by bisection to   */
+/* |dt| <= 1e-14 (t+1) (the reference mixes Newton and bisection, :121-171). */
+/* Returns 1 with the root, its rst and the field there.                     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  double t;
+  double rst[3];
+  double u;
+  double gw[3];
+} orc_iso_root_t;
+
+static int eval_along(const orc_mesh_t* m, int e, const double* o, const double* d, double t,
+                      double* rst, double* u)
+{
+  const double x[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
+  if (orc_invert(m, e, x, rst) < 0) return 0;
+  orc_field(m, e, rst, u, NULL, NULL);
+  return 1;
+}
+
+ORC_EXPORT int orc_iso_segment(const orc_mesh_t* m, int e, const double* o, const double* d,
+                               double ta, double tb, int M, double iso, double* out_t,
+                               double* out_rst, double* out_u, double* out_minabs)
+{
+  double prev_g = 0.0, prev_t = ta;
+  int have_prev = 0;
+  double minabs = 1e30;
+  for (int j = 0; j <= M; ++j) {
+    const double t = ta + (tb - ta) * ((double)j / M);
+    double rst[3], u;
+    if (!eval_along(m, e, o, d, t, rst, &u)) continue;
+    const double g = u - iso;
+    if (fabs(g) < minabs) minabs = fabs(g);
+    if (have_prev && prev_g * g <= 0.0) {
+      double lt = prev_t, lg = prev_g, rt = t;
+      for (int it = 0; it < 200 && fabs(rt - lt) > 1e-14 * (fabs(rt) + 1.0); ++it) {
+        const double mt = 0.5 * (lt + rt);
+        double mr[3], mu;
+        if (!eval_along(m, e, o, d, mt, mr, &mu)) break;
+        const double mg = mu - iso;
+        if (lg * mg <= 0.0) {
+          rt = mt;
+        } else {
+          lt = mt;
+          lg = mg;
+        }
+      }
+      const double root = 0.5 * (lt + rt);
+      double rr[3], ru;
+      if (!eval_along(m, e, o, d, root, rr, &ru)) return 0;
+      *out_t = root;
+      out_rst[0] = rr[0];
+      out_rst[1] = rr[1];
+      out_rst[2] = rr[2];
+      *out_u = ru;
+      if (out_minabs) *out_minabs = minabs;
+      return 1;
+    }
+    prev_g = g;
+    prev_t = t;
+    have_prev = 1;
+  }
+  if (out_minabs) *out_minabs = minabs;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* A10/A13  whole-mesh trace of explicit rays, brute force over elements     */
+/* (raytracer.cu:102-254 hex branch; high_order.h:409-465).  Candidate       */
+/* elements are those whose sampled box (11^3 samples, 10% pad) the ray      */
+/* crosses; the BVH is an optimisation the product must show equivalent to   */
+/* this.                                                                     */
+/* Output per ray (doubles, ORC_GEOM_STRIDE):                                */
+/*   [0] elem (-1 miss) [1] face [2] t [3..5] rst [6] u at hit               */
+/*   [7] t of nearest candidate on a different (elem,face) (1e30 none)       */
+/*   [8] margin of the accepted root  [9] smallest near-miss margin          */
+/* Per ray and isovalue (ORC_ISO_STRIDE):                                     */
+/*   [0] elem (-1 none) [1] t [2..4] rst [5] u [6] min |g| over samples of   */
+/*   the element where the root was found (tie classification)               */
+/* ------------------------------------------------------------------------ */
+#define ORC_GEOM_STRIDE 10
+#define ORC_ISO_STRIDE 7
+
+ORC_EXPORT int orc_geom_stride(void) { return ORC_GEOM_STRIDE; }
+ORC_EXPORT int orc_iso_stride(void) { return ORC_ISO_STRIDE; }
+ORC_EXPORT double orc_face_acc(void) { return ORC_FACE_ACC; }
+
+typedef struct {
+  double lo[3], hi[3];
+} box_t;
+
+ORC_EXPORT void orc_trace(const orc_mesh_t* m, const float* rays, int nrays, const double* isos,
+                          int niso, int M, double tmin, double* out_geom, double* out_iso)
+{
+  box_t* boxes = (box_t*)malloc(sizeof(box_t) * (size_t)m->n_elem);
+  for (int e = 0; e < m->n_elem; ++e) orc_elem_box(m, e, 11, 0.10, boxes[e].lo, boxes[e].hi);
+
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int r = 0; r < nrays; ++r) {
+    const double o[3] = {rays[0 * (size_t)nrays + r], rays[1 * (size_t)nrays + r],
+                         rays[2 * (size_t)nrays + r]};
+    const double d[3] = {rays[3 * (size_t)nrays + r], rays[4 * (size_t)nrays + r],
+                         rays[5 * (size_t)nrays + r]};
+    double* G = out_geom + (size_t)r * ORC_GEOM_STRIDE;
+    G[0] = -1; G[1] = -1; G[2] = 1e30; G[3] = G[4] = G[5] = 0; G[6] = 0;
+    G[7] = 1e30; G[8] = 0; G[9] = 1e30;
+    for (int k = 0; k < niso; ++k) {
+      double* I = out_iso + ((size_t)r * niso + k) * ORC_ISO_STRIDE;
+      I[0] = -1; I[1] = 1e30; I[2] = I[3] = I[4] = 0; I[5] = 0; I[6] = 1e30;
+    }
+    for (int e = 0; e < m->n_elem; ++e) {
+      double t0, t1;
+      if (!orc_slab(o, d, boxes[e].lo, boxes[e].hi, tmin, 1e30, &t0, &t1)) continue;
+      double ft[6], frst[6][3];
+      int fok[6];
+      int nroots = 0;
+      for (int f = 0; f < 6; ++f) {
+        double mg = 0, nm = 1e30;
+        fok[f] = orc_face_hit(m, e, f, o, d, tmin, 1e30, &ft[f], frst[f], &mg, &nm);
+        if (nm < G[9]) G[9] = nm;
+        if (!fok[f]) continue;
+        ++nroots;
+        /* geometry first hit bookkeeping */
+        if (ft[f] < G[2]) {
+          if (G[0] >= 0) G[7] = fmin(G[7], G[2]);
+          G[0] = e; G[1] = f; G[2] = ft[f];
+          G[3] = frst[f][0]; G[4] = frst[f][1]; G[5] = frst[f][2];
+          G[8] = mg;
+        } else {
+          G[7] = fmin(G[7], ft[f]);
+        }
+      }
+      if (niso == 0 || nroots < 2) continue;
+      /* inside segment: entry = nearest root, exit = nearest root beyond */
+      double ta = 1e30;
+      for (int f = 0; f < 6; ++f) if (fok[f] && ft[f] < ta) ta = ft[f];
+      double tb = 1e30;
+      for (int f = 0; f < 6; ++f)
+        if (fok[f] && ft[f] > ta + 1e-9 * (fabs(ta) + 1.0) && ft[f] < tb) tb = ft[f];
+      if (tb >= 1e30) continue;
+      for (int k = 0; k < niso; ++k) {
+        double* I = out_iso + ((size_t)r * niso + k) * ORC_ISO_STRIDE;
+        if (I[0] >= 0 && I[1] <= ta) continue; /* an earlier root already */
+        double t, rst[3], u, mn;
+        if (orc_iso_segment(m, e, o, d, ta, tb, M, isos[k], &t, rst, &u, &mn)) {
+          if (t < I[1]) {
+            I[0] = e; I[1] = t; I[2] = rst[0]; I[3] = rst[1]; I[4] = rst[2]; I[5] = u;
+            I[6] = mn;
+          }
+        }
+      }
+    }
+    if (G[0] >= 0) {
+      const double rst[3] = {G[3], G[4], G[5]};
+      double u;
+      orc_field(m, (int)G[0], rst, &u, NULL, NULL);
+      G[6] = u;
+    }
+  }
+  free(boxes);
+}
+
+/* ------------------------------------------------------------------------ */
+/* A14  pinhole camera (raytracer.h:31-48) with pixel-centre rays            */
+/* (raytracer.cu:439-447 without the jitter).  Rays are SoA floats:          */
+/* ox[n] oy[n] oz[n] dx[n] dy[n] dz[n]; pixel index = y*W + x, y=0 bottom.   */
+/* ------------------------------------------------------------------------ */
+ORC_EXPORT void orc_camera_rays(const double* lookfrom, const double* lookat, const double* vup,
+                                double vfov_deg, int W, int H, double* rays_out)
+{
+  const double theta = vfov_deg * 3.14159265358979323846 / 180.0;
+  const double h = tan(theta / 2.0);
+  const double vh = 2.0 * h;
+  const double vw = ((double)W / (double)H) * vh;
+  double w[3] = {lookfrom[0] - lookat[0], lookfrom[1] - lookat[1], lookfrom[2] - lookat[2]};
+  double fd = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  for (int c = 0; c < 3; ++c) w[c] /= fd;
+  double u[3] = {vup[1] * w[2] - vup[2] * w[1], vup[2] * w[0] - vup[0] * w[2],
+                 vup[0] * w[1] - vup[1] * w[0]};
+  const double un = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  for (int c = 0; c < 3; ++c) u[c] /= un;
+  const double v[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2],
+                       w[0] * u[1] - w[1] * u[0]};
+  double hor[3], ver[3], llc[3];
+  for (int c = 0; c < 3; ++c) {
+    hor[c] = fd * vw * u[c];
+    ver[c] = fd * vh * v[c];
+    llc[c] = lookfrom[c] - hor[c] / 2 - ver[c] / 2 - fd * w[c];
+  }
+  const size_t n = (size_t)W * H;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const size_t p = (size_t)y * W + x;
+      const double s = (x + 0.5) / W, t = (y + 0.5) / H;
+      double dir[3];
+      for (int c = 0; c < 3; ++c) dir[c] = llc[c] + s * hor[c] + t * ver[c] - lookfrom[c];
+      const double l = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+      for (int c = 0; c < 3; ++c) {
+        rays_out[c * n + p] = lookfrom[c];
+        rays_out[(3 + c) * n + p] = dir[c] / l;
+      }
+    }
+}

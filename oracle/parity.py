@@ -1,0 +1,85 @@
+"""Parity comparison of GPU trace results against the FP64 oracle.
+
+TEST INFRASTRUCTURE ONLY (used by tests/, __graft_entry__.smoke() and the
+checks of bench.py).  Tolerances are the ones the acceptance bar states:
+hit distance 1e-6 relative, reference coordinates 1e-7, field and isosurface
+values 1e-6.  Tie pixels, where the element or face id is not determined to
+the working precision, are classified by the oracle and excused from the id
+check only (their distance is still checked when both sides hit).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+T_REL = 1e-6
+RST_ABS = 1e-7
+VAL_ABS = 1e-6
+# a ray is a tie pixel when another (element, face) candidate lies within this
+# relative distance of the nearest hit, or the accepted/rejected face
+# coordinate is within TIE_MARGIN of the acceptance bound
+TIE_T = 1e-6
+TIE_MARGIN = 1e-5
+# an isovalue is tie-classified in a ray when some sample of g = u - iso in the
+# element that holds the oracle's root is within this of zero
+TIE_ISO = 1e-5
+
+
+def geometry_ties(og):
+    t = og["t"]
+    return (np.abs(og["t2"] - t) <= TIE_T * np.abs(t)) | (og["margin"] < TIE_MARGIN) | (
+        og["nearmiss"] < TIE_MARGIN)
+
+
+def compare(gh, giso, og, oiso, isos):
+    """Returns a dict of counts and worst errors; `ok` tells whether the bar
+    is met."""
+    rep = {}
+    tie = geometry_ties(og)
+    ho, hg = og["elem"] >= 0, gh["elem"] >= 0
+    rep["rays"] = int(ho.size)
+    rep["oracle_hits"] = int(ho.sum())
+    rep["gpu_hits"] = int(hg.sum())
+    rep["ties"] = int(tie.sum())
+    rep["hit_miss_mismatch"] = int(((ho != hg) & ~tie).sum())
+    both = ho & hg
+    trel = np.abs(gh["t"][both] - og["t"][both]) / np.abs(og["t"][both])
+    rep["t_rel_max"] = float(trel.max()) if trel.size else 0.0
+    same = both & (gh["elem"] == og["elem"]) & (gh["face"] == og["face"])
+    rep["id_mismatch"] = int((both & ~same & ~tie).sum())
+    rep["id_mismatch_ties"] = int((both & ~same & tie).sum())
+    if same.any():
+        drst = np.abs(gh["rst"][:, same] - og["rst"][:, same]).max()
+        du = np.abs(gh["u"][same] - og["u"][same]).max()
+    else:
+        drst = du = 0.0
+    rep["rst_abs_max"] = float(drst)
+    rep["u_abs_max"] = float(du)
+    ok = (rep["hit_miss_mismatch"] == 0 and rep["id_mismatch"] == 0 and
+          rep["t_rel_max"] <= T_REL and rep["rst_abs_max"] <= RST_ABS and
+          rep["u_abs_max"] <= VAL_ABS)
+    iso_rep = []
+    for k, iso in enumerate(isos):
+        fo, fg = oiso["elem"][k] >= 0, giso["elem"][k] >= 0
+        itie = oiso["minabs"][k] < TIE_ISO
+        r = {"iso": float(iso), "oracle_found": int(fo.sum()), "gpu_found": int(fg.sum()),
+             "found_mismatch": int(((fo != fg) & ~itie).sum()), "ties": int(itie.sum())}
+        b = fo & fg
+        if b.any():
+            tr = np.abs(giso["t"][k][b] - oiso["t"][k][b]) / np.abs(oiso["t"][k][b])
+            r["t_rel_max"] = float(tr.max())
+            r["t_rel_bad"] = int((tr > T_REL).sum())
+            r["t_rel_bad_nontie"] = int(((tr > T_REL) & ~itie[b]).sum())
+            r["u_minus_iso_max"] = float(np.abs(giso["u"][k][fg] - iso).max())
+            se = b & (giso["elem"][k] == oiso["elem"][k])
+            r["elem_mismatch"] = int((b & ~se & ~itie).sum())
+            r["rst_abs_max"] = float(np.abs(giso["rst"][k][:, se] - oiso["rst"][k][:, se]).max()) if se.any() else 0.0
+        else:
+            r.update(t_rel_max=0.0, t_rel_bad=0, t_rel_bad_nontie=0, u_minus_iso_max=0.0,
+                     elem_mismatch=0, rst_abs_max=0.0)
+        r["ok"] = (r["found_mismatch"] == 0 and r["t_rel_bad_nontie"] == 0 and
+                   r["u_minus_iso_max"] <= VAL_ABS and r["elem_mismatch"] == 0)
+        ok = ok and r["ok"]
+        iso_rep.append(r)
+    rep["iso"] = iso_rep
+    rep["ok"] = bool(ok)
+    return rep

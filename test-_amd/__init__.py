@@ -1,0 +1,376 @@
+"""MI355X curved-hex FEM ray tracer: Python binding of the C ABI (include/femrt.h).
+
+The product is ``_build/libfemrt.so`` (gfx950 HIP kernels + host C++); this
+module only loads it with ctypes and moves numpy arrays across the boundary.
+Nothing here computes the method itself, and there is no CPU fallback: when
+the library is missing, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "_build" / "libfemrt.so"
+
+OK = 0
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "ESTATE"}
+GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
+MAX_ISO = 8
+NCOUNTERS = 10
+COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",
+                 "geo64_it", "field64_evals", "slab_tests", "mlp_infer", "elems_marched"]
+
+
+class FemrtError(RuntimeError):
+    pass
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("n_iso", C.c_int), ("iso_enable", C.c_int), ("iso", C.c_float * MAX_ISO),
+                ("samples", C.c_int), ("guess", C.c_int), ("t_min", C.c_float),
+                ("t_max", C.c_float), ("max_march", C.c_int), ("flags", C.c_int)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("n", C.c_int), ("q", C.c_int), ("p", C.c_int), ("pert", C.c_int),
+                ("amp", C.c_double), ("seed", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the native library (raises FemrtError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise FemrtError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(str(LIB_PATH))
+    P, I, F, D = C.c_void_p, C.c_int, C.c_float, C.c_double
+    sig = {
+        "femrt_version": (I, []),
+        "femrt_ctx_create": (I, [I, C.POINTER(P)]),
+        "femrt_ctx_destroy": (I, [P]),
+        "femrt_last_error": (C.c_char_p, [P]),
+        "femrt_sync": (I, [P]),
+        "femrt_timing_enable": (I, [P, I]),
+        "femrt_timing_reset": (I, [P]),
+        "femrt_timing_read": (I, [P, I, P, P, P]),
+        "femrt_mesh_upload_hex": (I, [P, I, I, I, P, P, P, P, C.POINTER(P)]),
+        "femrt_mesh_destroy": (I, [P]),
+        "femrt_mesh_boxes": (I, [P, P]),
+        "femrt_mesh_build_bvh": (I, [P, I, C.POINTER(I)]),
+        "femrt_mesh_get_bvh": (I, [P, P, P]),
+        "femrt_eval_field": (I, [P, I, P, P, I, P]),
+        "femrt_invert_points": (I, [P, I, P, P, P, P]),
+        "femrt_trace_params_default": (None, [P]),
+        "femrt_frame_create": (I, [P, I, C.POINTER(P)]),
+        "femrt_frame_destroy": (I, [P]),
+        "femrt_frame_set_rays": (I, [P, P]),
+        "femrt_frame_camera_rays": (I, [P, P, I, I]),
+        "femrt_frame_get_rays": (I, [P, P]),
+        "femrt_trace": (I, [P, P, P]),
+        "femrt_frame_get_hits": (I, [P, P, P, P, P, P]),
+        "femrt_frame_get_iso": (I, [P, I, P, P, P, P]),
+        "femrt_frame_get_rgba": (I, [P, P]),
+        "femrt_frame_counters": (I, [P, I, P]),
+        "femrt_gen_structured_hex": (I, [P, P, P, P, P]),
+        "femrt_host_boxes": (I, [I, I, P, P]),
+        "femrt_host_build_bvh": (I, [I, P, C.POINTER(I), P, P]),
+        "femrt_load_xml_fld": (I, [C.c_char_p, C.c_char_p, I, C.POINTER(I), C.POINTER(I),
+                                   C.POINTER(I), P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names declared in include/femrt.h that the loaded library must export."""
+    return ["femrt_version", "femrt_ctx_create", "femrt_ctx_destroy", "femrt_last_error",
+            "femrt_sync", "femrt_timing_enable", "femrt_timing_reset", "femrt_timing_read",
+            "femrt_mesh_upload_hex", "femrt_mesh_destroy", "femrt_mesh_boxes",
+            "femrt_mesh_build_bvh", "femrt_mesh_get_bvh", "femrt_eval_field",
+            "femrt_invert_points", "femrt_trace_params_default", "femrt_frame_create",
+            "femrt_frame_destroy", "femrt_frame_set_rays", "femrt_frame_camera_rays",
+            "femrt_frame_get_rays", "femrt_trace", "femrt_frame_get_hits", "femrt_frame_get_iso",
+            "femrt_frame_get_rgba", "femrt_frame_counters", "femrt_gen_structured_hex",
+            "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh"]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != OK:
+        msg = ""
+        if ctx is not None:
+            m = lib().femrt_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise FemrtError(f"{what}: {ERRORS.get(rc, rc)} {msg}")
+
+
+# ----------------------------------------------------------------- host data
+
+class HexMesh:
+    """Uniform-order hex mesh in the SoA layout of femrt_mesh_upload_hex."""
+
+    def __init__(self, vertices, geom, field, adjacency, ng, nf):
+        self.vertices = np.ascontiguousarray(vertices, np.float32)
+        self.geom = np.ascontiguousarray(geom, np.float32)
+        self.field = np.ascontiguousarray(field, np.float32)
+        self.adjacency = None if adjacency is None else np.ascontiguousarray(adjacency, np.int32)
+        self.ng, self.nf = int(ng), int(nf)
+        self.n_elem = self.vertices.shape[0]
+
+    def geom_blocks(self):
+        return self.geom.reshape(self.n_elem, 3, self.ng ** 3)
+
+
+def gen_structured_hex(n, q, p, pert=0, amp=0.0, seed=0) -> HexMesh:
+    gp = GenParams(n, q, p, pert, amp, seed)
+    ne, ng, nf = n ** 3, q + 1, p + 1
+    v = np.zeros((ne, 8, 3), np.float32)
+    g = np.zeros((ne, 3, ng ** 3), np.float32)
+    f = np.zeros((ne, nf ** 3), np.float32)
+    a = np.zeros((ne, 6, 2), np.int32)
+    _check(lib().femrt_gen_structured_hex(C.byref(gp), _ptr(v), _ptr(g), _ptr(f), _ptr(a)),
+           what="gen_structured_hex")
+    return HexMesh(v, g, f, a, ng, nf)
+
+
+def host_boxes(hm: HexMesh):
+    out = np.zeros((hm.n_elem, 7, 6), np.float32)
+    _check(lib().femrt_host_boxes(hm.n_elem, hm.ng, _ptr(hm.geom), _ptr(out)), what="host_boxes")
+    return out
+
+
+def host_build_bvh(boxes):
+    boxes = np.ascontiguousarray(boxes, np.float32)
+    ne = boxes.shape[0]
+    nodes = np.zeros((2 * ne, 8), np.float32)
+    prims = np.zeros(ne, np.int32)
+    nn = C.c_int(0)
+    _check(lib().femrt_host_build_bvh(ne, _ptr(boxes), C.byref(nn), _ptr(nodes), _ptr(prims)),
+           what="host_build_bvh")
+    return nodes[:nn.value].copy(), prims
+
+
+def load_xml_fld(xml_path, fld_path, flags=0) -> HexMesh:
+    L = lib()
+    ne, ng, nf = C.c_int(0), C.c_int(0), C.c_int(0)
+    rc = L.femrt_load_xml_fld(str(xml_path).encode(), str(fld_path).encode(), flags,
+                              C.byref(ne), C.byref(ng), C.byref(nf), None, None, None, None)
+    _check(rc, what="load_xml_fld(sizes)")
+    v = np.zeros((ne.value, 8, 3), np.float32)
+    g = np.zeros((ne.value, 3, ng.value ** 3), np.float32)
+    f = np.zeros((ne.value, nf.value ** 3), np.float32)
+    a = np.zeros((ne.value, 6, 2), np.int32)
+    rc = L.femrt_load_xml_fld(str(xml_path).encode(), str(fld_path).encode(), flags,
+                              C.byref(ne), C.byref(ng), C.byref(nf), _ptr(v), _ptr(g), _ptr(f),
+                              _ptr(a))
+    _check(rc, what="load_xml_fld")
+    return HexMesh(v, g, f, a, ng.value, nf.value)
+
+
+def auto_camera(lo, hi):
+    """Camera of main.cu:248-302: look at the box centre from
+    centre + (2.5, -2.5, -2.0) * max_dim, vup (0,1,0), vfov 20 degrees."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    c = 0.5 * (lo + hi)
+    md = float(np.max(hi - lo)) or 1.0
+    lf = c + np.array([2.5, -2.5, -2.0]) * md
+    return np.array([*lf, *c, 0.0, 1.0, 0.0, 20.0], np.float32)
+
+
+# ------------------------------------------------------------- device objects
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().femrt_ctx_create(device, C.byref(self.h)), what="ctx_create")
+
+    def close(self):
+        if self.h:
+            lib().femrt_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().femrt_sync(self.h), self.h, "sync")
+
+    def timing(self, on=True):
+        _check(lib().femrt_timing_enable(self.h, int(on)), self.h, "timing_enable")
+
+    def timing_reset(self):
+        _check(lib().femrt_timing_reset(self.h), self.h, "timing_reset")
+
+    def timing_read(self):
+        cap = 64
+        names = C.create_string_buffer(48 * cap)
+        ms = np.zeros(cap, np.float64)
+        cnt = np.zeros(cap, np.int64)
+        n = lib().femrt_timing_read(self.h, cap, names, _ptr(ms), _ptr(cnt))
+        if n < 0:
+            _check(n, self.h, "timing_read")
+        out = {}
+        raw = names.raw
+        for i in range(min(n, cap)):
+            nm = raw[48 * i:48 * (i + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[i]), int(cnt[i]))
+        return out
+
+
+class Mesh:
+    def __init__(self, ctx: Context, hm: HexMesh, build_bvh=True):
+        self.ctx = ctx
+        self.host = hm
+        self.h = C.c_void_p()
+        adj = hm.adjacency
+        _check(lib().femrt_mesh_upload_hex(ctx.h, hm.n_elem, hm.ng, hm.nf, _ptr(hm.vertices),
+                                           _ptr(hm.geom), _ptr(hm.field), _ptr(adj),
+                                           C.byref(self.h)), ctx.h, "mesh_upload_hex")
+        self.n_nodes = 0
+        if build_bvh:
+            self.build_bvh()
+
+    def build_bvh(self, builder=0):
+        nn = C.c_int(0)
+        _check(lib().femrt_mesh_build_bvh(self.h, builder, C.byref(nn)), self.ctx.h, "build_bvh")
+        self.n_nodes = nn.value
+
+    def boxes(self):
+        out = np.zeros((self.host.n_elem, 7, 6), np.float32)
+        _check(lib().femrt_mesh_boxes(self.h, _ptr(out)), self.ctx.h, "mesh_boxes")
+        return out
+
+    def bvh(self):
+        nodes = np.zeros((self.n_nodes, 8), np.float32)
+        prims = np.zeros(self.host.n_elem, np.int32)
+        _check(lib().femrt_mesh_get_bvh(self.h, _ptr(nodes), _ptr(prims)), self.ctx.h, "get_bvh")
+        return nodes, prims
+
+    def eval_field(self, elem, rst, precision=1):
+        elem = np.ascontiguousarray(elem, np.int32)
+        rst = np.ascontiguousarray(rst, np.float32).reshape(3, -1)
+        n = elem.size
+        out = np.zeros((4, n), np.float64 if precision else np.float32)
+        _check(lib().femrt_eval_field(self.h, n, _ptr(elem), _ptr(rst), precision, _ptr(out)),
+               self.ctx.h, "eval_field")
+        return out
+
+    def invert(self, elem, x):
+        elem = np.ascontiguousarray(elem, np.int32)
+        x = np.ascontiguousarray(x, np.float32).reshape(3, -1)
+        n = elem.size
+        rst = np.zeros((3, n), np.float64)
+        it = np.zeros(n, np.int32)
+        _check(lib().femrt_invert_points(self.h, n, _ptr(elem), _ptr(x), _ptr(rst), _ptr(it)),
+               self.ctx.h, "invert_points")
+        return rst, it
+
+    def close(self):
+        if self.h:
+            lib().femrt_mesh_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def trace_params(isos=(), guess=GUESS_PLAIN, samples=16, iso_enable=True, max_march=4096,
+                 flags=1):
+    p = TraceParams()
+    lib().femrt_trace_params_default(C.byref(p))
+    p.n_iso = len(isos)
+    p.iso_enable = int(iso_enable)
+    for i, v in enumerate(isos):
+        p.iso[i] = float(v)
+    p.samples = samples
+    p.guess = guess
+    p.max_march = max_march
+    p.flags = flags
+    return p
+
+
+class Frame:
+    def __init__(self, ctx: Context, n_rays: int):
+        self.ctx = ctx
+        self.n = int(n_rays)
+        self.h = C.c_void_p()
+        _check(lib().femrt_frame_create(ctx.h, self.n, C.byref(self.h)), ctx.h, "frame_create")
+
+    def set_rays(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(6, self.n)
+        _check(lib().femrt_frame_set_rays(self.h, _ptr(rays)), self.ctx.h, "set_rays")
+
+    def camera_rays(self, cam10, W, H):
+        cam = np.ascontiguousarray(cam10, np.float32)
+        _check(lib().femrt_frame_camera_rays(self.h, _ptr(cam), W, H), self.ctx.h, "camera_rays")
+
+    def get_rays(self):
+        out = np.zeros((6, self.n), np.float32)
+        _check(lib().femrt_frame_get_rays(self.h, _ptr(out)), self.ctx.h, "get_rays")
+        return out
+
+    def trace(self, mesh: Mesh, params: TraceParams):
+        _check(lib().femrt_trace(mesh.h, self.h, C.byref(params)), self.ctx.h, "trace")
+
+    def hits(self):
+        n = self.n
+        e = np.zeros(n, np.int32)
+        f = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float64)
+        r = np.zeros((3, n), np.float64)
+        u = np.zeros(n, np.float64)
+        _check(lib().femrt_frame_get_hits(self.h, _ptr(e), _ptr(f), _ptr(t), _ptr(r), _ptr(u)),
+               self.ctx.h, "get_hits")
+        return dict(elem=e, face=f, t=t, rst=r, u=u)
+
+    def iso(self, n_iso):
+        n = self.n
+        e = np.zeros((n_iso, n), np.int32)
+        t = np.zeros((n_iso, n), np.float64)
+        r = np.zeros((n_iso, 3, n), np.float64)
+        u = np.zeros((n_iso, n), np.float64)
+        _check(lib().femrt_frame_get_iso(self.h, n_iso, _ptr(e), _ptr(t), _ptr(r), _ptr(u)),
+               self.ctx.h, "get_iso")
+        return dict(elem=e, t=t, rst=r, u=u)
+
+    def rgba(self):
+        out = np.zeros(self.n, np.uint32)
+        _check(lib().femrt_frame_get_rgba(self.h, _ptr(out)), self.ctx.h, "get_rgba")
+        return out
+
+    def counters(self, enable=None):
+        out = np.zeros(NCOUNTERS, np.int64)
+        en = 1 if enable is None else int(enable)
+        _check(lib().femrt_frame_counters(self.h, en, _ptr(out)), self.ctx.h, "counters")
+        return dict(zip(COUNTER_NAMES, out.tolist()))
+
+    def close(self):
+        if self.h:
+            lib().femrt_frame_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def save_image_ppm(rgba, W, H, path):
+    """P6 writer of raytracer.cu:480-494 (rows written in buffer order, y=0 is
+    the bottom row of the image)."""
+    px = np.asarray(rgba, np.uint32).reshape(H * W)
+    rgb = np.stack([(px >> s) & 0xFF for s in (0, 8, 16)], axis=1).astype(np.uint8)
+    with open(path, "wb") as fh:
+        fh.write(f"P6\n{W} {H}\n255\n".encode())
+        fh.write(rgb.tobytes())

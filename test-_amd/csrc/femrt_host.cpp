@@ -1,0 +1,621 @@
+// femrt_host.cpp -- host-side algorithms of the library: GLL nodes, modal
+// fitting, the structured curved-hex generator, exact Bernstein bounding
+// boxes and the binned-SAH BVH builder.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "femrt_internal.hpp"
+
+namespace femrt {
+
+// ---------------------------------------------------------------- 1-D tools
+
+// Modified_A basis value (same definition as basis1d in femrt_math.hpp, in
+// double, used by the host fitting code).
+static double modA(int p, double x)
+{
+  if (p == 0) return 0.5 * (1.0 - x);
+  if (p == 1) return 0.5 * (1.0 + x);
+  const double b = 0.25 * (1.0 - x * x);
+  double pm2 = 1.0, pm1 = 2.0 * x;
+  if (p == 2) return b;
+  if (p == 3) return b * pm1;
+  double pk = pm1;
+  for (int q = 4; q <= p; ++q) {
+    const int k = q - 2;
+    const double A = (2.0 * k + 1.0) * (k + 1.0) / (k * (k + 2.0));
+    const double B = (k + 1.0) / (k + 2.0);
+    pk = A * x * pm1 - B * pm2;
+    pm2 = pm1;
+    pm1 = pk;
+  }
+  return b * pk;
+}
+
+static double legendre(int n, double x, double* pnm1)
+{
+  double p0 = 1.0, p1 = x;
+  if (n == 0) {
+    if (pnm1) *pnm1 = 0.0;
+    return 1.0;
+  }
+  for (int k = 2; k <= n; ++k) {
+    const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+    p0 = p1;
+    p1 = p2;
+  }
+  if (pnm1) *pnm1 = p0;
+  return p1;
+}
+
+// Gauss-Lobatto-Legendre nodes of n points on [-1,1], ascending: the roots of
+// (1-x^2) P'_{n-1}(x).  Newton iteration on x P_N - P_{N-1} = 0 started from
+// the Chebyshev-Gauss-Lobatto points (Canuto, Hussaini, Quarteroni & Zang,
+// Spectral Methods, 2.3).  The reference computes the same nodes
+// (xml_fld_math.cpp:60-112) and stores them as float.
+std::vector<double> gll_nodes(int n)
+{
+  std::vector<double> x(n);
+  if (n == 1) {
+    x[0] = 0.0;
+    return x;
+  }
+  const int N = n - 1;
+  for (int i = 0; i <= N; ++i) {
+    double xi = -std::cos(M_PI * i / N);
+    for (int it = 0; it < 100; ++it) {
+      double pm1;
+      const double p = legendre(N, xi, &pm1);
+      // f = x P_N - P_{N-1};  f' = (N+1) P_N
+      const double dx = (xi * p - pm1) / ((N + 1) * p);
+      xi -= dx;
+      if (std::fabs(dx) < 1e-16) break;
+    }
+    x[i] = xi;
+  }
+  x[0] = -1.0;
+  x[N] = 1.0;
+  for (int i = 0; i < n / 2; ++i) {  // exact antisymmetry
+    const double s = 0.5 * (x[N - i] - x[i]);
+    x[i] = -s;
+    x[N - i] = s;
+  }
+  if (n % 2) x[n / 2] = 0.0;
+  return x;
+}
+
+// Gauss-Jordan inverse with partial pivoting (double).
+static bool invert(std::vector<double> a, int n, std::vector<double>& inv)
+{
+  inv.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)piv * n + c])) piv = r;
+    if (std::fabs(a[(size_t)piv * n + c]) < 1e-300) return false;
+    if (piv != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[(size_t)c * n + k], a[(size_t)piv * n + k]);
+        std::swap(inv[(size_t)c * n + k], inv[(size_t)piv * n + k]);
+      }
+    const double ip = 1.0 / a[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      a[(size_t)c * n + k] *= ip;
+      inv[(size_t)c * n + k] *= ip;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = a[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[(size_t)r * n + k] -= f * a[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+      }
+    }
+  }
+  return true;
+}
+
+// inv = V^{-1}, V[i][a] = phi_a(node_i): modal coefficients from nodal values
+// (the 1-D factor of TensorModalFit3D, xml_fld_math.cpp:379-483).
+bool modal_fit_1d_matrix(int n, const std::vector<double>& nodes, std::vector<double>& inv)
+{
+  std::vector<double> V((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int a = 0; a < n; ++a) V[(size_t)i * n + a] = modA(a, nodes[i]);
+  return invert(V, n, inv);
+}
+
+// Lagrange interpolation matrix from nodes src (n) to nodes dst (m): m x n.
+static std::vector<double> lagrange_matrix(const std::vector<double>& src,
+                                           const std::vector<double>& dst)
+{
+  const int n = (int)src.size(), m = (int)dst.size();
+  std::vector<double> L((size_t)m * n);
+  for (int i = 0; i < m; ++i)
+    for (int a = 0; a < n; ++a) {
+      double v = 1.0;
+      for (int b = 0; b < n; ++b)
+        if (b != a) v *= (dst[i] - src[b]) / (src[a] - src[b]);
+      L[(size_t)i * n + a] = v;
+    }
+  return L;
+}
+
+// Tensor modal fit of nodal values on an n^3 grid (index (k*n+j)*n+i) to
+// modal coefficients with the same ordering.
+static void tensor_fit(int n, const std::vector<double>& inv, const double* nodal, double* out)
+{
+  std::vector<double> t1((size_t)n * n * n), t2((size_t)n * n * n);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int a = 0; a < n; ++a) {
+        double s = 0;
+        for (int i = 0; i < n; ++i) s += inv[(size_t)a * n + i] * nodal[((size_t)k * n + j) * n + i];
+        t1[((size_t)k * n + j) * n + a] = s;
+      }
+  for (int k = 0; k < n; ++k)
+    for (int b = 0; b < n; ++b)
+      for (int a = 0; a < n; ++a) {
+        double s = 0;
+        for (int j = 0; j < n; ++j) s += inv[(size_t)b * n + j] * t1[((size_t)k * n + j) * n + a];
+        t2[((size_t)k * n + b) * n + a] = s;
+      }
+  for (int c = 0; c < n; ++c)
+    for (int b = 0; b < n; ++b)
+      for (int a = 0; a < n; ++a) {
+        double s = 0;
+        for (int k = 0; k < n; ++k) s += inv[(size_t)c * n + k] * t2[((size_t)k * n + b) * n + a];
+        out[((size_t)c * n + b) * n + a] = s;
+      }
+}
+
+// ------------------------------------------------------------ random numbers
+
+static inline uint64_t mix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Standard normal keyed by a tuple: order-independent counter-based draw
+// (splitmix64 finaliser + Box-Muller).
+static double normal_keyed(uint64_t seed, uint64_t a, uint64_t b, uint64_t c)
+{
+  uint64_t h = mix64(seed ^ mix64(a ^ mix64(b ^ mix64(c))));
+  const uint64_t h2 = mix64(h ^ 0xD1B54A32D192ED03ull);
+  const double u1 = ((h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((h2 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+}
+
+// ------------------------------------------------------ structured generator
+
+// Local faces in Nektar order with the corner vertices that span (u, v):
+// grid (0,0)=c0, (max,0)=c1, (max,max)=c2, (0,max)=c3
+// (xml_fld_geometry_builder.cpp:85-97).
+static const int kFaceCorners[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                                       {3, 2, 6, 7}, {0, 3, 7, 4}, {4, 5, 6, 7}};
+
+int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom, float* field,
+                       int* adjacency)
+{
+  const int N = gp->n, q = gp->q, ng = gp->q + 1, nf = gp->p + 1;
+  if (N < 1 || q < 1 || gp->p < 1 || ng > 16 || nf > 16) return FEMRT_EINVAL;
+  const double h = 1.0 / N;
+  const std::vector<double> gll = gll_nodes(ng);
+  std::vector<double> even(ng);
+  for (int a = 0; a < ng; ++a) even[a] = -1.0 + 2.0 * a / q;
+  const std::vector<double> L = lagrange_matrix(even, gll);  // ng x ng
+  std::vector<double> inv;
+  modal_fit_1d_matrix(ng, gll, inv);
+  const size_t n3 = (size_t)ng * ng * ng, f3 = (size_t)nf * nf * nf;
+
+  // Integer lattice of geometry nodes: a face node of element (i,j,k) at local
+  // evenly spaced position (a,b) has lattice coordinates (i*q+.., ...), unique
+  // across elements, which keys its displacement.
+  auto displacement = [&](int axis, const long lat[3], const double w[3]) -> double {
+    if (gp->pert == 1) {
+      const int a0 = axis == 0 ? 1 : 0, a1 = axis == 2 ? 1 : 2;
+      return gp->amp * std::sin(2.0 * M_PI * w[a0]) * std::sin(2.0 * M_PI * w[a1]);
+    }
+    if (gp->pert == 2) {
+      double g = normal_keyed(gp->seed, (uint64_t)axis + 1,
+                              ((uint64_t)lat[0] << 21) ^ (uint64_t)lat[1], (uint64_t)lat[2]);
+      g = std::max(-3.0, std::min(3.0, g));
+      return gp->amp * h * g;
+    }
+    return 0.0;
+  };
+
+  std::vector<double> nodal(3 * n3), coef(n3);
+  for (int k = 0; k < N; ++k)
+    for (int j = 0; j < N; ++j)
+      for (int i = 0; i < N; ++i) {
+        const size_t e = ((size_t)k * N + j) * N + i;
+        double V[8][3];
+        const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                               {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+        for (int v = 0; v < 8; ++v) {
+          V[v][0] = (i + off[v][0]) * h;
+          V[v][1] = (j + off[v][1]) * h;
+          V[v][2] = (k + off[v][2]) * h;
+          if (vertices)
+            for (int c = 0; c < 3; ++c) vertices[e * 24 + v * 3 + c] = (float)V[v][c];
+        }
+        if (geom) {
+          // trilinear nodal values on the GLL grid
+          for (int kk = 0; kk < ng; ++kk)
+            for (int jj = 0; jj < ng; ++jj)
+              for (int ii = 0; ii < ng; ++ii) {
+                const double r = gll[ii], s = gll[jj], t = gll[kk];
+                const size_t id = ((size_t)kk * ng + jj) * ng + ii;
+                nodal[0 * n3 + id] = (i + 0.5 * (r + 1)) * h;
+                nodal[1 * n3 + id] = (j + 0.5 * (s + 1)) * h;
+                nodal[2 * n3 + id] = (k + 0.5 * (t + 1)) * h;
+              }
+          // curved faces: evenly spaced face nodes displaced along the face
+          // normal axis, resampled to GLL and blended linearly into the volume
+          // (ResampleQuadFaceLegendre + blend_face,
+          // xml_fld_geometry_builder.cpp:99-233, 350-377).
+          for (int lf = 0; lf < 6; ++lf) {
+            const int fix = (lf == 0 || lf == 5) ? 2 : ((lf == 1 || lf == 3) ? 1 : 0);
+            const int fixside = (lf == 2 || lf == 3 || lf == 5) ? 1 : 0;
+            const int ua = (lf == 2 || lf == 4) ? 1 : 0;
+            const int va = (lf == 0 || lf == 5) ? 1 : 2;
+            // displacement on the evenly spaced face grid (zero on the face's
+            // boundary ring: edges stay straight)
+            std::vector<double> de((size_t)ng * ng, 0.0);
+            bool any = false;
+            for (int b = 1; b < q; ++b)
+              for (int a = 1; a < q; ++a) {
+                long lat[3];
+                double w[3];
+                const int base[3] = {i, j, k};
+                lat[fix] = (long)(base[fix] + fixside) * q;
+                lat[ua] = (long)base[ua] * q + a;
+                lat[va] = (long)base[va] * q + b;
+                for (int c = 0; c < 3; ++c) w[c] = (double)lat[c] / ((double)N * q);
+                de[(size_t)b * ng + a] = displacement(fix, lat, w);
+                if (de[(size_t)b * ng + a] != 0.0) any = true;
+              }
+            if (!any) continue;
+            // resample to GLL face nodes: D = L de L^T
+            std::vector<double> tmp((size_t)ng * ng, 0.0), dg((size_t)ng * ng, 0.0);
+            for (int b = 0; b < ng; ++b)
+              for (int a2 = 0; a2 < ng; ++a2) {
+                double s = 0;
+                for (int a = 0; a < ng; ++a) s += L[(size_t)a2 * ng + a] * de[(size_t)b * ng + a];
+                tmp[(size_t)b * ng + a2] = s;
+              }
+            for (int b2 = 0; b2 < ng; ++b2)
+              for (int a2 = 0; a2 < ng; ++a2) {
+                double s = 0;
+                for (int b = 0; b < ng; ++b) s += L[(size_t)b2 * ng + b] * tmp[(size_t)b * ng + a2];
+                dg[(size_t)b2 * ng + a2] = s;
+              }
+            // blend along the normal coordinate with weight (1 -+ w)/2
+            for (int b = 0; b < ng; ++b)
+              for (int a = 0; a < ng; ++a) {
+                const double dv = dg[(size_t)b * ng + a];
+                if (dv == 0.0) continue;
+                for (int m = 0; m < ng; ++m) {
+                  int idx3[3];
+                  idx3[ua] = a;
+                  idx3[va] = b;
+                  idx3[fix] = m;
+                  const double wn = fixside ? 0.5 * (1.0 + gll[m]) : 0.5 * (1.0 - gll[m]);
+                  const size_t id = ((size_t)idx3[2] * ng + idx3[1]) * ng + idx3[0];
+                  nodal[(size_t)fix * n3 + id] += dv * wn;
+                }
+              }
+          }
+          for (int c = 0; c < 3; ++c) {
+            tensor_fit(ng, inv, &nodal[(size_t)c * n3], coef.data());
+            for (size_t m = 0; m < n3; ++m) geom[(e * 3 + c) * n3 + m] = (float)coef[m];
+          }
+        }
+        if (field) {
+          for (int kk = 0; kk < nf; ++kk)
+            for (int jj = 0; jj < nf; ++jj)
+              for (int ii = 0; ii < nf; ++ii) {
+                const size_t m = ((size_t)kk * nf + jj) * nf + ii;
+                const double g = normal_keyed(gp->seed ^ 0xF1E1Dull, e, m, 7);
+                const double den = 1.0 + ii + jj + kk;
+                field[e * f3 + m] = (float)(g / (den * den));
+              }
+        }
+        if (adjacency) {
+          const int nb[6][3] = {{0, 0, -1}, {0, -1, 0}, {1, 0, 0}, {0, 1, 0}, {-1, 0, 0}, {0, 0, 1}};
+          const int opp[6] = {5, 3, 4, 1, 2, 0};
+          for (int f = 0; f < 6; ++f) {
+            const int ni = i + nb[f][0], nj = j + nb[f][1], nk = k + nb[f][2];
+            int ne = -1;
+            if (ni >= 0 && nj >= 0 && nk >= 0 && ni < N && nj < N && nk < N)
+              ne = (int)(((size_t)nk * N + nj) * N + ni);
+            adjacency[e * 12 + f * 2 + 0] = ne;
+            adjacency[e * 12 + f * 2 + 1] = ne >= 0 ? opp[f] : -1;
+          }
+        }
+      }
+  (void)kFaceCorners;
+  return FEMRT_OK;
+}
+
+// ---------------------------------------------------------- Bernstein boxes
+
+// Modal -> Bernstein change of basis for n modes, restricted to the
+// sub-interval [a, b] of [-1, 1] and reparametrised to [-1, 1]:
+//   phi_q(a + (b-a)(x+1)/2) = sum_m T[q][m] B_m(x),
+//   B_m(x) = C(n-1,m) ((1+x)/2)^m ((1-x)/2)^(n-1-m).
+// Bernstein coefficients of a tensor polynomial bound it on its domain
+// (convex hull property), so their min/max is a guaranteed box, and a face's
+// patch is bounded by the boundary layer of the net.  Splitting [-1,1] into S
+// pieces per direction (S^3 sub-nets) tightens the bound: the hull of a
+// sign-alternating bubble overshoots by up to ~4.5x per direction on the whole
+// interval, and the overshoot shrinks quadratically with the piece length.
+static std::vector<double> modal_to_bernstein(int n, double a, double b)
+{
+  std::vector<double> xs(n), VB((size_t)n * n), VP((size_t)n * n), iVB;
+  for (int p = 0; p < n; ++p) xs[p] = n == 1 ? 0.0 : -1.0 + 2.0 * p / (n - 1);
+  for (int p = 0; p < n; ++p) {
+    const double wa = 0.5 * (1 + xs[p]), wb = 0.5 * (1 - xs[p]);
+    for (int m = 0; m < n; ++m) {
+      double c = 1.0;
+      for (int r = 1; r <= m; ++r) c = c * (n - 1 - m + r) / r;
+      VB[(size_t)p * n + m] = c * std::pow(wa, m) * std::pow(wb, n - 1 - m);
+    }
+    const double y = a + (b - a) * 0.5 * (xs[p] + 1.0);
+    for (int q = 0; q < n; ++q) VP[(size_t)p * n + q] = modA(q, y);
+  }
+  invert(VB, n, iVB);
+  // VP = VB T^T  ->  T[q][m] = sum_p iVB[m][p] VP[p][q]
+  std::vector<double> T((size_t)n * n);
+  for (int q = 0; q < n; ++q)
+    for (int m = 0; m < n; ++m) {
+      double s = 0;
+      for (int p = 0; p < n; ++p) s += iVB[(size_t)m * n + p] * VP[(size_t)p * n + q];
+      T[(size_t)q * n + m] = s;
+    }
+  return T;
+}
+
+static const int kBoxSplit = 3;
+
+void bernstein_boxes(int n_elem, int ng, const float* geom, float* out)
+{
+  const int n = ng, S = kBoxSplit;
+  const size_t n3 = (size_t)n * n * n;
+  std::vector<std::vector<double>> T(S);
+  for (int s = 0; s < S; ++s) T[s] = modal_to_bernstein(n, -1.0 + 2.0 * s / S, -1.0 + 2.0 * (s + 1) / S);
+  std::vector<double> cc(3 * n3), t1(3 * n3), t2(3 * n3), bz(3 * n3);
+  for (int e = 0; e < n_elem; ++e) {
+    double lo[7][3], hi[7][3];
+    for (int b = 0; b < 7; ++b)
+      for (int c = 0; c < 3; ++c) {
+        lo[b][c] = 1e300;
+        hi[b][c] = -1e300;
+      }
+    for (size_t m = 0; m < 3 * n3; ++m) cc[m] = geom[(size_t)e * 3 * n3 + m];
+    for (int s1 = 0; s1 < S; ++s1) {
+      const double* A = T[s1].data();
+      for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < n; ++k)
+          for (int j = 0; j < n; ++j)
+            for (int m = 0; m < n; ++m) {
+              double acc = 0;
+              for (int i = 0; i < n; ++i) acc += cc[c * n3 + ((size_t)k * n + j) * n + i] * A[(size_t)i * n + m];
+              t1[c * n3 + ((size_t)k * n + j) * n + m] = acc;
+            }
+      for (int s2 = 0; s2 < S; ++s2) {
+        const double* B = T[s2].data();
+        for (int c = 0; c < 3; ++c)
+          for (int k = 0; k < n; ++k)
+            for (int m2 = 0; m2 < n; ++m2)
+              for (int m = 0; m < n; ++m) {
+                double acc = 0;
+                for (int j = 0; j < n; ++j) acc += t1[c * n3 + ((size_t)k * n + j) * n + m] * B[(size_t)j * n + m2];
+                t2[c * n3 + ((size_t)k * n + m2) * n + m] = acc;
+              }
+        for (int s3 = 0; s3 < S; ++s3) {
+          const double* Cm = T[s3].data();
+          for (int c = 0; c < 3; ++c)
+            for (int m3 = 0; m3 < n; ++m3)
+              for (int m2 = 0; m2 < n; ++m2)
+                for (int m = 0; m < n; ++m) {
+                  double acc = 0;
+                  for (int k = 0; k < n; ++k) acc += t2[c * n3 + ((size_t)k * n + m2) * n + m] * Cm[(size_t)k * n + m3];
+                  bz[c * n3 + ((size_t)m3 * n + m2) * n + m] = acc;
+                }
+          const int sub[3] = {s1, s2, s3};
+          for (int m3 = 0; m3 < n; ++m3)
+            for (int m2 = 0; m2 < n; ++m2)
+              for (int m = 0; m < n; ++m) {
+                const int idx[3] = {m, m2, m3};
+                double v[3];
+                for (int c = 0; c < 3; ++c) v[c] = bz[c * n3 + ((size_t)m3 * n + m2) * n + m];
+                for (int c = 0; c < 3; ++c) {
+                  lo[0][c] = std::min(lo[0][c], v[c]);
+                  hi[0][c] = std::max(hi[0][c], v[c]);
+                }
+                for (int f = 0; f < 6; ++f) {
+                  const int fix = (f == 0 || f == 5) ? 2 : ((f == 1 || f == 3) ? 1 : 0);
+                  const bool plus = (f == 2 || f == 3 || f == 5);
+                  if (sub[fix] != (plus ? S - 1 : 0) || idx[fix] != (plus ? n - 1 : 0)) continue;
+                  for (int c = 0; c < 3; ++c) {
+                    lo[1 + f][c] = std::min(lo[1 + f][c], v[c]);
+                    hi[1 + f][c] = std::max(hi[1 + f][c], v[c]);
+                  }
+                }
+              }
+        }
+      }
+    }
+    for (int b = 0; b < 7; ++b) {
+      double ext = 0;
+      for (int c = 0; c < 3; ++c) ext = std::max(ext, hi[b][c] - lo[b][c]);
+      for (int c = 0; c < 3; ++c) {
+        const double pad = 1e-6 * ext + 1e-6 * std::max(std::fabs(lo[b][c]), std::fabs(hi[b][c])) + 1e-9;
+        out[((size_t)e * 7 + b) * 6 + c] = std::nextafter((float)(lo[b][c] - pad), -INFINITY);
+        out[((size_t)e * 7 + b) * 6 + 3 + c] = std::nextafter((float)(hi[b][c] + pad), INFINITY);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ binned SAH BVH
+
+// Binned SAH over element-box centroids, 16 bins per axis, leaves of <= 4
+// elements, split only when the SAH cost beats the leaf cost (the policy of
+// BVH.cu:35-177).  Nodes: lo.xyz, a, hi.xyz, b; internal: a,b = children;
+// leaf: a = first slot in prims, b = -count.
+namespace {
+struct Builder {
+  const float* boxes;
+  std::vector<int> idx;
+  std::vector<float> cen;  // 3 per element
+  BvhHost* out;
+
+  void box_of(int s, int e, float* lo, float* hi)
+  {
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = INFINITY;
+      hi[c] = -INFINITY;
+    }
+    for (int i = s; i < e; ++i) {
+      const float* b = boxes + (size_t)idx[i] * 42;
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::min(lo[c], b[c]);
+        hi[c] = std::max(hi[c], b[3 + c]);
+      }
+    }
+  }
+  static float area(const float* lo, const float* hi)
+  {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0f * (dx * dy + dx * dz + dy * dz);
+  }
+  int emit(const float* lo, const float* hi, int a, int b)
+  {
+    const int id = (int)(out->nodes.size() / 8);
+    float rec[8];
+    rec[0] = lo[0]; rec[1] = lo[1]; rec[2] = lo[2];
+    std::memcpy(&rec[3], &a, 4);
+    rec[4] = hi[0]; rec[5] = hi[1]; rec[6] = hi[2];
+    std::memcpy(&rec[7], &b, 4);
+    out->nodes.insert(out->nodes.end(), rec, rec + 8);
+    return id;
+  }
+  void set_children(int node, int a, int b)
+  {
+    std::memcpy(&out->nodes[(size_t)node * 8 + 3], &a, 4);
+    std::memcpy(&out->nodes[(size_t)node * 8 + 7], &b, 4);
+  }
+  int build(int s, int e)
+  {
+    float lo[3], hi[3];
+    box_of(s, e, lo, hi);
+    const int count = e - s;
+    if (count <= 4) return emit(lo, hi, s, -count);
+    // centroid bounds
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = s; i < e; ++i)
+      for (int c = 0; c < 3; ++c) {
+        clo[c] = std::min(clo[c], cen[(size_t)idx[i] * 3 + c]);
+        chi[c] = std::max(chi[c], cen[(size_t)idx[i] * 3 + c]);
+      }
+    const int NB = 16;
+    float best = INFINITY;
+    int bax = -1, bsplit = -1;
+    const float parea = area(lo, hi);
+    for (int ax = 0; ax < 3; ++ax) {
+      const float ext = chi[ax] - clo[ax];
+      if (!(ext > 1e-12f)) continue;
+      int cnt[NB] = {0};
+      float blo[NB][3], bhi[NB][3];
+      for (int b = 0; b < NB; ++b)
+        for (int c = 0; c < 3; ++c) {
+          blo[b][c] = INFINITY;
+          bhi[b][c] = -INFINITY;
+        }
+      for (int i = s; i < e; ++i) {
+        int b = (int)(NB * (cen[(size_t)idx[i] * 3 + ax] - clo[ax]) / ext);
+        b = std::max(0, std::min(NB - 1, b));
+        ++cnt[b];
+        const float* bx = boxes + (size_t)idx[i] * 42;
+        for (int c = 0; c < 3; ++c) {
+          blo[b][c] = std::min(blo[b][c], bx[c]);
+          bhi[b][c] = std::max(bhi[b][c], bx[3 + c]);
+        }
+      }
+      // sweep
+      float rA[NB], rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      int rC[NB], rc = 0;
+      for (int b = NB - 1; b > 0; --b) {
+        rc += cnt[b];
+        for (int c = 0; c < 3; ++c) {
+          rlo[c] = std::min(rlo[c], blo[b][c]);
+          rhi[c] = std::max(rhi[c], bhi[b][c]);
+        }
+        rA[b] = rc ? area(rlo, rhi) : 0.f;
+        rC[b] = rc;
+      }
+      float llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      int lc = 0;
+      for (int b = 0; b < NB - 1; ++b) {
+        lc += cnt[b];
+        for (int c = 0; c < 3; ++c) {
+          llo[c] = std::min(llo[c], blo[b][c]);
+          lhi[c] = std::max(lhi[c], bhi[b][c]);
+        }
+        if (lc == 0 || rC[b + 1] == 0) continue;
+        const float cost = (area(llo, lhi) * lc + rA[b + 1] * rC[b + 1]) / parea;
+        if (cost < best) {
+          best = cost;
+          bax = ax;
+          bsplit = b;
+        }
+      }
+    }
+    if (bax < 0 || best >= (float)count) return emit(lo, hi, s, -count);
+    const float ext = chi[bax] - clo[bax];
+    auto mid_it = std::partition(idx.begin() + s, idx.begin() + e, [&](int id) {
+      int b = (int)(NB * (cen[(size_t)id * 3 + bax] - clo[bax]) / ext);
+      b = std::max(0, std::min(NB - 1, b));
+      return b <= bsplit;
+    });
+    int mid = (int)(mid_it - idx.begin());
+    if (mid == s || mid == e) return emit(lo, hi, s, -count);
+    const int node = emit(lo, hi, 0, 0);
+    const int l = build(s, mid);
+    const int r = build(mid, e);
+    set_children(node, l, r);
+    return node;
+  }
+};
+}  // namespace
+
+void build_bvh_sah(int n_elem, const float* boxes, BvhHost& out)
+{
+  Builder b;
+  b.boxes = boxes;
+  b.out = &out;
+  out.nodes.clear();
+  b.idx.resize(n_elem);
+  std::iota(b.idx.begin(), b.idx.end(), 0);
+  b.cen.resize((size_t)n_elem * 3);
+  for (int e = 0; e < n_elem; ++e)
+    for (int c = 0; c < 3; ++c)
+      b.cen[(size_t)e * 3 + c] = 0.5f * (boxes[(size_t)e * 42 + c] + boxes[(size_t)e * 42 + 3 + c]);
+  if (n_elem > 0) b.build(0, n_elem);
+  out.prims = b.idx;
+}
+
+}  // namespace femrt

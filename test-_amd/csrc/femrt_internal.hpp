@@ -1,0 +1,32 @@
+// femrt_internal.hpp -- private declarations shared by the library's sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/femrt.h"
+
+namespace femrt {
+
+// Per-mesh host-side products of the build (Bernstein boxes, BVH).
+struct BvhHost {
+  std::vector<float> nodes;  // n_nodes x 8 (lo.xyz, a, hi.xyz, b), a/b int bits
+  std::vector<int> prims;
+};
+
+// Host algorithms (femrt_host.cpp)
+void bernstein_boxes(int n_elem, int ng, const float* geom, float* out /* n x 7 x 6 */);
+void build_bvh_sah(int n_elem, const float* boxes /* n x 7 x 6, element box first */,
+                   BvhHost& out);
+int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom, float* field,
+                       int* adjacency);
+std::vector<double> gll_nodes(int n);
+bool modal_fit_1d_matrix(int n, const std::vector<double>& nodes, std::vector<double>& inv);
+int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_elem, int* ng,
+                 int* nf, float* vertices, float* geom, float* field, int* adjacency,
+                 std::string* err);
+
+}  // namespace femrt

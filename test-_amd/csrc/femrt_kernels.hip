@@ -1,0 +1,1502 @@
+// femrt_kernels.hip -- gfx950 kernels of the curved-hex ray tracer and the
+// C ABI that drives them.
+//
+// Pipeline of one trace (all on the context stream):
+//   k_first_hit<NG>      one lane per ray: BVH traversal over element boxes,
+//                        per-face box cull, face-constrained Newton in FP32;
+//                        hit rays are compacted into a queue with a wave
+//                        ballot + prefix and one atomic per wave.
+//   k_polish_hit<NG,NF>  FP64 Newton polish of each hit on its face, FP64
+//                        field value at the hit.
+//   k_march<NG,NF>       per hit ray: element walk through face adjacency;
+//                        in each element the exit face root (FP32), uniform
+//                        samples of g = u - iso over [t_in, t_out] with point
+//                        inversion + field evaluation (FP32), first sign
+//                        change per isovalue refined in FP32; brackets queued.
+//   k_polish_iso<NG,NF>  FP64 safeguarded Newton/bisection endgame per
+//                        (ray, isovalue) bracket.
+//   k_shade              RGBA from the AOVs.
+// Reference: raytracer.cu:102-254 (traversal + hex branch), high_order.h:
+// 189-304, 409-465 (face Newton, nearest face), fem/HexElement.cu:227-272
+// (point inversion), fem/RayIsosurface.cu:67-188 (iso search).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "femrt_internal.hpp"
+#include "femrt_math.hpp"
+
+using namespace femrt;
+
+// ============================================================== host objects
+
+struct TimedLaunch {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+struct femrt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool timing = false;
+  std::vector<TimedLaunch> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long long>> stats;
+};
+
+struct femrt_mesh {
+  femrt_ctx* ctx = nullptr;
+  int n_elem = 0, ng = 0, nf = 0;      // true mode counts
+  int NG = 0, NF = 0;                  // kernel mode counts (padded)
+  float4* d_geom = nullptr;            // n_elem x NG^3
+  float* d_field = nullptr;            // n_elem x NF^3
+  float* d_verts = nullptr;            // n_elem x 24
+  float* d_boxes = nullptr;            // n_elem x 42
+  int* d_adj = nullptr;                // n_elem x 12
+  float4* d_nodes = nullptr;           // n_nodes x 2
+  int* d_prims = nullptr;
+  int n_nodes = 0;
+  int bvh_depth = 0;
+  std::vector<float> h_boxes;
+  BvhHost bvh;
+};
+
+struct femrt_frame {
+  femrt_ctx* ctx = nullptr;
+  int n = 0;
+  float* d_rays = nullptr;
+  int* d_hit_elem = nullptr;
+  int* d_hit_face = nullptr;
+  float4* d_hit_f32 = nullptr;  // u, v, tau, unused
+  double* d_hit_t = nullptr;
+  double* d_hit_rst = nullptr;  // 3 x n
+  double* d_hit_u = nullptr;
+  int* d_iso_elem = nullptr;    // MAX_ISO x n
+  double* d_iso_t = nullptr;
+  double* d_iso_rst = nullptr;  // MAX_ISO x 3 x n
+  double* d_iso_u = nullptr;
+  float4* d_iso_f32 = nullptr;  // MAX_ISO x n: (t_a, t_b, t*, packed signs)
+  float4* d_iso_rst32 = nullptr;  // MAX_ISO x n: rst* (FP32)
+  int* d_queue_hit = nullptr;     // n
+  int* d_queue_iso = nullptr;     // MAX_ISO x n, entries r*MAX_ISO+k
+  int* d_counts = nullptr;        // [0] hits [1] iso items
+  uint32_t* d_rgba = nullptr;
+  long long* d_work = nullptr;
+  bool counters = false;
+};
+
+static int fail(femrt_ctx* ctx, int code, const std::string& msg)
+{
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((ctx), FEMRT_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+static hipEvent_t take_event(femrt_ctx* c)
+{
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static void collect_timing(femrt_ctx* c)
+{
+  for (auto& t : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      auto& s = c->stats[t.name];
+      s.first += ms;
+      s.second += 1;
+    }
+    c->pool.push_back(t.a);
+    c->pool.push_back(t.b);
+  }
+  c->pending.clear();
+}
+
+// Launch wrapper: records HIP events around the launch on the context stream
+// when timing is enabled.
+template <typename F>
+static int timed_launch(femrt_ctx* c, const char* name, F&& launch)
+{
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = take_event(c);
+    b = take_event(c);
+    if (a) (void)hipEventRecord(a, c->stream);
+  }
+  launch();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, FEMRT_EHIP, std::string("launch ") + name + ": " + hipGetErrorString(e));
+  if (c->timing && a && b) {
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({name, a, b});
+  }
+  return FEMRT_OK;
+}
+
+// ============================================================ device helpers
+
+__device__ __forceinline__ void face_to_rst(int f, float u, float v, float& r, float& s, float& t)
+{
+  r = (f == 2) ? 1.f : ((f == 4) ? -1.f : u);
+  s = (f == 0 || f == 5) ? v : ((f == 1) ? -1.f : ((f == 3) ? 1.f : u));
+  t = (f == 0) ? -1.f : ((f == 5) ? 1.f : v);
+}
+
+template <typename T>
+__device__ __forceinline__ void face_to_rst_t(int f, T u, T v, T* rst)
+{
+  rst[0] = (f == 2) ? T(1) : ((f == 4) ? T(-1) : u);
+  rst[1] = (f == 0 || f == 5) ? v : ((f == 1) ? T(-1) : ((f == 3) ? T(1) : u));
+  rst[2] = (f == 0) ? T(-1) : ((f == 5) ? T(1) : v);
+}
+
+__device__ __forceinline__ void rst_to_face(int f, const float* rst, float& u, float& v)
+{
+  u = (f == 2 || f == 4) ? rst[1] : rst[0];
+  v = (f == 0 || f == 5) ? rst[1] : rst[2];
+}
+
+struct Counters {
+  long long face_it = 0, inv_it = 0, field = 0, geo64 = 0, field64 = 0, slab = 0, mlp = 0,
+            marched = 0;
+};
+
+// Face-constrained Newton on (u, v, tau) from one seed, FP32, with the
+// reference's update rules (high_order.h:213-300): step capped to 0.25 in the
+// inf-norm, tau clamped to [tmin, tmax], u, v clamped to [-1.5, 1.5],
+// divergence break after 5 iterations when the residual grows 1.5x, at most
+// 50 iterations.  Converged roots within max(|u|,|v|) <= 1 + 1e-4 are accepted.
+template <int NG>
+__device__ bool face_newton(const float4* __restrict__ gc, const float* o, const float* d, int f,
+                            float u, float v, float tau, float tmin, float tmax, float& tau_o,
+                            float& u_o, float& v_o, int& conv, long long& iters)
+{
+  float prev = FLT_MAX;
+  const float oscale = 1.f + fmaxf(fabsf(o[0]), fmaxf(fabsf(o[1]), fabsf(o[2])));
+  tau = fminf(fmaxf(tau, tmin), tmax);
+  conv = 0;
+  for (int it = 0; it < 50; ++it) {
+    float r, s, t;
+    face_to_rst(f, u, v, r, s, t);
+    float X[3], J[9];
+    geom_eval<float, NG>(gc, r, s, t, X, J);
+    ++iters;
+    const float R0 = X[0] - fmaf(tau, d[0], o[0]);
+    const float R1 = X[1] - fmaf(tau, d[1], o[1]);
+    const float R2 = X[2] - fmaf(tau, d[2], o[2]);
+    const float res = fmaxf(fabsf(R0), fmaxf(fabsf(R1), fabsf(R2)));
+    const float tol = 2e-6f * (oscale + fabsf(tau));
+    if (res < tol) {
+      conv = 1;
+      if (fmaxf(fabsf(u), fabsf(v)) <= 1.0001f && tau >= tmin && tau <= tmax) {
+        tau_o = tau;
+        u_o = u;
+        v_o = v;
+        return true;
+      }
+      return false;
+    }
+    if (it > 5 && res > prev * 1.5f) return false;
+    prev = res;
+    const bool us = (f == 2 || f == 4);
+    const bool vs = (f == 0 || f == 5);
+    const float A[9] = {us ? J[1] : J[0], vs ? J[1] : J[2], -d[0],
+                        us ? J[4] : J[3], vs ? J[4] : J[5], -d[1],
+                        us ? J[7] : J[6], vs ? J[7] : J[8], -d[2]};
+    const float b[3] = {-R0, -R1, -R2};
+    float dl[3];
+    if (!solve3<float>(A, b, dl)) return false;
+    const float mx = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
+    const float sc = mx > 0.25f ? 0.25f / mx : 1.f;
+    u = fminf(fmaxf(u + dl[0] * sc, -1.5f), 1.5f);
+    v = fminf(fmaxf(v + dl[1] * sc, -1.5f), 1.5f);
+    tau = fminf(fmaxf(tau + dl[2] * sc, tmin), tmax);
+  }
+  return false;
+}
+
+// Projection of X(seed) onto the ray (high_order.h:154-161, 221-225).
+template <int NG>
+__device__ float seed_tau(const float4* __restrict__ gc, const float* o, const float* d, int f,
+                          float u, float v)
+{
+  float r, s, t, X[3];
+  face_to_rst(f, u, v, r, s, t);
+  geom_point<float, NG>(gc, r, s, t, X);
+  const float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  return ((X[0] - o[0]) * d[0] + (X[1] - o[1]) * d[1] + (X[2] - o[2]) * d[2]) / dd;
+}
+
+// Flat-face seed: the ray meets the plane through the face's corner centroid
+// with normal (c2-c0)x(c3-c1); (u,v) from the parallelogram spanned by the
+// averaged edge vectors.  Returns false when the ray is parallel to it.
+__device__ bool flat_face_seed(const float* __restrict__ vx, const float* o, const float* d, int f,
+                               float& u, float& v, float& tau)
+{
+  const int C[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5}, {3, 2, 6, 7}, {0, 3, 7, 4}, {4, 5, 6, 7}};
+  float c[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int vi = C[f][q];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[q][k] = vx[vi * 3 + k];
+  }
+  float cen[3], eu[3], ev[3], a[3], b[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    cen[k] = 0.25f * (c[0][k] + c[1][k] + c[2][k] + c[3][k]);
+    eu[k] = 0.5f * ((c[1][k] - c[0][k]) + (c[2][k] - c[3][k]));
+    ev[k] = 0.5f * ((c[3][k] - c[0][k]) + (c[2][k] - c[1][k]));
+    a[k] = c[2][k] - c[0][k];
+    b[k] = c[3][k] - c[1][k];
+  }
+  const float n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const float dn = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
+  const float nn = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (!(fabsf(dn) > 1e-6f * nn)) return false;
+  tau = ((cen[0] - o[0]) * n[0] + (cen[1] - o[1]) * n[1] + (cen[2] - o[2]) * n[2]) / dn;
+  float q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) q[k] = o[k] + tau * d[k] - cen[k];
+  const float uu = eu[0] * eu[0] + eu[1] * eu[1] + eu[2] * eu[2];
+  const float uv = eu[0] * ev[0] + eu[1] * ev[1] + eu[2] * ev[2];
+  const float vv = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2];
+  const float qu = q[0] * eu[0] + q[1] * eu[1] + q[2] * eu[2];
+  const float qv = q[0] * ev[0] + q[1] * ev[1] + q[2] * ev[2];
+  const float det = uu * vv - uv * uv;
+  if (!(det > 0.f)) return false;
+  u = fminf(fmaxf(2.f * (qu * vv - qv * uv) / det, -1.5f), 1.5f);
+  v = fminf(fmaxf(2.f * (qv * uu - qu * uv) / det, -1.5f), 1.5f);
+  return true;
+}
+
+// Nearest accepted root of one face.  PLAIN: the reference's 9 seeds
+// {-0.8,0,0.8}^2, all run, nearest kept.  WARM/LEARNED: one flat-face seed,
+// the 9 seeds only when that seed does not converge.
+template <int NG>
+__device__ bool face_root(const float4* __restrict__ gc, const float* __restrict__ vx, const float* o,
+                          const float* d, int f, float tmin, float tmax, int guess, float& tau_o,
+                          float& u_o, float& v_o, long long& iters)
+{
+  if (guess != FEMRT_GUESS_PLAIN) {
+    float u, v, tau;
+    if (flat_face_seed(vx, o, d, f, u, v, tau)) {
+      int conv;
+      if (face_newton<NG>(gc, o, d, f, u, v, tau, tmin, tmax, tau_o, u_o, v_o, conv, iters)) return true;
+      if (conv) return false;
+    }
+  }
+  bool found = false;
+  float best = FLT_MAX;
+  for (int sj = 0; sj < 3; ++sj)
+    for (int si = 0; si < 3; ++si) {
+      const float su = -0.8f + 0.8f * si, sv = -0.8f + 0.8f * sj;
+      const float tau0 = seed_tau<NG>(gc, o, d, f, su, sv);
+      float tt, uu, vv;
+      int conv;
+      if (face_newton<NG>(gc, o, d, f, su, sv, tau0, tmin, tmax, tt, uu, vv, conv, iters)) {
+        if (tt < best) {
+          best = tt;
+          tau_o = tt;
+          u_o = uu;
+          v_o = vv;
+          found = true;
+        }
+      }
+    }
+  return found;
+}
+
+__device__ __forceinline__ void load_ray(const float* __restrict__ rays, int n, int r, float* o, float* d)
+{
+  o[0] = rays[r];
+  o[1] = rays[(size_t)n + r];
+  o[2] = rays[2 * (size_t)n + r];
+  d[0] = rays[3 * (size_t)n + r];
+  d[1] = rays[4 * (size_t)n + r];
+  d[2] = rays[5 * (size_t)n + r];
+}
+
+__device__ __forceinline__ void add_work(long long* work, int idx, long long v)
+{
+  if (v) atomicAdd((unsigned long long*)&work[idx], (unsigned long long)v);
+}
+
+// Append `pred` lanes to a queue: one atomic per wave, ballot + prefix count
+// for the slot of each lane.
+__device__ __forceinline__ int wave_append(bool pred, int* counter)
+{
+  const unsigned long long mask = __ballot(pred);
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (mask) {
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+  }
+  return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// ================================================================== kernels
+
+__global__ void k_raygen(float* __restrict__ rays, int W, int H, float ox, float oy, float oz,
+                         float lx, float ly, float lz, float hx, float hy, float hz, float vx,
+                         float vy, float vz)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = W * H;
+  if (p >= n) return;
+  const int y = p / W, x = p - y * W;
+  const float s = (x + 0.5f) / W, t = (y + 0.5f) / H;
+  float d0 = lx + s * hx + t * vx - ox;
+  float d1 = ly + s * hy + t * vy - oy;
+  float d2 = lz + s * hz + t * vz - oz;
+  const float il = 1.f / sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+  rays[p] = ox;
+  rays[n + p] = oy;
+  rays[2 * n + p] = oz;
+  rays[3 * n + p] = d0 * il;
+  rays[4 * n + p] = d1 * il;
+  rays[5 * n + p] = d2 * il;
+}
+
+#define FEMRT_STACK 64
+
+template <int NG>
+__global__ __launch_bounds__(256) void k_first_hit(
+    const float* __restrict__ rays, int n, const float4* __restrict__ geom,
+    const float* __restrict__ boxes, const float* __restrict__ verts,
+    const float4* __restrict__ nodes, const int* __restrict__ prims, int guess, float tmin,
+    float tmax, int* __restrict__ hit_elem, int* __restrict__ hit_face,
+    float4* __restrict__ hit_f32, int* __restrict__ queue, int* __restrict__ counts,
+    long long* __restrict__ work)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = r < n;
+  int best_e = -1, best_f = -1;
+  float best_t = tmax, best_u = 0.f, best_v = 0.f;
+  long long it_face = 0, n_slab = 0;
+  if (active) {
+    float o[3], d[3], id[3];
+    load_ray(rays, n, r, o, d);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
+    int stack[FEMRT_STACK];
+    float stack_t[FEMRT_STACK];
+    int sp = 0;
+    int node = 0;
+    float t0, t1;
+    const float4 ra = nodes[0], rb = nodes[1];
+    const float rlo[3] = {ra.x, ra.y, ra.z}, rhi[3] = {rb.x, rb.y, rb.z};
+    bool go = slab(o, id, rlo, rhi, tmin, best_t, t0, t1);
+    while (go) {
+      const float4 na = nodes[2 * node], nb = nodes[2 * node + 1];
+      const int a = __float_as_int(na.w), b = __float_as_int(nb.w);
+      if (b < 0) {
+        for (int i = 0; i < -b; ++i) {
+          const int e = prims[a + i];
+          const float* bx = boxes + (size_t)e * 42;
+          float e0, e1;
+          ++n_slab;
+          if (!slab(o, id, bx, bx + 3, tmin, best_t, e0, e1)) continue;
+          const float4* gc = geom + (size_t)e * (NG * NG * NG);
+          const float* vx = verts + (size_t)e * 24;
+          for (int f = 0; f < 6; ++f) {
+            const float* fb = bx + 6 * (1 + f);
+            float f0, f1;
+            ++n_slab;
+            if (!slab(o, id, fb, fb + 3, tmin, best_t, f0, f1)) continue;
+            float tt, uu, vv;
+            if (face_root<NG>(gc, vx, o, d, f, tmin, 1e30f, guess, tt, uu, vv, it_face) &&
+                tt < best_t) {
+              best_t = tt;
+              best_e = e;
+              best_f = f;
+              best_u = uu;
+              best_v = vv;
+            }
+          }
+        }
+      } else {
+        const float4 la = nodes[2 * a], lb = nodes[2 * a + 1];
+        const float4 qa = nodes[2 * b], qb = nodes[2 * b + 1];
+        const float llo[3] = {la.x, la.y, la.z}, lhi[3] = {lb.x, lb.y, lb.z};
+        const float qlo[3] = {qa.x, qa.y, qa.z}, qhi[3] = {qb.x, qb.y, qb.z};
+        float l0, l1, q0, q1;
+        n_slab += 2;
+        const bool hl = slab(o, id, llo, lhi, tmin, best_t, l0, l1);
+        const bool hq = slab(o, id, qlo, qhi, tmin, best_t, q0, q1);
+        if (hl && hq) {
+          const bool lfirst = l0 <= q0;
+          if (sp < FEMRT_STACK) {
+            stack[sp] = lfirst ? b : a;
+            stack_t[sp] = lfirst ? q0 : l0;
+            ++sp;
+          }
+          node = lfirst ? a : b;
+          continue;
+        }
+        if (hl) { node = a; continue; }
+        if (hq) { node = b; continue; }
+      }
+      // pop the next subtree still in front of the best hit
+      go = false;
+      while (sp > 0) {
+        --sp;
+        if (stack_t[sp] < best_t) {
+          node = stack[sp];
+          go = true;
+          break;
+        }
+      }
+    }
+    hit_elem[r] = best_e;
+    hit_face[r] = best_f;
+    hit_f32[r] = make_float4(best_u, best_v, best_t, 0.f);
+  }
+  const int slot = wave_append(active && best_e >= 0, &counts[0]);
+  if (active && best_e >= 0) queue[slot] = r;
+  if (work && active) {
+    add_work(work, 0, 1);
+    add_work(work, 1, best_e >= 0);
+    add_work(work, 2, it_face);
+    add_work(work, 7, n_slab);
+  }
+}
+
+// FP64 face Newton polish of each queued hit; FP64 field value at the hit.
+template <int NG, int NF>
+__global__ __launch_bounds__(256) void k_polish_hit(
+    const float* __restrict__ rays, int n, const float4* __restrict__ geom,
+    const float* __restrict__ field, const int* __restrict__ queue, const int* __restrict__ counts,
+    const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
+    const float4* __restrict__ hit_f32, double* __restrict__ hit_t, double* __restrict__ hit_rst,
+    double* __restrict__ hit_u, long long* __restrict__ work)
+{
+  const int count = counts[0];
+  long long g64 = 0;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+    const int r = queue[q];
+    float of[3], df[3];
+    load_ray(rays, n, r, of, df);
+    const double o[3] = {of[0], of[1], of[2]}, d[3] = {df[0], df[1], df[2]};
+    const int e = hit_elem[r], f = hit_face[r];
+    const float4 h = hit_f32[r];
+    const float4* gc = geom + (size_t)e * (NG * NG * NG);
+    double u = h.x, v = h.y, tau = h.z;
+    const bool us = (f == 2 || f == 4), vs = (f == 0 || f == 5);
+    for (int it = 0; it < 4; ++it) {
+      double rst[3], X[3], J[9];
+      face_to_rst_t<double>(f, u, v, rst);
+      geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
+      ++g64;
+      const double R[3] = {X[0] - (o[0] + tau * d[0]), X[1] - (o[1] + tau * d[1]),
+                           X[2] - (o[2] + tau * d[2])};
+      const double A[9] = {us ? J[1] : J[0], vs ? J[1] : J[2], -d[0],
+                           us ? J[4] : J[3], vs ? J[4] : J[5], -d[1],
+                           us ? J[7] : J[6], vs ? J[7] : J[8], -d[2]};
+      const double b[3] = {-R[0], -R[1], -R[2]};
+      double dl[3];
+      if (!solve3<double>(A, b, dl)) break;
+      u += dl[0];
+      v += dl[1];
+      tau += dl[2];
+      if (fabs(dl[0]) + fabs(dl[1]) < 1e-15 && fabs(dl[2]) < 1e-15 * (1.0 + fabs(tau))) break;
+    }
+    double rst[3];
+    face_to_rst_t<double>(f, u, v, rst);
+    hit_t[r] = tau;
+    hit_rst[r] = rst[0];
+    hit_rst[(size_t)n + r] = rst[1];
+    hit_rst[2 * (size_t)n + r] = rst[2];
+    hit_u[r] = field_value<double, NF>(field + (size_t)e * (NF * NF * NF), rst[0], rst[1], rst[2]);
+  }
+  if (work) {
+    add_work(work, 5, g64);
+    add_work(work, 6, 0);
+  }
+}
+
+// Point inversion x -> rst in FP32 from a guess (Newton of HexElement.cu:
+// 227-272: undamped, |delta|_inf < 1e-5, <= 50 iterations).  On return J holds
+// the Jacobian at the final iterate.
+template <int NG>
+__device__ bool invert_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
+                           long long& iters)
+{
+  for (int it = 0; it < 50; ++it) {
+    float X[3];
+    geom_eval<float, NG>(gc, rst[0], rst[1], rst[2], X, J);
+    ++iters;
+    const float b[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
+    float dl[3];
+    if (!solve3<float>(J, b, dl)) return false;
+    rst[0] -= dl[0];
+    rst[1] -= dl[1];
+    rst[2] -= dl[2];
+    if (!(fabsf(rst[0]) < 1e4f && fabsf(rst[1]) < 1e4f && fabsf(rst[2]) < 1e4f)) return false;
+    if (fabsf(dl[0]) < 1e-5f && fabsf(dl[1]) < 1e-5f && fabsf(dl[2]) < 1e-5f) return true;
+  }
+  return false;
+}
+
+struct IsoParams {
+  float iso[FEMRT_MAX_ISO];
+  int n_iso;
+  int samples;
+  int guess;
+  int max_march;
+  float tmin;
+};
+
+// Exit of the ray from element e: nearest accepted face root with
+// t > t_in + eps on the faces other than the entry face.
+template <int NG>
+__device__ bool find_exit(const float4* __restrict__ gc, const float* __restrict__ vx,
+                          const float* __restrict__ bx, const float* o, const float* d,
+                          const float* id, int f_in, float t_in, float eps, int guess, int& f_out,
+                          float& t_out, float& u_out, float& v_out, long long& iters,
+                          long long& nslab)
+{
+  bool found = false;
+  float best = FLT_MAX;
+  for (int f = 0; f < 6; ++f) {
+    if (f == f_in) continue;
+    const float* fb = bx + 6 * (1 + f);
+    float f0, f1;
+    ++nslab;
+    if (!slab(o, id, fb, fb + 3, t_in, best, f0, f1)) continue;
+    float tt, uu, vv;
+    if (face_root<NG>(gc, vx, o, d, f, t_in + eps, 1e30f, guess, tt, uu, vv, iters) && tt < best) {
+      best = tt;
+      f_out = f;
+      t_out = tt;
+      u_out = uu;
+      v_out = vv;
+      found = true;
+    }
+  }
+  return found;
+}
+
+template <int NG, int NF>
+__global__ __launch_bounds__(256) void k_march(
+    const float* __restrict__ rays, int n, const float4* __restrict__ geom,
+    const float* __restrict__ field, const float* __restrict__ boxes,
+    const float* __restrict__ verts, const int* __restrict__ adj, IsoParams P,
+    const int* __restrict__ queue, int* __restrict__ counts, const int* __restrict__ hit_elem,
+    const int* __restrict__ hit_face, const float4* __restrict__ hit_f32,
+    int* __restrict__ iso_elem, float4* __restrict__ iso_f32, float4* __restrict__ iso_rst32,
+    int* __restrict__ iso_queue, long long* __restrict__ work)
+{
+  const int count = counts[0];
+  long long c_face = 0, c_inv = 0, c_field = 0, c_slab = 0, c_marched = 0;
+  const int stride = gridDim.x * blockDim.x;
+  // All lanes loop the same number of times so the wave-level queue append
+  // stays uniform.
+  const int rounds = (count + stride - 1) / stride;
+  for (int rd = 0; rd < rounds; ++rd) {
+    const int q = rd * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = q < count;
+    unsigned found_mask = 0;
+    if (active) {
+      const int r = queue[q];
+      float o[3], d[3], id[3];
+      load_ray(rays, n, r, o, d);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
+      int e = hit_elem[r];
+      int f_in = hit_face[r];
+      const float4 h = hit_f32[r];
+      float t_in = h.z;
+      const unsigned all = (1u << P.n_iso) - 1u;
+      for (int step = 0; step < P.max_march && found_mask != all; ++step) {
+        ++c_marched;
+        const float4* gc = geom + (size_t)e * (NG * NG * NG);
+        const float* fc = field + (size_t)e * (NF * NF * NF);
+        const float* bx = boxes + (size_t)e * 42;
+        const float* vx = verts + (size_t)e * 24;
+        const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
+        int f_out = -1;
+        float t_out, u_out, v_out;
+        if (!find_exit<NG>(gc, vx, bx, o, d, id, f_in, t_in, 1e-4f * ext, P.guess, f_out, t_out,
+                           u_out, v_out, c_face, c_slab))
+          break;
+        // uniform samples over [t_in, t_out]
+        float gprev[FEMRT_MAX_ISO];
+        float rst[3] = {0.f, 0.f, 0.f}, rprev[3] = {0.f, 0.f, 0.f};
+        float tprev = t_in;
+        bool have_prev = false;
+        const int M = P.samples;
+        for (int j = 0; j <= M; ++j) {
+          const float t = t_in + (t_out - t_in) * ((float)j / (float)M);
+          const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
+          float J[9];
+          if (P.guess == FEMRT_GUESS_PLAIN || !have_prev) {
+            rst[0] = rst[1] = rst[2] = 0.f;
+          }
+          if (!invert_f32<NG>(gc, x, rst, J, c_inv)) {
+            have_prev = false;
+            continue;
+          }
+          const float u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
+          ++c_field;
+#pragma unroll
+          for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
+            if (k >= P.n_iso || (found_mask >> k) & 1u) continue;
+            const float g = u - P.iso[k];
+            if (have_prev && gprev[k] * g <= 0.f) {
+              const size_t slot = (size_t)k * n + r;
+              iso_elem[slot] = e;
+              iso_f32[slot] = make_float4(tprev, t, 0.5f * (tprev + t), gprev[k]);
+              iso_rst32[slot] = make_float4(0.5f * (rprev[0] + rst[0]), 0.5f * (rprev[1] + rst[1]),
+                                            0.5f * (rprev[2] + rst[2]), g);
+              found_mask |= 1u << k;
+            }
+            gprev[k] = g;
+          }
+          rprev[0] = rst[0];
+          rprev[1] = rst[1];
+          rprev[2] = rst[2];
+          tprev = t;
+          have_prev = true;
+        }
+        if (found_mask == all) break;
+        if (!adj) break;
+        const int ne = adj[(size_t)e * 12 + f_out * 2];
+        const int nfc = adj[(size_t)e * 12 + f_out * 2 + 1];
+        if (ne < 0) break;
+        e = ne;
+        f_in = nfc;
+        t_in = t_out;
+      }
+    }
+    // queue the (ray, iso) brackets for the FP64 endgame
+    for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
+      if (k >= P.n_iso) break;
+      const bool pred = active && ((found_mask >> k) & 1u);
+      const int slot = wave_append(pred, &counts[1]);
+      if (pred) iso_queue[slot] = queue[q] * FEMRT_MAX_ISO + k;
+    }
+  }
+  if (work) {
+    add_work(work, 2, c_face);
+    add_work(work, 3, c_inv);
+    add_work(work, 4, c_field);
+    add_work(work, 7, c_slab);
+    add_work(work, 9, c_marched);
+  }
+}
+
+// FP64 endgame of one isovalue bracket: point inversion in FP64 (warm from
+// the FP32 estimate) and safeguarded Newton on g(t) = u - iso with
+// g'(t) = grad_world(u) . d, kept inside the sign-change bracket; bisection
+// when the Newton step leaves it.  Converged when the step is below
+// 1e-13 (1 + |t|).
+template <int NG>
+__device__ bool invert_f64(const float4* __restrict__ gc, const double* x, double* rst, double* J,
+                           long long& iters)
+{
+  for (int it = 0; it < 30; ++it) {
+    double X[3];
+    geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
+    ++iters;
+    const double b[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
+    double dl[3];
+    if (!solve3<double>(J, b, dl)) return false;
+    rst[0] -= dl[0];
+    rst[1] -= dl[1];
+    rst[2] -= dl[2];
+    if (fabs(dl[0]) < 1e-15 && fabs(dl[1]) < 1e-15 && fabs(dl[2]) < 1e-15) {
+      geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
+      ++iters;
+      return true;
+    }
+  }
+  return false;
+}
+
+template <int NG, int NF>
+__global__ __launch_bounds__(256) void k_polish_iso(
+    const float* __restrict__ rays, int n, const float4* __restrict__ geom,
+    const float* __restrict__ field, IsoParams P, const int* __restrict__ iso_queue,
+    const int* __restrict__ counts, const int* __restrict__ iso_elem_in,
+    const float4* __restrict__ iso_f32, const float4* __restrict__ iso_rst32,
+    int* __restrict__ iso_elem, double* __restrict__ iso_t, double* __restrict__ iso_rst,
+    double* __restrict__ iso_u, long long* __restrict__ work)
+{
+  const int count = counts[1];
+  long long g64 = 0, f64 = 0;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+    const int item = iso_queue[q];
+    const int r = item / FEMRT_MAX_ISO, k = item % FEMRT_MAX_ISO;
+    const size_t slot = (size_t)k * n + r;
+    float of[3], df[3];
+    load_ray(rays, n, r, of, df);
+    const double o[3] = {of[0], of[1], of[2]}, d[3] = {df[0], df[1], df[2]};
+    const int e = iso_elem_in[slot];
+    const float4 br = iso_f32[slot];
+    const float4 rg = iso_rst32[slot];
+    const float4* gc = geom + (size_t)e * (NG * NG * NG);
+    const float* fc = field + (size_t)e * (NF * NF * NF);
+    const double iso = P.iso[k];
+    double lo = br.x, hi = br.y, t = br.z;
+    const double glo_sign = br.w;  // g at the bracket's left end (FP32 sample)
+    double rst[3] = {rg.x, rg.y, rg.z}, J[9];
+    double u = 0.0, best_t = t;
+    bool ok = false;
+    for (int it = 0; it < 60; ++it) {
+      const double x[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
+      if (!invert_f64<NG>(gc, x, rst, J, g64)) break;
+      double gref[3], gw[3];
+      field_eval<double, NF>(fc, rst[0], rst[1], rst[2], &u, gref);
+      ++f64;
+      grad_world<double>(J, gref, gw);
+      const double g = u - iso;
+      const double gp = gw[0] * d[0] + gw[1] * d[1] + gw[2] * d[2];
+      best_t = t;
+      ok = true;
+      if (g == 0.0) break;
+      if ((g < 0.0) == (glo_sign < 0.0)) lo = t; else hi = t;
+      double tn = (gp != 0.0) ? t - g / gp : 0.5 * (lo + hi);
+      if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+      const double step = fabs(tn - t);
+      t = tn;
+      if (step < 1e-13 * (1.0 + fabs(t)) || (hi - lo) < 1e-13 * (1.0 + fabs(t))) {
+        // final evaluation at the converged t
+        const double x2[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
+        if (invert_f64<NG>(gc, x2, rst, J, g64)) {
+          u = field_value<double, NF>(fc, rst[0], rst[1], rst[2]);
+          ++f64;
+          best_t = t;
+        }
+        break;
+      }
+    }
+    if (ok) {
+      iso_elem[slot] = e;
+      iso_t[slot] = best_t;
+      iso_rst[(size_t)k * 3 * n + r] = rst[0];
+      iso_rst[(size_t)k * 3 * n + n + r] = rst[1];
+      iso_rst[(size_t)k * 3 * n + 2 * n + r] = rst[2];
+      iso_u[slot] = u;
+    } else {
+      iso_elem[slot] = -1;
+    }
+  }
+  if (work) {
+    add_work(work, 5, g64);
+    add_work(work, 6, f64);
+  }
+}
+
+__global__ void k_clear_frame(int n, int n_iso, int* hit_elem, int* iso_elem, int* counts)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) {
+    hit_elem[r] = -1;
+    for (int k = 0; k < n_iso; ++k) iso_elem[(size_t)k * n + r] = -1;
+  }
+  if (r < 4) counts[r] = 0;
+}
+
+__global__ void k_shade(int n, int n_iso, const float* __restrict__ rays,
+                        const int* __restrict__ hit_elem, const double* __restrict__ hit_u,
+                        const int* __restrict__ iso_elem, uint32_t* __restrict__ rgba)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float c[3];
+  int first = -1;
+  for (int k = 0; k < n_iso; ++k)
+    if (iso_elem[(size_t)k * n + r] >= 0) { first = k; break; }
+  if (first >= 0) {
+    const float a = n_iso > 1 ? (float)first / (n_iso - 1) : 0.5f;
+    c[0] = 0.2f + 0.7f * a;
+    c[1] = 0.3f;
+    c[2] = 0.9f - 0.7f * a;
+  } else if (hit_elem[r] >= 0) {
+    const float u = (float)hit_u[r];
+    const float a = fminf(fmaxf(0.5f + 0.5f * u, 0.f), 1.f);
+    c[0] = c[1] = c[2] = 0.3f + 0.5f * a;
+  } else {
+    // sky gradient (raytracer.cu:398-402)
+    const float dy = rays[4 * (size_t)n + r];
+    const float t = 0.5f * (dy + 1.f);
+    c[0] = (1.f - t) + t * 0.5f;
+    c[1] = (1.f - t) + t * 0.7f;
+    c[2] = 1.f;
+  }
+  uint32_t px = 0xFF000000u;
+  for (int i = 0; i < 3; ++i) {
+    const float g = sqrtf(fminf(fmaxf(c[i], 0.f), 1.f));
+    px |= (uint32_t)(255.99f * g) << (8 * i);
+  }
+  rgba[r] = px;
+}
+
+// Point kernels: field value + world gradient; inversion.
+template <typename T, int NG, int NF>
+__global__ void k_eval_field(int n, const int* __restrict__ elem, const float* __restrict__ rst,
+                             const float4* __restrict__ geom, const float* __restrict__ field,
+                             T* __restrict__ out)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int e = elem[p];
+  const T r = rst[p], s = rst[n + p], t = rst[2 * n + p];
+  T u, gref[3], X[3], J[9], gw[3];
+  field_eval<T, NF>(field + (size_t)e * (NF * NF * NF), r, s, t, &u, gref);
+  geom_eval<T, NG>(geom + (size_t)e * (NG * NG * NG), r, s, t, X, J);
+  if (!grad_world<T>(J, gref, gw)) gw[0] = gw[1] = gw[2] = T(NAN);
+  out[p] = u;
+  out[n + p] = gw[0];
+  out[2 * n + p] = gw[1];
+  out[3 * n + p] = gw[2];
+}
+
+template <int NG>
+__global__ void k_invert(int n, const int* __restrict__ elem, const float* __restrict__ x,
+                         const float4* __restrict__ geom, double* __restrict__ rst_out,
+                         int* __restrict__ iters_out)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int e = elem[p];
+  const float4* gc = geom + (size_t)e * (NG * NG * NG);
+  const float xf[3] = {x[p], x[n + p], x[2 * n + p]};
+  float rst[3] = {0.f, 0.f, 0.f}, J[9];
+  long long it = 0;
+  const bool ok = invert_f32<NG>(gc, xf, rst, J, it);
+  double rd[3] = {rst[0], rst[1], rst[2]}, Jd[9];
+  const double xd[3] = {xf[0], xf[1], xf[2]};
+  long long it64 = 0;
+  const bool ok64 = ok && invert_f64<NG>(gc, xd, rd, Jd, it64);
+  rst_out[p] = rd[0];
+  rst_out[n + p] = rd[1];
+  rst_out[2 * n + p] = rd[2];
+  if (iters_out) iters_out[p] = ok64 ? (int)it : -1;
+}
+
+// ============================================================ mode dispatch
+
+static int pad_ng(int ng)
+{
+  if (ng <= 4) return 4;
+  if (ng == 5) return 5;
+  if (ng <= 7) return 7;
+  return 0;
+}
+static int pad_nf(int nf)
+{
+  if (nf <= 3) return 3;
+  if (nf <= 5) return 5;
+  if (nf <= 7) return 7;
+  if (nf <= 9) return 9;
+  return 0;
+}
+
+#define FEMRT_DISPATCH_NG(NGV, BODY)      \
+  switch (NGV) {                          \
+    case 4: { constexpr int NG_ = 4; BODY; } break; \
+    case 5: { constexpr int NG_ = 5; BODY; } break; \
+    case 7: { constexpr int NG_ = 7; BODY; } break; \
+    default: break;                       \
+  }
+#define FEMRT_DISPATCH_NF(NFV, BODY)      \
+  switch (NFV) {                          \
+    case 3: { constexpr int NF_ = 3; BODY; } break; \
+    case 5: { constexpr int NF_ = 5; BODY; } break; \
+    case 7: { constexpr int NF_ = 7; BODY; } break; \
+    case 9: { constexpr int NF_ = 9; BODY; } break; \
+    default: break;                       \
+  }
+
+// Pad mode counts: coefficient (i,j,k) of an n-mode expansion moves to the
+// same (i,j,k) of an N-mode one; higher modes are zero, so the map is
+// unchanged.
+static void pad_coeffs(const float* src, int n, int N, int comps, size_t n_elem, std::vector<float>& dst,
+                       int stride_out)
+{
+  const size_t n3 = (size_t)n * n * n, N3 = (size_t)N * N * N;
+  for (size_t e = 0; e < n_elem; ++e)
+    for (int c = 0; c < comps; ++c)
+      for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < n; ++i)
+            dst[(e * N3 + ((size_t)k * N + j) * N + i) * stride_out + c] =
+                src[(e * comps + c) * n3 + ((size_t)k * n + j) * n + i];
+}
+
+// ================================================================== C ABI
+
+extern "C" {
+
+FEMRT_API int femrt_version(void) { return 100; }
+
+FEMRT_API int femrt_ctx_create(int device, femrt_ctx** out)
+{
+  if (!out) return FEMRT_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FEMRT_EHIP;
+  if (device < 0 || device >= ndev) return FEMRT_EINVAL;
+  femrt_ctx* c = new femrt_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return FEMRT_EHIP;
+  }
+  *out = c;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_ctx_destroy(femrt_ctx* c)
+{
+  if (!c) return FEMRT_EINVAL;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  collect_timing(c);
+  for (auto e : c->pool) hipEventDestroy(e);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return FEMRT_OK;
+}
+
+FEMRT_API const char* femrt_last_error(const femrt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+FEMRT_API int femrt_sync(femrt_ctx* c)
+{
+  if (!c) return FEMRT_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_timing_enable(femrt_ctx* c, int on)
+{
+  if (!c) return FEMRT_EINVAL;
+  c->timing = on != 0;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_timing_reset(femrt_ctx* c)
+{
+  if (!c) return FEMRT_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  c->stats.clear();
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_timing_read(femrt_ctx* c, int cap, char* names48, double* total_ms,
+                                long long* launches)
+{
+  if (!c) return FEMRT_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  int i = 0;
+  for (auto& kv : c->stats) {
+    if (i < cap) {
+      if (names48) {
+        std::memset(names48 + 48 * i, 0, 48);
+        std::strncpy(names48 + 48 * i, kv.first.c_str(), 47);
+      }
+      if (total_ms) total_ms[i] = kv.second.first;
+      if (launches) launches[i] = kv.second.second;
+    }
+    ++i;
+  }
+  return i;
+}
+
+FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, const float* vertices,
+                                    const float* geom, const float* field, const int* adjacency,
+                                    femrt_mesh** out)
+{
+  if (!c || !out || !vertices || !geom || !field || n_elem <= 0) return fail(c, FEMRT_EINVAL, "mesh_upload_hex: bad argument");
+  *out = nullptr;
+  const int NG = pad_ng(ng), NF = pad_nf(nf);
+  if (ng < 2 || nf < 1 || !NG || !NF)
+    return fail(c, FEMRT_EUNSUP, "mesh_upload_hex: geometry modes must be 2..7 and field modes 1..9");
+  HIPCHK(c, hipSetDevice(c->device));
+  femrt_mesh* m = new femrt_mesh();
+  m->ctx = c;
+  m->n_elem = n_elem;
+  m->ng = ng;
+  m->nf = nf;
+  m->NG = NG;
+  m->NF = NF;
+  const size_t N3 = (size_t)NG * NG * NG, F3 = (size_t)NF * NF * NF;
+  std::vector<float> g4((size_t)n_elem * N3 * 4, 0.f), fp((size_t)n_elem * F3, 0.f);
+  pad_coeffs(geom, ng, NG, 3, n_elem, g4, 4);
+  pad_coeffs(field, nf, NF, 1, n_elem, fp, 1);
+  m->h_boxes.resize((size_t)n_elem * 42);
+  bernstein_boxes(n_elem, ng, geom, m->h_boxes.data());
+  auto cleanup = [&](const std::string& msg) {
+    hipFree(m->d_geom); hipFree(m->d_field); hipFree(m->d_verts); hipFree(m->d_boxes); hipFree(m->d_adj);
+    delete m;
+    return fail(c, FEMRT_ENOMEM, msg);
+  };
+  if (hipMalloc(&m->d_geom, g4.size() * 4) != hipSuccess) return cleanup("hipMalloc geom");
+  if (hipMalloc(&m->d_field, fp.size() * 4) != hipSuccess) return cleanup("hipMalloc field");
+  if (hipMalloc(&m->d_verts, (size_t)n_elem * 24 * 4) != hipSuccess) return cleanup("hipMalloc verts");
+  if (hipMalloc(&m->d_boxes, m->h_boxes.size() * 4) != hipSuccess) return cleanup("hipMalloc boxes");
+  if (adjacency && hipMalloc(&m->d_adj, (size_t)n_elem * 12 * 4) != hipSuccess) return cleanup("hipMalloc adj");
+  bool ok = hipMemcpy(m->d_geom, g4.data(), g4.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(m->d_field, fp.data(), fp.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(m->d_verts, vertices, (size_t)n_elem * 24 * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(m->d_boxes, m->h_boxes.data(), m->h_boxes.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+  if (ok && adjacency)
+    ok = hipMemcpy(m->d_adj, adjacency, (size_t)n_elem * 12 * 4, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) return cleanup("hipMemcpy mesh");
+  *out = m;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_mesh_destroy(femrt_mesh* m)
+{
+  if (!m) return FEMRT_EINVAL;
+  hipSetDevice(m->ctx->device);
+  hipStreamSynchronize(m->ctx->stream);
+  hipFree(m->d_geom);
+  hipFree(m->d_field);
+  hipFree(m->d_verts);
+  hipFree(m->d_boxes);
+  hipFree(m->d_adj);
+  hipFree(m->d_nodes);
+  hipFree(m->d_prims);
+  delete m;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_mesh_boxes(const femrt_mesh* m, float* out)
+{
+  if (!m || !out) return FEMRT_EINVAL;
+  std::memcpy(out, m->h_boxes.data(), m->h_boxes.size() * 4);
+  return FEMRT_OK;
+}
+
+static int bvh_depth(const std::vector<float>& nodes, int node, int depth)
+{
+  int a, b;
+  std::memcpy(&a, &nodes[(size_t)node * 8 + 3], 4);
+  std::memcpy(&b, &nodes[(size_t)node * 8 + 7], 4);
+  if (b < 0) return depth;
+  return std::max(bvh_depth(nodes, a, depth + 1), bvh_depth(nodes, b, depth + 1));
+}
+
+FEMRT_API int femrt_mesh_build_bvh(femrt_mesh* m, int builder, int* n_nodes)
+{
+  if (!m) return FEMRT_EINVAL;
+  femrt_ctx* c = m->ctx;
+  if (builder != 0) return fail(c, FEMRT_EUNSUP, "build_bvh: only builder 0 (host binned SAH)");
+  build_bvh_sah(m->n_elem, m->h_boxes.data(), m->bvh);
+  m->n_nodes = (int)(m->bvh.nodes.size() / 8);
+  m->bvh_depth = bvh_depth(m->bvh.nodes, 0, 0);
+  if (m->bvh_depth >= FEMRT_STACK)
+    return fail(c, FEMRT_EUNSUP, "build_bvh: tree deeper than the traversal stack");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipFree(m->d_nodes);
+  hipFree(m->d_prims);
+  m->d_nodes = nullptr;
+  m->d_prims = nullptr;
+  HIPCHK(c, hipMalloc(&m->d_nodes, m->bvh.nodes.size() * 4));
+  HIPCHK(c, hipMalloc(&m->d_prims, m->bvh.prims.size() * 4));
+  HIPCHK(c, hipMemcpy(m->d_nodes, m->bvh.nodes.data(), m->bvh.nodes.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(m->d_prims, m->bvh.prims.data(), m->bvh.prims.size() * 4, hipMemcpyHostToDevice));
+  if (n_nodes) *n_nodes = m->n_nodes;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_mesh_get_bvh(const femrt_mesh* m, float* nodes8, int* prims)
+{
+  if (!m) return FEMRT_EINVAL;
+  if (m->n_nodes == 0) return fail(m->ctx, FEMRT_ESTATE, "get_bvh: BVH not built");
+  if (nodes8) std::memcpy(nodes8, m->bvh.nodes.data(), m->bvh.nodes.size() * 4);
+  if (prims) std::memcpy(prims, m->bvh.prims.data(), m->bvh.prims.size() * 4);
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_eval_field(femrt_mesh* m, int n, const int* elem, const float* rst, int precision,
+                               void* out)
+{
+  if (!m || n < 0 || (n > 0 && (!elem || !rst || !out))) return FEMRT_EINVAL;
+  femrt_ctx* c = m->ctx;
+  if (n == 0) return FEMRT_OK;
+  for (int i = 0; i < n; ++i)
+    if (elem[i] < 0 || elem[i] >= m->n_elem) return fail(c, FEMRT_EINVAL, "eval_field: element index out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t osz = precision ? sizeof(double) : sizeof(float);
+  int* d_e = nullptr;
+  float* d_r = nullptr;
+  void* d_o = nullptr;
+  HIPCHK(c, hipMalloc(&d_e, (size_t)n * 4));
+  HIPCHK(c, hipMalloc(&d_r, (size_t)n * 12));
+  HIPCHK(c, hipMalloc(&d_o, (size_t)n * 4 * osz));
+  HIPCHK(c, hipMemcpyAsync(d_e, elem, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_r, rst, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+  const int blocks = (n + 255) / 256;
+  int rc = FEMRT_EUNSUP;
+  FEMRT_DISPATCH_NG(m->NG, FEMRT_DISPATCH_NF(m->NF, {
+    if (precision)
+      rc = timed_launch(c, "k_eval_field_f64", [&] {
+        k_eval_field<double, NG_, NF_><<<blocks, 256, 0, c->stream>>>(n, d_e, d_r, m->d_geom, m->d_field, (double*)d_o);
+      });
+    else
+      rc = timed_launch(c, "k_eval_field_f32", [&] {
+        k_eval_field<float, NG_, NF_><<<blocks, 256, 0, c->stream>>>(n, d_e, d_r, m->d_geom, m->d_field, (float*)d_o);
+      });
+  }));
+  if (rc == FEMRT_OK) {
+    HIPCHK(c, hipMemcpyAsync(out, d_o, (size_t)n * 4 * osz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  hipFree(d_e);
+  hipFree(d_r);
+  hipFree(d_o);
+  return rc;
+}
+
+FEMRT_API int femrt_invert_points(femrt_mesh* m, int n, const int* elem, const float* x, double* rst_out,
+                                  int* iters_out)
+{
+  if (!m || n < 0 || (n > 0 && (!elem || !x || !rst_out))) return FEMRT_EINVAL;
+  femrt_ctx* c = m->ctx;
+  if (n == 0) return FEMRT_OK;
+  for (int i = 0; i < n; ++i)
+    if (elem[i] < 0 || elem[i] >= m->n_elem) return fail(c, FEMRT_EINVAL, "invert_points: element index out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  int *d_e = nullptr, *d_it = nullptr;
+  float* d_x = nullptr;
+  double* d_r = nullptr;
+  HIPCHK(c, hipMalloc(&d_e, (size_t)n * 4));
+  HIPCHK(c, hipMalloc(&d_x, (size_t)n * 12));
+  HIPCHK(c, hipMalloc(&d_r, (size_t)n * 24));
+  HIPCHK(c, hipMalloc(&d_it, (size_t)n * 4));
+  HIPCHK(c, hipMemcpyAsync(d_e, elem, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_x, x, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+  const int blocks = (n + 255) / 256;
+  int rc = FEMRT_EUNSUP;
+  FEMRT_DISPATCH_NG(m->NG, {
+    rc = timed_launch(c, "k_invert", [&] {
+      k_invert<NG_><<<blocks, 256, 0, c->stream>>>(n, d_e, d_x, m->d_geom, d_r, d_it);
+    });
+  });
+  if (rc == FEMRT_OK) {
+    HIPCHK(c, hipMemcpyAsync(rst_out, d_r, (size_t)n * 24, hipMemcpyDeviceToHost, c->stream));
+    if (iters_out) HIPCHK(c, hipMemcpyAsync(iters_out, d_it, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  hipFree(d_e);
+  hipFree(d_x);
+  hipFree(d_r);
+  hipFree(d_it);
+  return rc;
+}
+
+FEMRT_API void femrt_trace_params_default(femrt_trace_params* p)
+{
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->n_iso = 0;
+  p->iso_enable = 1;
+  p->samples = 16;
+  p->guess = FEMRT_GUESS_PLAIN;
+  p->t_min = 0.001f;
+  p->t_max = 1e30f;
+  p->max_march = 4096;
+  p->flags = 1;
+}
+
+FEMRT_API int femrt_frame_create(femrt_ctx* c, int n, femrt_frame** out)
+{
+  if (!c || !out || n <= 0) return fail(c, FEMRT_EINVAL, "frame_create: bad argument");
+  *out = nullptr;
+  HIPCHK(c, hipSetDevice(c->device));
+  femrt_frame* f = new femrt_frame();
+  f->ctx = c;
+  f->n = n;
+  const size_t N = n, K = FEMRT_MAX_ISO;
+  bool ok = hipMalloc(&f->d_rays, 6 * N * 4) == hipSuccess &&
+            hipMalloc(&f->d_hit_elem, N * 4) == hipSuccess &&
+            hipMalloc(&f->d_hit_face, N * 4) == hipSuccess &&
+            hipMalloc(&f->d_hit_f32, N * 16) == hipSuccess &&
+            hipMalloc(&f->d_hit_t, N * 8) == hipSuccess &&
+            hipMalloc(&f->d_hit_rst, 3 * N * 8) == hipSuccess &&
+            hipMalloc(&f->d_hit_u, N * 8) == hipSuccess &&
+            hipMalloc(&f->d_iso_elem, K * N * 4) == hipSuccess &&
+            hipMalloc(&f->d_iso_t, K * N * 8) == hipSuccess &&
+            hipMalloc(&f->d_iso_rst, 3 * K * N * 8) == hipSuccess &&
+            hipMalloc(&f->d_iso_u, K * N * 8) == hipSuccess &&
+            hipMalloc(&f->d_iso_f32, K * N * 16) == hipSuccess &&
+            hipMalloc(&f->d_iso_rst32, K * N * 16) == hipSuccess &&
+            hipMalloc(&f->d_queue_hit, N * 4) == hipSuccess &&
+            hipMalloc(&f->d_queue_iso, K * N * 4) == hipSuccess &&
+            hipMalloc(&f->d_counts, 16) == hipSuccess &&
+            hipMalloc(&f->d_rgba, N * 4) == hipSuccess &&
+            hipMalloc(&f->d_work, FEMRT_NCOUNTERS * 8) == hipSuccess;
+  if (!ok) {
+    femrt_frame_destroy(f);
+    return fail(c, FEMRT_ENOMEM, "frame_create: hipMalloc failed");
+  }
+  hipMemset(f->d_counts, 0, 16);
+  hipMemset(f->d_work, 0, FEMRT_NCOUNTERS * 8);
+  hipMemset(f->d_hit_elem, 0xff, N * 4);
+  hipMemset(f->d_iso_elem, 0xff, K * N * 4);
+  *out = f;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_frame_destroy(femrt_frame* f)
+{
+  if (!f) return FEMRT_EINVAL;
+  hipSetDevice(f->ctx->device);
+  hipStreamSynchronize(f->ctx->stream);
+  void* ptrs[] = {f->d_rays, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst,
+                  f->d_hit_u, f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, f->d_iso_f32,
+                  f->d_iso_rst32, f->d_queue_hit, f->d_queue_iso, f->d_counts, f->d_rgba, f->d_work};
+  for (void* p : ptrs) hipFree(p);
+  delete f;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_frame_set_rays(femrt_frame* f, const float* rays)
+{
+  if (!f || !rays) return FEMRT_EINVAL;
+  femrt_ctx* c = f->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(f->d_rays, rays, 6 * (size_t)f->n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_frame_get_rays(femrt_frame* f, float* rays)
+{
+  if (!f || !rays) return FEMRT_EINVAL;
+  femrt_ctx* c = f->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(rays, f->d_rays, 6 * (size_t)f->n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_frame_camera_rays(femrt_frame* f, const float* cam, int W, int H)
+{
+  if (!f || !cam || W <= 0 || H <= 0 || (long long)W * H != f->n)
+    return fail(f ? f->ctx : nullptr, FEMRT_EINVAL, "camera_rays: W*H must equal the frame size");
+  femrt_ctx* c = f->ctx;
+  // Camera basis (raytracer.h:31-48) on the host in double, rounded once.
+  const double lf[3] = {cam[0], cam[1], cam[2]}, la[3] = {cam[3], cam[4], cam[5]},
+               vu[3] = {cam[6], cam[7], cam[8]};
+  const double th = cam[9] * M_PI / 180.0, hh = std::tan(th / 2.0);
+  const double vh = 2.0 * hh, vw = ((double)W / H) * vh;
+  double w[3] = {lf[0] - la[0], lf[1] - la[1], lf[2] - la[2]};
+  const double fd = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  for (double& x : w) x /= fd;
+  double u[3] = {vu[1] * w[2] - vu[2] * w[1], vu[2] * w[0] - vu[0] * w[2], vu[0] * w[1] - vu[1] * w[0]};
+  const double un = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  for (double& x : u) x /= un;
+  const double v[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]};
+  float hor[3], ver[3], llc[3];
+  for (int k = 0; k < 3; ++k) {
+    hor[k] = (float)(fd * vw * u[k]);
+    ver[k] = (float)(fd * vh * v[k]);
+    llc[k] = (float)(lf[k] - fd * vw * u[k] / 2 - fd * vh * v[k] / 2 - fd * w[k]);
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = W * H;
+  return timed_launch(c, "k_raygen", [&] {
+    k_raygen<<<(n + 255) / 256, 256, 0, c->stream>>>(f->d_rays, W, H, cam[0], cam[1], cam[2], llc[0],
+                                                    llc[1], llc[2], hor[0], hor[1], hor[2], ver[0],
+                                                    ver[1], ver[2]);
+  });
+}
+
+FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out)
+{
+  if (!f) return FEMRT_EINVAL;
+  femrt_ctx* c = f->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (out) {
+    HIPCHK(c, hipMemcpyAsync(out, f->d_work, FEMRT_NCOUNTERS * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  f->counters = enable != 0;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_params* p)
+{
+  if (!m || !f || !p) return FEMRT_EINVAL;
+  femrt_ctx* c = m->ctx;
+  if (f->ctx != c) return fail(c, FEMRT_EINVAL, "trace: mesh and frame belong to different contexts");
+  if (m->n_nodes == 0) return fail(c, FEMRT_ESTATE, "trace: BVH not built");
+  if (p->n_iso < 0 || p->n_iso > FEMRT_MAX_ISO || p->samples < 1 || p->samples > 1024)
+    return fail(c, FEMRT_EINVAL, "trace: n_iso must be 0..8 and samples 1..1024");
+  if (p->guess < 0 || p->guess > FEMRT_GUESS_WARM)
+    return fail(c, FEMRT_EUNSUP, "trace: guess mode not available in this build");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = f->n;
+  const int n_iso = p->iso_enable ? p->n_iso : 0;
+  long long* work = f->counters ? f->d_work : nullptr;
+  if (work) HIPCHK(c, hipMemsetAsync(f->d_work, 0, FEMRT_NCOUNTERS * 8, c->stream));
+  const int blocks = (n + 255) / 256;
+  int rc = timed_launch(c, "k_clear_frame", [&] {
+    k_clear_frame<<<blocks, 256, 0, c->stream>>>(n, FEMRT_MAX_ISO, f->d_hit_elem, f->d_iso_elem, f->d_counts);
+  });
+  if (rc) return rc;
+  int dev_cus = 256;
+  hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, c->device);
+  const int pblocks = dev_cus * 8;
+  IsoParams ip;
+  std::memset(&ip, 0, sizeof(ip));
+  for (int k = 0; k < FEMRT_MAX_ISO; ++k) ip.iso[k] = p->iso[k];
+  ip.n_iso = n_iso;
+  ip.samples = p->samples;
+  ip.guess = p->guess;
+  ip.max_march = p->max_march;
+  ip.tmin = p->t_min;
+  rc = FEMRT_EUNSUP;
+  FEMRT_DISPATCH_NG(m->NG, {
+    rc = timed_launch(c, "k_first_hit", [&] {
+      k_first_hit<NG_><<<blocks, 256, 0, c->stream>>>(
+          f->d_rays, n, m->d_geom, m->d_boxes, m->d_verts, m->d_nodes, m->d_prims, p->guess,
+          p->t_min, p->t_max, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_queue_hit,
+          f->d_counts, work);
+    });
+    if (rc) break;
+    FEMRT_DISPATCH_NF(m->NF, {
+      rc = timed_launch(c, "k_polish_hit", [&] {
+        k_polish_hit<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
+            f->d_rays, n, m->d_geom, m->d_field, f->d_queue_hit, f->d_counts, f->d_hit_elem,
+            f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst, f->d_hit_u, work);
+      });
+      if (rc) break;
+      if (n_iso > 0) {
+        rc = timed_launch(c, "k_march", [&] {
+          k_march<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
+              f->d_rays, n, m->d_geom, m->d_field, m->d_boxes, m->d_verts, m->d_adj, ip,
+              f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face, f->d_hit_f32,
+              f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work);
+        });
+        if (rc) break;
+        rc = timed_launch(c, "k_polish_iso", [&] {
+          k_polish_iso<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
+              f->d_rays, n, m->d_geom, m->d_field, ip, f->d_queue_iso, f->d_counts,
+              f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_iso_elem, f->d_iso_t,
+              f->d_iso_rst, f->d_iso_u, work);
+        });
+        if (rc) break;
+      }
+    });
+  });
+  if (rc) return rc;
+  if (p->flags & 1)
+    rc = timed_launch(c, "k_shade", [&] {
+      k_shade<<<blocks, 256, 0, c->stream>>>(n, n_iso, f->d_rays, f->d_hit_elem, f->d_hit_u,
+                                             f->d_iso_elem, f->d_rgba);
+    });
+  return rc;
+}
+
+FEMRT_API int femrt_frame_get_hits(femrt_frame* f, int* he, int* hf, double* ht, double* hr, double* hu)
+{
+  if (!f) return FEMRT_EINVAL;
+  femrt_ctx* c = f->ctx;
+  const size_t N = f->n;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (he) HIPCHK(c, hipMemcpy(he, f->d_hit_elem, N * 4, hipMemcpyDeviceToHost));
+  if (hf) HIPCHK(c, hipMemcpy(hf, f->d_hit_face, N * 4, hipMemcpyDeviceToHost));
+  if (ht) HIPCHK(c, hipMemcpy(ht, f->d_hit_t, N * 8, hipMemcpyDeviceToHost));
+  if (hr) HIPCHK(c, hipMemcpy(hr, f->d_hit_rst, 3 * N * 8, hipMemcpyDeviceToHost));
+  if (hu) HIPCHK(c, hipMemcpy(hu, f->d_hit_u, N * 8, hipMemcpyDeviceToHost));
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_frame_get_iso(femrt_frame* f, int n_iso, int* ie, double* it, double* ir, double* iu)
+{
+  if (!f || n_iso < 0 || n_iso > FEMRT_MAX_ISO) return FEMRT_EINVAL;
+  femrt_ctx* c = f->ctx;
+  const size_t N = f->n, K = n_iso;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!K) return FEMRT_OK;
+  if (ie) HIPCHK(c, hipMemcpy(ie, f->d_iso_elem, K * N * 4, hipMemcpyDeviceToHost));
+  if (it) HIPCHK(c, hipMemcpy(it, f->d_iso_t, K * N * 8, hipMemcpyDeviceToHost));
+  if (ir) HIPCHK(c, hipMemcpy(ir, f->d_iso_rst, 3 * K * N * 8, hipMemcpyDeviceToHost));
+  if (iu) HIPCHK(c, hipMemcpy(iu, f->d_iso_u, K * N * 8, hipMemcpyDeviceToHost));
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba)
+{
+  if (!f || !rgba) return FEMRT_EINVAL;
+  femrt_ctx* c = f->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(rgba, f->d_rgba, (size_t)f->n * 4, hipMemcpyDeviceToHost));
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_host_boxes(int n_elem, int ng, const float* geom, float* boxes)
+{
+  if (n_elem <= 0 || ng < 1 || ng > 16 || !geom || !boxes) return FEMRT_EINVAL;
+  bernstein_boxes(n_elem, ng, geom, boxes);
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_host_build_bvh(int n_elem, const float* boxes, int* n_nodes, float* nodes8,
+                                   int* prims)
+{
+  if (n_elem <= 0 || !boxes || !n_nodes || !nodes8 || !prims) return FEMRT_EINVAL;
+  BvhHost b;
+  build_bvh_sah(n_elem, boxes, b);
+  const size_t nn = b.nodes.size() / 8;
+  if (nn > (size_t)2 * n_elem) return FEMRT_EINVAL;
+  std::memcpy(nodes8, b.nodes.data(), b.nodes.size() * 4);
+  std::memcpy(prims, b.prims.data(), b.prims.size() * 4);
+  *n_nodes = (int)nn;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
+                                       float* field, int* adjacency)
+{
+  if (!gp) return FEMRT_EINVAL;
+  return gen_structured_hex(gp, vertices, geom, field, adjacency);
+}
+
+FEMRT_API int femrt_load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_elem,
+                                 int* ng, int* nf, float* vertices, float* geom, float* field,
+                                 int* adjacency)
+{
+  if (!xml_path || !fld_path || !n_elem || !ng || !nf) return FEMRT_EINVAL;
+  return load_xml_fld(xml_path, fld_path, flags, n_elem, ng, nf, vertices, geom, field, adjacency,
+                      nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,292 @@
+// femrt_math.hpp -- element math shared by the host and the gfx950 kernels.
+//
+// Modified_A basis tables by recurrence, sum-factorised evaluation of the
+// modal geometry map x(r,s,t) with its Jacobian, and of the scalar field with
+// its reference gradient.  Templated on the arithmetic type T (float for the
+// bulk, double for the polish) and on the per-direction mode counts, so the
+// loops unroll and the 1-D tables live in registers.
+//
+// Reference: fem/ModalBasis.cuh:16-70 (basis), fem/HexElement.cu:8-330 (modal
+// sums, map, Jacobian, inverse), StdHexExp.cpp:199-245 (the sum-factorisation
+// the reference's own spec describes but its device code does not use).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FEMRT_HD __host__ __device__ __forceinline__
+
+namespace femrt {
+
+// Modified_A basis of n modes and its derivative at x.
+//   phi_0 = (1-x)/2, phi_1 = (1+x)/2, phi_p = (1-x^2)/4 P^{(1,1)}_{p-2}(x).
+// P^{(1,1)}_k = A_k x P_{k-1} - B_k P_{k-2} with
+//   A_k = (2k+1)(k+1)/(k(k+2)),  B_k = (k+1)/(k+2)   (k >= 2),
+// P_0 = 1, P_1 = 2x; derivatives follow by differentiating the recurrence.
+template <typename T, int N>
+FEMRT_HD void basis1d(T x, T* __restrict__ phi, T* __restrict__ dphi)
+{
+  phi[0] = T(0.5) - T(0.5) * x;
+  dphi[0] = T(-0.5);
+  if (N > 1) {
+    phi[1] = T(0.5) + T(0.5) * x;
+    dphi[1] = T(0.5);
+  }
+  if (N > 2) {
+    const T b = T(0.25) - T(0.25) * x * x;
+    const T db = T(-0.5) * x;
+    T pm2 = T(1), dpm2 = T(0);
+    T pm1 = T(2) * x, dpm1 = T(2);
+    phi[2] = b;
+    dphi[2] = db;
+    if (N > 3) {
+      phi[3] = b * pm1;
+      dphi[3] = db * pm1 + b * dpm1;
+    }
+#pragma unroll
+    for (int p = 4; p < N; ++p) {
+      const int k = p - 2;
+      const T A = T((2.0 * k + 1.0) * (k + 1.0) / (k * (k + 2.0)));
+      const T B = T((k + 1.0) / (k + 2.0));
+      const T pk = A * x * pm1 - B * pm2;
+      const T dpk = A * (pm1 + x * dpm1) - B * dpm2;
+      phi[p] = b * pk;
+      dphi[p] = db * pk + b * dpk;
+      pm2 = pm1;
+      dpm2 = dpm1;
+      pm1 = pk;
+      dpm1 = dpk;
+    }
+  }
+}
+
+template <typename T>
+struct Vec3 {
+  T x, y, z;
+};
+
+// x(r,s,t) and J (row-major, rows x,y,z; columns d/dr, d/ds, d/dt).
+// c: NG^3 float4 (x, y, z, unused) for one element, mode (i,j,k) at
+// (k*NG+j)*NG+i.  1035 FMA at NG=5.
+template <typename T, int NG>
+FEMRT_HD void geom_eval(const float4* __restrict__ c, T r, T s, T t, T* __restrict__ X,
+                        T* __restrict__ J)
+{
+  T pr[NG], dpr[NG], ps[NG], dps[NG], pt[NG], dpt[NG];
+  basis1d<T, NG>(r, pr, dpr);
+  basis1d<T, NG>(s, ps, dps);
+  basis1d<T, NG>(t, pt, dpt);
+  T x0 = 0, x1 = 0, x2 = 0;
+  T jr0 = 0, jr1 = 0, jr2 = 0, js0 = 0, js1 = 0, js2 = 0, jt0 = 0, jt1 = 0, jt2 = 0;
+#pragma unroll
+  for (int k = 0; k < NG; ++k) {
+    T b0 = 0, b1 = 0, b2 = 0;     // sum_j phi_j(s) a_jk
+    T bs0 = 0, bs1 = 0, bs2 = 0;  // sum_j phi'_j(s) a_jk
+    T br0 = 0, br1 = 0, br2 = 0;  // sum_j phi_j(s) a'_jk
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      T a0 = 0, a1 = 0, a2 = 0, ad0 = 0, ad1 = 0, ad2 = 0;
+#pragma unroll
+      for (int i = 0; i < NG; ++i) {
+        const float4 q = c[(k * NG + j) * NG + i];
+        a0 += pr[i] * T(q.x);
+        a1 += pr[i] * T(q.y);
+        a2 += pr[i] * T(q.z);
+        ad0 += dpr[i] * T(q.x);
+        ad1 += dpr[i] * T(q.y);
+        ad2 += dpr[i] * T(q.z);
+      }
+      b0 += ps[j] * a0;
+      b1 += ps[j] * a1;
+      b2 += ps[j] * a2;
+      bs0 += dps[j] * a0;
+      bs1 += dps[j] * a1;
+      bs2 += dps[j] * a2;
+      br0 += ps[j] * ad0;
+      br1 += ps[j] * ad1;
+      br2 += ps[j] * ad2;
+    }
+    x0 += pt[k] * b0;
+    x1 += pt[k] * b1;
+    x2 += pt[k] * b2;
+    jr0 += pt[k] * br0;
+    jr1 += pt[k] * br1;
+    jr2 += pt[k] * br2;
+    js0 += pt[k] * bs0;
+    js1 += pt[k] * bs1;
+    js2 += pt[k] * bs2;
+    jt0 += dpt[k] * b0;
+    jt1 += dpt[k] * b1;
+    jt2 += dpt[k] * b2;
+  }
+  X[0] = x0;
+  X[1] = x1;
+  X[2] = x2;
+  J[0] = jr0; J[1] = js0; J[2] = jt0;
+  J[3] = jr1; J[4] = js1; J[5] = jt1;
+  J[6] = jr2; J[7] = js2; J[8] = jt2;
+}
+
+// x(r,s,t) only (no Jacobian): 3 NG^3 + 3 NG^2 + 3 NG FMA.
+template <typename T, int NG>
+FEMRT_HD void geom_point(const float4* __restrict__ c, T r, T s, T t, T* __restrict__ X)
+{
+  T pr[NG], dpr[NG], ps[NG], dps[NG], pt[NG], dpt[NG];
+  basis1d<T, NG>(r, pr, dpr);
+  basis1d<T, NG>(s, ps, dps);
+  basis1d<T, NG>(t, pt, dpt);
+  T x0 = 0, x1 = 0, x2 = 0;
+#pragma unroll
+  for (int k = 0; k < NG; ++k) {
+    T b0 = 0, b1 = 0, b2 = 0;
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+      T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+      for (int i = 0; i < NG; ++i) {
+        const float4 q = c[(k * NG + j) * NG + i];
+        a0 += pr[i] * T(q.x);
+        a1 += pr[i] * T(q.y);
+        a2 += pr[i] * T(q.z);
+      }
+      b0 += ps[j] * a0;
+      b1 += ps[j] * a1;
+      b2 += ps[j] * a2;
+    }
+    x0 += pt[k] * b0;
+    x1 += pt[k] * b1;
+    x2 += pt[k] * b2;
+  }
+  X[0] = x0;
+  X[1] = x1;
+  X[2] = x2;
+}
+
+// Field value and reference gradient (du/dr, du/ds, du/dt): 2 NF^3 + 3 NF^2 +
+// 4 NF FMA.
+template <typename T, int NF>
+FEMRT_HD void field_eval(const float* __restrict__ c, T r, T s, T t, T* __restrict__ u,
+                         T* __restrict__ g)
+{
+  T pr[NF], dpr[NF], ps[NF], dps[NF], pt[NF], dpt[NF];
+  basis1d<T, NF>(r, pr, dpr);
+  basis1d<T, NF>(s, ps, dps);
+  basis1d<T, NF>(t, pt, dpt);
+  T v = 0, gr = 0, gs = 0, gt = 0;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    T b = 0, bs = 0, br = 0;
+#pragma unroll
+    for (int j = 0; j < NF; ++j) {
+      T a = 0, ad = 0;
+#pragma unroll
+      for (int i = 0; i < NF; ++i) {
+        const T q = T(c[(k * NF + j) * NF + i]);
+        a += pr[i] * q;
+        ad += dpr[i] * q;
+      }
+      b += ps[j] * a;
+      bs += dps[j] * a;
+      br += ps[j] * ad;
+    }
+    v += pt[k] * b;
+    gr += pt[k] * br;
+    gs += pt[k] * bs;
+    gt += dpt[k] * b;
+  }
+  *u = v;
+  g[0] = gr;
+  g[1] = gs;
+  g[2] = gt;
+}
+
+// Field value only: NF^3 + NF^2 + NF FMA.
+template <typename T, int NF>
+FEMRT_HD T field_value(const float* __restrict__ c, T r, T s, T t)
+{
+  T pr[NF], dpr[NF], ps[NF], dps[NF], pt[NF], dpt[NF];
+  basis1d<T, NF>(r, pr, dpr);
+  basis1d<T, NF>(s, ps, dps);
+  basis1d<T, NF>(t, pt, dpt);
+  T v = 0;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    T b = 0;
+#pragma unroll
+    for (int j = 0; j < NF; ++j) {
+      T a = 0;
+#pragma unroll
+      for (int i = 0; i < NF; ++i) a += pr[i] * T(c[(k * NF + j) * NF + i]);
+      b += ps[j] * a;
+    }
+    v += pt[k] * b;
+  }
+  return v;
+}
+
+template <typename T>
+FEMRT_HD T det3(const T* J)
+{
+  return J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+         J[2] * (J[3] * J[7] - J[4] * J[6]);
+}
+
+// Solve J d = b by Cramer's rule (the cofactor inverse of HexElement.cu:208-225
+// applied to one right-hand side).  Returns false when det is zero/non-finite.
+template <typename T>
+FEMRT_HD bool solve3(const T* J, const T* b, T* d)
+{
+  const T c00 = J[4] * J[8] - J[5] * J[7];
+  const T c01 = J[5] * J[6] - J[3] * J[8];
+  const T c02 = J[3] * J[7] - J[4] * J[6];
+  const T det = J[0] * c00 + J[1] * c01 + J[2] * c02;
+  if (!(det != T(0)) || !(det == det)) return false;
+  const T id = T(1) / det;
+  const T c10 = J[2] * J[7] - J[1] * J[8];
+  const T c11 = J[0] * J[8] - J[2] * J[6];
+  const T c12 = J[1] * J[6] - J[0] * J[7];
+  const T c20 = J[1] * J[5] - J[2] * J[4];
+  const T c21 = J[2] * J[3] - J[0] * J[5];
+  const T c22 = J[0] * J[4] - J[1] * J[3];
+  d[0] = (c00 * b[0] + c10 * b[1] + c20 * b[2]) * id;
+  d[1] = (c01 * b[0] + c11 * b[1] + c21 * b[2]) * id;
+  d[2] = (c02 * b[0] + c12 * b[1] + c22 * b[2]) * id;
+  return true;
+}
+
+// world gradient = J^{-T} grad_ref (HexElement.cu:308-330)
+template <typename T>
+FEMRT_HD bool grad_world(const T* J, const T* gref, T* gw)
+{
+  // Solve J^T gw = gref
+  const T JT[9] = {J[0], J[3], J[6], J[1], J[4], J[7], J[2], J[5], J[8]};
+  return solve3<T>(JT, gref, gw);
+}
+
+// Face table, Nektar order (high_order.h:172-184):
+//   face 0: t=-1 (u=r, v=s)   1: s=-1 (r,t)   2: r=+1 (s,t)
+//   face 3: s=+1 (r,t)        4: r=-1 (s,t)   5: t=+1 (r,s)
+FEMRT_HD int face_fix(int f) { return (f == 0 || f == 5) ? 2 : ((f == 1 || f == 3) ? 1 : 0); }
+FEMRT_HD float face_val(int f) { return (f == 0 || f == 1 || f == 4) ? -1.f : 1.f; }
+FEMRT_HD int face_u(int f) { return (f == 2 || f == 4) ? 1 : 0; }
+FEMRT_HD int face_v(int f) { return (f == 0 || f == 5) ? 1 : 2; }
+
+// Slab test (geometry.h:162-175, high_order.h:115-152).
+FEMRT_HD bool slab(const float* o, const float* inv_d, const float* lo, const float* hi, float tmin,
+                   float tmax, float& t0, float& t1)
+{
+  float a = tmin, b = tmax;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float ta = (lo[c] - o[c]) * inv_d[c];
+    float tb = (hi[c] - o[c]) * inv_d[c];
+    const float mn = fminf(ta, tb), mx = fmaxf(ta, tb);
+    a = fmaxf(a, mn);
+    b = fminf(b, mx);
+  }
+  t0 = a;
+  t1 = b;
+  return a <= b;
+}
+
+}  // namespace femrt

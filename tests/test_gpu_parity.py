@@ -1,0 +1,97 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle on the
+same seeded inputs (same FP32 coefficients, same FP32 rays)."""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _parity():
+    import oracle.parity as parity
+    return parity
+
+
+def _mesh_c1(femrt, n=4):
+    # config-1 style: structured 4^3, Q=3, sinusoidal face displacement 0.05,
+    # field order 4
+    return femrt.gen_structured_hex(n, 3, 4, pert=1, amp=0.05, seed=2024)
+
+
+def test_eval_field_matches_oracle(femrt, oracle, gpu_ctx):
+    hm = _mesh_c1(femrt)
+    mesh = femrt.Mesh(gpu_ctx, hm)
+    orc = oracle.Oracle.from_mesh(hm)
+    rng = np.random.default_rng(10)
+    n = 500
+    elem = rng.integers(0, hm.n_elem, n).astype(np.int32)
+    rst = rng.uniform(-1, 1, (3, n)).astype(np.float32)
+    out64 = mesh.eval_field(elem, rst, precision=1)
+    out32 = mesh.eval_field(elem, rst, precision=0)
+    for i in range(n):
+        u, _, gw = orc.field_at(elem[i], rst[:, i].astype(np.float64))
+        assert out64[0, i] == pytest.approx(u, abs=1e-12)
+        np.testing.assert_allclose(out64[1:, i], gw, rtol=1e-10, atol=1e-10)
+        assert out32[0, i] == pytest.approx(u, abs=2e-6)
+        np.testing.assert_allclose(out32[1:, i], gw, rtol=1e-4, atol=1e-4)
+    mesh.close()
+
+
+def test_point_inversion_matches_oracle(femrt, oracle, gpu_ctx):
+    hm = _mesh_c1(femrt)
+    mesh = femrt.Mesh(gpu_ctx, hm)
+    orc = oracle.Oracle.from_mesh(hm)
+    rng = np.random.default_rng(11)
+    n = 400
+    elem = rng.integers(0, hm.n_elem, n).astype(np.int32)
+    x = np.zeros((3, n), np.float32)
+    for i in range(n):
+        x[:, i] = orc.map(elem[i], rng.uniform(-0.95, 0.95, 3))[0]
+    rst, it = mesh.invert(elem, x)
+    assert np.all(it > 0)
+    for i in range(n):
+        want, k = orc.invert(elem[i], x[:, i].astype(np.float64))
+        assert k > 0
+        np.testing.assert_allclose(rst[:, i], want, atol=1e-11)
+    mesh.close()
+
+
+def _trace_and_compare(femrt, oracle, ctx, hm, W, H, isos, guess):
+    mesh = femrt.Mesh(ctx, hm)
+    lo = hm.vertices.reshape(-1, 3).min(0)
+    hi = hm.vertices.reshape(-1, 3).max(0)
+    cam = femrt.auto_camera(lo, hi)
+    fr = femrt.Frame(ctx, W * H)
+    fr.camera_rays(cam, W, H)
+    fr.trace(mesh, femrt.trace_params(isos, guess=guess))
+    gh = fr.hits()
+    giso = fr.iso(len(isos))
+    rays = fr.get_rays()
+    orc = oracle.Oracle.from_mesh(hm)
+    og, oiso = orc.trace(rays, isos)
+    rep = _parity().compare(gh, giso, og, oiso, isos)
+    fr.close()
+    mesh.close()
+    return rep
+
+
+@pytest.mark.parametrize("guess", [0, 1])
+def test_trace_config1_geometry_and_iso(femrt, oracle, gpu_ctx, guess):
+    hm = _mesh_c1(femrt)
+    isos = [0.0]
+    rep = _trace_and_compare(femrt, oracle, gpu_ctx, hm, 64, 48, isos, guess)
+    print(json.dumps(rep, indent=1))
+    assert rep["oracle_hits"] > 200
+    assert rep["ok"], rep
+
+
+def test_trace_three_isovalues_curved_q4(femrt, oracle, gpu_ctx):
+    hm = femrt.gen_structured_hex(4, 4, 6, pert=2, amp=0.08, seed=99)
+    isos = [-0.1, 0.0, 0.1]
+    rep = _trace_and_compare(femrt, oracle, gpu_ctx, hm, 48, 32, isos, 1)
+    print(json.dumps(rep, indent=1))
+    assert rep["oracle_hits"] > 100
+    assert rep["ok"], rep

@@ -1,0 +1,176 @@
+"""Host-side parts of the library, on the CPU: the C ABI exports, the
+structured curved-hex generator against its closed form, exact Bernstein
+boxes against dense sampling, and BVH invariants.  No GPU call is made."""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import math
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def test_library_exports_every_declared_symbol(femrt):
+    header = (ROOT / "include" / "femrt.h").read_text()
+    declared = set(re.findall(r"FEMRT_API\s+[\w\s\*]+?\b(femrt_\w+)\s*\(", header))
+    assert declared, "no declarations parsed"
+    assert declared == set(femrt.exported_symbols())
+    lib = C.CDLL(str(femrt.LIB_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert femrt.lib().femrt_version() >= 100
+
+
+def _lagrange(nodes, x):
+    out = np.ones(len(nodes))
+    for a in range(len(nodes)):
+        for b in range(len(nodes)):
+            if a != b:
+                out[a] *= (x - nodes[b]) / (nodes[a] - nodes[b])
+    return out
+
+
+FACES = [  # fix axis, side (0:-1, 1:+1), u axis, v axis (Nektar order)
+    (2, 0, 0, 1), (1, 0, 0, 2), (0, 1, 1, 2), (1, 1, 0, 2), (0, 0, 1, 2), (2, 1, 0, 1)]
+
+
+def closed_form_map(N, q, amp, ijk, rst):
+    """Trilinear map plus, per face, the degree-q interpolant of the evenly
+    spaced face-node displacements (sinusoid amp sin(2 pi a) sin(2 pi b) of
+    the node's in-plane world coordinates, zero on the face boundary) times
+    the linear weight (1 -+ w)/2 in the face-normal coordinate."""
+    h = 1.0 / N
+    x = np.array([(ijk[c] + 0.5 * (rst[c] + 1)) * h for c in range(3)])
+    even = np.array([-1 + 2 * a / q for a in range(q + 1)])
+    for fix, side, ua, va in FACES:
+        lu = _lagrange(even, rst[ua])
+        lv = _lagrange(even, rst[va])
+        D = 0.0
+        for a in range(1, q):
+            for b in range(1, q):
+                w = np.zeros(3)
+                w[fix] = (ijk[fix] + side) * h
+                w[ua] = (ijk[ua] + a / q) * h
+                w[va] = (ijk[va] + b / q) * h
+                a0, a1 = [c for c in range(3) if c != fix]
+                disp = amp * math.sin(2 * math.pi * w[a0]) * math.sin(2 * math.pi * w[a1])
+                D += disp * lu[a] * lv[b]
+        wn = 0.5 * (1 + rst[fix]) if side else 0.5 * (1 - rst[fix])
+        x[fix] += D * wn
+    return x
+
+
+@pytest.mark.parametrize("N,q,p,amp", [(4, 3, 4, 0.05), (2, 4, 6, 0.03)])
+def test_generator_matches_closed_form(femrt, oracle, N, q, p, amp):
+    hm = femrt.gen_structured_hex(N, q, p, pert=1, amp=amp, seed=7)
+    orc = oracle.Oracle.from_mesh(hm)
+    rng = np.random.default_rng(4)
+    worst = 0.0
+    for _ in range(200):
+        ijk = rng.integers(0, N, 3)
+        e = int((ijk[2] * N + ijk[1]) * N + ijk[0])
+        rst = rng.uniform(-1, 1, 3)
+        x, _ = orc.map(e, rst)
+        worst = max(worst, np.max(np.abs(x - closed_form_map(N, q, amp, ijk, rst))))
+    assert worst < 2e-6  # FP32 coefficient rounding
+
+
+def test_generator_corners_adjacency_and_conformity(femrt, oracle):
+    N = 3
+    hm = femrt.gen_structured_hex(N, 4, 3, pert=2, amp=0.08, seed=11)
+    orc = oracle.Oracle.from_mesh(hm)
+    corners = [(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1),
+               (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)]
+    rng = np.random.default_rng(5)
+    opp = [5, 3, 4, 1, 2, 0]
+    for e in range(hm.n_elem):
+        for c, rst in enumerate(corners):  # main.cu:93-121 vertex reproduction
+            x, _ = orc.map(e, rst)
+            np.testing.assert_allclose(x, hm.vertices[e, c], atol=2e-7)
+        for f in range(6):
+            ne, nfc = hm.adjacency[e, f]
+            if ne < 0:
+                continue
+            assert nfc == opp[f] and hm.adjacency[ne, nfc, 0] == e
+            fix, side, ua, va = FACES[f]
+            for _ in range(5):
+                u, v = rng.uniform(-1, 1, 2)
+                a = np.zeros(3)
+                a[fix] = 1 if side else -1
+                a[ua], a[va] = u, v
+                fix2, side2, ua2, va2 = FACES[nfc]
+                b = np.zeros(3)
+                b[fix2] = 1 if side2 else -1
+                b[ua2], b[va2] = u, v
+                xa, _ = orc.map(e, a)
+                xb, _ = orc.map(ne, b)
+                np.testing.assert_allclose(xa, xb, atol=1e-6)
+
+
+def test_generator_field_is_seeded_and_decays(femrt):
+    a = femrt.gen_structured_hex(2, 3, 6, seed=3)
+    b = femrt.gen_structured_hex(2, 3, 6, seed=3)
+    c = femrt.gen_structured_hex(2, 3, 6, seed=4)
+    np.testing.assert_array_equal(a.field, b.field)
+    assert not np.array_equal(a.field, c.field)
+    nf = 7
+    deg = np.add.outer(np.add.outer(np.arange(nf), np.arange(nf)), np.arange(nf)).ravel()
+    scaled = a.field * (1.0 + deg)[None, :] ** 2  # (k,j,i) symmetric sum of indices
+    assert abs(scaled.mean()) < 0.1 and 0.8 < scaled.std() < 1.2
+
+
+@pytest.mark.parametrize("pert,amp,q", [(1, 0.05, 3), (2, 0.08, 4)])
+def test_bernstein_boxes_contain_dense_samples(femrt, oracle, pert, amp, q):
+    N = 3
+    hm = femrt.gen_structured_hex(N, q, 3, pert=pert, amp=amp, seed=9)
+    boxes = femrt.host_boxes(hm)
+    orc = oracle.Oracle.from_mesh(hm)
+    g = np.linspace(-1, 1, 9)
+    rng = np.random.default_rng(6)
+    for e in range(hm.n_elem):
+        lo, hi = boxes[e, 0, :3], boxes[e, 0, 3:]
+        pts = [orc.map(e, rst)[0] for rst in itertools.product(g, g, g)]
+        pts += [orc.map(e, rng.uniform(-1, 1, 3))[0] for _ in range(100)]
+        pts = np.array(pts)
+        assert np.all(pts >= lo) and np.all(pts <= hi)
+        ext = pts.max(0) - pts.min(0)
+        assert np.all((hi - lo) <= 1.6 * ext + 1e-6)  # tight enough to be useful
+        for f, (fix, side, ua, va) in enumerate(FACES):
+            flo, fhi = boxes[e, 1 + f, :3], boxes[e, 1 + f, 3:]
+            for u, v in itertools.product(g, g):
+                rst = np.zeros(3)
+                rst[fix] = 1 if side else -1
+                rst[ua], rst[va] = u, v
+                x, _ = orc.map(e, rst)
+                assert np.all(x >= flo) and np.all(x <= fhi), (e, f)
+
+
+def test_bvh_invariants(femrt):
+    hm = femrt.gen_structured_hex(6, 3, 2, pert=2, amp=0.08, seed=1)
+    boxes = femrt.host_boxes(hm)
+    nodes, prims = femrt.host_build_bvh(boxes)
+    assert sorted(prims.tolist()) == list(range(hm.n_elem))
+    ab = nodes.view(np.int32)
+    seen = np.zeros(hm.n_elem, int)
+
+    def visit(i, depth):
+        lo, hi = nodes[i, :3], nodes[i, 4:7]
+        a, b = ab[i, 3], ab[i, 7]
+        if b < 0:
+            assert 1 <= -b <= 4
+            for s in range(a, a - b):
+                e = prims[s]
+                seen[e] += 1
+                assert np.all(boxes[e, 0, :3] >= lo) and np.all(boxes[e, 0, 3:] <= hi)
+            return depth
+        for c in (a, b):
+            assert np.all(nodes[c, :3] >= lo) and np.all(nodes[c, 4:7] <= hi)
+        return max(visit(a, depth + 1), visit(b, depth + 1))
+
+    depth = visit(0, 0)
+    assert np.all(seen == 1)
+    assert depth < 40
