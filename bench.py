@@ -1,0 +1,237 @@
+"""Throughput of the curved-hex ray tracer on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): structured
+32^3 curved hexes of the unit cube, geometry order Q=4 with seeded gaussian
+face-node displacement 0.08 h, field order P=6 with seeded coefficients
+N(0,1)/(1+i+j+k)^2, 1920x1080 pixel-centre primary rays (1 spp), geometry
+first hit plus isosurfaces at {-0.1, 0, 0.1}.  A step is one trace of the
+whole frame: ray generation, BVH traversal + face Newton (FP32), FP64 hit
+polish, element walk + iso sampling (FP32), FP64 iso endgame, shading.  The
+mesh, its BVH and the frame buffers are resident in HBM before timing starts.
+
+Prints one JSON line (see the field docs in main()).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s, 32³ curved hex P=6 @1080p: learned-guess vs plain Newton at 1e-6 err"
+N_EL, Q, P, AMP, SEED = 32, 4, 6, 0.08, 20240611
+W, H = 1920, 1080
+ISOS = [-0.1, 0.0, 0.1]
+PEAK_FP32_TFLOPS = 157.3   # MI355X published vector/matrix FP32 (packed); 78.6 unpacked
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def flops_model(ng, nf):
+    """Event-count cost model (flops per event) at the true mode counts."""
+    def VG(n):
+        return 2 * (2 * n ** 3 + 3 * n ** 2 + 4 * n)
+
+    def V(n):
+        return 2 * (n ** 3 + n ** 2 + n)
+    geo = 3 * VG(ng) + 30 * ng + 60          # x + J + 3x3 solve
+    field_val = V(nf) + 30 * nf              # value only (iso samples)
+    field_grad = VG(nf) + 30 * nf + 15       # value + world gradient
+    return dict(geo=geo, field_val=field_val, field_grad=field_grad, slab=24)
+
+
+def kernel_flops(counters, ng, nf):
+    fm = flops_model(ng, nf)
+    f32 = ((counters["face_newton_it"] + counters["invert_it"]) * fm["geo"] +
+           counters["field_evals"] * fm["field_val"] + counters["slab_tests"] * fm["slab"])
+    f64 = counters["geo64_it"] * fm["geo"] + counters["field64_evals"] * fm["field_grad"]
+    return f32, f64
+
+
+def cpu_baseline(femrt, orc_mod, hm, rays, budget_s=15.0):
+    """Reference-faithful FP32 CPU port (oracle/femrt_refport.c), all host
+    cores, on a stratified sample of the same rays, bounded in time."""
+    import ctypes as C
+    L = orc_mod.lib()
+    L.rp_loader_boxes.argtypes = [C.c_int] * 3 + [C.c_void_p, C.c_void_p]
+    L.rp_trace.restype = C.c_int
+    L.rp_trace.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                           C.c_void_p]
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    boxes = np.zeros((hm.n_elem, 6), np.float32)
+    L.rp_loader_boxes(hm.n_elem, hm.ng, hm.nf, p(hm.geom), p(boxes))
+    n = rays.shape[1]
+    # stratified pixel sample: every 16th pixel in x and y, in a shuffled order
+    ys, xs = np.meshgrid(np.arange(8, H, 16), np.arange(8, W, 16), indexing="ij")
+    idx = (ys * W + xs).ravel().astype(np.int32)
+    np.random.default_rng(0).shuffle(idx)
+    isos = np.array(ISOS, np.float32)
+    done, t_used, chunk = 0, 0.0, 64
+    cores = len(os.sched_getaffinity(0))
+    while done < idx.size and t_used < budget_s:
+        sel = np.ascontiguousarray(idx[done:done + chunk])
+        out = np.zeros((sel.size, len(ISOS)), np.float32)
+        t0 = time.perf_counter()
+        L.rp_trace(hm.n_elem, hm.ng, hm.nf, p(hm.geom), p(hm.field), p(boxes), p(rays), n,
+                   p(sel), sel.size, p(isos), len(ISOS), p(out))
+        t_used += time.perf_counter() - t0
+        done += sel.size
+        chunk = min(4 * chunk, 1024)
+    mrays = done / t_used / 1e6
+    return {"value": mrays, "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": (f"{done} of the {idx.size} rays at every 16th pixel in x and y "
+                       f"({done / n * 100:.3f}% of the frame), 3 isovalue passes per ray, "
+                       f"{t_used:.1f} s on {cores} threads, {platform.processor() or platform.machine()}")}
+
+
+def parity_sample(femrt, orc_mod, parity, hm, fr, n_iso, n_check=192):
+    """Check a random sample of hit rays (and some misses) against the oracle."""
+    gh, giso, rays = fr.hits(), fr.iso(n_iso), fr.get_rays()
+    rng = np.random.default_rng(1)
+    hit = np.flatnonzero(gh["elem"] >= 0)
+    miss = np.flatnonzero(gh["elem"] < 0)
+    sel = np.concatenate([rng.choice(hit, min(n_check, hit.size), replace=False),
+                          rng.choice(miss, min(n_check // 8, miss.size), replace=False)])
+    sub = lambda d: {k: (v[..., sel] if isinstance(v, np.ndarray) else v) for k, v in d.items()}  # noqa: E731
+    og, oiso = orc_mod.Oracle.from_mesh(hm).trace(rays[:, sel], ISOS[:n_iso])
+    return parity.compare(sub(gh), sub(giso), og, oiso, ISOS[:n_iso])
+
+
+def run_variant(femrt, ctx, mesh, fr, guess, steps, warmup):
+    prm = femrt.trace_params(ISOS, guess=guess)
+    for _ in range(warmup):
+        fr.trace(mesh, prm)
+    ctx.sync()
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fr.camera_rays(CAM, W, H)
+        fr.trace(mesh, prm)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    kt = ctx.timing_read()
+    ctx.timing(False)
+    # one validation trace with work counters on (not timed)
+    fr.counters(enable=True)
+    fr.trace(mesh, prm)
+    cnt = fr.counters(enable=False)
+    return wall, kt, cnt
+
+
+CAM = None
+
+
+def main():
+    global CAM
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--variants", default="warm,plain")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the path shards by image tiles with the mesh replicated, see the design notes",
+              file=sys.stderr)
+        sys.exit(2)
+
+    femrt = importlib.import_module("test-_amd")
+    femrt.lib()
+    hm = femrt.gen_structured_hex(N_EL, Q, P, pert=2, amp=AMP, seed=SEED)
+    CAM = femrt.auto_camera([0, 0, 0], [1, 1, 1])
+    ctx = femrt.Context(0)
+    mesh = femrt.Mesh(ctx, hm)
+    n_rays = W * H
+    fr = femrt.Frame(ctx, n_rays)
+    fr.camera_rays(CAM, W, H)
+
+    names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED}
+    variants = [v for v in args.variants.split(",") if v]
+    results = {}
+    for v in variants:
+        wall, kt, cnt = run_variant(femrt, ctx, mesh, fr, names[v], args.steps, args.warmup)
+        f32, f64 = kernel_flops(cnt, hm.ng, hm.nf)
+        results[v] = dict(wall=wall, kernels=kt, counters=cnt, f32=f32, f64=f64,
+                          mrays=args.steps * n_rays / wall / 1e6)
+        print(json.dumps({"variant": v, "mrays_s": results[v]["mrays"],
+                          "ms_per_step": wall / args.steps * 1e3,
+                          "kernels_ms_per_step": {k: a / max(1, c // 1) for k, (a, c) in kt.items()},
+                          "counters": cnt}), file=sys.stderr, flush=True)
+
+    head = "learned" if "learned" in results else ("warm" if "warm" in results else variants[0])
+    R = results[head]
+    # dominant kernel roofline: the kernel with the largest time share
+    kt = R["kernels"]
+    dom = max(kt, key=lambda k: kt[k][0])
+    dom_ms = kt[dom][0] / kt[dom][1]
+    # FP32 flops per launch attributed to the dominant kernel by its counters
+    fm = flops_model(hm.ng, hm.nf)
+    cnt = R["counters"]
+    if dom == "k_march":
+        fl = ((cnt["invert_it"] + cnt["face_newton_it"]) * fm["geo"] +
+              cnt["field_evals"] * fm["field_val"]) - 0
+    else:
+        fl = R["f32"]
+    achieved = fl / (dom_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "frac_of_unpacked_fp32": achieved / (PEAK_FP32_TFLOPS / 2)}
+
+    check = None
+    if not args.no_check:
+        orc_mod = importlib.import_module("oracle.oracle")
+        parity = importlib.import_module("oracle.parity")
+        prm = femrt.trace_params(ISOS, guess=names[head])
+        fr.trace(mesh, prm)
+        check = parity_sample(femrt, orc_mod, parity, hm, fr, len(ISOS))
+    cpu = None
+    if not args.no_cpu:
+        orc_mod = importlib.import_module("oracle.oracle")
+        cpu = cpu_baseline(femrt, orc_mod, hm, fr.get_rays(), args.cpu_budget)
+
+    hits = R["counters"]["hit_rays"]
+    line = {
+        "metric": METRIC,
+        "value": R["mrays"],
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": R["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded 32^3 curved-hex mesh and modal field)",
+        "config": {"workload": "32^3 curved hex Q=4 P=6, 1920x1080 1spp, isos {-0.1,0,0.1}",
+                   "variant": head, "rays": n_rays, "hit_fraction": hits / n_rays,
+                   "mrays_s_hit_rays": R["mrays"] * hits / n_rays,
+                   "variants_mrays_s": {k: v["mrays"] for k, v in results.items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "parity_sample": check,
+    }
+    if "plain" in results and head != "plain":
+        line["config"]["speedup_vs_plain_newton"] = R["mrays"] / results["plain"]["mrays"]
+    print(json.dumps(line), flush=True)
+    fr.close()
+    mesh.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

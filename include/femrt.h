@@ -84,10 +84,12 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* ctx, int n_elem, int ng, int nf,
                                     const float* vertices, const float* geom, const float* field,
                                     const int* adjacency, femrt_mesh** out);
 FEMRT_API int femrt_mesh_destroy(femrt_mesh* mesh);
-/* Conservative element and face boxes: exact Bernstein hull of the modal
- * geometry (replaces the sampled +15% box of ComputeHexAabb,
- * high_order.h:51-113).  out: n_elem x 7 x 6 floats (element box, then the 6
- * face boxes; each lo.xyz, hi.xyz). */
+/* Conservative boxes from the exact Bernstein hull of the modal geometry,
+ * split 3 ways per reference direction (replaces the sampled +15% box of
+ * ComputeHexAabb, high_order.h:51-113).  out: n_elem x 61 x 6 floats: [0]
+ * element box, [1..6] face boxes (Nektar face order), [7 + 9 f + 3 b + a] box
+ * of sub-patch (a, b) of face f on the 3 x 3 split of its (u, v) square; each
+ * lo.xyz, hi.xyz. */
 FEMRT_API int femrt_mesh_boxes(const femrt_mesh* mesh, float* out);
 /* builder 0: binned SAH on the host, 16 bins, leaves of <= 4 elements
  * (BVH.cu:35-177).  Returns node count via *n_nodes (may be NULL). */
@@ -166,7 +168,7 @@ FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
 
 /* ------------------------------------------------ host-only utilities */
 /* The same box and BVH builders the mesh upload uses, on host arrays (no
- * device needed).  boxes: n_elem x 7 x 6 floats.  nodes8 must hold
+ * device needed).  boxes: n_elem x 61 x 6 floats (layout of femrt_mesh_boxes).  nodes8 must hold
  * 2 * n_elem nodes; *n_nodes receives the count. */
 FEMRT_API int femrt_host_boxes(int n_elem, int ng, const float* geom, float* boxes);
 FEMRT_API int femrt_host_build_bvh(int n_elem, const float* boxes, int* n_nodes, float* nodes8,
@@ -177,8 +179,10 @@ typedef struct {
   int n;          /* elements per axis on the unit cube                     */
   int q;          /* geometry order (ng = q + 1)                            */
   int p;          /* field order (nf = p + 1)                               */
-  int pert;       /* 0 none, 1 A sin(2 pi a) sin(2 pi b), 2 seeded gaussian
-                     A*h*N(0,1) clipped to 3 sigma                          */
+  int pert;       /* 0 none; 1 A sin(2 pi a) sin(2 pi b) of the node's two
+                     in-face world coordinates; 2 seeded bubble per face
+                     A h xi_f (1-u^2)(1-v^2), xi_f ~ N(0,1) clipped to +-2,
+                     (u, v) the node's face coordinates                     */
   double amp;
   uint64_t seed;
 } femrt_gen_params;

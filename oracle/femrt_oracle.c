@@ -96,25 +96,26 @@ ORC_EXPORT double orc_modA_d(int i, double x)
 /* ------------------------------------------------------------------------ */
 ORC_EXPORT void orc_eval_modal(const float* c, int n, const double* rst, double* out4)
 {
+  double pr[ORC_MAXN], dpr[ORC_MAXN], ps[ORC_MAXN], dps[ORC_MAXN], pt[ORC_MAXN], dpt[ORC_MAXN];
+  for (int i = 0; i < n; ++i) {
+    pr[i] = orc_modA(i, rst[0]);
+    dpr[i] = orc_modA_d(i, rst[0]);
+    ps[i] = orc_modA(i, rst[1]);
+    dps[i] = orc_modA_d(i, rst[1]);
+    pt[i] = orc_modA(i, rst[2]);
+    dpt[i] = orc_modA_d(i, rst[2]);
+  }
   double v = 0.0, dr = 0.0, ds = 0.0, dt = 0.0;
   int idx = 0;
-  for (int k = 0; k < n; ++k) {
-    const double tk = orc_modA(k, rst[2]);
-    const double tkd = orc_modA_d(k, rst[2]);
-    for (int j = 0; j < n; ++j) {
-      const double sj = orc_modA(j, rst[1]);
-      const double sjd = orc_modA_d(j, rst[1]);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) {
-        const double ri = orc_modA(i, rst[0]);
-        const double rid = orc_modA_d(i, rst[0]);
         const double cc = (double)c[idx++];
-        v += cc * ri * sj * tk;
-        dr += cc * rid * sj * tk;
-        ds += cc * ri * sjd * tk;
-        dt += cc * ri * sj * tkd;
+        v += cc * pr[i] * ps[j] * pt[k];
+        dr += cc * dpr[i] * ps[j] * pt[k];
+        ds += cc * pr[i] * dps[j] * pt[k];
+        dt += cc * pr[i] * ps[j] * dpt[k];
       }
-    }
-  }
   out4[0] = v;
   out4[1] = dr;
   out4[2] = ds;
@@ -181,15 +182,18 @@ static int inv3(const double* J, double* inv)
   return 1;
 }
 
-/* A5  point inversion x -> rst: HexElement.cu:227-272.  Same iteration
- * (start at rst=0, undamped Newton), converged in FP64 to |drst|_inf < 1e-14;
- * if the undamped iteration fails to converge in 60 steps a damped restart
- * (step capped at 0.5 in inf-norm) is tried.  Returns the iteration count or
- * -1. */
-ORC_EXPORT int orc_invert(const orc_mesh_t* m, int e, const double* x, double* rst)
+/* A5  point inversion x -> rst: HexElement.cu:227-272.  Newton from a seed
+ * (the reference starts at rst = 0, undamped), converged in FP64 to
+ * |drst|_inf < 1e-14 or to the rounding floor; when the undamped iteration
+ * fails a damped restart (step capped at 0.5 in inf-norm) is tried.  Returns
+ * the iteration count or -1. */
+static int invert_from(const orc_mesh_t* m, int e, const double* x, const double* seed, double* rst)
 {
   for (int attempt = 0; attempt < 2; ++attempt) {
-    rst[0] = rst[1] = rst[2] = 0.0;
+    rst[0] = seed[0];
+    rst[1] = seed[1];
+    rst[2] = seed[2];
+    double prev = 1e300;
     for (int it = 0; it < 80; ++it) {
       double X[3], J[9], iJ[9];
       orc_map(m, e, rst, X, J);
@@ -209,10 +213,47 @@ ORC_EXPORT int orc_invert(const orc_mesh_t* m, int e, const double* x, double* r
       rst[1] -= d1;
       rst[2] -= d2;
       if (!(fabs(rst[0]) < 1e6 && fabs(rst[1]) < 1e6 && fabs(rst[2]) < 1e6)) break;
-      if (fabs(d0) < 1e-14 && fabs(d1) < 1e-14 && fabs(d2) < 1e-14) return it + 1;
+      /* converged below 1e-14, or at the FP64 rounding floor (step below
+       * 1e-10 that no longer halves) */
+      const double st = fmax(fabs(d0), fmax(fabs(d1), fabs(d2)));
+      if (st < 1e-14 || (st < 1e-10 && st > 0.5 * prev)) return it + 1;
+      prev = st;
     }
   }
   return -1;
+}
+
+/* The preimage of x in the element: for a valid element (positive Jacobian)
+ * the map is one-to-one on the closed cube, so the in-cube preimage is unique.
+ * Newton from rst = 0 (the reference's start) can converge to a spurious
+ * preimage of the polynomial map outside the cube for points near a face; the
+ * oracle then restarts from the 8 octant centres (+-0.5)^3 and returns the
+ * first converged preimage with max|rst| <= 1 + 1e-3 (or, if none is inside,
+ * the converged one nearest the cube). */
+ORC_EXPORT int orc_invert(const orc_mesh_t* m, int e, const double* x, double* rst)
+{
+  double best[3] = {0, 0, 0}, best_out = 1e300;
+  int best_it = -1;
+  for (int s = 0; s < 9; ++s) {
+    const double seed[3] = {s == 0 ? 0.0 : ((s - 1) & 1 ? 0.5 : -0.5),
+                            s == 0 ? 0.0 : ((s - 1) & 2 ? 0.5 : -0.5),
+                            s == 0 ? 0.0 : ((s - 1) & 4 ? 0.5 : -0.5)};
+    double r[3];
+    const int it = invert_from(m, e, x, seed, r);
+    if (it < 0) continue;
+    const double out = fmax(fabs(r[0]), fmax(fabs(r[1]), fabs(r[2])));
+    if (out <= 1.0 + 1e-3) {
+      rst[0] = r[0]; rst[1] = r[1]; rst[2] = r[2];
+      return it;
+    }
+    if (out < best_out) {
+      best_out = out;
+      best[0] = r[0]; best[1] = r[1]; best[2] = r[2];
+      best_it = it;
+    }
+  }
+  rst[0] = best[0]; rst[1] = best[1]; rst[2] = best[2];
+  return best_it;
 }
 
 /* A6  field value, reference gradient and world gradient J^{-T} grad_ref:
@@ -253,7 +294,7 @@ ORC_EXPORT void orc_face_axes(int face, int* fix, double* val, int* ua, int* va)
 }
 
 /* Nearest in-bounds root of X(face(u,v)) = o + tau d over the reference's 9
- * seeds {-0.8,0,0.8}^2 (high_order.h:206-211).  Each seed follows the
+ * seeds {-0.8,0,0.8}^2 (high_order.h:206-211) and a 5x5 seed grid.  Each seed follows the
  * reference iteration: tau0 = projection of X(seed) onto the ray clamped to
  * [tmin,tmax] (:221-225); Newton on (u,v,tau) with the step scaled to
  * |.|_inf <= 0.25 (:288-293), tau clamped to [tmin,tmax] (:294), u,v clamped
@@ -265,23 +306,45 @@ ORC_EXPORT void orc_face_axes(int face, int* fix, double* val, int* ua, int* va)
  * 1+ORC_FACE_ACC-max(|u|,|v|) for the returned root, and *nearmiss the
  * smallest |1+ORC_FACE_ACC-max(|u|,|v|)| over converged but rejected roots
  * (both used to classify tie pixels). */
-ORC_EXPORT int orc_face_hit(const orc_mesh_t* m, int e, int face, const double* o,
-                            const double* d, double tmin, double tmax, double* t_out,
-                            double* rst_out, double* margin, double* nearmiss)
+/* Rejected (converged, just outside the face) roots for tie classification. */
+typedef struct {
+  double t[64], mg[64];
+  int n;
+} orc_rej_t;
+
+static int face_roots_impl(const orc_mesh_t* m, int e, int face, const double* o, const double* d,
+                           double tmin, double tmax, double* t_out, double* rst_out, double* margin,
+                           double* nearmiss, double* all_t, int cap, int* n_all, orc_rej_t* rej)
 {
   const int fa = FACE_FIX[face], ua = FACE_U[face], va = FACE_V[face];
   const double fv = FACE_VAL[face];
-  const double sv[3] = {-0.8, 0.0, 0.8};
+  /* the reference's 3x3 grid {-0.8,0,0.8}^2 followed by a 5x5 grid
+   * {-0.9,-0.45,0,0.45,0.9}^2: the denser grid reaches roots the reference's
+   * seeds miss on strongly curved faces */
+  double seeds[34][2];
+  int ns = 0;
+  for (int sj = 0; sj < 3; ++sj)
+    for (int si = 0; si < 3; ++si) {
+      seeds[ns][0] = -0.8 + 0.8 * si;
+      seeds[ns][1] = -0.8 + 0.8 * sj;
+      ++ns;
+    }
+  for (int sj = 0; sj < 5; ++sj)
+    for (int si = 0; si < 5; ++si) {
+      seeds[ns][0] = -0.9 + 0.45 * si;
+      seeds[ns][1] = -0.9 + 0.45 * sj;
+      ++ns;
+    }
   int found = 0;
   double best_t = tmax;
   double nm = 1e30;
   const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  for (int sj = 0; sj < 3; ++sj)
-    for (int si = 0; si < 3; ++si) {
+  for (int sq = 0; sq < ns; ++sq) {
+    {
       double rst[3] = {0, 0, 0};
       rst[fa] = fv;
-      rst[ua] = sv[si];
-      rst[va] = sv[sj];
+      rst[ua] = seeds[sq][0];
+      rst[va] = seeds[sq][1];
       double X[3], J[9];
       orc_map(m, e, rst, X, NULL);
       double tau = dd > 0 ? ((X[0] - o[0]) * d[0] + (X[1] - o[1]) * d[1] + (X[2] - o[2]) * d[2]) / dd : 0.0;
@@ -331,6 +394,12 @@ ORC_EXPORT int orc_face_hit(const orc_mesh_t* m, int e, int face, const double* 
       const double mu = fmax(fabs(rst[ua]), fabs(rst[va]));
       const double mg = 1.0 + ORC_FACE_ACC - mu;
       if (mg >= 0.0 && tau >= tmin && tau <= tmax) {
+        if (all_t) {
+          int dup = 0;
+          for (int q = 0; q < *n_all; ++q)
+            if (fabs(all_t[q] - tau) <= 1e-9 * (1.0 + fabs(tau))) dup = 1;
+          if (!dup && *n_all < cap) all_t[(*n_all)++] = tau;
+        }
         if (!found || tau < best_t) {
           found = 1;
           best_t = tau;
@@ -341,11 +410,25 @@ ORC_EXPORT int orc_face_hit(const orc_mesh_t* m, int e, int face, const double* 
         }
       } else if (tau >= tmin && tau <= tmax) {
         if (fabs(mg) < nm) nm = fabs(mg);
+        if (rej && fabs(mg) < 1e-3 && rej->n < 64) {
+          rej->t[rej->n] = tau;
+          rej->mg[rej->n] = fabs(mg);
+          ++rej->n;
+        }
       }
     }
+  }
   if (nearmiss) *nearmiss = nm;
   if (found) *t_out = best_t;
   return found;
+}
+
+ORC_EXPORT int orc_face_hit(const orc_mesh_t* m, int e, int face, const double* o,
+                            const double* d, double tmin, double tmax, double* t_out,
+                            double* rst_out, double* margin, double* nearmiss)
+{
+  return face_roots_impl(m, e, face, o, d, tmin, tmax, t_out, rst_out, margin, nearmiss, NULL, 0,
+                         NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -355,13 +438,18 @@ ORC_EXPORT int orc_face_hit(const orc_mesh_t* m, int e, int face, const double* 
 ORC_EXPORT void orc_elem_box(const orc_mesh_t* m, int e, int samples, double pad_frac, double* lo,
                              double* hi)
 {
+  /* The image of the reference cube is bounded by the image of its boundary,
+   * so the box samples the 6 faces on a samples x samples grid and pads by
+   * pad_frac of the largest extent. */
   lo[0] = lo[1] = lo[2] = DBL_MAX;
   hi[0] = hi[1] = hi[2] = -DBL_MAX;
-  for (int k = 0; k < samples; ++k)
+  for (int f = 0; f < 6; ++f)
     for (int j = 0; j < samples; ++j)
       for (int i = 0; i < samples; ++i) {
-        const double rst[3] = {-1.0 + 2.0 * i / (samples - 1), -1.0 + 2.0 * j / (samples - 1),
-                               -1.0 + 2.0 * k / (samples - 1)};
+        double rst[3];
+        rst[FACE_FIX[f]] = FACE_VAL[f];
+        rst[FACE_U[f]] = -1.0 + 2.0 * i / (samples - 1);
+        rst[FACE_V[f]] = -1.0 + 2.0 * j / (samples - 1);
         double X[3];
         orc_map(m, e, rst, X, NULL);
         for (int c = 0; c < 3; ++c) {
@@ -477,7 +565,9 @@ ORC_EXPORT int orc_iso_segment(const orc_mesh_t* m, int e, const double* o, cons
 /* Output per ray (doubles, ORC_GEOM_STRIDE):                                */
 /*   [0] elem (-1 miss) [1] face [2] t [3..5] rst [6] u at hit               */
 /*   [7] t of nearest candidate on a different (elem,face) (1e30 none)       */
-/*   [8] margin of the accepted root  [9] smallest near-miss margin          */
+/*   [8] margin of the accepted root  [9] smallest |margin| of a converged    */
+/*   root rejected just outside its face, at or before the hit (any root for */
+/*   a miss)                                                                 */
 /* Per ray and isovalue (ORC_ISO_STRIDE):                                     */
 /*   [0] elem (-1 none) [1] t [2..4] rst [5] u [6] min |g| over samples of   */
 /*   the element where the root was found (tie classification)               */
@@ -491,15 +581,51 @@ ORC_EXPORT double orc_face_acc(void) { return ORC_FACE_ACC; }
 
 typedef struct {
   double lo[3], hi[3];
+  int ready;
 } box_t;
 
+typedef struct {
+  double t;
+  int e;
+} cand_t;
+
+static int cmp_cand(const void* a, const void* b)
+{
+  const double x = ((const cand_t*)a)->t, y = ((const cand_t*)b)->t;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* Candidate culling (an optimisation only): a coarse box per element from its
+ * 8 corner images padded by its full extent on every side, then, lazily, the
+ * face-sampled box (9x9 per face, 10% pad).  Candidates are visited by
+ * increasing coarse entry distance; once that distance exceeds every result
+ * still open (geometry hit, each isovalue) no later element can improve any
+ * result, so the scan stops. */
 ORC_EXPORT void orc_trace(const orc_mesh_t* m, const float* rays, int nrays, const double* isos,
                           int niso, int M, double tmin, double* out_geom, double* out_iso)
 {
-  box_t* boxes = (box_t*)malloc(sizeof(box_t) * (size_t)m->n_elem);
-  for (int e = 0; e < m->n_elem; ++e) orc_elem_box(m, e, 11, 0.10, boxes[e].lo, boxes[e].hi);
+  double* coarse = (double*)malloc(sizeof(double) * 6 * (size_t)m->n_elem);
+  box_t* boxes = (box_t*)calloc((size_t)m->n_elem, sizeof(box_t));
+  for (int e = 0; e < m->n_elem; ++e) {
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int c8 = 0; c8 < 8; ++c8) {
+      const double rst[3] = {(c8 & 1) ? 1.0 : -1.0, (c8 & 2) ? 1.0 : -1.0, (c8 & 4) ? 1.0 : -1.0};
+      double X[3];
+      orc_map(m, e, rst, X, NULL);
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = fmin(lo[c], X[c]);
+        hi[c] = fmax(hi[c], X[c]);
+      }
+    }
+    double ext = 0.0;
+    for (int c = 0; c < 3; ++c) ext = fmax(ext, hi[c] - lo[c]);
+    for (int c = 0; c < 3; ++c) {
+      coarse[6 * (size_t)e + c] = lo[c] - ext;
+      coarse[6 * (size_t)e + 3 + c] = hi[c] + ext;
+    }
+  }
 
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for schedule(dynamic, 1)
   for (int r = 0; r < nrays; ++r) {
     const double o[3] = {rays[0 * (size_t)nrays + r], rays[1 * (size_t)nrays + r],
                          rays[2 * (size_t)nrays + r]};
@@ -512,19 +638,60 @@ ORC_EXPORT void orc_trace(const orc_mesh_t* m, const float* rays, int nrays, con
       double* I = out_iso + ((size_t)r * niso + k) * ORC_ISO_STRIDE;
       I[0] = -1; I[1] = 1e30; I[2] = I[3] = I[4] = 0; I[5] = 0; I[6] = 1e30;
     }
+    int cap = 256, nc = 0;
+    cand_t* cand = (cand_t*)malloc(sizeof(cand_t) * cap);
+    orc_rej_t rej;
+    rej.n = 0;
     for (int e = 0; e < m->n_elem; ++e) {
+      double t0, t1;
+      if (!orc_slab(o, d, coarse + 6 * (size_t)e, coarse + 6 * (size_t)e + 3, tmin, 1e30, &t0, &t1))
+        continue;
+      if (nc == cap) {
+        cap *= 2;
+        cand = (cand_t*)realloc(cand, sizeof(cand_t) * cap);
+      }
+      cand[nc].t = t0;
+      cand[nc].e = e;
+      ++nc;
+    }
+    qsort(cand, nc, sizeof(cand_t), cmp_cand);
+    for (int ci = 0; ci < nc; ++ci) {
+      /* sound early exit */
+      double open = G[7] < 1e30 ? G[7] : G[2];
+      for (int k = 0; k < niso; ++k) open = fmax(open, out_iso[((size_t)r * niso + k) * ORC_ISO_STRIDE + 1]);
+      if (G[0] >= 0 && cand[ci].t > open) break;
+      const int e = cand[ci].e;
+      if (!boxes[e].ready) {
+        double lo[3], hi[3];
+        orc_elem_box(m, e, 9, 0.10, lo, hi);
+#pragma omp critical(orc_boxes)
+        {
+          if (!boxes[e].ready) {
+            memcpy(boxes[e].lo, lo, sizeof(lo));
+            memcpy(boxes[e].hi, hi, sizeof(hi));
+            boxes[e].ready = 1;
+          }
+        }
+      }
       double t0, t1;
       if (!orc_slab(o, d, boxes[e].lo, boxes[e].hi, tmin, 1e30, &t0, &t1)) continue;
       double ft[6], frst[6][3];
       int fok[6];
+      double roots[64];
       int nroots = 0;
       for (int f = 0; f < 6; ++f) {
         double mg = 0, nm = 1e30;
-        fok[f] = orc_face_hit(m, e, f, o, d, tmin, 1e30, &ft[f], frst[f], &mg, &nm);
-        if (nm < G[9]) G[9] = nm;
+        double fr[16];
+        int nf = 0;
+        fok[f] = face_roots_impl(m, e, f, o, d, tmin, 1e30, &ft[f], frst[f], &mg, &nm, fr, 16, &nf,
+                                 &rej);
+        for (int q = 0; q < nf; ++q) {
+          int dup = 0;
+          for (int z = 0; z < nroots; ++z)
+            if (fabs(roots[z] - fr[q]) <= 1e-9 * (1.0 + fabs(fr[q]))) dup = 1;
+          if (!dup && nroots < 64) roots[nroots++] = fr[q];
+        }
         if (!fok[f]) continue;
-        ++nroots;
-        /* geometry first hit bookkeeping */
         if (ft[f] < G[2]) {
           if (G[0] >= 0) G[7] = fmin(G[7], G[2]);
           G[0] = e; G[1] = f; G[2] = ft[f];
@@ -535,25 +702,33 @@ ORC_EXPORT void orc_trace(const orc_mesh_t* m, const float* rays, int nrays, con
         }
       }
       if (niso == 0 || nroots < 2) continue;
-      /* inside segment: entry = nearest root, exit = nearest root beyond */
-      double ta = 1e30;
-      for (int f = 0; f < 6; ++f) if (fok[f] && ft[f] < ta) ta = ft[f];
-      double tb = 1e30;
-      for (int f = 0; f < 6; ++f)
-        if (fok[f] && ft[f] > ta + 1e-9 * (fabs(ta) + 1.0) && ft[f] < tb) tb = ft[f];
-      if (tb >= 1e30) continue;
-      for (int k = 0; k < niso; ++k) {
-        double* I = out_iso + ((size_t)r * niso + k) * ORC_ISO_STRIDE;
-        if (I[0] >= 0 && I[1] <= ta) continue; /* an earlier root already */
-        double t, rst[3], u, mn;
-        if (orc_iso_segment(m, e, o, d, ta, tb, M, isos[k], &t, rst, &u, &mn)) {
-          if (t < I[1]) {
-            I[0] = e; I[1] = t; I[2] = rst[0]; I[3] = rst[1]; I[4] = rst[2]; I[5] = u;
-            I[6] = mn;
+      /* the ray is inside the element between consecutive crossings of its
+       * boundary: [r0, r1], [r2, r3], ... (sorted distinct face roots) */
+      for (int a2 = 1; a2 < nroots; ++a2)
+        for (int b2 = a2; b2 > 0 && roots[b2] < roots[b2 - 1]; --b2) {
+          const double tmp = roots[b2];
+          roots[b2] = roots[b2 - 1];
+          roots[b2 - 1] = tmp;
+        }
+      for (int iv = 0; iv + 1 < nroots; iv += 2) {
+        const double ta = roots[iv], tb = roots[iv + 1];
+        for (int k = 0; k < niso; ++k) {
+          double* I = out_iso + ((size_t)r * niso + k) * ORC_ISO_STRIDE;
+          if (I[0] >= 0 && I[1] <= ta) continue; /* an earlier root already */
+          double t, rst[3], u, mn;
+          if (orc_iso_segment(m, e, o, d, ta, tb, M, isos[k], &t, rst, &u, &mn)) {
+            if (t < I[1]) {
+              I[0] = e; I[1] = t; I[2] = rst[0]; I[3] = rst[1]; I[4] = rst[2]; I[5] = u;
+              I[6] = mn;
+            }
           }
         }
       }
     }
+    free(cand);
+    /* near miss: a rejected root at or before the hit (any, for a miss) */
+    for (int q = 0; q < rej.n; ++q)
+      if (G[0] < 0 || rej.t[q] <= G[2] * (1.0 + 1e-6) + 1e-12) G[9] = fmin(G[9], rej.mg[q]);
     if (G[0] >= 0) {
       const double rst[3] = {G[3], G[4], G[5]};
       double u;
@@ -562,6 +737,7 @@ ORC_EXPORT void orc_trace(const orc_mesh_t* m, const float* rays, int nrays, con
     }
   }
   free(boxes);
+  free(coarse);
 }
 
 /* ------------------------------------------------------------------------ */

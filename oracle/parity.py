@@ -25,9 +25,13 @@ TIE_ISO = 1e-5
 
 
 def geometry_ties(og):
+    """Hit rays: another candidate within TIE_T of the hit distance, or the hit
+    within TIE_MARGIN of its face boundary.  Miss rays: a converged root just
+    outside a face (within TIE_MARGIN of acceptance)."""
     t = og["t"]
-    return (np.abs(og["t2"] - t) <= TIE_T * np.abs(t)) | (og["margin"] < TIE_MARGIN) | (
-        og["nearmiss"] < TIE_MARGIN)
+    hit = og["elem"] >= 0
+    near_hit = (np.abs(og["t2"] - t) <= TIE_T * np.abs(t)) | (og["margin"] < TIE_MARGIN)
+    return near_hit & hit | (og["nearmiss"] < TIE_MARGIN)
 
 
 def compare(gh, giso, og, oiso, isos):
@@ -42,8 +46,12 @@ def compare(gh, giso, og, oiso, isos):
     rep["ties"] = int(tie.sum())
     rep["hit_miss_mismatch"] = int(((ho != hg) & ~tie).sum())
     both = ho & hg
-    trel = np.abs(gh["t"][both] - og["t"][both]) / np.abs(og["t"][both])
-    rep["t_rel_max"] = float(trel.max()) if trel.size else 0.0
+    trel = np.abs(gh["t"] - og["t"]) / np.maximum(np.abs(og["t"]), 1e-300)
+    nt = both & ~tie
+    rep["t_rel_max"] = float(trel[nt].max()) if nt.any() else 0.0
+    # at tie pixels two crossings lie within TIE_T of each other, so either
+    # may be reported; their distance agreement is shown, not gated
+    rep["t_rel_max_ties"] = float(trel[both & tie].max()) if (both & tie).any() else 0.0
     same = both & (gh["elem"] == og["elem"]) & (gh["face"] == og["face"])
     rep["id_mismatch"] = int((both & ~same & ~tie).sum())
     rep["id_mismatch_ties"] = int((both & ~same & tie).sum())
@@ -60,7 +68,8 @@ def compare(gh, giso, og, oiso, isos):
     iso_rep = []
     for k, iso in enumerate(isos):
         fo, fg = oiso["elem"][k] >= 0, giso["elem"][k] >= 0
-        itie = oiso["minabs"][k] < TIE_ISO
+        # the walk starts at the geometry hit: a tie there ties the iso result
+        itie = (oiso["minabs"][k] < TIE_ISO) | tie
         r = {"iso": float(iso), "oracle_found": int(fo.sum()), "gpu_found": int(fg.sum()),
              "found_mismatch": int(((fo != fg) & ~itie).sum()), "ties": int(itie.sum())}
         b = fo & fg

@@ -148,8 +148,11 @@ def gen_structured_hex(n, q, p, pert=0, amp=0.0, seed=0) -> HexMesh:
     return HexMesh(v, g, f, a, ng, nf)
 
 
+NBOX = 61  # element box, 6 face boxes, 6 x 9 face sub-patch boxes
+
+
 def host_boxes(hm: HexMesh):
-    out = np.zeros((hm.n_elem, 7, 6), np.float32)
+    out = np.zeros((hm.n_elem, NBOX, 6), np.float32)
     _check(lib().femrt_host_boxes(hm.n_elem, hm.ng, _ptr(hm.geom), _ptr(out)), what="host_boxes")
     return out
 
@@ -255,7 +258,7 @@ class Mesh:
         self.n_nodes = nn.value
 
     def boxes(self):
-        out = np.zeros((self.host.n_elem, 7, 6), np.float32)
+        out = np.zeros((self.host.n_elem, NBOX, 6), np.float32)
         _check(lib().femrt_mesh_boxes(self.h, _ptr(out)), self.ctx.h, "mesh_boxes")
         return out
 

@@ -60,7 +60,7 @@ def build_oracle() -> Path:
     ORACLE_LIB.parent.mkdir(parents=True, exist_ok=True)
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
     _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-fopenmp",
-          ORACLE_DIR / "femrt_oracle.c", "-o", tmp, "-lm"])
+          ORACLE_DIR / "femrt_oracle.c", ORACLE_DIR / "femrt_refport.c", "-o", tmp, "-lm"])
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB
 

@@ -227,10 +227,17 @@ int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
       return gp->amp * std::sin(2.0 * M_PI * w[a0]) * std::sin(2.0 * M_PI * w[a1]);
     }
     if (gp->pert == 2) {
-      double g = normal_keyed(gp->seed, (uint64_t)axis + 1,
-                              ((uint64_t)lat[0] << 21) ^ (uint64_t)lat[1], (uint64_t)lat[2]);
-      g = std::max(-3.0, std::min(3.0, g));
-      return gp->amp * h * g;
+      // seeded bubble per face: amp h xi_f (1-u^2)(1-v^2), xi_f ~ N(0,1)
+      // clipped to +-2, keyed by the face's axis and lattice position so both
+      // elements of an interior face draw the same xi_f
+      const int a0 = axis == 0 ? 1 : 0, a1 = axis == 2 ? 1 : 2;
+      const long fq[3] = {lat[axis] / q, lat[a0] / q, lat[a1] / q};
+      double xi = normal_keyed(gp->seed, (uint64_t)axis + 1,
+                               ((uint64_t)fq[0] << 21) ^ (uint64_t)fq[1], (uint64_t)fq[2]);
+      xi = std::max(-2.0, std::min(2.0, xi));
+      const double u = -1.0 + 2.0 * (double)(lat[a0] % q) / q;
+      const double v = -1.0 + 2.0 * (double)(lat[a1] % q) / q;
+      return gp->amp * h * xi * (1.0 - u * u) * (1.0 - v * v);
     }
     return 0.0;
   };
@@ -388,6 +395,7 @@ static std::vector<double> modal_to_bernstein(int n, double a, double b)
 }
 
 static const int kBoxSplit = 3;
+static_assert(kBoxSplit == 3, "face sub-patch layout assumes 3x3 sub-patches per face");
 
 void bernstein_boxes(int n_elem, int ng, const float* geom, float* out)
 {
@@ -397,8 +405,8 @@ void bernstein_boxes(int n_elem, int ng, const float* geom, float* out)
   for (int s = 0; s < S; ++s) T[s] = modal_to_bernstein(n, -1.0 + 2.0 * s / S, -1.0 + 2.0 * (s + 1) / S);
   std::vector<double> cc(3 * n3), t1(3 * n3), t2(3 * n3), bz(3 * n3);
   for (int e = 0; e < n_elem; ++e) {
-    double lo[7][3], hi[7][3];
-    for (int b = 0; b < 7; ++b)
+    double lo[FEMRT_NBOX][3], hi[FEMRT_NBOX][3];
+    for (int b = 0; b < FEMRT_NBOX; ++b)
       for (int c = 0; c < 3; ++c) {
         lo[b][c] = 1e300;
         hi[b][c] = -1e300;
@@ -447,24 +455,29 @@ void bernstein_boxes(int n_elem, int ng, const float* geom, float* out)
                 }
                 for (int f = 0; f < 6; ++f) {
                   const int fix = (f == 0 || f == 5) ? 2 : ((f == 1 || f == 3) ? 1 : 0);
+                  const int ua = (f == 2 || f == 4) ? 1 : 0, va = (f == 0 || f == 5) ? 1 : 2;
                   const bool plus = (f == 2 || f == 3 || f == 5);
                   if (sub[fix] != (plus ? S - 1 : 0) || idx[fix] != (plus ? n - 1 : 0)) continue;
+                  // face box and the box of the face's sub-patch (sub[ua], sub[va])
+                  const int sp = 7 + 9 * f + 3 * sub[va] + sub[ua];
                   for (int c = 0; c < 3; ++c) {
                     lo[1 + f][c] = std::min(lo[1 + f][c], v[c]);
                     hi[1 + f][c] = std::max(hi[1 + f][c], v[c]);
+                    lo[sp][c] = std::min(lo[sp][c], v[c]);
+                    hi[sp][c] = std::max(hi[sp][c], v[c]);
                   }
                 }
               }
         }
       }
     }
-    for (int b = 0; b < 7; ++b) {
+    for (int b = 0; b < FEMRT_NBOX; ++b) {
       double ext = 0;
       for (int c = 0; c < 3; ++c) ext = std::max(ext, hi[b][c] - lo[b][c]);
       for (int c = 0; c < 3; ++c) {
         const double pad = 1e-6 * ext + 1e-6 * std::max(std::fabs(lo[b][c]), std::fabs(hi[b][c])) + 1e-9;
-        out[((size_t)e * 7 + b) * 6 + c] = std::nextafter((float)(lo[b][c] - pad), -INFINITY);
-        out[((size_t)e * 7 + b) * 6 + 3 + c] = std::nextafter((float)(hi[b][c] + pad), INFINITY);
+        out[((size_t)e * FEMRT_NBOX + b) * 6 + c] = std::nextafter((float)(lo[b][c] - pad), -INFINITY);
+        out[((size_t)e * FEMRT_NBOX + b) * 6 + 3 + c] = std::nextafter((float)(hi[b][c] + pad), INFINITY);
       }
     }
   }
@@ -490,7 +503,7 @@ struct Builder {
       hi[c] = -INFINITY;
     }
     for (int i = s; i < e; ++i) {
-      const float* b = boxes + (size_t)idx[i] * 42;
+      const float* b = boxes + (size_t)idx[i] * FEMRT_BOX_STRIDE;
       for (int c = 0; c < 3; ++c) {
         lo[c] = std::min(lo[c], b[c]);
         hi[c] = std::max(hi[c], b[3 + c]);
@@ -549,7 +562,7 @@ struct Builder {
         int b = (int)(NB * (cen[(size_t)idx[i] * 3 + ax] - clo[ax]) / ext);
         b = std::max(0, std::min(NB - 1, b));
         ++cnt[b];
-        const float* bx = boxes + (size_t)idx[i] * 42;
+        const float* bx = boxes + (size_t)idx[i] * FEMRT_BOX_STRIDE;
         for (int c = 0; c < 3; ++c) {
           blo[b][c] = std::min(blo[b][c], bx[c]);
           bhi[b][c] = std::max(bhi[b][c], bx[3 + c]);
@@ -613,7 +626,7 @@ void build_bvh_sah(int n_elem, const float* boxes, BvhHost& out)
   b.cen.resize((size_t)n_elem * 3);
   for (int e = 0; e < n_elem; ++e)
     for (int c = 0; c < 3; ++c)
-      b.cen[(size_t)e * 3 + c] = 0.5f * (boxes[(size_t)e * 42 + c] + boxes[(size_t)e * 42 + 3 + c]);
+      b.cen[(size_t)e * 3 + c] = 0.5f * (boxes[(size_t)e * FEMRT_BOX_STRIDE + c] + boxes[(size_t)e * FEMRT_BOX_STRIDE + 3 + c]);
   if (n_elem > 0) b.build(0, n_elem);
   out.prims = b.idx;
 }

@@ -18,8 +18,13 @@ struct BvhHost {
 };
 
 // Host algorithms (femrt_host.cpp)
-void bernstein_boxes(int n_elem, int ng, const float* geom, float* out /* n x 7 x 6 */);
-void build_bvh_sah(int n_elem, const float* boxes /* n x 7 x 6, element box first */,
+// Boxes per element: [0] element, [1..6] faces (Nektar order), [7 + 9 f +
+// 3 b + a] sub-patch (a, b) of face f on the 3 x 3 split of its (u, v)
+// square; each box lo.xyz, hi.xyz.
+#define FEMRT_NBOX 61
+#define FEMRT_BOX_STRIDE (FEMRT_NBOX * 6)
+void bernstein_boxes(int n_elem, int ng, const float* geom, float* out /* n x FEMRT_NBOX x 6 */);
+void build_bvh_sah(int n_elem, const float* boxes /* n x FEMRT_NBOX x 6, element box first */,
                    BvhHost& out);
 int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom, float* field,
                        int* adjacency);

@@ -32,6 +32,7 @@
 
 #include "femrt_internal.hpp"
 #include "femrt_math.hpp"
+#include "femrt_trace_core.hpp"
 
 using namespace femrt;
 
@@ -59,7 +60,7 @@ struct femrt_mesh {
   float4* d_geom = nullptr;            // n_elem x NG^3
   float* d_field = nullptr;            // n_elem x NF^3
   float* d_verts = nullptr;            // n_elem x 24
-  float* d_boxes = nullptr;            // n_elem x 42
+  float* d_boxes = nullptr;            // n_elem x FEMRT_BOX_STRIDE
   int* d_adj = nullptr;                // n_elem x 12
   float4* d_nodes = nullptr;           // n_nodes x 2
   int* d_prims = nullptr;
@@ -156,195 +157,26 @@ static int timed_launch(femrt_ctx* c, const char* name, F&& launch)
 
 // ============================================================ device helpers
 
-__device__ __forceinline__ void face_to_rst(int f, float u, float v, float& r, float& s, float& t)
-{
-  r = (f == 2) ? 1.f : ((f == 4) ? -1.f : u);
-  s = (f == 0 || f == 5) ? v : ((f == 1) ? -1.f : ((f == 3) ? 1.f : u));
-  t = (f == 0) ? -1.f : ((f == 5) ? 1.f : v);
-}
-
-template <typename T>
-__device__ __forceinline__ void face_to_rst_t(int f, T u, T v, T* rst)
-{
-  rst[0] = (f == 2) ? T(1) : ((f == 4) ? T(-1) : u);
-  rst[1] = (f == 0 || f == 5) ? v : ((f == 1) ? T(-1) : ((f == 3) ? T(1) : u));
-  rst[2] = (f == 0) ? T(-1) : ((f == 5) ? T(1) : v);
-}
-
-__device__ __forceinline__ void rst_to_face(int f, const float* rst, float& u, float& v)
-{
-  u = (f == 2 || f == 4) ? rst[1] : rst[0];
-  v = (f == 0 || f == 5) ? rst[1] : rst[2];
-}
-
-struct Counters {
-  long long face_it = 0, inv_it = 0, field = 0, geo64 = 0, field64 = 0, slab = 0, mlp = 0,
-            marched = 0;
-};
-
-// Face-constrained Newton on (u, v, tau) from one seed, FP32, with the
-// reference's update rules (high_order.h:213-300): step capped to 0.25 in the
-// inf-norm, tau clamped to [tmin, tmax], u, v clamped to [-1.5, 1.5],
-// divergence break after 5 iterations when the residual grows 1.5x, at most
-// 50 iterations.  Converged roots within max(|u|,|v|) <= 1 + 1e-4 are accepted.
-template <int NG>
-__device__ bool face_newton(const float4* __restrict__ gc, const float* o, const float* d, int f,
-                            float u, float v, float tau, float tmin, float tmax, float& tau_o,
-                            float& u_o, float& v_o, int& conv, long long& iters)
-{
-  float prev = FLT_MAX;
-  const float oscale = 1.f + fmaxf(fabsf(o[0]), fmaxf(fabsf(o[1]), fabsf(o[2])));
-  tau = fminf(fmaxf(tau, tmin), tmax);
-  conv = 0;
-  for (int it = 0; it < 50; ++it) {
-    float r, s, t;
-    face_to_rst(f, u, v, r, s, t);
-    float X[3], J[9];
-    geom_eval<float, NG>(gc, r, s, t, X, J);
-    ++iters;
-    const float R0 = X[0] - fmaf(tau, d[0], o[0]);
-    const float R1 = X[1] - fmaf(tau, d[1], o[1]);
-    const float R2 = X[2] - fmaf(tau, d[2], o[2]);
-    const float res = fmaxf(fabsf(R0), fmaxf(fabsf(R1), fabsf(R2)));
-    const float tol = 2e-6f * (oscale + fabsf(tau));
-    if (res < tol) {
-      conv = 1;
-      if (fmaxf(fabsf(u), fabsf(v)) <= 1.0001f && tau >= tmin && tau <= tmax) {
-        tau_o = tau;
-        u_o = u;
-        v_o = v;
-        return true;
-      }
-      return false;
-    }
-    if (it > 5 && res > prev * 1.5f) return false;
-    prev = res;
-    const bool us = (f == 2 || f == 4);
-    const bool vs = (f == 0 || f == 5);
-    const float A[9] = {us ? J[1] : J[0], vs ? J[1] : J[2], -d[0],
-                        us ? J[4] : J[3], vs ? J[4] : J[5], -d[1],
-                        us ? J[7] : J[6], vs ? J[7] : J[8], -d[2]};
-    const float b[3] = {-R0, -R1, -R2};
-    float dl[3];
-    if (!solve3<float>(A, b, dl)) return false;
-    const float mx = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
-    const float sc = mx > 0.25f ? 0.25f / mx : 1.f;
-    u = fminf(fmaxf(u + dl[0] * sc, -1.5f), 1.5f);
-    v = fminf(fmaxf(v + dl[1] * sc, -1.5f), 1.5f);
-    tau = fminf(fmaxf(tau + dl[2] * sc, tmin), tmax);
-  }
-  return false;
-}
-
-// Projection of X(seed) onto the ray (high_order.h:154-161, 221-225).
-template <int NG>
-__device__ float seed_tau(const float4* __restrict__ gc, const float* o, const float* d, int f,
-                          float u, float v)
-{
-  float r, s, t, X[3];
-  face_to_rst(f, u, v, r, s, t);
-  geom_point<float, NG>(gc, r, s, t, X);
-  const float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  return ((X[0] - o[0]) * d[0] + (X[1] - o[1]) * d[1] + (X[2] - o[2]) * d[2]) / dd;
-}
-
-// Flat-face seed: the ray meets the plane through the face's corner centroid
-// with normal (c2-c0)x(c3-c1); (u,v) from the parallelogram spanned by the
-// averaged edge vectors.  Returns false when the ray is parallel to it.
-__device__ bool flat_face_seed(const float* __restrict__ vx, const float* o, const float* d, int f,
-                               float& u, float& v, float& tau)
-{
-  const int C[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5}, {3, 2, 6, 7}, {0, 3, 7, 4}, {4, 5, 6, 7}};
-  float c[4][3];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int vi = C[f][q];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) c[q][k] = vx[vi * 3 + k];
-  }
-  float cen[3], eu[3], ev[3], a[3], b[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    cen[k] = 0.25f * (c[0][k] + c[1][k] + c[2][k] + c[3][k]);
-    eu[k] = 0.5f * ((c[1][k] - c[0][k]) + (c[2][k] - c[3][k]));
-    ev[k] = 0.5f * ((c[3][k] - c[0][k]) + (c[2][k] - c[1][k]));
-    a[k] = c[2][k] - c[0][k];
-    b[k] = c[3][k] - c[1][k];
-  }
-  const float n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
-  const float dn = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
-  const float nn = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  if (!(fabsf(dn) > 1e-6f * nn)) return false;
-  tau = ((cen[0] - o[0]) * n[0] + (cen[1] - o[1]) * n[1] + (cen[2] - o[2]) * n[2]) / dn;
-  float q[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) q[k] = o[k] + tau * d[k] - cen[k];
-  const float uu = eu[0] * eu[0] + eu[1] * eu[1] + eu[2] * eu[2];
-  const float uv = eu[0] * ev[0] + eu[1] * ev[1] + eu[2] * ev[2];
-  const float vv = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2];
-  const float qu = q[0] * eu[0] + q[1] * eu[1] + q[2] * eu[2];
-  const float qv = q[0] * ev[0] + q[1] * ev[1] + q[2] * ev[2];
-  const float det = uu * vv - uv * uv;
-  if (!(det > 0.f)) return false;
-  u = fminf(fmaxf(2.f * (qu * vv - qv * uv) / det, -1.5f), 1.5f);
-  v = fminf(fmaxf(2.f * (qv * uu - qu * uv) / det, -1.5f), 1.5f);
-  return true;
-}
-
-// Nearest accepted root of one face.  PLAIN: the reference's 9 seeds
-// {-0.8,0,0.8}^2, all run, nearest kept.  WARM/LEARNED: one flat-face seed,
-// the 9 seeds only when that seed does not converge.
-template <int NG>
-__device__ bool face_root(const float4* __restrict__ gc, const float* __restrict__ vx, const float* o,
-                          const float* d, int f, float tmin, float tmax, int guess, float& tau_o,
-                          float& u_o, float& v_o, long long& iters)
-{
-  if (guess != FEMRT_GUESS_PLAIN) {
-    float u, v, tau;
-    if (flat_face_seed(vx, o, d, f, u, v, tau)) {
-      int conv;
-      if (face_newton<NG>(gc, o, d, f, u, v, tau, tmin, tmax, tau_o, u_o, v_o, conv, iters)) return true;
-      if (conv) return false;
-    }
-  }
-  bool found = false;
-  float best = FLT_MAX;
-  for (int sj = 0; sj < 3; ++sj)
-    for (int si = 0; si < 3; ++si) {
-      const float su = -0.8f + 0.8f * si, sv = -0.8f + 0.8f * sj;
-      const float tau0 = seed_tau<NG>(gc, o, d, f, su, sv);
-      float tt, uu, vv;
-      int conv;
-      if (face_newton<NG>(gc, o, d, f, su, sv, tau0, tmin, tmax, tt, uu, vv, conv, iters)) {
-        if (tt < best) {
-          best = tt;
-          tau_o = tt;
-          u_o = uu;
-          v_o = vv;
-          found = true;
-        }
-      }
-    }
-  return found;
-}
-
-__device__ __forceinline__ void load_ray(const float* __restrict__ rays, int n, int r, float* o, float* d)
-{
-  o[0] = rays[r];
-  o[1] = rays[(size_t)n + r];
-  o[2] = rays[2 * (size_t)n + r];
-  d[0] = rays[3 * (size_t)n + r];
-  d[1] = rays[4 * (size_t)n + r];
-  d[2] = rays[5 * (size_t)n + r];
-}
-
 __device__ __forceinline__ void add_work(long long* work, int idx, long long v)
 {
   if (v) atomicAdd((unsigned long long*)&work[idx], (unsigned long long)v);
 }
 
+__device__ __forceinline__ void flush_work(long long* work, const Work& w)
+{
+  if (!work) return;
+  add_work(work, 2, w.face_it);
+  add_work(work, 3, w.inv_it);
+  add_work(work, 4, w.field);
+  add_work(work, 5, w.geo64);
+  add_work(work, 6, w.field64);
+  add_work(work, 7, w.slab);
+  add_work(work, 8, w.mlp);
+  add_work(work, 9, w.marched);
+}
+
 // Append `pred` lanes to a queue: one atomic per wave, ballot + prefix count
-// for the slot of each lane.
+// for the slot of each lane.  Must be reached by every lane of the wave.
 __device__ __forceinline__ int wave_append(bool pred, int* counter)
 {
   const unsigned long long mask = __ballot(pred);
@@ -381,121 +213,46 @@ __global__ void k_raygen(float* __restrict__ rays, int W, int H, float ox, float
   rays[5 * n + p] = d2 * il;
 }
 
-#define FEMRT_STACK 64
-
 template <int NG>
-__global__ __launch_bounds__(256) void k_first_hit(
-    const float* __restrict__ rays, int n, const float4* __restrict__ geom,
-    const float* __restrict__ boxes, const float* __restrict__ verts,
-    const float4* __restrict__ nodes, const int* __restrict__ prims, int guess, float tmin,
-    float tmax, int* __restrict__ hit_elem, int* __restrict__ hit_face,
-    float4* __restrict__ hit_f32, int* __restrict__ queue, int* __restrict__ counts,
-    long long* __restrict__ work)
+__global__ __launch_bounds__(256) void k_first_hit(const float* __restrict__ rays, int n, TraceMesh M,
+                                                   int guess, float tmin, float tmax,
+                                                   int* __restrict__ hit_elem, int* __restrict__ hit_face,
+                                                   float4* __restrict__ hit_f32, int* __restrict__ queue,
+                                                   int* __restrict__ counts, long long* __restrict__ work)
 {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = r < n;
-  int best_e = -1, best_f = -1;
-  float best_t = tmax, best_u = 0.f, best_v = 0.f;
-  long long it_face = 0, n_slab = 0;
+  Work w;
+  HitF32 h;
   if (active) {
-    float o[3], d[3], id[3];
+    float o[3], d[3];
     load_ray(rays, n, r, o, d);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
-    int stack[FEMRT_STACK];
-    float stack_t[FEMRT_STACK];
-    int sp = 0;
-    int node = 0;
-    float t0, t1;
-    const float4 ra = nodes[0], rb = nodes[1];
-    const float rlo[3] = {ra.x, ra.y, ra.z}, rhi[3] = {rb.x, rb.y, rb.z};
-    bool go = slab(o, id, rlo, rhi, tmin, best_t, t0, t1);
-    while (go) {
-      const float4 na = nodes[2 * node], nb = nodes[2 * node + 1];
-      const int a = __float_as_int(na.w), b = __float_as_int(nb.w);
-      if (b < 0) {
-        for (int i = 0; i < -b; ++i) {
-          const int e = prims[a + i];
-          const float* bx = boxes + (size_t)e * 42;
-          float e0, e1;
-          ++n_slab;
-          if (!slab(o, id, bx, bx + 3, tmin, best_t, e0, e1)) continue;
-          const float4* gc = geom + (size_t)e * (NG * NG * NG);
-          const float* vx = verts + (size_t)e * 24;
-          for (int f = 0; f < 6; ++f) {
-            const float* fb = bx + 6 * (1 + f);
-            float f0, f1;
-            ++n_slab;
-            if (!slab(o, id, fb, fb + 3, tmin, best_t, f0, f1)) continue;
-            float tt, uu, vv;
-            if (face_root<NG>(gc, vx, o, d, f, tmin, 1e30f, guess, tt, uu, vv, it_face) &&
-                tt < best_t) {
-              best_t = tt;
-              best_e = e;
-              best_f = f;
-              best_u = uu;
-              best_v = vv;
-            }
-          }
-        }
-      } else {
-        const float4 la = nodes[2 * a], lb = nodes[2 * a + 1];
-        const float4 qa = nodes[2 * b], qb = nodes[2 * b + 1];
-        const float llo[3] = {la.x, la.y, la.z}, lhi[3] = {lb.x, lb.y, lb.z};
-        const float qlo[3] = {qa.x, qa.y, qa.z}, qhi[3] = {qb.x, qb.y, qb.z};
-        float l0, l1, q0, q1;
-        n_slab += 2;
-        const bool hl = slab(o, id, llo, lhi, tmin, best_t, l0, l1);
-        const bool hq = slab(o, id, qlo, qhi, tmin, best_t, q0, q1);
-        if (hl && hq) {
-          const bool lfirst = l0 <= q0;
-          if (sp < FEMRT_STACK) {
-            stack[sp] = lfirst ? b : a;
-            stack_t[sp] = lfirst ? q0 : l0;
-            ++sp;
-          }
-          node = lfirst ? a : b;
-          continue;
-        }
-        if (hl) { node = a; continue; }
-        if (hq) { node = b; continue; }
-      }
-      // pop the next subtree still in front of the best hit
-      go = false;
-      while (sp > 0) {
-        --sp;
-        if (stack_t[sp] < best_t) {
-          node = stack[sp];
-          go = true;
-          break;
-        }
-      }
-    }
-    hit_elem[r] = best_e;
-    hit_face[r] = best_f;
-    hit_f32[r] = make_float4(best_u, best_v, best_t, 0.f);
+    h = first_hit_ray<NG>(M, o, d, guess, tmin, tmax, w);
+    hit_elem[r] = h.elem;
+    hit_face[r] = h.face;
+    hit_f32[r] = make_float4(h.u, h.v, h.t, 0.f);
   }
-  const int slot = wave_append(active && best_e >= 0, &counts[0]);
-  if (active && best_e >= 0) queue[slot] = r;
+  const bool hit = active && h.elem >= 0;
+  const int slot = wave_append(hit, &counts[0]);
+  if (hit) queue[slot] = r;
   if (work && active) {
     add_work(work, 0, 1);
-    add_work(work, 1, best_e >= 0);
-    add_work(work, 2, it_face);
-    add_work(work, 7, n_slab);
+    add_work(work, 1, hit ? 1 : 0);
+    flush_work(work, w);
   }
 }
 
 // FP64 face Newton polish of each queued hit; FP64 field value at the hit.
 template <int NG, int NF>
 __global__ __launch_bounds__(256) void k_polish_hit(
-    const float* __restrict__ rays, int n, const float4* __restrict__ geom,
-    const float* __restrict__ field, const int* __restrict__ queue, const int* __restrict__ counts,
-    const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
-    const float4* __restrict__ hit_f32, double* __restrict__ hit_t, double* __restrict__ hit_rst,
-    double* __restrict__ hit_u, long long* __restrict__ work)
+    const float* __restrict__ rays, int n, TraceMesh M, const int* __restrict__ queue,
+    const int* __restrict__ counts, const int* __restrict__ hit_elem,
+    const int* __restrict__ hit_face, const float4* __restrict__ hit_f32,
+    double* __restrict__ hit_t, double* __restrict__ hit_rst, double* __restrict__ hit_u,
+    long long* __restrict__ work)
 {
   const int count = counts[0];
-  long long g64 = 0;
+  Work w;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
     const int r = queue[q];
     float of[3], df[3];
@@ -503,251 +260,79 @@ __global__ __launch_bounds__(256) void k_polish_hit(
     const double o[3] = {of[0], of[1], of[2]}, d[3] = {df[0], df[1], df[2]};
     const int e = hit_elem[r], f = hit_face[r];
     const float4 h = hit_f32[r];
-    const float4* gc = geom + (size_t)e * (NG * NG * NG);
     double u = h.x, v = h.y, tau = h.z;
-    const bool us = (f == 2 || f == 4), vs = (f == 0 || f == 5);
-    for (int it = 0; it < 4; ++it) {
-      double rst[3], X[3], J[9];
-      face_to_rst_t<double>(f, u, v, rst);
-      geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
-      ++g64;
-      const double R[3] = {X[0] - (o[0] + tau * d[0]), X[1] - (o[1] + tau * d[1]),
-                           X[2] - (o[2] + tau * d[2])};
-      const double A[9] = {us ? J[1] : J[0], vs ? J[1] : J[2], -d[0],
-                           us ? J[4] : J[3], vs ? J[4] : J[5], -d[1],
-                           us ? J[7] : J[6], vs ? J[7] : J[8], -d[2]};
-      const double b[3] = {-R[0], -R[1], -R[2]};
-      double dl[3];
-      if (!solve3<double>(A, b, dl)) break;
-      u += dl[0];
-      v += dl[1];
-      tau += dl[2];
-      if (fabs(dl[0]) + fabs(dl[1]) < 1e-15 && fabs(dl[2]) < 1e-15 * (1.0 + fabs(tau))) break;
-    }
+    polish_hit<NG>(M.geom + (size_t)e * (NG * NG * NG), o, d, f, u, v, tau, w);
     double rst[3];
     face_to_rst_t<double>(f, u, v, rst);
     hit_t[r] = tau;
     hit_rst[r] = rst[0];
     hit_rst[(size_t)n + r] = rst[1];
     hit_rst[2 * (size_t)n + r] = rst[2];
-    hit_u[r] = field_value<double, NF>(field + (size_t)e * (NF * NF * NF), rst[0], rst[1], rst[2]);
+    hit_u[r] = field_value<double, NF>(M.field + (size_t)e * (NF * NF * NF), rst[0], rst[1], rst[2]);
+    ++w.field64;
   }
-  if (work) {
-    add_work(work, 5, g64);
-    add_work(work, 6, 0);
-  }
-}
-
-// Point inversion x -> rst in FP32 from a guess (Newton of HexElement.cu:
-// 227-272: undamped, |delta|_inf < 1e-5, <= 50 iterations).  On return J holds
-// the Jacobian at the final iterate.
-template <int NG>
-__device__ bool invert_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
-                           long long& iters)
-{
-  for (int it = 0; it < 50; ++it) {
-    float X[3];
-    geom_eval<float, NG>(gc, rst[0], rst[1], rst[2], X, J);
-    ++iters;
-    const float b[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
-    float dl[3];
-    if (!solve3<float>(J, b, dl)) return false;
-    rst[0] -= dl[0];
-    rst[1] -= dl[1];
-    rst[2] -= dl[2];
-    if (!(fabsf(rst[0]) < 1e4f && fabsf(rst[1]) < 1e4f && fabsf(rst[2]) < 1e4f)) return false;
-    if (fabsf(dl[0]) < 1e-5f && fabsf(dl[1]) < 1e-5f && fabsf(dl[2]) < 1e-5f) return true;
-  }
-  return false;
-}
-
-struct IsoParams {
-  float iso[FEMRT_MAX_ISO];
-  int n_iso;
-  int samples;
-  int guess;
-  int max_march;
-  float tmin;
-};
-
-// Exit of the ray from element e: nearest accepted face root with
-// t > t_in + eps on the faces other than the entry face.
-template <int NG>
-__device__ bool find_exit(const float4* __restrict__ gc, const float* __restrict__ vx,
-                          const float* __restrict__ bx, const float* o, const float* d,
-                          const float* id, int f_in, float t_in, float eps, int guess, int& f_out,
-                          float& t_out, float& u_out, float& v_out, long long& iters,
-                          long long& nslab)
-{
-  bool found = false;
-  float best = FLT_MAX;
-  for (int f = 0; f < 6; ++f) {
-    if (f == f_in) continue;
-    const float* fb = bx + 6 * (1 + f);
-    float f0, f1;
-    ++nslab;
-    if (!slab(o, id, fb, fb + 3, t_in, best, f0, f1)) continue;
-    float tt, uu, vv;
-    if (face_root<NG>(gc, vx, o, d, f, t_in + eps, 1e30f, guess, tt, uu, vv, iters) && tt < best) {
-      best = tt;
-      f_out = f;
-      t_out = tt;
-      u_out = uu;
-      v_out = vv;
-      found = true;
-    }
-  }
-  return found;
+  flush_work(work, w);
 }
 
 template <int NG, int NF>
 __global__ __launch_bounds__(256) void k_march(
-    const float* __restrict__ rays, int n, const float4* __restrict__ geom,
-    const float* __restrict__ field, const float* __restrict__ boxes,
-    const float* __restrict__ verts, const int* __restrict__ adj, IsoParams P,
-    const int* __restrict__ queue, int* __restrict__ counts, const int* __restrict__ hit_elem,
-    const int* __restrict__ hit_face, const float4* __restrict__ hit_f32,
-    int* __restrict__ iso_elem, float4* __restrict__ iso_f32, float4* __restrict__ iso_rst32,
-    int* __restrict__ iso_queue, long long* __restrict__ work)
+    const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ queue,
+    int* __restrict__ counts, const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
+    const float4* __restrict__ hit_f32, int* __restrict__ iso_elem, float4* __restrict__ iso_f32,
+    float4* __restrict__ iso_rst32, int* __restrict__ iso_queue, long long* __restrict__ work)
 {
   const int count = counts[0];
-  long long c_face = 0, c_inv = 0, c_field = 0, c_slab = 0, c_marched = 0;
+  Work w;
   const int stride = gridDim.x * blockDim.x;
-  // All lanes loop the same number of times so the wave-level queue append
-  // stays uniform.
+  // every lane loops the same number of rounds so the queue append below is
+  // reached by whole waves
   const int rounds = (count + stride - 1) / stride;
   for (int rd = 0; rd < rounds; ++rd) {
     const int q = rd * stride + blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = q < count;
-    unsigned found_mask = 0;
+    unsigned found = 0;
+    int r = 0;
     if (active) {
-      const int r = queue[q];
-      float o[3], d[3], id[3];
+      r = queue[q];
+      float o[3], d[3];
       load_ray(rays, n, r, o, d);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
-      int e = hit_elem[r];
-      int f_in = hit_face[r];
-      const float4 h = hit_f32[r];
-      float t_in = h.z;
-      const unsigned all = (1u << P.n_iso) - 1u;
-      for (int step = 0; step < P.max_march && found_mask != all; ++step) {
-        ++c_marched;
-        const float4* gc = geom + (size_t)e * (NG * NG * NG);
-        const float* fc = field + (size_t)e * (NF * NF * NF);
-        const float* bx = boxes + (size_t)e * 42;
-        const float* vx = verts + (size_t)e * 24;
-        const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
-        int f_out = -1;
-        float t_out, u_out, v_out;
-        if (!find_exit<NG>(gc, vx, bx, o, d, id, f_in, t_in, 1e-4f * ext, P.guess, f_out, t_out,
-                           u_out, v_out, c_face, c_slab))
-          break;
-        // uniform samples over [t_in, t_out]
-        float gprev[FEMRT_MAX_ISO];
-        float rst[3] = {0.f, 0.f, 0.f}, rprev[3] = {0.f, 0.f, 0.f};
-        float tprev = t_in;
-        bool have_prev = false;
-        const int M = P.samples;
-        for (int j = 0; j <= M; ++j) {
-          const float t = t_in + (t_out - t_in) * ((float)j / (float)M);
-          const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
-          float J[9];
-          if (P.guess == FEMRT_GUESS_PLAIN || !have_prev) {
-            rst[0] = rst[1] = rst[2] = 0.f;
-          }
-          if (!invert_f32<NG>(gc, x, rst, J, c_inv)) {
-            have_prev = false;
-            continue;
-          }
-          const float u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
-          ++c_field;
-#pragma unroll
-          for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
-            if (k >= P.n_iso || (found_mask >> k) & 1u) continue;
-            const float g = u - P.iso[k];
-            if (have_prev && gprev[k] * g <= 0.f) {
-              const size_t slot = (size_t)k * n + r;
-              iso_elem[slot] = e;
-              iso_f32[slot] = make_float4(tprev, t, 0.5f * (tprev + t), gprev[k]);
-              iso_rst32[slot] = make_float4(0.5f * (rprev[0] + rst[0]), 0.5f * (rprev[1] + rst[1]),
-                                            0.5f * (rprev[2] + rst[2]), g);
-              found_mask |= 1u << k;
-            }
-            gprev[k] = g;
-          }
-          rprev[0] = rst[0];
-          rprev[1] = rst[1];
-          rprev[2] = rst[2];
-          tprev = t;
-          have_prev = true;
-        }
-        if (found_mask == all) break;
-        if (!adj) break;
-        const int ne = adj[(size_t)e * 12 + f_out * 2];
-        const int nfc = adj[(size_t)e * 12 + f_out * 2 + 1];
-        if (ne < 0) break;
-        e = ne;
-        f_in = nfc;
-        t_in = t_out;
+      HitF32 h;
+      h.elem = hit_elem[r];
+      h.face = hit_face[r];
+      const float4 hf = hit_f32[r];
+      h.u = hf.x;
+      h.v = hf.y;
+      h.t = hf.z;
+      IsoBracket br[FEMRT_MAX_ISO];
+      found = march_ray<NG, NF>(M, P, o, d, h, br, w);
+      for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
+        if (!((found >> k) & 1u)) continue;
+        const size_t slot = (size_t)k * n + r;
+        iso_elem[slot] = br[k].elem;
+        iso_f32[slot] = make_float4(br[k].ta, br[k].tb, br[k].tm, br[k].ga);
+        iso_rst32[slot] = make_float4(br[k].rst[0], br[k].rst[1], br[k].rst[2], 0.f);
       }
     }
-    // queue the (ray, iso) brackets for the FP64 endgame
     for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
       if (k >= P.n_iso) break;
-      const bool pred = active && ((found_mask >> k) & 1u);
+      const bool pred = active && ((found >> k) & 1u);
       const int slot = wave_append(pred, &counts[1]);
-      if (pred) iso_queue[slot] = queue[q] * FEMRT_MAX_ISO + k;
+      if (pred) iso_queue[slot] = r * FEMRT_MAX_ISO + k;
     }
   }
-  if (work) {
-    add_work(work, 2, c_face);
-    add_work(work, 3, c_inv);
-    add_work(work, 4, c_field);
-    add_work(work, 7, c_slab);
-    add_work(work, 9, c_marched);
-  }
-}
-
-// FP64 endgame of one isovalue bracket: point inversion in FP64 (warm from
-// the FP32 estimate) and safeguarded Newton on g(t) = u - iso with
-// g'(t) = grad_world(u) . d, kept inside the sign-change bracket; bisection
-// when the Newton step leaves it.  Converged when the step is below
-// 1e-13 (1 + |t|).
-template <int NG>
-__device__ bool invert_f64(const float4* __restrict__ gc, const double* x, double* rst, double* J,
-                           long long& iters)
-{
-  for (int it = 0; it < 30; ++it) {
-    double X[3];
-    geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
-    ++iters;
-    const double b[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
-    double dl[3];
-    if (!solve3<double>(J, b, dl)) return false;
-    rst[0] -= dl[0];
-    rst[1] -= dl[1];
-    rst[2] -= dl[2];
-    if (fabs(dl[0]) < 1e-15 && fabs(dl[1]) < 1e-15 && fabs(dl[2]) < 1e-15) {
-      geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
-      ++iters;
-      return true;
-    }
-  }
-  return false;
+  flush_work(work, w);
 }
 
 template <int NG, int NF>
 __global__ __launch_bounds__(256) void k_polish_iso(
-    const float* __restrict__ rays, int n, const float4* __restrict__ geom,
-    const float* __restrict__ field, IsoParams P, const int* __restrict__ iso_queue,
-    const int* __restrict__ counts, const int* __restrict__ iso_elem_in,
+    const float* __restrict__ rays, int n, TraceMesh M, IsoParams P,
+    const int* __restrict__ iso_queue, const int* __restrict__ counts,
     const float4* __restrict__ iso_f32, const float4* __restrict__ iso_rst32,
     int* __restrict__ iso_elem, double* __restrict__ iso_t, double* __restrict__ iso_rst,
     double* __restrict__ iso_u, long long* __restrict__ work)
 {
   const int count = counts[1];
-  long long g64 = 0, f64 = 0;
+  Work w;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
     const int item = iso_queue[q];
     const int r = item / FEMRT_MAX_ISO, k = item % FEMRT_MAX_ISO;
@@ -755,48 +340,20 @@ __global__ __launch_bounds__(256) void k_polish_iso(
     float of[3], df[3];
     load_ray(rays, n, r, of, df);
     const double o[3] = {of[0], of[1], of[2]}, d[3] = {df[0], df[1], df[2]};
-    const int e = iso_elem_in[slot];
-    const float4 br = iso_f32[slot];
-    const float4 rg = iso_rst32[slot];
-    const float4* gc = geom + (size_t)e * (NG * NG * NG);
-    const float* fc = field + (size_t)e * (NF * NF * NF);
-    const double iso = P.iso[k];
-    double lo = br.x, hi = br.y, t = br.z;
-    const double glo_sign = br.w;  // g at the bracket's left end (FP32 sample)
-    double rst[3] = {rg.x, rg.y, rg.z}, J[9];
-    double u = 0.0, best_t = t;
-    bool ok = false;
-    for (int it = 0; it < 60; ++it) {
-      const double x[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
-      if (!invert_f64<NG>(gc, x, rst, J, g64)) break;
-      double gref[3], gw[3];
-      field_eval<double, NF>(fc, rst[0], rst[1], rst[2], &u, gref);
-      ++f64;
-      grad_world<double>(J, gref, gw);
-      const double g = u - iso;
-      const double gp = gw[0] * d[0] + gw[1] * d[1] + gw[2] * d[2];
-      best_t = t;
-      ok = true;
-      if (g == 0.0) break;
-      if ((g < 0.0) == (glo_sign < 0.0)) lo = t; else hi = t;
-      double tn = (gp != 0.0) ? t - g / gp : 0.5 * (lo + hi);
-      if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
-      const double step = fabs(tn - t);
-      t = tn;
-      if (step < 1e-13 * (1.0 + fabs(t)) || (hi - lo) < 1e-13 * (1.0 + fabs(t))) {
-        // final evaluation at the converged t
-        const double x2[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
-        if (invert_f64<NG>(gc, x2, rst, J, g64)) {
-          u = field_value<double, NF>(fc, rst[0], rst[1], rst[2]);
-          ++f64;
-          best_t = t;
-        }
-        break;
-      }
-    }
-    if (ok) {
-      iso_elem[slot] = e;
-      iso_t[slot] = best_t;
+    IsoBracket br;
+    br.elem = iso_elem[slot];
+    const float4 b4 = iso_f32[slot];
+    const float4 r4 = iso_rst32[slot];
+    br.ta = b4.x;
+    br.tb = b4.y;
+    br.tm = b4.z;
+    br.ga = b4.w;
+    br.rst[0] = r4.x;
+    br.rst[1] = r4.y;
+    br.rst[2] = r4.z;
+    double t, rst[3], u;
+    if (polish_iso<NG, NF>(M, o, d, (double)P.iso[k], br, t, rst, u, w)) {
+      iso_t[slot] = t;
       iso_rst[(size_t)k * 3 * n + r] = rst[0];
       iso_rst[(size_t)k * 3 * n + n + r] = rst[1];
       iso_rst[(size_t)k * 3 * n + 2 * n + r] = rst[2];
@@ -805,10 +362,7 @@ __global__ __launch_bounds__(256) void k_polish_iso(
       iso_elem[slot] = -1;
     }
   }
-  if (work) {
-    add_work(work, 5, g64);
-    add_work(work, 6, f64);
-  }
+  flush_work(work, w);
 }
 
 __global__ void k_clear_frame(int n, int n_iso, int* hit_elem, int* iso_elem, int* counts)
@@ -887,11 +441,10 @@ __global__ void k_invert(int n, const int* __restrict__ elem, const float* __res
   const float4* gc = geom + (size_t)e * (NG * NG * NG);
   const float xf[3] = {x[p], x[n + p], x[2 * n + p]};
   float rst[3] = {0.f, 0.f, 0.f}, J[9];
-  long long it = 0;
+  long long it = 0, it64 = 0;
   const bool ok = invert_f32<NG>(gc, xf, rst, J, it);
   double rd[3] = {rst[0], rst[1], rst[2]}, Jd[9];
   const double xd[3] = {xf[0], xf[1], xf[2]};
-  long long it64 = 0;
   const bool ok64 = ok && invert_f64<NG>(gc, xd, rd, Jd, it64);
   rst_out[p] = rd[0];
   rst_out[n + p] = rd[1];
@@ -1053,7 +606,7 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   std::vector<float> g4((size_t)n_elem * N3 * 4, 0.f), fp((size_t)n_elem * F3, 0.f);
   pad_coeffs(geom, ng, NG, 3, n_elem, g4, 4);
   pad_coeffs(field, nf, NF, 1, n_elem, fp, 1);
-  m->h_boxes.resize((size_t)n_elem * 42);
+  m->h_boxes.resize((size_t)n_elem * FEMRT_BOX_STRIDE);
   bernstein_boxes(n_elem, ng, geom, m->h_boxes.data());
   auto cleanup = [&](const std::string& msg) {
     hipFree(m->d_geom); hipFree(m->d_field); hipFree(m->d_verts); hipFree(m->d_boxes); hipFree(m->d_adj);
@@ -1379,35 +932,40 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   ip.guess = p->guess;
   ip.max_march = p->max_march;
   ip.tmin = p->t_min;
+  TraceMesh TM;
+  TM.geom = m->d_geom;
+  TM.field = m->d_field;
+  TM.boxes = m->d_boxes;
+  TM.verts = m->d_verts;
+  TM.adj = m->d_adj;
+  TM.nodes = m->d_nodes;
+  TM.prims = m->d_prims;
   rc = FEMRT_EUNSUP;
   FEMRT_DISPATCH_NG(m->NG, {
     rc = timed_launch(c, "k_first_hit", [&] {
       k_first_hit<NG_><<<blocks, 256, 0, c->stream>>>(
-          f->d_rays, n, m->d_geom, m->d_boxes, m->d_verts, m->d_nodes, m->d_prims, p->guess,
-          p->t_min, p->t_max, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_queue_hit,
-          f->d_counts, work);
+          f->d_rays, n, TM, p->guess, p->t_min, p->t_max, f->d_hit_elem, f->d_hit_face,
+          f->d_hit_f32, f->d_queue_hit, f->d_counts, work);
     });
     if (rc) break;
     FEMRT_DISPATCH_NF(m->NF, {
       rc = timed_launch(c, "k_polish_hit", [&] {
         k_polish_hit<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
-            f->d_rays, n, m->d_geom, m->d_field, f->d_queue_hit, f->d_counts, f->d_hit_elem,
-            f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst, f->d_hit_u, work);
+            f->d_rays, n, TM, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
+            f->d_hit_f32, f->d_hit_t, f->d_hit_rst, f->d_hit_u, work);
       });
       if (rc) break;
       if (n_iso > 0) {
         rc = timed_launch(c, "k_march", [&] {
           k_march<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
-              f->d_rays, n, m->d_geom, m->d_field, m->d_boxes, m->d_verts, m->d_adj, ip,
-              f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face, f->d_hit_f32,
-              f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work);
+              f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
+              f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work);
         });
         if (rc) break;
         rc = timed_launch(c, "k_polish_iso", [&] {
           k_polish_iso<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
-              f->d_rays, n, m->d_geom, m->d_field, ip, f->d_queue_iso, f->d_counts,
-              f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_iso_elem, f->d_iso_t,
-              f->d_iso_rst, f->d_iso_u, work);
+              f->d_rays, n, TM, ip, f->d_queue_iso, f->d_counts, f->d_iso_f32, f->d_iso_rst32,
+              f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, work);
         });
         if (rc) break;
       }

@@ -1,0 +1,797 @@
+// femrt_trace_core.hpp -- per-ray bodies of the trace kernels.
+//
+// Everything here is __host__ __device__ and free of wave-level intrinsics, so
+// the kernels in femrt_kernels.hip are thin wrappers around these functions.
+// Reference: raytracer.cu:102-254 (traversal + hex branch), high_order.h:
+// 189-304, 409-465 (face Newton, nearest face), fem/HexElement.cu:227-272
+// (point inversion), fem/RayIsosurface.cu:67-188 (iso search).
+#pragma once
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include "../../include/femrt.h"
+#include "femrt_math.hpp"
+
+namespace femrt {
+
+// Device-resident mesh as the trace kernels see it.
+struct TraceMesh {
+  const float4* geom;   // n_elem x NG^3 (x, y, z, unused)
+  const float* field;   // n_elem x NF^3
+  const float* boxes;   // n_elem x FEMRT_BOX_STRIDE: element, face, face sub-patch boxes
+  const float* verts;   // n_elem x 24 corner vertices, XML order
+  const int* adj;       // n_elem x 12 (neighbour, neighbour face) or null
+  const float4* nodes;  // BVH nodes x 2
+  const int* prims;
+};
+
+struct IsoParams {
+  float iso[FEMRT_MAX_ISO];
+  int n_iso;
+  int samples;
+  int guess;
+  int max_march;
+  float tmin;
+};
+
+// Work counters accumulated per lane.
+struct Work {
+  long long face_it = 0, inv_it = 0, field = 0, geo64 = 0, field64 = 0, slab = 0, mlp = 0,
+            marched = 0;
+};
+
+// FP32 result of the first-hit stage.
+struct HitF32 {
+  int elem = -1, face = -1;
+  float u = 0.f, v = 0.f, t = 0.f;
+};
+
+// FP32 result of the iso stage for one isovalue: sample bracket [ta, tb],
+// FP32 estimate tm, g at ta, rst estimate.
+struct IsoBracket {
+  int elem;
+  float ta, tb, tm, ga;
+  float rst[3];
+};
+
+#define FEMRT_STACK 64
+#define FEMRT_FACE_ACC 1e-4f
+// Boxes per element: [0] element, [1..6] faces (Nektar order), [7 + 9 f +
+// 3 b + a] sub-patch (a, b) of face f on the 3 x 3 split of its (u, v) square.
+#ifndef FEMRT_NBOX
+#define FEMRT_NBOX 61
+#define FEMRT_BOX_STRIDE (FEMRT_NBOX * 6)
+#endif
+
+FEMRT_HD int as_int(float x)
+{
+  int i;
+  memcpy(&i, &x, 4);
+  return i;
+}
+
+FEMRT_HD void face_to_rst(int f, float u, float v, float& r, float& s, float& t)
+{
+  r = (f == 2) ? 1.f : ((f == 4) ? -1.f : u);
+  s = (f == 0 || f == 5) ? v : ((f == 1) ? -1.f : ((f == 3) ? 1.f : u));
+  t = (f == 0) ? -1.f : ((f == 5) ? 1.f : v);
+}
+
+template <typename T>
+FEMRT_HD void face_to_rst_t(int f, T u, T v, T* rst)
+{
+  rst[0] = (f == 2) ? T(1) : ((f == 4) ? T(-1) : u);
+  rst[1] = (f == 0 || f == 5) ? v : ((f == 1) ? T(-1) : ((f == 3) ? T(1) : u));
+  rst[2] = (f == 0) ? T(-1) : ((f == 5) ? T(1) : v);
+}
+
+FEMRT_HD void rst_to_face(int f, const float* rst, float& u, float& v)
+{
+  u = (f == 2 || f == 4) ? rst[1] : rst[0];
+  v = (f == 0 || f == 5) ? rst[1] : rst[2];
+}
+
+// Face-constrained Newton on (u, v, tau) from one seed, FP32, with the
+// reference's update rules (high_order.h:213-300): step capped to 0.25 in the
+// inf-norm, tau clamped to [tmin, tmax], u, v clamped to [-1.5, 1.5],
+// divergence break after 5 iterations when the residual grows 1.5x, at most
+// 50 iterations.  Returns 1 for an accepted root (max(|u|,|v|) <= 1 + 1e-4,
+// tmin <= tau <= tmax), 0 for a converged root outside the face, -1 when the
+// iteration did not converge.
+template <int NG>
+FEMRT_HD int face_newton(const float4* __restrict__ gc, const float* o, const float* d, int f,
+                         float u, float v, float tau, float tmin, float tmax, float& tau_o,
+                         float& u_o, float& v_o, long long& iters)
+{
+  float prev = FLT_MAX;
+  const float oscale = 1.f + fmaxf(fabsf(o[0]), fmaxf(fabsf(o[1]), fabsf(o[2])));
+  tau = fminf(fmaxf(tau, tmin), tmax);
+  const bool us = (f == 2 || f == 4);
+  const bool vs = (f == 0 || f == 5);
+  for (int it = 0; it < 50; ++it) {
+    float r, s, t;
+    face_to_rst(f, u, v, r, s, t);
+    float X[3], J[9];
+    geom_eval<float, NG>(gc, r, s, t, X, J);
+    ++iters;
+    const float R0 = X[0] - fmaf(tau, d[0], o[0]);
+    const float R1 = X[1] - fmaf(tau, d[1], o[1]);
+    const float R2 = X[2] - fmaf(tau, d[2], o[2]);
+    const float res = fmaxf(fabsf(R0), fmaxf(fabsf(R1), fabsf(R2)));
+    const float tol = 2e-6f * (oscale + fabsf(tau));
+    const float A[9] = {us ? J[1] : J[0], vs ? J[1] : J[2], -d[0],
+                        us ? J[4] : J[3], vs ? J[4] : J[5], -d[1],
+                        us ? J[7] : J[6], vs ? J[7] : J[8], -d[2]};
+    const float b[3] = {-R0, -R1, -R2};
+    float dl[3];
+    const bool solved = solve3<float>(A, b, dl);
+    if (res < tol) {
+      // converged: one more (free) Newton step with the Jacobian at hand takes
+      // the root to the FP32 floor
+      if (solved && fmaxf(fabsf(dl[0]), fabsf(dl[1])) < 0.01f) {
+        u += dl[0];
+        v += dl[1];
+        tau += dl[2];
+      }
+      tau_o = tau;
+      u_o = u;
+      v_o = v;
+      return (fmaxf(fabsf(u), fabsf(v)) <= 1.f + FEMRT_FACE_ACC && tau >= tmin && tau <= tmax) ? 1 : 0;
+    }
+    if (it > 5 && res > prev * 1.5f) return -1;
+    prev = res;
+    if (!solved) return -1;
+    const float mx = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
+    const float sc = mx > 0.25f ? 0.25f / mx : 1.f;
+    u = fminf(fmaxf(u + dl[0] * sc, -1.5f), 1.5f);
+    v = fminf(fmaxf(v + dl[1] * sc, -1.5f), 1.5f);
+    tau = fminf(fmaxf(tau + dl[2] * sc, tmin), tmax);
+  }
+  return -1;
+}
+
+// Projection of X(seed) onto the ray (high_order.h:154-161, 221-225).
+template <int NG>
+FEMRT_HD float seed_tau(const float4* __restrict__ gc, const float* o, const float* d, int f,
+                        float u, float v)
+{
+  float r, s, t, X[3];
+  face_to_rst(f, u, v, r, s, t);
+  geom_point<float, NG>(gc, r, s, t, X);
+  const float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  return ((X[0] - o[0]) * d[0] + (X[1] - o[1]) * d[1] + (X[2] - o[2]) * d[2]) / dd;
+}
+
+// The reference's seed grid {-0.8,0,0.8}^2, every seed run, nearest accepted
+// root kept.
+template <int NG>
+FEMRT_HD bool face_root_grid(const float4* __restrict__ gc, const float* o, const float* d, int f,
+                             float tmin, float tmax, float& tau_o, float& u_o, float& v_o,
+                             long long& iters)
+{
+  bool found = false;
+  float best = FLT_MAX;
+  for (int sj = 0; sj < 3; ++sj)
+    for (int si = 0; si < 3; ++si) {
+      const float su = -0.8f + 0.8f * si, sv = -0.8f + 0.8f * sj;
+      const float tau0 = seed_tau<NG>(gc, o, d, f, su, sv);
+      float tt, uu, vv;
+      if (face_newton<NG>(gc, o, d, f, su, sv, tau0, tmin, tmax, tt, uu, vv, iters) == 1 && tt < best) {
+        best = tt;
+        tau_o = tt;
+        u_o = uu;
+        v_o = vv;
+        found = true;
+      }
+    }
+  return found;
+}
+
+// Nearest accepted root of one face seeded from its Bernstein sub-patches:
+// the ray is tested against the 3 x 3 sub-patch boxes (each bounds its piece
+// of the face exactly), and a face Newton starts at the centre of every
+// sub-patch whose box the ray crosses before `tcull`.  A root of the face lies
+// in a crossed sub-patch, so this misses no root the seeds can converge to,
+// and a single flat seed's failure mode (converging to a crossing of the
+// face's polynomial extension outside the element while a crossing inside
+// exists) is avoided.
+template <int NG>
+FEMRT_HD bool face_root_patches(const float4* __restrict__ gc, const float* __restrict__ bx,
+                                const float* o, const float* d, const float* id, int f, float tmin,
+                                float tcull, float& tau_o, float& u_o, float& v_o, Work& w)
+{
+  const float* sb = bx + 6 * (7 + 9 * f);
+  bool found = false;
+  float best = tcull;
+  for (int q = 0; q < 9; ++q) {
+    float s0, s1;
+    ++w.slab;
+    if (!slab(o, id, sb + 6 * q, sb + 6 * q + 3, tmin, best, s0, s1)) continue;
+    const float uc = -1.f + (2.f * (q % 3) + 1.f) / 3.f;
+    const float vc = -1.f + (2.f * (q / 3) + 1.f) / 3.f;
+    float tt, uu, vv;
+    int rc = face_newton<NG>(gc, o, d, f, uc, vc, 0.5f * (s0 + s1), tmin, 1e30f, tt, uu, vv, w.face_it);
+    // converged to a crossing behind the range (the ray enters and leaves the
+    // face within this sub-patch): seed again at the far end of the box
+    if (rc == 0 && tt < tmin)
+      rc = face_newton<NG>(gc, o, d, f, uc, vc, s1, tmin, 1e30f, tt, uu, vv, w.face_it);
+    if (rc == 1 && tt < best) {
+      best = tt;
+      tau_o = tt;
+      u_o = uu;
+      v_o = vv;
+      found = true;
+    }
+  }
+  return found;
+}
+
+// Nearest accepted root of one face below `tcull`.  PLAIN: the reference's
+// seed grid, then the sub-patch seeds when the grid finds nothing.  WARM and
+// LEARNED: the sub-patch seeds.
+template <int NG>
+FEMRT_HD bool face_root(const float4* __restrict__ gc, const float* __restrict__ bx, const float* o,
+                        const float* d, const float* id, int f, float tmin, float tcull, int guess,
+                        float& tau_o, float& u_o, float& v_o, Work& w)
+{
+  if (guess == FEMRT_GUESS_PLAIN) {
+    float tt, uu, vv;
+    if (face_root_grid<NG>(gc, o, d, f, tmin, 1e30f, tt, uu, vv, w.face_it)) {
+      if (!(tt < tcull)) return false;
+      tau_o = tt;
+      u_o = uu;
+      v_o = vv;
+      return true;
+    }
+  }
+  return face_root_patches<NG>(gc, bx, o, d, id, f, tmin, tcull, tau_o, u_o, v_o, w);
+}
+
+FEMRT_HD void load_ray(const float* __restrict__ rays, int n, int r, float* o, float* d)
+{
+  o[0] = rays[r];
+  o[1] = rays[(size_t)n + r];
+  o[2] = rays[2 * (size_t)n + r];
+  d[0] = rays[3 * (size_t)n + r];
+  d[1] = rays[4 * (size_t)n + r];
+  d[2] = rays[5 * (size_t)n + r];
+}
+
+// ------------------------------------------------------------- first hit
+
+// BVH traversal over element boxes, near child first, short stack; per leaf
+// element: element box, then per face its box and the face root.  The
+// nearest accepted root over all elements (but `exclude`) and faces is the
+// hit.
+template <int NG>
+FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d, int guess,
+                              float tmin, float tmax, Work& w, int exclude = -1)
+{
+  HitF32 h;
+  float id[3];
+  for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
+  float best_t = tmax;
+  int stack[FEMRT_STACK];
+  float stack_t[FEMRT_STACK];
+  int sp = 0, node = 0;
+  float t0, t1;
+  {
+    const float4 ra = M.nodes[0], rb = M.nodes[1];
+    const float rlo[3] = {ra.x, ra.y, ra.z}, rhi[3] = {rb.x, rb.y, rb.z};
+    ++w.slab;
+    if (!slab(o, id, rlo, rhi, tmin, best_t, t0, t1)) return h;
+  }
+  bool go = true;
+  while (go) {
+    const float4 na = M.nodes[2 * node], nb = M.nodes[2 * node + 1];
+    const int a = as_int(na.w), b = as_int(nb.w);
+    if (b < 0) {
+      for (int i = 0; i < -b; ++i) {
+        const int e = M.prims[a + i];
+        if (e == exclude) continue;
+        const float* bx = M.boxes + (size_t)e * FEMRT_BOX_STRIDE;
+        float e0, e1;
+        ++w.slab;
+        if (!slab(o, id, bx, bx + 3, tmin, best_t, e0, e1)) continue;
+        const float4* gc = M.geom + (size_t)e * (NG * NG * NG);
+        for (int f = 0; f < 6; ++f) {
+          const float* fb = bx + 6 * (1 + f);
+          float f0, f1;
+          ++w.slab;
+          if (!slab(o, id, fb, fb + 3, tmin, best_t, f0, f1)) continue;
+          float tt, uu, vv;
+          if (face_root<NG>(gc, bx, o, d, id, f, tmin, best_t, guess, tt, uu, vv, w) && tt < best_t) {
+            best_t = tt;
+            h.elem = e;
+            h.face = f;
+            h.u = uu;
+            h.v = vv;
+            h.t = tt;
+          }
+        }
+      }
+    } else {
+      const float4 la = M.nodes[2 * a], lb = M.nodes[2 * a + 1];
+      const float4 qa = M.nodes[2 * b], qb = M.nodes[2 * b + 1];
+      const float llo[3] = {la.x, la.y, la.z}, lhi[3] = {lb.x, lb.y, lb.z};
+      const float qlo[3] = {qa.x, qa.y, qa.z}, qhi[3] = {qb.x, qb.y, qb.z};
+      float l0, l1, q0, q1;
+      w.slab += 2;
+      const bool hl = slab(o, id, llo, lhi, tmin, best_t, l0, l1);
+      const bool hq = slab(o, id, qlo, qhi, tmin, best_t, q0, q1);
+      if (hl && hq) {
+        const bool lfirst = l0 <= q0;
+        if (sp < FEMRT_STACK) {
+          stack[sp] = lfirst ? b : a;
+          stack_t[sp] = lfirst ? q0 : l0;
+          ++sp;
+        }
+        node = lfirst ? a : b;
+        continue;
+      }
+      if (hl) { node = a; continue; }
+      if (hq) { node = b; continue; }
+    }
+    go = false;
+    while (sp > 0) {
+      --sp;
+      if (stack_t[sp] < best_t) {
+        node = stack[sp];
+        go = true;
+        break;
+      }
+    }
+  }
+  return h;
+}
+
+// FP64 face Newton polish of a hit (unconstrained, quadratic convergence from
+// the FP32 root).
+template <int NG>
+FEMRT_HD void polish_hit(const float4* __restrict__ gc, const double* o, const double* d, int f,
+                         double& u, double& v, double& tau, Work& w)
+{
+  const bool us = (f == 2 || f == 4), vs = (f == 0 || f == 5);
+  for (int it = 0; it < 4; ++it) {
+    double rst[3], X[3], J[9];
+    face_to_rst_t<double>(f, u, v, rst);
+    geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
+    ++w.geo64;
+    const double R[3] = {X[0] - (o[0] + tau * d[0]), X[1] - (o[1] + tau * d[1]),
+                         X[2] - (o[2] + tau * d[2])};
+    const double A[9] = {us ? J[1] : J[0], vs ? J[1] : J[2], -d[0],
+                         us ? J[4] : J[3], vs ? J[4] : J[5], -d[1],
+                         us ? J[7] : J[6], vs ? J[7] : J[8], -d[2]};
+    const double b[3] = {-R[0], -R[1], -R[2]};
+    double dl[3];
+    if (!solve3<double>(A, b, dl)) break;
+    u += dl[0];
+    v += dl[1];
+    tau += dl[2];
+    if (fabs(dl[0]) + fabs(dl[1]) < 1e-15 && fabs(dl[2]) < 1e-15 * (1.0 + fabs(tau))) break;
+  }
+}
+
+// ------------------------------------------------------ point inversion
+
+// Newton x -> rst in FP32 from the guess in rst (HexElement.cu:227-272:
+// undamped, |delta|_inf < 1e-5, <= 50 iterations).  J holds the Jacobian at
+// the last evaluated iterate.
+template <int NG>
+FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
+                         long long& iters)
+{
+  for (int it = 0; it < 50; ++it) {
+    float X[3];
+    geom_eval<float, NG>(gc, rst[0], rst[1], rst[2], X, J);
+    ++iters;
+    const float b[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
+    float dl[3];
+    if (!solve3<float>(J, b, dl)) return false;
+    rst[0] -= dl[0];
+    rst[1] -= dl[1];
+    rst[2] -= dl[2];
+    if (!(fabsf(rst[0]) < 1e4f && fabsf(rst[1]) < 1e4f && fabsf(rst[2]) < 1e4f)) return false;
+    if (fabsf(dl[0]) < 1e-5f && fabsf(dl[1]) < 1e-5f && fabsf(dl[2]) < 1e-5f) return true;
+  }
+  return false;
+}
+
+// The preimage of x inside the element (unique for a valid element): Newton
+// from the guess in rst; when that converges outside the cube (Newton from
+// rst = 0 finds spurious preimages of the polynomial map for points near a
+// face) or fails, restart from the 8 octant centres (+-0.5)^3.
+template <int NG>
+FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
+                                 long long& iters)
+{
+  if (invert_f32<NG>(gc, x, rst, J, iters) &&
+      fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
+    return true;
+  for (int s = 0; s < 8; ++s) {
+    rst[0] = (s & 1) ? 0.5f : -0.5f;
+    rst[1] = (s & 2) ? 0.5f : -0.5f;
+    rst[2] = (s & 4) ? 0.5f : -0.5f;
+    if (invert_f32<NG>(gc, x, rst, J, iters) &&
+        fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
+      return true;
+  }
+  return false;
+}
+
+// FP64 Newton x -> rst from a guess.  Converged when the step drops below
+// 1e-13, or once it is below 1e-9 and stops halving (the FP64 rounding floor
+// of x(rst) - x divided by |J|).  On success J is evaluated at the result.
+template <int NG>
+FEMRT_HD bool invert_f64(const float4* __restrict__ gc, const double* x, double* rst, double* J,
+                         long long& iters)
+{
+  double prev = 1e300;
+  for (int it = 0; it < 30; ++it) {
+    double X[3];
+    geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
+    ++iters;
+    const double b[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
+    double dl[3];
+    if (!solve3<double>(J, b, dl)) return false;
+    rst[0] -= dl[0];
+    rst[1] -= dl[1];
+    rst[2] -= dl[2];
+    const double st = fmax(fabs(dl[0]), fmax(fabs(dl[1]), fabs(dl[2])));
+    if (st < 1e-13 || (st < 1e-9 && st > 0.5 * prev)) {
+      geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
+      ++iters;
+      return true;
+    }
+    prev = st;
+  }
+  return false;
+}
+
+// ------------------------------------------------------------ element walk
+
+// Exit of the ray from element e: nearest accepted face root with
+// t > t_lo on the faces other than the entry face (and on the entry face too
+// when include_entry is set).
+template <int NG>
+FEMRT_HD bool find_exit_search(const float4* __restrict__ gc, const float* __restrict__ bx,
+                               const float* o, const float* d, const float* id, int f_in,
+                               float t_lo, int guess, bool include_entry, int& f_out, float& t_out,
+                               float& u_out, float& v_out, Work& w)
+{
+  bool found = false;
+  float best = FLT_MAX;
+  for (int f = 0; f < 6; ++f) {
+    if (f == f_in && !include_entry) continue;
+    const float* fb = bx + 6 * (1 + f);
+    float f0, f1;
+    ++w.slab;
+    if (!slab(o, id, fb, fb + 3, t_lo, best, f0, f1)) continue;
+    float tt, uu, vv;
+    if (face_root<NG>(gc, bx, o, d, id, f, t_lo, best, guess, tt, uu, vv, w) && tt < best) {
+      best = tt;
+      f_out = f;
+      t_out = tt;
+      u_out = uu;
+      v_out = vv;
+      found = true;
+    }
+  }
+  return found;
+}
+
+// Exit found by following the ray with point inversion: steps of 1/32 of the
+// element box extent until a point has no preimage in the element, bisection
+// of that inside/outside predicate, then a face Newton on the face whose
+// coordinate was crossed, seeded at the bisected point.  Used when the face
+// searches find no exit (two crossings of one face inside one sub-patch).
+template <int NG>
+FEMRT_HD bool find_exit_marching(const float4* __restrict__ gc, const float* __restrict__ bx,
+                                 const float* o, const float* d, const float* id,
+                                 const float* rst_in, float t_in, float t_lo, int& f_out,
+                                 float& t_out, float& u_out, float& v_out, Work& w)
+{
+  float b0, b1;
+  ++w.slab;
+  if (!slab(o, id, bx, bx + 3, t_in, 1e30f, b0, b1)) return false;
+  const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
+  const float dt = ext / 32.f;
+  float ta = t_in, ra[3] = {rst_in[0], rst_in[1], rst_in[2]};
+  float tb = -1.f;
+  float J[9];
+  for (float t = t_lo; t <= b1 + dt; t += dt) {
+    const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
+    float r[3] = {ra[0], ra[1], ra[2]};
+    if (invert_in_cube_f32<NG>(gc, x, r, J, w.inv_it)) {
+      ta = t;
+      ra[0] = r[0];
+      ra[1] = r[1];
+      ra[2] = r[2];
+    } else {
+      tb = t;
+      break;
+    }
+  }
+  if (tb < 0.f) return false;
+  for (int it = 0; it < 14; ++it) {
+    const float tm = 0.5f * (ta + tb);
+    const float x[3] = {fmaf(tm, d[0], o[0]), fmaf(tm, d[1], o[1]), fmaf(tm, d[2], o[2])};
+    float r[3] = {ra[0], ra[1], ra[2]};
+    if (invert_in_cube_f32<NG>(gc, x, r, J, w.inv_it)) {
+      ta = tm;
+      ra[0] = r[0];
+      ra[1] = r[1];
+      ra[2] = r[2];
+    } else {
+      tb = tm;
+    }
+  }
+  // the crossed face: the coordinate of the last inside point nearest +-1
+  int c = 0;
+  for (int k = 1; k < 3; ++k)
+    if (fabsf(ra[k]) > fabsf(ra[c])) c = k;
+  const int f = c == 0 ? (ra[0] > 0.f ? 2 : 4) : (c == 1 ? (ra[1] > 0.f ? 3 : 1) : (ra[2] > 0.f ? 5 : 0));
+  float su, sv;
+  rst_to_face(f, ra, su, sv);
+  float tt, uu, vv;
+  if (face_newton<NG>(gc, o, d, f, su, sv, ta, t_lo, 1e30f, tt, uu, vv, w.face_it) == 1 &&
+      fabsf(tt - ta) <= 4.f * dt) {
+    f_out = f;
+    t_out = tt;
+    u_out = uu;
+    v_out = vv;
+    return true;
+  }
+  f_out = f;
+  t_out = ta;
+  u_out = fminf(fmaxf(su, -1.f), 1.f);
+  v_out = fminf(fmaxf(sv, -1.f), 1.f);
+  return true;
+}
+
+// Exit predicted from the entry point: the reference-space line
+// rst_in + s J^{-1}(rst_in) d leaves the cube [-1,1]^3 through face f* at
+// s*; a face Newton on f* seeded there gives the exit.  Returns false when
+// the prediction does not converge to an accepted root.
+template <int NG>
+FEMRT_HD bool find_exit_predicted(const float4* __restrict__ gc, const float* o, const float* d,
+                                  int f_in, const float* rst_in, float t_in, float t_lo, int& f_out,
+                                  float& t_out, float& u_out, float& v_out, Work& w)
+{
+  float X[3], J[9], dl[3];
+  geom_eval<float, NG>(gc, rst_in[0], rst_in[1], rst_in[2], X, J);
+  ++w.face_it;
+  if (!solve3<float>(J, d, dl)) return false;
+  // first crossing of the cube boundary by rst_in + s dl, s > 0, skipping the
+  // entry face
+  float s_best = FLT_MAX;
+  int f_best = -1;
+  for (int c = 0; c < 3; ++c) {
+    if (dl[c] == 0.f) continue;
+    const float target = dl[c] > 0.f ? 1.f : -1.f;
+    const float s = (target - rst_in[c]) / dl[c];
+    const int f = c == 0 ? (target > 0 ? 2 : 4) : (c == 1 ? (target > 0 ? 3 : 1) : (target > 0 ? 5 : 0));
+    if (f == f_in || !(s > 0.f)) continue;
+    if (s < s_best) {
+      s_best = s;
+      f_best = f;
+    }
+  }
+  if (f_best < 0) return false;
+  float pr[3];
+  for (int c = 0; c < 3; ++c) pr[c] = rst_in[c] + s_best * dl[c];
+  float su, sv;
+  rst_to_face(f_best, pr, su, sv);
+  su = fminf(fmaxf(su, -1.5f), 1.5f);
+  sv = fminf(fmaxf(sv, -1.5f), 1.5f);
+  float tt, uu, vv;
+  if (face_newton<NG>(gc, o, d, f_best, su, sv, t_in + s_best, t_lo, 1e30f, tt, uu, vv, w.face_it) != 1)
+    return false;
+  f_out = f_best;
+  t_out = tt;
+  u_out = uu;
+  v_out = vv;
+  return true;
+}
+
+// Walk the hit ray through face-adjacent elements, sampling g = u - iso at
+// P.samples uniform sub-intervals of each element segment [t_in, t_out].
+// The first sub-interval with g(prev) g(cur) <= 0 of each isovalue is
+// recorded.  Returns the mask of isovalues found.
+template <int NG, int NF>
+FEMRT_HD unsigned march_ray(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
+                            HitF32 h, IsoBracket* out, Work& w)
+{
+  float id[3];
+  for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
+  int e = h.elem, f_in = h.face;
+  float t_in = h.t, u_in = h.u, v_in = h.v;
+  unsigned found = 0;
+  const unsigned all = (1u << P.n_iso) - 1u;
+  for (int step = 0; step < P.max_march && found != all; ++step) {
+    ++w.marched;
+    const float4* gc = M.geom + (size_t)e * (NG * NG * NG);
+    const float* fc = M.field + (size_t)e * (NF * NF * NF);
+    const float* bx = M.boxes + (size_t)e * FEMRT_BOX_STRIDE;
+    const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
+    // exits closer than eps to the entry are not resolved in FP32
+    const float eps = 1e-4f * ext;
+    const float t_lo = t_in + eps;
+    float rst_in[3];
+    face_to_rst(f_in, u_in, v_in, rst_in[0], rst_in[1], rst_in[2]);
+    int f_out = -1;
+    float t_out = 0.f, u_out = 0.f, v_out = 0.f;
+    bool ok = false;
+    if (P.guess != FEMRT_GUESS_PLAIN)
+      ok = find_exit_predicted<NG>(gc, o, d, f_in, rst_in, t_in, t_lo, f_out, t_out, u_out, v_out, w);
+    if (!ok)
+      ok = find_exit_search<NG>(gc, bx, o, d, id, f_in, t_lo, P.guess, false, f_out, t_out, u_out, v_out, w);
+    // A ray can leave through the face it entered by (a curved boundary face
+    // seen at a grazing angle); the entry face is searched when no other face
+    // holds an exit, or when a sample shows the segment leaves the element.
+    if (!ok)
+      ok = find_exit_search<NG>(gc, bx, o, d, id, f_in, t_lo, P.guess, true, f_out, t_out, u_out, v_out, w);
+    if (!ok)
+      ok = find_exit_marching<NG>(gc, bx, o, d, id, rst_in, t_in, t_lo, f_out, t_out, u_out, v_out, w);
+    if (ok) {
+      const int Ms = P.samples;
+      for (int pass = 0; pass < 2; ++pass) {
+        bool redo = false;
+        float gprev[FEMRT_MAX_ISO];
+        float rst[3] = {rst_in[0], rst_in[1], rst_in[2]}, rprev[3] = {0.f, 0.f, 0.f};
+        float tprev = t_in;
+        bool have_prev = false;
+        unsigned found_here = 0;
+        IsoBracket br[FEMRT_MAX_ISO];
+        for (int j = 0; j <= Ms; ++j) {
+          const float t = j == Ms ? t_out : t_in + (t_out - t_in) * ((float)j / (float)Ms);
+          const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
+          float J[9];
+          // the segment ends lie on faces whose (u, v) the face roots give:
+          // evaluate there, never by extrapolating past the face
+          bool known = false;
+          if (j == 0) {
+            rst[0] = rst_in[0];
+            rst[1] = rst_in[1];
+            rst[2] = rst_in[2];
+            known = true;
+          } else if (j == Ms) {
+            face_to_rst(f_out, u_out, v_out, rst[0], rst[1], rst[2]);
+            known = true;
+          } else if (P.guess == FEMRT_GUESS_PLAIN) {
+            rst[0] = rst[1] = rst[2] = 0.f;
+          }
+          if (!known && !invert_in_cube_f32<NG>(gc, x, rst, J, w.inv_it)) {
+            // no preimage inside the element: the segment end is wrong
+            if (pass == 0) {
+              redo = true;
+              break;
+            }
+            have_prev = false;
+            rst[0] = rst[1] = rst[2] = 0.f;
+            continue;
+          }
+          const float u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
+          ++w.field;
+          for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
+            if (k >= P.n_iso || ((found | found_here) >> k) & 1u) continue;
+            const float g = u - P.iso[k];
+            if (have_prev && gprev[k] * g <= 0.f) {
+              br[k].elem = e;
+              br[k].ta = tprev;
+              br[k].tb = t;
+              br[k].tm = 0.5f * (tprev + t);
+              br[k].ga = gprev[k];
+              for (int c = 0; c < 3; ++c) br[k].rst[c] = 0.5f * (rprev[c] + rst[c]);
+              found_here |= 1u << k;
+            }
+            gprev[k] = g;
+          }
+          rprev[0] = rst[0];
+          rprev[1] = rst[1];
+          rprev[2] = rst[2];
+          tprev = t;
+          have_prev = true;
+        }
+        if (redo) {
+          if (!find_exit_search<NG>(gc, bx, o, d, id, f_in, t_lo, P.guess, true, f_out, t_out, u_out,
+                                    v_out, w) &&
+              !find_exit_marching<NG>(gc, bx, o, d, id, rst_in, t_in, t_lo, f_out, t_out, u_out, v_out,
+                                      w)) {
+            ok = false;
+            break;
+          }
+          continue;
+        }
+        for (int k = 0; k < FEMRT_MAX_ISO; ++k)
+          if ((found_here >> k) & 1u) out[k] = br[k];
+        found |= found_here;
+        break;
+      }
+    }
+    if (found == all) break;
+    int ne = -1, nfc = -1;
+    if (ok && M.adj) {
+      ne = M.adj[(size_t)e * 12 + f_out * 2];
+      nfc = M.adj[(size_t)e * 12 + f_out * 2 + 1];
+    }
+    if (ne >= 0) {
+      // entry point on the neighbour's face: the shared face is parametrised
+      // alike on both sides in a conforming mesh of aligned elements; the
+      // walk only uses it to seed the next exit prediction
+      e = ne;
+      f_in = nfc;
+      t_in = t_out;
+      u_in = u_out;
+      v_in = v_out;
+      continue;
+    }
+    // the ray left the mesh (boundary face, no adjacency, or a segment too
+    // short to resolve): re-enter where it next meets another element, by the
+    // same traversal as the first hit
+    const float t_from = ok ? t_out : t_in;
+    const HitF32 h2 =
+        first_hit_ray<NG>(M, o, d, P.guess, t_from + 1e-6f * (1.f + fabsf(t_from)), 1e30f, w, e);
+    if (h2.elem < 0) break;
+    e = h2.elem;
+    f_in = h2.face;
+    t_in = h2.t;
+    u_in = h2.u;
+    v_in = h2.v;
+  }
+  return found;
+}
+
+// FP64 endgame of one isovalue bracket: point inversion in FP64 (warm from
+// the FP32 estimate) and safeguarded Newton on g(t) = u - iso with
+// g'(t) = grad_world(u) . d, kept inside the sign-change bracket; bisection
+// when the Newton step leaves it.  Converged when the step or the bracket is
+// below 1e-13 (1 + |t|).
+template <int NG, int NF>
+FEMRT_HD bool polish_iso(const TraceMesh& M, const double* o, const double* d, double iso,
+                         const IsoBracket& br, double& t_out, double* rst_out, double& u_out,
+                         Work& w)
+{
+  const float4* gc = M.geom + (size_t)br.elem * (NG * NG * NG);
+  const float* fc = M.field + (size_t)br.elem * (NF * NF * NF);
+  double lo = br.ta, hi = br.tb, t = br.tm;
+  const bool lo_neg = br.ga < 0.f;
+  double rst[3] = {br.rst[0], br.rst[1], br.rst[2]}, J[9];
+  double u = 0.0;
+  bool ok = false;
+  for (int it = 0; it < 60; ++it) {
+    const double x[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
+    if (!invert_f64<NG>(gc, x, rst, J, w.geo64)) break;
+    double gref[3], gw[3];
+    field_eval<double, NF>(fc, rst[0], rst[1], rst[2], &u, gref);
+    ++w.field64;
+    grad_world<double>(J, gref, gw);
+    const double g = u - iso;
+    const double gp = gw[0] * d[0] + gw[1] * d[1] + gw[2] * d[2];
+    ok = true;
+    t_out = t;
+    for (int c = 0; c < 3; ++c) rst_out[c] = rst[c];
+    u_out = u;
+    if (g == 0.0) break;
+    if ((g < 0.0) == lo_neg) lo = t; else hi = t;
+    double tn = (gp != 0.0) ? t - g / gp : 0.5 * (lo + hi);
+    if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+    const double step = fabs(tn - t);
+    t = tn;
+    if (step < 1e-13 * (1.0 + fabs(t)) || (hi - lo) < 1e-13 * (1.0 + fabs(t))) {
+      const double x2[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
+      if (invert_f64<NG>(gc, x2, rst, J, w.geo64)) {
+        u_out = field_value<double, NF>(fc, rst[0], rst[1], rst[2]);
+        ++w.field64;
+        t_out = t;
+        for (int c = 0; c < 3; ++c) rst_out[c] = rst[c];
+      }
+      break;
+    }
+  }
+  return ok;
+}
+
+}  // namespace femrt

@@ -111,6 +111,21 @@ def test_generator_corners_adjacency_and_conformity(femrt, oracle):
                 np.testing.assert_allclose(xa, xb, atol=1e-6)
 
 
+@pytest.mark.parametrize("N,q,pert,amp,seed,n_check", [(4, 3, 1, 0.05, 1, 64), (6, 4, 2, 0.08, 5, 216),
+                                                      (32, 4, 2, 0.08, 20240611, 150)])
+def test_generator_elements_are_valid(femrt, oracle, N, q, pert, amp, seed, n_check):
+    """Every checked element has a positive Jacobian determinant on a 5^3 grid
+    (the map is one-to-one, so point inversion and face roots are defined)."""
+    hm = femrt.gen_structured_hex(N, q, 2, pert=pert, amp=amp, seed=seed)
+    orc = oracle.Oracle.from_mesh(hm)
+    rng = np.random.default_rng(0)
+    g = np.linspace(-1, 1, 5)
+    els = range(hm.n_elem) if hm.n_elem <= n_check else rng.choice(hm.n_elem, n_check, replace=False)
+    worst = min(np.linalg.det(orc.map(e, np.array(p))[1]) * (2.0 * N) ** 3
+                for e in els for p in itertools.product(g, g, g))
+    assert worst > 0.15
+
+
 def test_generator_field_is_seeded_and_decays(femrt):
     a = femrt.gen_structured_hex(2, 3, 6, seed=3)
     b = femrt.gen_structured_hex(2, 3, 6, seed=3)
@@ -147,6 +162,14 @@ def test_bernstein_boxes_contain_dense_samples(femrt, oracle, pert, amp, q):
                 rst[ua], rst[va] = u, v
                 x, _ = orc.map(e, rst)
                 assert np.all(x >= flo) and np.all(x <= fhi), (e, f)
+                # the sub-patch boxes that contain (u, v) contain the point
+                for a in range(3):
+                    for b in range(3):
+                        if not (-1 + 2 * a / 3 - 1e-12 <= u <= -1 + 2 * (a + 1) / 3 + 1e-12 and
+                                -1 + 2 * b / 3 - 1e-12 <= v <= -1 + 2 * (b + 1) / 3 + 1e-12):
+                            continue
+                        sb = boxes[e, 7 + 9 * f + 3 * b + a]
+                        assert np.all(x >= sb[:3]) and np.all(x <= sb[3:]), (e, f, a, b)
 
 
 def test_bvh_invariants(femrt):
