@@ -107,9 +107,12 @@ FEMRT_API int femrt_mesh_get_bvh(const femrt_mesh* mesh, float* nodes8, int* pri
  * float or double). Host buffers; synchronous. */
 FEMRT_API int femrt_eval_field(femrt_mesh* mesh, int n, const int* elem, const float* rst,
                                int precision, void* out);
-/* Point inversion x -> rst (Newton from rst = 0, as WorldToReferenceNewton):
- * x is 3 x n SoA; rst_out 3 x n SoA doubles after an FP64 polish; iters_out
- * (may be NULL) the FP32 iteration count, -1 when the iteration failed. */
+/* Point inversion x -> rst: the preimage inside the element.  Newton from
+ * rst = 0 as WorldToReferenceNewton, restarted from the 8 octant centres when
+ * it lands outside the element (Newton from 0 finds spurious preimages of the
+ * polynomial map near faces), then an FP64 polish.  x is 3 x n SoA; rst_out
+ * 3 x n SoA doubles; iters_out (may be NULL) the FP32 iteration count, -1
+ * when no preimage inside the element was found. */
 FEMRT_API int femrt_invert_points(femrt_mesh* mesh, int n, const int* elem, const float* x,
                                   double* rst_out, int* iters_out);
 
@@ -126,7 +129,8 @@ enum {
 typedef struct {
   int n_iso;                     /* isovalues searched (0..FEMRT_MAX_ISO)         */
   int iso_enable;                /* 0 disables the iso stage (0.0 is a valid iso) */
-  float iso[FEMRT_MAX_ISO];
+  double iso[FEMRT_MAX_ISO];      /* exact isovalues (the FP32 scan uses their
+                                    rounding, the FP64 endgame the values)  */
   int samples;                   /* uniform sub-intervals per element segment (16) */
   int guess;                     /* FEMRT_GUESS_*                                 */
   float t_min;                   /* 0.001 in the reference (raytracer.cu:221)     */

@@ -30,7 +30,7 @@ class FemrtError(RuntimeError):
 
 
 class TraceParams(C.Structure):
-    _fields_ = [("n_iso", C.c_int), ("iso_enable", C.c_int), ("iso", C.c_float * MAX_ISO),
+    _fields_ = [("n_iso", C.c_int), ("iso_enable", C.c_int), ("iso", C.c_double * MAX_ISO),
                 ("samples", C.c_int), ("guess", C.c_int), ("t_min", C.c_float),
                 ("t_max", C.c_float), ("max_march", C.c_int), ("flags", C.c_int)]
 

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void k_polish_iso(
     br.rst[1] = r4.y;
     br.rst[2] = r4.z;
     double t, rst[3], u;
-    if (polish_iso<NG, NF>(M, o, d, (double)P.iso[k], br, t, rst, u, w)) {
+    if (polish_iso<NG, NF>(M, o, d, P.iso64[k], br, t, rst, u, w)) {
       iso_t[slot] = t;
       iso_rst[(size_t)k * 3 * n + r] = rst[0];
       iso_rst[(size_t)k * 3 * n + n + r] = rst[1];
@@ -442,7 +442,7 @@ __global__ void k_invert(int n, const int* __restrict__ elem, const float* __res
   const float xf[3] = {x[p], x[n + p], x[2 * n + p]};
   float rst[3] = {0.f, 0.f, 0.f}, J[9];
   long long it = 0, it64 = 0;
-  const bool ok = invert_f32<NG>(gc, xf, rst, J, it);
+  const bool ok = invert_in_cube_f32<NG>(gc, xf, rst, J, it);
   double rd[3] = {rst[0], rst[1], rst[2]}, Jd[9];
   const double xd[3] = {xf[0], xf[1], xf[2]};
   const bool ok64 = ok && invert_f64<NG>(gc, xd, rd, Jd, it64);
@@ -926,7 +926,10 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   const int pblocks = dev_cus * 8;
   IsoParams ip;
   std::memset(&ip, 0, sizeof(ip));
-  for (int k = 0; k < FEMRT_MAX_ISO; ++k) ip.iso[k] = p->iso[k];
+  for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
+    ip.iso[k] = (float)p->iso[k];
+    ip.iso64[k] = p->iso[k];
+  }
   ip.n_iso = n_iso;
   ip.samples = p->samples;
   ip.guess = p->guess;

@@ -28,7 +28,8 @@ struct TraceMesh {
 };
 
 struct IsoParams {
-  float iso[FEMRT_MAX_ISO];
+  float iso[FEMRT_MAX_ISO];      // FP32 rounding, used by the scan
+  double iso64[FEMRT_MAX_ISO];   // exact values, used by the FP64 endgame
   int n_iso;
   int samples;
   int guess;
@@ -596,6 +597,46 @@ FEMRT_HD bool find_exit_predicted(const float4* __restrict__ gc, const float* o,
   return true;
 }
 
+// Coupled Newton for the isosurface point on the ray: unknowns (r, s, t) and
+// the ray parameter tau, equations x(rst) - (o + tau d) = 0 and
+// u(rst) - iso = 0.  The 4x4 system [J -d; g^T 0] is solved by block
+// elimination with the cofactors of J: a = J^-1 F_x, b = J^-1 d,
+// dtau = (F_u - g.a) / (g.b), drst = a + b dtau (g.b = grad u . d is the
+// derivative of u along the ray).  tau stays inside [lo, hi]; returns false
+// when an iterate leaves it or the system is singular.
+template <typename T, int NG, int NF>
+FEMRT_HD bool iso_newton(const float4* __restrict__ gc, const float* __restrict__ fc, const T* o,
+                         const T* d, T iso, T lo, T hi, int max_it, T tol_t, T tol_r, T& tau, T* rst,
+                         T& u, long long& geo_it, long long& field_it)
+{
+  for (int it = 0; it < max_it; ++it) {
+    T X[3], J[9], g[3];
+    geom_eval<T, NG>(gc, rst[0], rst[1], rst[2], X, J);
+    field_eval<T, NF>(fc, rst[0], rst[1], rst[2], &u, g);
+    ++geo_it;
+    ++field_it;
+    const T Fx[3] = {X[0] - (o[0] + tau * d[0]), X[1] - (o[1] + tau * d[1]),
+                     X[2] - (o[2] + tau * d[2])};
+    const T Fu = u - iso;
+    T a[3], b[3];
+    if (!solve3<T>(J, Fx, a) || !solve3<T>(J, d, b)) return false;
+    const T gb = g[0] * b[0] + g[1] * b[1] + g[2] * b[2];
+    if (!(gb != T(0))) return false;
+    // Newton step: [J -d; g^T 0] [drst; dtau] = [Fx; Fu], then subtract
+    const T dtau = (Fu - (g[0] * a[0] + g[1] * a[1] + g[2] * a[2])) / gb;
+    const T dr[3] = {a[0] + b[0] * dtau, a[1] + b[1] * dtau, a[2] + b[2] * dtau};
+    rst[0] -= dr[0];
+    rst[1] -= dr[1];
+    rst[2] -= dr[2];
+    tau -= dtau;
+    if (!(tau >= lo && tau <= hi)) return false;
+    if (fabs((double)dtau) <= (double)tol_t * (1.0 + fabs((double)tau)) &&
+        fmax(fabs((double)dr[0]), fmax(fabs((double)dr[1]), fabs((double)dr[2]))) <= (double)tol_r)
+      return true;
+  }
+  return false;
+}
+
 // Walk the hit ray through face-adjacent elements, sampling g = u - iso at
 // P.samples uniform sub-intervals of each element segment [t_in, t_out].
 // The first sub-interval with g(prev) g(cur) <= 0 of each isovalue is
@@ -682,9 +723,21 @@ FEMRT_HD unsigned march_ray(const TraceMesh& M, const IsoParams& P, const float*
               br[k].elem = e;
               br[k].ta = tprev;
               br[k].tb = t;
-              br[k].tm = 0.5f * (tprev + t);
               br[k].ga = gprev[k];
-              for (int c = 0; c < 3; ++c) br[k].rst[c] = 0.5f * (rprev[c] + rst[c]);
+              // FP32 coupled Newton from the bracket midpoint; the FP64
+              // endgame starts from its result (or from the midpoint when it
+              // leaves the bracket)
+              float tm = 0.5f * (tprev + t), rm[3], um;
+              for (int c = 0; c < 3; ++c) rm[c] = 0.5f * (rprev[c] + rst[c]);
+              const float ofs[3] = {o[0], o[1], o[2]}, dfs[3] = {d[0], d[1], d[2]};
+              // FP32 resolves rst to ~1e-7 |x| / |J| (~1e-5 on small elements)
+              if (!iso_newton<float, NG, NF>(gc, fc, ofs, dfs, P.iso[k], tprev, t, 6, 1e-6f, 1e-3f, tm,
+                                            rm, um, w.inv_it, w.field)) {
+                tm = 0.5f * (tprev + t);
+                for (int c = 0; c < 3; ++c) rm[c] = 0.5f * (rprev[c] + rst[c]);
+              }
+              br[k].tm = tm;
+              for (int c = 0; c < 3; ++c) br[k].rst[c] = rm[c];
               found_here |= 1u << k;
             }
             gprev[k] = g;
@@ -756,6 +809,16 @@ FEMRT_HD bool polish_iso(const TraceMesh& M, const double* o, const double* d, d
 {
   const float4* gc = M.geom + (size_t)br.elem * (NG * NG * NG);
   const float* fc = M.field + (size_t)br.elem * (NF * NF * NF);
+  {
+    double tau = br.tm, r4[3] = {br.rst[0], br.rst[1], br.rst[2]}, u4;
+    if (iso_newton<double, NG, NF>(gc, fc, o, d, iso, (double)br.ta, (double)br.tb, 6, 1e-13, 1e-9,
+                                   tau, r4, u4, w.geo64, w.field64)) {
+      t_out = tau;
+      for (int c = 0; c < 3; ++c) rst_out[c] = r4[c];
+      u_out = u4;
+      return true;
+    }
+  }
   double lo = br.ta, hi = br.tb, t = br.tm;
   const bool lo_neg = br.ga < 0.f;
   double rst[3] = {br.rst[0], br.rst[1], br.rst[2]}, J[9];
