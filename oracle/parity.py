@@ -78,7 +78,10 @@ def compare(gh, giso, og, oiso, isos):
             r["t_rel_max"] = float(tr.max())
             r["t_rel_bad"] = int((tr > T_REL).sum())
             r["t_rel_bad_nontie"] = int(((tr > T_REL) & ~itie[b]).sum())
-            r["u_minus_iso_max"] = float(np.abs(giso["u"][k][fg] - iso).max())
+            du = np.abs(giso["u"][k] - iso)
+            r["u_minus_iso_max"] = float(du[fg & ~itie].max()) if (fg & ~itie).any() else 0.0
+            # near-tangent crossings (tie class) are reported, not gated
+            r["u_minus_iso_max_ties"] = float(du[fg & itie].max()) if (fg & itie).any() else 0.0
             se = b & (giso["elem"][k] == oiso["elem"][k])
             r["elem_mismatch"] = int((b & ~se & ~itie).sum())
             r["rst_abs_max"] = float(np.abs(giso["rst"][k][:, se] - oiso["rst"][k][:, se]).max()) if se.any() else 0.0

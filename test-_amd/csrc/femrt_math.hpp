@@ -60,35 +60,93 @@ FEMRT_HD void basis1d(T x, T* __restrict__ phi, T* __restrict__ dphi)
   }
 }
 
+// Modified_A basis generated one mode at a time (phi_p, phi'_p for
+// p = 0, 1, 2, ...), so rolled loops over modes keep only the recurrence state
+// in registers instead of whole tables.
 template <typename T>
-struct Vec3 {
-  T x, y, z;
+struct BasisIter {
+  T x, b, db;
+  T P0, P1, dP0, dP1;  // P_{k-1}, P_k of the (1,1) Jacobi recurrence and derivatives
+  int p;
+  FEMRT_HD void init(T x_)
+  {
+    x = x_;
+    b = T(0.25) - T(0.25) * x * x;
+    db = T(-0.5) * x;
+    P0 = T(0);
+    dP0 = T(0);
+    P1 = T(1);
+    dP1 = T(0);
+    p = 0;
+  }
+  FEMRT_HD void next(T& phi, T& dphi)
+  {
+    if (p == 0) {
+      phi = T(0.5) - T(0.5) * x;
+      dphi = T(-0.5);
+    } else if (p == 1) {
+      phi = T(0.5) + T(0.5) * x;
+      dphi = T(0.5);
+    } else {
+      if (p >= 3) {
+        // advance P_{k} -> P_{k+1}, k = p - 3
+        const int k = p - 2;  // index of the polynomial produced
+        T Pn, dPn;
+        if (k == 1) {
+          Pn = T(2) * x;
+          dPn = T(2);
+        } else {
+          const T kk = T(k);
+          const T A = (T(2) * kk + T(1)) * (kk + T(1)) / (kk * (kk + T(2)));
+          const T B = (kk + T(1)) / (kk + T(2));
+          Pn = A * x * P1 - B * P0;
+          dPn = A * (P1 + x * dP1) - B * dP0;
+        }
+        P0 = P1;
+        dP0 = dP1;
+        P1 = Pn;
+        dP1 = dPn;
+      }
+      phi = b * P1;
+      dphi = db * P1 + b * dP1;
+    }
+    ++p;
+  }
 };
 
 // x(r,s,t) and J (row-major, rows x,y,z; columns d/dr, d/ds, d/dt).
 // c: NG^3 float4 (x, y, z, unused) for one element, mode (i,j,k) at
-// (k*NG+j)*NG+i.  1035 FMA at NG=5.
+// (k*NG+j)*NG+i.  1035 FMA at NG=5.  The k and j loops stay rolled (the s and
+// t basis is generated by the recurrence as they advance) so only NG float4
+// coefficients are live at a time.
 template <typename T, int NG>
 FEMRT_HD void geom_eval(const float4* __restrict__ c, T r, T s, T t, T* __restrict__ X,
                         T* __restrict__ J)
 {
-  T pr[NG], dpr[NG], ps[NG], dps[NG], pt[NG], dpt[NG];
+  T pr[NG], dpr[NG];
   basis1d<T, NG>(r, pr, dpr);
-  basis1d<T, NG>(s, ps, dps);
-  basis1d<T, NG>(t, pt, dpt);
   T x0 = 0, x1 = 0, x2 = 0;
   T jr0 = 0, jr1 = 0, jr2 = 0, js0 = 0, js1 = 0, js2 = 0, jt0 = 0, jt1 = 0, jt2 = 0;
-#pragma unroll
+  BasisIter<T> bt;
+  bt.init(t);
+#pragma unroll 1
   for (int k = 0; k < NG; ++k) {
+    T tk, dtk;
+    bt.next(tk, dtk);
     T b0 = 0, b1 = 0, b2 = 0;     // sum_j phi_j(s) a_jk
     T bs0 = 0, bs1 = 0, bs2 = 0;  // sum_j phi'_j(s) a_jk
     T br0 = 0, br1 = 0, br2 = 0;  // sum_j phi_j(s) a'_jk
-#pragma unroll
+    BasisIter<T> bs;
+    bs.init(s);
+#pragma unroll 1
     for (int j = 0; j < NG; ++j) {
+      T sj, dsj;
+      bs.next(sj, dsj);
+      const float4* cj = c + (k * NG + j) * NG;
       T a0 = 0, a1 = 0, a2 = 0, ad0 = 0, ad1 = 0, ad2 = 0;
 #pragma unroll
       for (int i = 0; i < NG; ++i) {
-        const float4 q = c[(k * NG + j) * NG + i];
+        const float4 q = cj[i];
         a0 += pr[i] * T(q.x);
         a1 += pr[i] * T(q.y);
         a2 += pr[i] * T(q.z);
@@ -96,28 +154,28 @@ FEMRT_HD void geom_eval(const float4* __restrict__ c, T r, T s, T t, T* __restri
         ad1 += dpr[i] * T(q.y);
         ad2 += dpr[i] * T(q.z);
       }
-      b0 += ps[j] * a0;
-      b1 += ps[j] * a1;
-      b2 += ps[j] * a2;
-      bs0 += dps[j] * a0;
-      bs1 += dps[j] * a1;
-      bs2 += dps[j] * a2;
-      br0 += ps[j] * ad0;
-      br1 += ps[j] * ad1;
-      br2 += ps[j] * ad2;
+      b0 += sj * a0;
+      b1 += sj * a1;
+      b2 += sj * a2;
+      bs0 += dsj * a0;
+      bs1 += dsj * a1;
+      bs2 += dsj * a2;
+      br0 += sj * ad0;
+      br1 += sj * ad1;
+      br2 += sj * ad2;
     }
-    x0 += pt[k] * b0;
-    x1 += pt[k] * b1;
-    x2 += pt[k] * b2;
-    jr0 += pt[k] * br0;
-    jr1 += pt[k] * br1;
-    jr2 += pt[k] * br2;
-    js0 += pt[k] * bs0;
-    js1 += pt[k] * bs1;
-    js2 += pt[k] * bs2;
-    jt0 += dpt[k] * b0;
-    jt1 += dpt[k] * b1;
-    jt2 += dpt[k] * b2;
+    x0 += tk * b0;
+    x1 += tk * b1;
+    x2 += tk * b2;
+    jr0 += tk * br0;
+    jr1 += tk * br1;
+    jr2 += tk * br2;
+    js0 += tk * bs0;
+    js1 += tk * bs1;
+    js2 += tk * bs2;
+    jt0 += dtk * b0;
+    jt1 += dtk * b1;
+    jt2 += dtk * b2;
   }
   X[0] = x0;
   X[1] = x1;
@@ -131,31 +189,38 @@ FEMRT_HD void geom_eval(const float4* __restrict__ c, T r, T s, T t, T* __restri
 template <typename T, int NG>
 FEMRT_HD void geom_point(const float4* __restrict__ c, T r, T s, T t, T* __restrict__ X)
 {
-  T pr[NG], dpr[NG], ps[NG], dps[NG], pt[NG], dpt[NG];
+  T pr[NG], dpr[NG];
   basis1d<T, NG>(r, pr, dpr);
-  basis1d<T, NG>(s, ps, dps);
-  basis1d<T, NG>(t, pt, dpt);
   T x0 = 0, x1 = 0, x2 = 0;
-#pragma unroll
+  BasisIter<T> bt;
+  bt.init(t);
+#pragma unroll 1
   for (int k = 0; k < NG; ++k) {
+    T tk, dtk;
+    bt.next(tk, dtk);
     T b0 = 0, b1 = 0, b2 = 0;
-#pragma unroll
+    BasisIter<T> bs;
+    bs.init(s);
+#pragma unroll 1
     for (int j = 0; j < NG; ++j) {
+      T sj, dsj;
+      bs.next(sj, dsj);
+      const float4* cj = c + (k * NG + j) * NG;
       T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
       for (int i = 0; i < NG; ++i) {
-        const float4 q = c[(k * NG + j) * NG + i];
+        const float4 q = cj[i];
         a0 += pr[i] * T(q.x);
         a1 += pr[i] * T(q.y);
         a2 += pr[i] * T(q.z);
       }
-      b0 += ps[j] * a0;
-      b1 += ps[j] * a1;
-      b2 += ps[j] * a2;
+      b0 += sj * a0;
+      b1 += sj * a1;
+      b2 += sj * a2;
     }
-    x0 += pt[k] * b0;
-    x1 += pt[k] * b1;
-    x2 += pt[k] * b2;
+    x0 += tk * b0;
+    x1 += tk * b1;
+    x2 += tk * b2;
   }
   X[0] = x0;
   X[1] = x1;
@@ -168,31 +233,38 @@ template <typename T, int NF>
 FEMRT_HD void field_eval(const float* __restrict__ c, T r, T s, T t, T* __restrict__ u,
                          T* __restrict__ g)
 {
-  T pr[NF], dpr[NF], ps[NF], dps[NF], pt[NF], dpt[NF];
+  T pr[NF], dpr[NF];
   basis1d<T, NF>(r, pr, dpr);
-  basis1d<T, NF>(s, ps, dps);
-  basis1d<T, NF>(t, pt, dpt);
   T v = 0, gr = 0, gs = 0, gt = 0;
-#pragma unroll
+  BasisIter<T> bt;
+  bt.init(t);
+#pragma unroll 1
   for (int k = 0; k < NF; ++k) {
+    T tk, dtk;
+    bt.next(tk, dtk);
     T b = 0, bs = 0, br = 0;
-#pragma unroll
+    BasisIter<T> bsi;
+    bsi.init(s);
+#pragma unroll 1
     for (int j = 0; j < NF; ++j) {
+      T sj, dsj;
+      bsi.next(sj, dsj);
+      const float* cj = c + (k * NF + j) * NF;
       T a = 0, ad = 0;
 #pragma unroll
       for (int i = 0; i < NF; ++i) {
-        const T q = T(c[(k * NF + j) * NF + i]);
+        const T q = T(cj[i]);
         a += pr[i] * q;
         ad += dpr[i] * q;
       }
-      b += ps[j] * a;
-      bs += dps[j] * a;
-      br += ps[j] * ad;
+      b += sj * a;
+      bs += dsj * a;
+      br += sj * ad;
     }
-    v += pt[k] * b;
-    gr += pt[k] * br;
-    gs += pt[k] * bs;
-    gt += dpt[k] * b;
+    v += tk * b;
+    gr += tk * br;
+    gs += tk * bs;
+    gt += dtk * b;
   }
   *u = v;
   g[0] = gr;
@@ -204,22 +276,29 @@ FEMRT_HD void field_eval(const float* __restrict__ c, T r, T s, T t, T* __restri
 template <typename T, int NF>
 FEMRT_HD T field_value(const float* __restrict__ c, T r, T s, T t)
 {
-  T pr[NF], dpr[NF], ps[NF], dps[NF], pt[NF], dpt[NF];
+  T pr[NF], dpr[NF];
   basis1d<T, NF>(r, pr, dpr);
-  basis1d<T, NF>(s, ps, dps);
-  basis1d<T, NF>(t, pt, dpt);
   T v = 0;
-#pragma unroll
+  BasisIter<T> bt;
+  bt.init(t);
+#pragma unroll 1
   for (int k = 0; k < NF; ++k) {
+    T tk, dtk;
+    bt.next(tk, dtk);
     T b = 0;
-#pragma unroll
+    BasisIter<T> bsi;
+    bsi.init(s);
+#pragma unroll 1
     for (int j = 0; j < NF; ++j) {
+      T sj, dsj;
+      bsi.next(sj, dsj);
+      const float* cj = c + (k * NF + j) * NF;
       T a = 0;
 #pragma unroll
-      for (int i = 0; i < NF; ++i) a += pr[i] * T(c[(k * NF + j) * NF + i]);
-      b += ps[j] * a;
+      for (int i = 0; i < NF; ++i) a += pr[i] * T(cj[i]);
+      b += sj * a;
     }
-    v += pt[k] * b;
+    v += tk * b;
   }
   return v;
 }
