@@ -34,6 +34,20 @@ ISOS = [-0.1, 0.0, 0.1]
 PEAK_FP32_TFLOPS = 157.3   # MI355X published vector/matrix FP32 (packed); 78.6 unpacked
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0
+MLP_FLOPS = 2 * (28 * 64 + 64 * 64 + 64 * 3)   # one inference, 27(+1) -> 64 -> 64 -> 3
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
+
+
+def measured_traffic(kernel, variant):
+    """HBM bytes per launch of `kernel` from the committed counter profile
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known size), when
+    one was taken for this variant; None otherwise."""
+    try:
+        d = json.loads(TRAFFIC_FILE.read_text())
+        return d[variant][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def flops_model(ng, nf):
@@ -138,7 +152,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variants", default="warm,plain")
+    ap.add_argument("--variants", default="learned,warm,plain")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -162,6 +176,26 @@ def main():
     names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED}
     variants = [v for v in args.variants.split(",") if v]
     results = {}
+    mlp_info = None
+    if "learned" in variants:
+        # the guess network is trained on another mesh of the same family
+        # (different seed, 16^3) by the library's own GPU kernels, untimed
+        t0 = time.perf_counter()
+        thm = femrt.gen_structured_hex(16, Q, P, pert=2, amp=AMP, seed=SEED + 1)
+        tmesh = femrt.Mesh(ctx, thm, build_bvh=False)
+        mlp = femrt.Mlp(ctx, seed=SEED)
+        losses = mlp.train(tmesh, n_samples=1 << 21, batch=4096, epochs=4, lr=3e-3, lr_final=1e-4,
+                           seed=SEED + 2)
+        feats, tgt = femrt.mlp_samples(mesh, 1 << 16, seed=SEED + 3)
+        err = np.abs(mlp.forward(feats) - tgt).max(1)
+        mesh.set_guess_mlp(mlp)
+        tmesh.close()
+        mlp_info = {"train_mesh": "16^3 Q=4 P=6 amp 0.08 h, seed+1", "train_pairs": 1 << 21,
+                    "epochs": 4, "epoch_loss": [float(x) for x in losses],
+                    "train_s": time.perf_counter() - t0,
+                    "guess_err_median_on_bench_mesh": float(np.median(err)),
+                    "guess_err_p99_on_bench_mesh": float(np.percentile(err, 99))}
+        print(json.dumps({"mlp": mlp_info}), file=sys.stderr, flush=True)
     for v in variants:
         wall, kt, cnt = run_variant(femrt, ctx, mesh, fr, names[v], args.steps, args.warmup)
         f32, f64 = kernel_flops(cnt, hm.ng, hm.nf)
@@ -183,13 +217,20 @@ def main():
     cnt = R["counters"]
     if dom == "k_march":
         fl = ((cnt["invert_it"] + cnt["face_newton_it"]) * fm["geo"] +
-              cnt["field_evals"] * fm["field_val"]) - 0
+              cnt["field_evals"] * fm["field_val"] + cnt["slab_tests"] * fm["slab"])
     else:
         fl = R["f32"]
     achieved = fl / (dom_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-            "frac_of_unpacked_fp32": achieved / (PEAK_FP32_TFLOPS / 2)}
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": measured_traffic(dom, head),
+            "frac_of_unpacked_fp32": achieved / (PEAK_FP32_TFLOPS / 2),
+            "kernel_ms_per_launch": dom_ms, "flops_per_launch": fl}
+    if cnt.get("mlp_infer"):
+        # network inferences of the walk, on the BF16 matrix cores
+        mf = cnt["mlp_infer"] * MLP_FLOPS
+        roof["mlp_bf16"] = {"flops_per_launch": mf, "achieved": mf / (dom_ms * 1e-3) / 1e12,
+                            "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s"}
 
     check = None
     if not args.no_check:
@@ -225,6 +266,8 @@ def main():
         "cpu_baseline": cpu,
         "parity_sample": check,
     }
+    if mlp_info:
+        line["config"]["mlp"] = mlp_info
     if "plain" in results and head != "plain":
         line["config"]["speedup_vs_plain_newton"] = R["mrays"] / results["plain"]["mrays"]
     print(json.dumps(line), flush=True)

@@ -170,6 +170,56 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
 #define FEMRT_NCOUNTERS 10
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
 
+/* ------------------------------------------------- learned Newton guess */
+/* No counterpart in the reference (its Newton solves start from fixed seeds,
+ * HexElement.cu:227-272, high_order.h:206): a small network maps a point in
+ * an element to a reference-coordinate guess for point inversion.
+ * FEMRT_MLP_GUESS: 27 inputs (the point and the 8 corner vertices, each
+ * normalised to the element's box as 2 (x - lo) / (hi - lo) - 1) -> 64 tanh
+ * -> 64 tanh -> 3.  BF16 operands, FP32 accumulation, on the matrix cores.
+ * Flat FP32 parameters (6147): W1 [28][64] (row 27 = first-layer bias),
+ * W2 [64][64] (in, out), b2 [64], W3 [64][3], b3 [3]. */
+#define FEMRT_MLP_GUESS 0
+typedef struct femrt_mlp femrt_mlp;
+typedef struct {
+  int n_samples;      /* training pairs generated from the mesh             */
+  int batch;          /* multiple of 64                                      */
+  int epochs;
+  float lr;           /* Adam step, cosine-decayed to lr_final               */
+  float lr_final;
+  uint64_t seed;
+} femrt_mlp_train_params;
+
+FEMRT_API int femrt_mlp_num_params(int kind);
+/* Glorot-uniform weights from a keyed generator, zero biases. */
+FEMRT_API int femrt_mlp_create(femrt_ctx* ctx, int kind, unsigned long long seed, femrt_mlp** out);
+FEMRT_API int femrt_mlp_destroy(femrt_mlp* m);
+FEMRT_API int femrt_mlp_get_params(femrt_mlp* m, float* params);
+/* Replaces the parameters and resets the optimiser state. */
+FEMRT_API int femrt_mlp_set_params(femrt_mlp* m, const float* params);
+/* Inference with the element walk's own device routine: feats n x 27,
+ * out n x 3 (host arrays, synchronous). */
+FEMRT_API int femrt_mlp_forward(femrt_mlp* m, int n, const float* feats, float* out);
+/* Loss sum_i sum_c (y_ic - target_ic)^2 / n and its gradient with respect to
+ * the flat parameters, from the training kernel's forward and backward pass
+ * (no update).  feats n x 27, targets n x 3, grad 6147 floats. */
+FEMRT_API int femrt_mlp_loss_grad(femrt_mlp* m, int n, const float* feats, const float* targets,
+                                  float* loss, float* grad);
+/* Training pairs from the tracer's own point inversions: a seeded element and
+ * reference point rst0, x = X_e(rst0), target = the FP32 Newton preimage of x
+ * inside the element.  feats n x 27, targets n x 3 (host). */
+FEMRT_API int femrt_mlp_make_samples(femrt_mesh* mesh, int n, unsigned long long seed, float* feats,
+                                     float* targets);
+FEMRT_API void femrt_mlp_train_params_default(femrt_mlp_train_params* p);
+/* Generates p->n_samples pairs on the device and runs p->epochs of Adam over
+ * them (MSE loss); epoch_loss (may be NULL) receives the mean batch loss of
+ * each epoch.  Synchronous. */
+FEMRT_API int femrt_mlp_train(femrt_mlp* m, femrt_mesh* mesh, const femrt_mlp_train_params* p,
+                              float* epoch_loss);
+/* Network used by FEMRT_GUESS_LEARNED traces of this mesh (NULL detaches);
+ * it must stay alive while such traces run. */
+FEMRT_API int femrt_mesh_set_guess_mlp(femrt_mesh* mesh, const femrt_mlp* m);
+
 /* ------------------------------------------------ host-only utilities */
 /* The same box and BVH builders the mesh upload uses, on host arrays (no
  * device needed).  boxes: n_elem x 61 x 6 floats (layout of femrt_mesh_boxes).  nodes8 must hold
@@ -200,13 +250,25 @@ FEMRT_API int femrt_gen_structured_hex(const femrt_gen_params* gp, float* vertic
                                        float* field, int* adjacency);
 
 /* Drop-in loader: Nektar++ XML mesh + FLD field (xml_fld_loader.h:14-16).
- * Two-phase: call with NULL arrays to get sizes, then with arrays.
+ * Two-phase: call with NULL arrays to get sizes, then with arrays.  fld_path
+ * may be NULL or empty (field modes then come from the XML EXPANSIONS and the
+ * field is zero).  Hex elements only; uniform field order.
  * flags bit0: replicate the reference's vertex-0 parse defect
  * (xml_fld_geometry_parser.cpp:54-77); bit1: replicate geomModes inflation
  * to max(field, geometry) modes (xml_fld_loader.cpp:252-254). */
 FEMRT_API int femrt_load_xml_fld(const char* xml_path, const char* fld_path, int flags,
                                  int* n_elem, int* ng, int* nf, float* vertices, float* geom,
                                  float* field, int* adjacency);
+/* Writer of the same formats (the reference has none; Nektar++ itself writes
+ * FLD as Hex_CurvedFace.fld:12 shows): vertices and edges shared by exact
+ * position, every face CURVED with ng^2 evenly spaced points of the modal
+ * geometry, one FLD ELEMENTS block B64Z-LittleEndian 64-bit,
+ * NUMMODESPERDIR="UNIORDER:nf,nf,nf". */
+FEMRT_API int femrt_save_xml_fld(const char* xml_path, const char* fld_path, int n_elem, int ng,
+                                 int nf, const float* vertices, const float* geom,
+                                 const float* field);
+/* Message of the last failed load/save call on this thread. */
+FEMRT_API const char* femrt_io_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,16 @@ class TraceParams(C.Structure):
                 ("t_max", C.c_float), ("max_march", C.c_int), ("flags", C.c_int)]
 
 
+class MlpTrainParams(C.Structure):
+    _fields_ = [("n_samples", C.c_int), ("batch", C.c_int), ("epochs", C.c_int),
+                ("lr", C.c_float), ("lr_final", C.c_float), ("seed", C.c_uint64)]
+
+
+MLP_GUESS = 0
+MLP_NPARAM = 6147
+MLP_IN = 27
+
+
 class GenParams(C.Structure):
     _fields_ = [("n", C.c_int), ("q", C.c_int), ("p", C.c_int), ("pert", C.c_int),
                 ("amp", C.c_double), ("seed", C.c_uint64)]
@@ -84,6 +94,19 @@ def lib():
         "femrt_host_build_bvh": (I, [I, P, C.POINTER(I), P, P]),
         "femrt_load_xml_fld": (I, [C.c_char_p, C.c_char_p, I, C.POINTER(I), C.POINTER(I),
                                    C.POINTER(I), P, P, P, P]),
+        "femrt_save_xml_fld": (I, [C.c_char_p, C.c_char_p, I, I, I, P, P, P]),
+        "femrt_io_last_error": (C.c_char_p, []),
+        "femrt_mlp_num_params": (I, [I]),
+        "femrt_mlp_create": (I, [P, I, C.c_ulonglong, C.POINTER(P)]),
+        "femrt_mlp_destroy": (I, [P]),
+        "femrt_mlp_get_params": (I, [P, P]),
+        "femrt_mlp_set_params": (I, [P, P]),
+        "femrt_mlp_forward": (I, [P, I, P, P]),
+        "femrt_mlp_loss_grad": (I, [P, I, P, P, P, P]),
+        "femrt_mlp_make_samples": (I, [P, I, C.c_ulonglong, P, P]),
+        "femrt_mlp_train_params_default": (None, [P]),
+        "femrt_mlp_train": (I, [P, P, P, P]),
+        "femrt_mesh_set_guess_mlp": (I, [P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -103,7 +126,12 @@ def exported_symbols():
             "femrt_frame_destroy", "femrt_frame_set_rays", "femrt_frame_camera_rays",
             "femrt_frame_get_rays", "femrt_trace", "femrt_frame_get_hits", "femrt_frame_get_iso",
             "femrt_frame_get_rgba", "femrt_frame_counters", "femrt_gen_structured_hex",
-            "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh"]
+            "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh",
+            "femrt_save_xml_fld", "femrt_io_last_error",
+            "femrt_mlp_num_params", "femrt_mlp_create", "femrt_mlp_destroy",
+            "femrt_mlp_get_params", "femrt_mlp_set_params", "femrt_mlp_forward",
+            "femrt_mlp_loss_grad", "femrt_mlp_make_samples", "femrt_mlp_train_params_default",
+            "femrt_mlp_train", "femrt_mesh_set_guess_mlp"]
 
 
 def _ptr(a):
@@ -168,20 +196,39 @@ def host_build_bvh(boxes):
     return nodes[:nn.value].copy(), prims
 
 
+LOAD_VERTEX0_DEFECT = 1   # replicate the reference's vertex-0 parse
+LOAD_GEOM_MODES_FROM_FIELD = 2   # replicate geometry modes raised to field modes
+
+
+def _io_check(rc, what):
+    if rc != OK:
+        m = lib().femrt_io_last_error()
+        raise FemrtError(f"{what}: {ERRORS.get(rc, rc)} {m.decode() if m else ''}")
+
+
+def save_xml_fld(hm: HexMesh, xml_path, fld_path):
+    """Write a Nektar++ XML mesh (every face CURVED) and FLD field."""
+    rc = lib().femrt_save_xml_fld(str(xml_path).encode(), str(fld_path).encode(), hm.n_elem,
+                                  hm.ng, hm.nf, _ptr(hm.vertices), _ptr(hm.geom), _ptr(hm.field))
+    _io_check(rc, "save_xml_fld")
+
+
 def load_xml_fld(xml_path, fld_path, flags=0) -> HexMesh:
+    """Drop-in of load_xml_fld (xml_fld_loader.h:14-16): hex meshes."""
     L = lib()
     ne, ng, nf = C.c_int(0), C.c_int(0), C.c_int(0)
-    rc = L.femrt_load_xml_fld(str(xml_path).encode(), str(fld_path).encode(), flags,
+    fp = None if fld_path is None else str(fld_path).encode()
+    rc = L.femrt_load_xml_fld(str(xml_path).encode(), fp, flags,
                               C.byref(ne), C.byref(ng), C.byref(nf), None, None, None, None)
-    _check(rc, what="load_xml_fld(sizes)")
+    _io_check(rc, "load_xml_fld(sizes)")
     v = np.zeros((ne.value, 8, 3), np.float32)
     g = np.zeros((ne.value, 3, ng.value ** 3), np.float32)
     f = np.zeros((ne.value, nf.value ** 3), np.float32)
     a = np.zeros((ne.value, 6, 2), np.int32)
-    rc = L.femrt_load_xml_fld(str(xml_path).encode(), str(fld_path).encode(), flags,
+    rc = L.femrt_load_xml_fld(str(xml_path).encode(), fp, flags,
                               C.byref(ne), C.byref(ng), C.byref(nf), _ptr(v), _ptr(g), _ptr(f),
                               _ptr(a))
-    _check(rc, what="load_xml_fld")
+    _io_check(rc, "load_xml_fld")
     return HexMesh(v, g, f, a, ng.value, nf.value)
 
 
@@ -287,10 +334,80 @@ class Mesh:
                self.ctx.h, "invert_points")
         return rst, it
 
+    def set_guess_mlp(self, mlp):
+        """Network used by GUESS_LEARNED traces (None detaches)."""
+        _check(lib().femrt_mesh_set_guess_mlp(self.h, None if mlp is None else mlp.h),
+               self.ctx.h, "set_guess_mlp")
+        self._mlp = mlp  # keep the network alive while the mesh uses it
+
     def close(self):
         if self.h:
             lib().femrt_mesh_destroy(self.h)
             self.h = C.c_void_p()
+
+
+class Mlp:
+    """Learned Newton-guess network (27 -> 64 tanh -> 64 tanh -> 3, BF16 MFMA)."""
+
+    def __init__(self, ctx: Context, seed=0, kind=MLP_GUESS):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(lib().femrt_mlp_create(ctx.h, kind, seed, C.byref(self.h)), ctx.h, "mlp_create")
+        self.n_param = lib().femrt_mlp_num_params(kind)
+
+    def params(self):
+        out = np.zeros(self.n_param, np.float32)
+        _check(lib().femrt_mlp_get_params(self.h, _ptr(out)), self.ctx.h, "mlp_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_param
+        _check(lib().femrt_mlp_set_params(self.h, _ptr(p)), self.ctx.h, "mlp_set_params")
+
+    def forward(self, feats):
+        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, MLP_IN)
+        out = np.zeros((feats.shape[0], 3), np.float32)
+        _check(lib().femrt_mlp_forward(self.h, feats.shape[0], _ptr(feats), _ptr(out)),
+               self.ctx.h, "mlp_forward")
+        return out
+
+    def loss_grad(self, feats, targets):
+        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, MLP_IN)
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        loss = C.c_float(0)
+        grad = np.zeros(self.n_param, np.float32)
+        _check(lib().femrt_mlp_loss_grad(self.h, feats.shape[0], _ptr(feats), _ptr(targets),
+                                         C.byref(loss), _ptr(grad)), self.ctx.h, "mlp_loss_grad")
+        return float(loss.value), grad
+
+    def train(self, mesh: Mesh, n_samples=None, batch=None, epochs=None, lr=None, lr_final=None,
+              seed=None):
+        p = MlpTrainParams()
+        lib().femrt_mlp_train_params_default(C.byref(p))
+        for k, v in dict(n_samples=n_samples, batch=batch, epochs=epochs, lr=lr,
+                         lr_final=lr_final, seed=seed).items():
+            if v is not None:
+                setattr(p, k, v)
+        losses = np.zeros(p.epochs, np.float32)
+        _check(lib().femrt_mlp_train(self.h, mesh.h, C.byref(p), _ptr(losses)), self.ctx.h,
+               "mlp_train")
+        return losses
+
+    def close(self):
+        if self.h:
+            lib().femrt_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def mlp_samples(mesh: Mesh, n, seed=0):
+    """Training pairs from the tracer's point inversions: (features n x 27,
+    Newton preimages n x 3)."""
+    feats = np.zeros((n, MLP_IN), np.float32)
+    tgt = np.zeros((n, 3), np.float32)
+    _check(lib().femrt_mlp_make_samples(mesh.h, n, seed, _ptr(feats), _ptr(tgt)), mesh.ctx.h,
+           "mlp_make_samples")
+    return feats, tgt
 
 
 def trace_params(isos=(), guess=GUESS_PLAIN, samples=16, iso_enable=True, max_march=4096,

@@ -40,7 +40,8 @@ def _run(cmd, cwd=None):
 
 def build_library(verbose: bool = True) -> Path:
     OUT.mkdir(parents=True, exist_ok=True)
-    objs = []
+    objs, jobs = [], []
+    # translation units compile concurrently
     for src in sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp")):
         obj = OUT / (src.stem + ".o")
         if src.suffix == ".hip":
@@ -48,8 +49,12 @@ def build_library(verbose: bool = True) -> Path:
         else:
             cmd = [HIPCC, "-x", "c++", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
                    "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", src, "-o", obj]
-        _run(cmd)
+        print("+", " ".join(str(c) for c in cmd), flush=True)
+        jobs.append((src, subprocess.Popen([str(c) for c in cmd])))
         objs.append(obj)
+    failed = [str(src) for src, p in jobs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp, "-lz"])
     os.replace(tmp, LIB)

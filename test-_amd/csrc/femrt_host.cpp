@@ -16,7 +16,7 @@ namespace femrt {
 
 // Modified_A basis value (same definition as basis1d in femrt_math.hpp, in
 // double, used by the host fitting code).
-static double modA(int p, double x)
+double modA(int p, double x)
 {
   if (p == 0) return 0.5 * (1.0 - x);
   if (p == 1) return 0.5 * (1.0 + x);
@@ -132,8 +132,8 @@ bool modal_fit_1d_matrix(int n, const std::vector<double>& nodes, std::vector<do
 }
 
 // Lagrange interpolation matrix from nodes src (n) to nodes dst (m): m x n.
-static std::vector<double> lagrange_matrix(const std::vector<double>& src,
-                                           const std::vector<double>& dst)
+std::vector<double> lagrange_matrix(const std::vector<double>& src,
+                                    const std::vector<double>& dst)
 {
   const int n = (int)src.size(), m = (int)dst.size();
   std::vector<double> L((size_t)m * n);
@@ -149,7 +149,7 @@ static std::vector<double> lagrange_matrix(const std::vector<double>& src,
 
 // Tensor modal fit of nodal values on an n^3 grid (index (k*n+j)*n+i) to
 // modal coefficients with the same ordering.
-static void tensor_fit(int n, const std::vector<double>& inv, const double* nodal, double* out)
+void tensor_fit(int n, const std::vector<double>& inv, const double* nodal, double* out)
 {
   std::vector<double> t1((size_t)n * n * n), t2((size_t)n * n * n);
   for (int k = 0; k < n; ++k)

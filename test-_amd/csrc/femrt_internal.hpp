@@ -30,6 +30,11 @@ int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
                        int* adjacency);
 std::vector<double> gll_nodes(int n);
 bool modal_fit_1d_matrix(int n, const std::vector<double>& nodes, std::vector<double>& inv);
+double modA(int p, double x);
+std::vector<double> lagrange_matrix(const std::vector<double>& src, const std::vector<double>& dst);
+void tensor_fit(int n, const std::vector<double>& inv, const double* nodal, double* out);
+int save_xml_fld(const char* xml_path, const char* fld_path, int n, int ng, int nf,
+                 const float* vertices, const float* geom, const float* field, std::string* err);
 int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_elem, int* ng,
                  int* nf, float* vertices, float* geom, float* field, int* adjacency,
                  std::string* err);

@@ -36,159 +36,9 @@
 
 using namespace femrt;
 
-// ============================================================== host objects
+#include "femrt_mlp.hpp"
+#include "femrt_runtime.hpp"
 
-struct TimedLaunch {
-  std::string name;
-  hipEvent_t a, b;
-};
-
-struct femrt_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  bool timing = false;
-  std::vector<TimedLaunch> pending;
-  std::vector<hipEvent_t> pool;
-  std::map<std::string, std::pair<double, long long>> stats;
-};
-
-struct femrt_mesh {
-  femrt_ctx* ctx = nullptr;
-  int n_elem = 0, ng = 0, nf = 0;      // true mode counts
-  int NG = 0, NF = 0;                  // kernel mode counts (padded)
-  float4* d_geom = nullptr;            // n_elem x NG^3
-  float* d_field = nullptr;            // n_elem x NF^3
-  float* d_verts = nullptr;            // n_elem x 24
-  float* d_boxes = nullptr;            // n_elem x FEMRT_BOX_STRIDE
-  int* d_adj = nullptr;                // n_elem x 12
-  float4* d_nodes = nullptr;           // n_nodes x 2
-  int* d_prims = nullptr;
-  int n_nodes = 0;
-  int bvh_depth = 0;
-  std::vector<float> h_boxes;
-  BvhHost bvh;
-};
-
-struct femrt_frame {
-  femrt_ctx* ctx = nullptr;
-  int n = 0;
-  float* d_rays = nullptr;
-  int* d_hit_elem = nullptr;
-  int* d_hit_face = nullptr;
-  float4* d_hit_f32 = nullptr;  // u, v, tau, unused
-  double* d_hit_t = nullptr;
-  double* d_hit_rst = nullptr;  // 3 x n
-  double* d_hit_u = nullptr;
-  int* d_iso_elem = nullptr;    // MAX_ISO x n
-  double* d_iso_t = nullptr;
-  double* d_iso_rst = nullptr;  // MAX_ISO x 3 x n
-  double* d_iso_u = nullptr;
-  float4* d_iso_f32 = nullptr;  // MAX_ISO x n: (t_a, t_b, t*, packed signs)
-  float4* d_iso_rst32 = nullptr;  // MAX_ISO x n: rst* (FP32)
-  int* d_queue_hit = nullptr;     // n
-  int* d_queue_iso = nullptr;     // MAX_ISO x n, entries r*MAX_ISO+k
-  int* d_counts = nullptr;        // [0] hits [1] iso items
-  uint32_t* d_rgba = nullptr;
-  long long* d_work = nullptr;
-  bool counters = false;
-};
-
-static int fail(femrt_ctx* ctx, int code, const std::string& msg)
-{
-  if (ctx) ctx->err = msg;
-  return code;
-}
-
-#define HIPCHK(ctx, call)                                                                   \
-  do {                                                                                      \
-    hipError_t e_ = (call);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
-      return fail((ctx), FEMRT_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
-  } while (0)
-
-static hipEvent_t take_event(femrt_ctx* c)
-{
-  if (!c->pool.empty()) {
-    hipEvent_t e = c->pool.back();
-    c->pool.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  return e;
-}
-
-static void collect_timing(femrt_ctx* c)
-{
-  for (auto& t : c->pending) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
-      auto& s = c->stats[t.name];
-      s.first += ms;
-      s.second += 1;
-    }
-    c->pool.push_back(t.a);
-    c->pool.push_back(t.b);
-  }
-  c->pending.clear();
-}
-
-// Launch wrapper: records HIP events around the launch on the context stream
-// when timing is enabled.
-template <typename F>
-static int timed_launch(femrt_ctx* c, const char* name, F&& launch)
-{
-  hipEvent_t a = nullptr, b = nullptr;
-  if (c->timing) {
-    a = take_event(c);
-    b = take_event(c);
-    if (a) (void)hipEventRecord(a, c->stream);
-  }
-  launch();
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(c, FEMRT_EHIP, std::string("launch ") + name + ": " + hipGetErrorString(e));
-  if (c->timing && a && b) {
-    (void)hipEventRecord(b, c->stream);
-    c->pending.push_back({name, a, b});
-  }
-  return FEMRT_OK;
-}
-
-// ============================================================ device helpers
-
-__device__ __forceinline__ void add_work(long long* work, int idx, long long v)
-{
-  if (v) atomicAdd((unsigned long long*)&work[idx], (unsigned long long)v);
-}
-
-__device__ __forceinline__ void flush_work(long long* work, const Work& w)
-{
-  if (!work) return;
-  add_work(work, 2, w.face_it);
-  add_work(work, 3, w.inv_it);
-  add_work(work, 4, w.field);
-  add_work(work, 5, w.geo64);
-  add_work(work, 6, w.field64);
-  add_work(work, 7, w.slab);
-  add_work(work, 8, w.mlp);
-  add_work(work, 9, w.marched);
-}
-
-// Append `pred` lanes to a queue: one atomic per wave, ballot + prefix count
-// for the slot of each lane.  Must be reached by every lane of the wave.
-__device__ __forceinline__ int wave_append(bool pred, int* counter)
-{
-  const unsigned long long mask = __ballot(pred);
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (mask) {
-    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader);
-  }
-  return base + __popcll(mask & ((1ull << lane) - 1ull));
-}
 
 // ================================================================== kernels
 
@@ -274,50 +124,161 @@ __global__ __launch_bounds__(256) void k_polish_hit(
   flush_work(work, w);
 }
 
-template <int NG, int NF>
-__global__ __launch_bounds__(256) void k_march(
+// Bracket output of the walk: FP32 bracket and estimate per (isovalue, ray).
+struct GlobalSink {
+  int* elem;
+  float4* f32;
+  float4* rst32;
+  size_t n;
+  int r;
+  __device__ void put(int k, const IsoBracket& b)
+  {
+    const size_t slot = (size_t)k * n + r;
+    elem[slot] = b.elem;
+    f32[slot] = make_float4(b.ta, b.tb, b.tm, b.ga);
+    rst32[slot] = make_float4(b.rst[0], b.rst[1], b.rst[2], 0.f);
+  }
+};
+
+// Network guesses of the interior samples of the lane's current segment.
+struct LdsGuess {
+  const _Float16* g;  // (samples - 1) x 3
+  __device__ bool get(int j, float* rst) const
+  {
+    rst[0] = (float)g[(j - 1) * 3 + 0];
+    rst[1] = (float)g[(j - 1) * 3 + 1];
+    rst[2] = (float)g[(j - 1) * 3 + 2];
+    return true;
+  }
+};
+
+// Orders this wave's LDS writes before its following LDS reads (lanes read
+// what other lanes of the same wave wrote).
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+#define FEMRT_MLP_MAX_SAMPLES 16
+
+// Persistent element walk.  Each round every lane of a wave advances its ray
+// by one element segment (exit, samples, next element); a lane whose ray
+// ended takes the next hit ray from the queue (one atomic per wave), so the
+// lanes stay busy until the queue drains.  With LEARNED, the point-inversion
+// guesses of the round's interior samples come from the network, evaluated
+// on the matrix cores for the whole wave between the exit and sample steps.
+template <int NG, int NF, bool LEARNED>
+__global__ __launch_bounds__(256, 2) void k_march(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ queue,
     int* __restrict__ counts, const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
     const float4* __restrict__ hit_f32, int* __restrict__ iso_elem, float4* __restrict__ iso_f32,
-    float4* __restrict__ iso_rst32, int* __restrict__ iso_queue, long long* __restrict__ work)
+    float4* __restrict__ iso_rst32, int* __restrict__ iso_queue, long long* __restrict__ work,
+    const void* __restrict__ mlp_pack)
 {
   const int count = counts[0];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   Work w;
-  const int stride = gridDim.x * blockDim.x;
-  // every lane loops the same number of rounds so the queue append below is
-  // reached by whole waves
-  const int rounds = (count + stride - 1) / stride;
-  for (int rd = 0; rd < rounds; ++rd) {
-    const int q = rd * stride + blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = q < count;
-    unsigned found = 0;
-    int r = 0;
-    if (active) {
-      r = queue[q];
-      float o[3], d[3];
-      load_ray(rays, n, r, o, d);
-      HitF32 h;
-      h.elem = hit_elem[r];
-      h.face = hit_face[r];
-      const float4 hf = hit_f32[r];
-      h.u = hf.x;
-      h.v = hf.y;
-      h.t = hf.z;
-      IsoBracket br[FEMRT_MAX_ISO];
-      found = march_ray<NG, NF>(M, P, o, d, h, br, w);
-      for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
-        if (!((found >> k) & 1u)) continue;
-        const size_t slot = (size_t)k * n + r;
-        iso_elem[slot] = br[k].elem;
-        iso_f32[slot] = make_float4(br[k].ta, br[k].tb, br[k].tm, br[k].ga);
-        iso_rst32[slot] = make_float4(br[k].rst[0], br[k].rst[1], br[k].rst[2], 0.f);
+  GlobalSink sink{iso_elem, iso_f32, iso_rst32, (size_t)n, 0};
+  bf16x8* ops = nullptr;
+  float* bias = nullptr;
+  bf16x8* xs = nullptr;
+  _Float16* G = nullptr;
+  if constexpr (LEARNED) {
+    __shared__ bf16x8 s_ops[PK_INFER_BF16 / 8];
+    __shared__ float s_bias[PK_NF32];
+    __shared__ bf16x8 s_xs[4][64 * 4];
+    __shared__ _Float16 s_g[4][64][(FEMRT_MLP_MAX_SAMPLES - 1) * 3];
+    const bf16x8* gp = (const bf16x8*)mlp_pack;
+    for (int t = threadIdx.x; t < PK_INFER_BF16 / 8; t += blockDim.x) s_ops[t] = gp[t];
+    const float* gf = (const float*)((const __bf16*)mlp_pack + PK_NBF16);
+    for (int t = threadIdx.x; t < PK_NF32; t += blockDim.x) s_bias[t] = gf[t];
+    __syncthreads();
+    ops = s_ops;
+    bias = s_bias;
+    xs = s_xs[wave];
+    G = &s_g[wave][lane][0];
+  }
+  bool active = false, exhausted = false;
+  float o[3] = {0.f, 0.f, 0.f}, d[3] = {1.f, 1.f, 1.f}, id[3] = {1.f, 1.f, 1.f};
+  MarchState S;
+  march_begin(S, HitF32());
+  for (;;) {
+    const bool need = !active && !exhausted;
+    const int q = wave_append(need, &counts[2]);
+    if (need) {
+      if (q < count) {
+        const int r = queue[q];
+        sink.r = r;
+        load_ray(rays, n, r, o, d);
+        for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
+        HitF32 h;
+        h.elem = hit_elem[r];
+        h.face = hit_face[r];
+        const float4 hf = hit_f32[r];
+        h.u = hf.x;
+        h.v = hf.y;
+        h.t = hf.z;
+        march_begin(S, h);
+        active = true;
+      } else {
+        exhausted = true;
       }
     }
-    for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
-      if (k >= P.n_iso) break;
-      const bool pred = active && ((found >> k) & 1u);
+    if (!__any(active)) break;
+    if (active) march_exit<NG>(M, P, o, d, id, S, w);
+    if constexpr (LEARNED) {
+      // every lane of the wave is here: the network runs on the whole wave
+      const bool seg = active && S.ok;
+      const float* bx = M.boxes + (size_t)(seg ? S.e : 0) * FEMRT_BOX_STRIDE;
+      const float* vt = M.verts + (size_t)(seg ? S.e : 0) * 24;
+      for (int j = 1; j < P.samples; ++j) {
+        float f[MLP_IN];
+        if (seg) {
+          const float t = march_sample_t(S, j, P.samples);
+          const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
+          mlp_features(bx, vt, x, f);
+          ++w.mlp;
+        } else {
+          for (int k = 0; k < MLP_IN; ++k) f[k] = 0.f;
+        }
+        mlp_stage_inputs(xs, lane, f);
+        wave_lds_sync();
+        float y0[3], y1[3];
+        wave_mlp_forward(xs, ops, bias, bias + 64, y0, y1);
+        if (lane < 32) {
+          _Float16* g0 = &G[(j - 1) * 3];
+          _Float16* g1 = g0 + 32 * (FEMRT_MLP_MAX_SAMPLES - 1) * 3;
+          for (int c = 0; c < 3; ++c) {
+            g0[c] = (_Float16)y0[c];
+            g1[c] = (_Float16)y1[c];
+          }
+        }
+        wave_lds_sync();
+      }
+    }
+    bool finished = false;
+    if (active) {
+      bool redo;
+      if constexpr (LEARNED)
+        redo = S.ok && march_samples<NG, NF>(M, P, o, d, S, LdsGuess{G}, sink, w);
+      else
+        redo = S.ok && march_samples<NG, NF>(M, P, o, d, S, NoGuess(), sink, w);
+      if (redo) {
+        S.redo = true;
+      } else if (!march_advance<NG>(M, P, o, d, S, w)) {
+        active = false;
+        finished = true;
+        // a bracket written by a pass that was redone is not a result
+        for (int k = 0; k < P.n_iso; ++k)
+          if (!((S.found >> k) & 1u)) iso_elem[(size_t)k * n + sink.r] = -1;
+      }
+    }
+    for (int k = 0; k < P.n_iso; ++k) {
+      const bool pred = finished && ((S.found >> k) & 1u);
       const int slot = wave_append(pred, &counts[1]);
-      if (pred) iso_queue[slot] = r * FEMRT_MAX_ISO + k;
+      if (pred) iso_queue[slot] = sink.r * FEMRT_MAX_ISO + k;
     }
   }
   flush_work(work, w);
@@ -909,8 +870,13 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   if (m->n_nodes == 0) return fail(c, FEMRT_ESTATE, "trace: BVH not built");
   if (p->n_iso < 0 || p->n_iso > FEMRT_MAX_ISO || p->samples < 1 || p->samples > 1024)
     return fail(c, FEMRT_EINVAL, "trace: n_iso must be 0..8 and samples 1..1024");
-  if (p->guess < 0 || p->guess > FEMRT_GUESS_WARM)
-    return fail(c, FEMRT_EUNSUP, "trace: guess mode not available in this build");
+  if (p->guess < 0 || p->guess > FEMRT_GUESS_LEARNED)
+    return fail(c, FEMRT_EINVAL, "trace: unknown guess mode");
+  const bool learned = p->guess == FEMRT_GUESS_LEARNED;
+  if (learned && !m->guess_mlp)
+    return fail(c, FEMRT_ESTATE, "trace: learned guesses need a network (femrt_mesh_set_guess_mlp)");
+  if (learned && p->samples > FEMRT_MLP_MAX_SAMPLES)
+    return fail(c, FEMRT_EUNSUP, "trace: learned guesses support at most 16 samples per segment");
   HIPCHK(c, hipSetDevice(c->device));
   const int n = f->n;
   const int n_iso = p->iso_enable ? p->n_iso : 0;
@@ -959,10 +925,19 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
       });
       if (rc) break;
       if (n_iso > 0) {
+        // persistent: lanes refill from the hit queue until it drains
+        const int mblocks = dev_cus * 4;
         rc = timed_launch(c, "k_march", [&] {
-          k_march<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
-              f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
-              f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work);
+          if (learned)
+            k_march<NG_, NF_, true><<<mblocks, 256, 0, c->stream>>>(
+                f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
+                f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
+                m->guess_mlp->d_pack);
+          else
+            k_march<NG_, NF_, false><<<mblocks, 256, 0, c->stream>>>(
+                f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
+                f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
+                nullptr);
         });
         if (rc) break;
         rc = timed_launch(c, "k_polish_iso", [&] {
@@ -1051,13 +1026,29 @@ FEMRT_API int femrt_gen_structured_hex(const femrt_gen_params* gp, float* vertic
   return gen_structured_hex(gp, vertices, geom, field, adjacency);
 }
 
+static thread_local std::string g_io_err;
+
+FEMRT_API const char* femrt_io_last_error(void) { return g_io_err.c_str(); }
+
 FEMRT_API int femrt_load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_elem,
                                  int* ng, int* nf, float* vertices, float* geom, float* field,
                                  int* adjacency)
 {
-  if (!xml_path || !fld_path || !n_elem || !ng || !nf) return FEMRT_EINVAL;
+  g_io_err.clear();
+  if (!xml_path || !n_elem || !ng || !nf) {
+    g_io_err = "load_xml_fld: null argument";
+    return FEMRT_EINVAL;
+  }
   return load_xml_fld(xml_path, fld_path, flags, n_elem, ng, nf, vertices, geom, field, adjacency,
-                      nullptr);
+                      &g_io_err);
+}
+
+FEMRT_API int femrt_save_xml_fld(const char* xml_path, const char* fld_path, int n_elem, int ng,
+                                 int nf, const float* vertices, const float* geom,
+                                 const float* field)
+{
+  g_io_err.clear();
+  return save_xml_fld(xml_path, fld_path, n_elem, ng, nf, vertices, geom, field, &g_io_err);
 }
 
 }  // extern "C"

@@ -1,0 +1,182 @@
+// femrt_runtime.hpp -- host objects behind the C ABI handles and the launch
+// helpers shared by the library's HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "femrt_internal.hpp"
+#include "femrt_trace_core.hpp"
+
+using namespace femrt;
+
+// ============================================================== host objects
+
+struct TimedLaunch {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+struct femrt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool timing = false;
+  std::vector<TimedLaunch> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long long>> stats;
+};
+
+// Guess network: FP32 master parameters, Adam moments and gradient
+// accumulators, and the packed BF16 operands the kernels read.
+struct femrt_mlp {
+  femrt_ctx* ctx = nullptr;
+  int kind = 0;
+  float* d_param = nullptr;   // MLP_NPARAM
+  float* d_m = nullptr;       // Adam first moment
+  float* d_v = nullptr;       // Adam second moment
+  float* d_grad = nullptr;    // MLP_NPARAM gradient accumulators + [MLP_NPARAM] loss
+  void* d_pack = nullptr;     // PK_BYTES
+  long long adam_t = 0;
+};
+
+struct femrt_mesh {
+  femrt_ctx* ctx = nullptr;
+  const femrt_mlp* guess_mlp = nullptr;  // network used by FEMRT_GUESS_LEARNED
+  int n_elem = 0, ng = 0, nf = 0;      // true mode counts
+  int NG = 0, NF = 0;                  // kernel mode counts (padded)
+  float4* d_geom = nullptr;            // n_elem x NG^3
+  float* d_field = nullptr;            // n_elem x NF^3
+  float* d_verts = nullptr;            // n_elem x 24
+  float* d_boxes = nullptr;            // n_elem x FEMRT_BOX_STRIDE
+  int* d_adj = nullptr;                // n_elem x 12
+  float4* d_nodes = nullptr;           // n_nodes x 2
+  int* d_prims = nullptr;
+  int n_nodes = 0;
+  int bvh_depth = 0;
+  std::vector<float> h_boxes;
+  BvhHost bvh;
+};
+
+struct femrt_frame {
+  femrt_ctx* ctx = nullptr;
+  int n = 0;
+  float* d_rays = nullptr;
+  int* d_hit_elem = nullptr;
+  int* d_hit_face = nullptr;
+  float4* d_hit_f32 = nullptr;  // u, v, tau, unused
+  double* d_hit_t = nullptr;
+  double* d_hit_rst = nullptr;  // 3 x n
+  double* d_hit_u = nullptr;
+  int* d_iso_elem = nullptr;    // MAX_ISO x n
+  double* d_iso_t = nullptr;
+  double* d_iso_rst = nullptr;  // MAX_ISO x 3 x n
+  double* d_iso_u = nullptr;
+  float4* d_iso_f32 = nullptr;  // MAX_ISO x n: (t_a, t_b, t*, packed signs)
+  float4* d_iso_rst32 = nullptr;  // MAX_ISO x n: rst* (FP32)
+  int* d_queue_hit = nullptr;     // n
+  int* d_queue_iso = nullptr;     // MAX_ISO x n, entries r*MAX_ISO+k
+  int* d_counts = nullptr;        // [0] hits [1] iso items
+  uint32_t* d_rgba = nullptr;
+  long long* d_work = nullptr;
+  bool counters = false;
+};
+
+static int fail(femrt_ctx* ctx, int code, const std::string& msg)
+{
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((ctx), FEMRT_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+static hipEvent_t take_event(femrt_ctx* c)
+{
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static void collect_timing(femrt_ctx* c)
+{
+  for (auto& t : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      auto& s = c->stats[t.name];
+      s.first += ms;
+      s.second += 1;
+    }
+    c->pool.push_back(t.a);
+    c->pool.push_back(t.b);
+  }
+  c->pending.clear();
+}
+
+// Launch wrapper: records HIP events around the launch on the context stream
+// when timing is enabled.
+template <typename F>
+static int timed_launch(femrt_ctx* c, const char* name, F&& launch)
+{
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = take_event(c);
+    b = take_event(c);
+    if (a) (void)hipEventRecord(a, c->stream);
+  }
+  launch();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, FEMRT_EHIP, std::string("launch ") + name + ": " + hipGetErrorString(e));
+  if (c->timing && a && b) {
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({name, a, b});
+  }
+  return FEMRT_OK;
+}
+
+// ============================================================ device helpers
+
+__device__ __forceinline__ void add_work(long long* work, int idx, long long v)
+{
+  if (v) atomicAdd((unsigned long long*)&work[idx], (unsigned long long)v);
+}
+
+__device__ __forceinline__ void flush_work(long long* work, const Work& w)
+{
+  if (!work) return;
+  add_work(work, 2, w.face_it);
+  add_work(work, 3, w.inv_it);
+  add_work(work, 4, w.field);
+  add_work(work, 5, w.geo64);
+  add_work(work, 6, w.field64);
+  add_work(work, 7, w.slab);
+  add_work(work, 8, w.mlp);
+  add_work(work, 9, w.marched);
+}
+
+// Append `pred` lanes to a queue: one atomic per wave, ballot + prefix count
+// for the slot of each lane.  Must be reached by every lane of the wave.
+__device__ __forceinline__ int wave_append(bool pred, int* counter)
+{
+  const unsigned long long mask = __ballot(pred);
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (mask) {
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+  }
+  return base + __popcll(mask & ((1ull << lane) - 1ull));
+}

@@ -403,14 +403,21 @@ FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* r
 // The preimage of x inside the element (unique for a valid element): Newton
 // from the guess in rst; when that converges outside the cube (Newton from
 // rst = 0 finds spurious preimages of the polynomial map for points near a
-// face) or fails, restart from the 8 octant centres (+-0.5)^3.
+// face) or fails, restart from rst = 0 (when zero_retry: the plain fallback
+// of a learned guess), then from the 8 octant centres (+-0.5)^3.
 template <int NG>
 FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
-                                 long long& iters)
+                                 long long& iters, bool zero_retry = false)
 {
   if (invert_f32<NG>(gc, x, rst, J, iters) &&
       fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
     return true;
+  if (zero_retry) {
+    rst[0] = rst[1] = rst[2] = 0.f;
+    if (invert_f32<NG>(gc, x, rst, J, iters) &&
+        fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
+      return true;
+  }
   for (int s = 0; s < 8; ++s) {
     rst[0] = (s & 1) ? 0.5f : -0.5f;
     rst[1] = (s & 2) ? 0.5f : -0.5f;
@@ -451,7 +458,7 @@ FEMRT_HD bool invert_f64(const float4* __restrict__ gc, const double* x, double*
   return false;
 }
 
-// ------------------------------------------------------------ element walk
+// ------------------------------------------------------------ segment exit
 
 // Exit of the ray from element e: nearest accepted face root with
 // t > t_lo on the faces other than the entry face (and on the entry face too
@@ -637,164 +644,239 @@ FEMRT_HD bool iso_newton(const float4* __restrict__ gc, const float* __restrict_
   return false;
 }
 
-// Walk the hit ray through face-adjacent elements, sampling g = u - iso at
-// P.samples uniform sub-intervals of each element segment [t_in, t_out].
-// The first sub-interval with g(prev) g(cur) <= 0 of each isovalue is
-// recorded.  Returns the mask of isovalues found.
-template <int NG, int NF>
-FEMRT_HD unsigned march_ray(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
-                            HitF32 h, IsoBracket* out, Work& w)
+// ------------------------------------------------------------- element walk
+//
+// The walk of one hit ray through face-adjacent elements, sampling
+// g = u - iso at P.samples uniform sub-intervals of each element segment
+// [t_in, t_out]; the first sub-interval with g(prev) g(cur) <= 0 of each
+// isovalue is refined and recorded (fem/RayIsosurface.cu:67-188 does the
+// same scan over the padded box of one element).  It is split in three
+// steps per segment -- exit, samples, advance -- so the kernel can run them
+// in wave-uniform rounds (and fetch a new ray into a lane whose ray ended)
+// with the batched network guesses in between.
+
+// Per-ray state of the walk between rounds.
+struct MarchState {
+  int e, f_in;            // current element and its entry face
+  float t_in, u_in, v_in;
+  int f_out;              // exit of the current segment (valid when ok)
+  float t_out, u_out, v_out;
+  int step;
+  unsigned found;         // isovalues whose bracket has been recorded
+  bool ok;
+  bool redo;              // samples showed a wrong exit: search again, entry face included
+};
+
+FEMRT_HD void march_begin(MarchState& S, const HitF32& h)
 {
-  float id[3];
-  for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
-  int e = h.elem, f_in = h.face;
-  float t_in = h.t, u_in = h.u, v_in = h.v;
-  unsigned found = 0;
-  const unsigned all = (1u << P.n_iso) - 1u;
-  for (int step = 0; step < P.max_march && found != all; ++step) {
+  S.e = h.elem;
+  S.f_in = h.face;
+  S.t_in = h.t;
+  S.u_in = h.u;
+  S.v_in = h.v;
+  S.f_out = -1;
+  S.t_out = S.u_out = S.v_out = 0.f;
+  S.step = 0;
+  S.found = 0;
+  S.ok = false;
+  S.redo = false;
+}
+
+// Exit of the current segment.  First pass: predicted exit (engineered and
+// learned guesses), then the face search without and with the entry face (a
+// ray can leave through the face it entered by: a curved boundary face seen
+// at a grazing angle), then marching.  After a redo: the search including the
+// entry face, then marching.
+template <int NG>
+FEMRT_HD void march_exit(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
+                         const float* id, MarchState& S, Work& w)
+{
+  const float4* gc = M.geom + (size_t)S.e * (NG * NG * NG);
+  const float* bx = M.boxes + (size_t)S.e * FEMRT_BOX_STRIDE;
+  const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
+  // exits closer than eps to the entry are not resolved in FP32
+  const float t_lo = S.t_in + 1e-4f * ext;
+  float rst_in[3];
+  face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
+  bool ok = false;
+  if (!S.redo) {
     ++w.marched;
-    const float4* gc = M.geom + (size_t)e * (NG * NG * NG);
-    const float* fc = M.field + (size_t)e * (NF * NF * NF);
-    const float* bx = M.boxes + (size_t)e * FEMRT_BOX_STRIDE;
-    const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
-    // exits closer than eps to the entry are not resolved in FP32
-    const float eps = 1e-4f * ext;
-    const float t_lo = t_in + eps;
-    float rst_in[3];
-    face_to_rst(f_in, u_in, v_in, rst_in[0], rst_in[1], rst_in[2]);
-    int f_out = -1;
-    float t_out = 0.f, u_out = 0.f, v_out = 0.f;
-    bool ok = false;
     if (P.guess != FEMRT_GUESS_PLAIN)
-      ok = find_exit_predicted<NG>(gc, o, d, f_in, rst_in, t_in, t_lo, f_out, t_out, u_out, v_out, w);
+      ok = find_exit_predicted<NG>(gc, o, d, S.f_in, rst_in, S.t_in, t_lo, S.f_out, S.t_out, S.u_out,
+                                   S.v_out, w);
     if (!ok)
-      ok = find_exit_search<NG>(gc, bx, o, d, id, f_in, t_lo, P.guess, false, f_out, t_out, u_out, v_out, w);
-    // A ray can leave through the face it entered by (a curved boundary face
-    // seen at a grazing angle); the entry face is searched when no other face
-    // holds an exit, or when a sample shows the segment leaves the element.
-    if (!ok)
-      ok = find_exit_search<NG>(gc, bx, o, d, id, f_in, t_lo, P.guess, true, f_out, t_out, u_out, v_out, w);
-    if (!ok)
-      ok = find_exit_marching<NG>(gc, bx, o, d, id, rst_in, t_in, t_lo, f_out, t_out, u_out, v_out, w);
-    if (ok) {
-      const int Ms = P.samples;
-      for (int pass = 0; pass < 2; ++pass) {
-        bool redo = false;
-        float gprev[FEMRT_MAX_ISO];
-        float rst[3] = {rst_in[0], rst_in[1], rst_in[2]}, rprev[3] = {0.f, 0.f, 0.f};
-        float tprev = t_in;
-        bool have_prev = false;
-        unsigned found_here = 0;
-        IsoBracket br[FEMRT_MAX_ISO];
-        for (int j = 0; j <= Ms; ++j) {
-          const float t = j == Ms ? t_out : t_in + (t_out - t_in) * ((float)j / (float)Ms);
-          const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
-          float J[9];
-          // the segment ends lie on faces whose (u, v) the face roots give:
-          // evaluate there, never by extrapolating past the face
-          bool known = false;
-          if (j == 0) {
-            rst[0] = rst_in[0];
-            rst[1] = rst_in[1];
-            rst[2] = rst_in[2];
-            known = true;
-          } else if (j == Ms) {
-            face_to_rst(f_out, u_out, v_out, rst[0], rst[1], rst[2]);
-            known = true;
-          } else if (P.guess == FEMRT_GUESS_PLAIN) {
-            rst[0] = rst[1] = rst[2] = 0.f;
-          }
-          if (!known && !invert_in_cube_f32<NG>(gc, x, rst, J, w.inv_it)) {
-            // no preimage inside the element: the segment end is wrong
-            if (pass == 0) {
-              redo = true;
-              break;
-            }
-            have_prev = false;
-            rst[0] = rst[1] = rst[2] = 0.f;
-            continue;
-          }
-          const float u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
-          ++w.field;
-          for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
-            if (k >= P.n_iso || ((found | found_here) >> k) & 1u) continue;
-            const float g = u - P.iso[k];
-            if (have_prev && gprev[k] * g <= 0.f) {
-              br[k].elem = e;
-              br[k].ta = tprev;
-              br[k].tb = t;
-              br[k].ga = gprev[k];
-              // FP32 coupled Newton from the bracket midpoint; the FP64
-              // endgame starts from its result (or from the midpoint when it
-              // leaves the bracket)
-              float tm = 0.5f * (tprev + t), rm[3], um;
-              for (int c = 0; c < 3; ++c) rm[c] = 0.5f * (rprev[c] + rst[c]);
-              const float ofs[3] = {o[0], o[1], o[2]}, dfs[3] = {d[0], d[1], d[2]};
-              // FP32 resolves rst to ~1e-7 |x| / |J| (~1e-5 on small elements)
-              if (!iso_newton<float, NG, NF>(gc, fc, ofs, dfs, P.iso[k], tprev, t, 6, 1e-6f, 1e-3f, tm,
-                                            rm, um, w.inv_it, w.field)) {
-                tm = 0.5f * (tprev + t);
-                for (int c = 0; c < 3; ++c) rm[c] = 0.5f * (rprev[c] + rst[c]);
-              }
-              br[k].tm = tm;
-              for (int c = 0; c < 3; ++c) br[k].rst[c] = rm[c];
-              found_here |= 1u << k;
-            }
-            gprev[k] = g;
-          }
-          rprev[0] = rst[0];
-          rprev[1] = rst[1];
-          rprev[2] = rst[2];
-          tprev = t;
-          have_prev = true;
+      ok = find_exit_search<NG>(gc, bx, o, d, id, S.f_in, t_lo, P.guess, false, S.f_out, S.t_out,
+                                S.u_out, S.v_out, w);
+  }
+  if (!ok)
+    ok = find_exit_search<NG>(gc, bx, o, d, id, S.f_in, t_lo, P.guess, true, S.f_out, S.t_out,
+                              S.u_out, S.v_out, w);
+  if (!ok)
+    ok = find_exit_marching<NG>(gc, bx, o, d, id, rst_in, S.t_in, t_lo, S.f_out, S.t_out, S.u_out,
+                                S.v_out, w);
+  S.ok = ok;
+}
+
+// Ray parameter of sample j of the current segment (j = 0 .. samples).
+FEMRT_HD float march_sample_t(const MarchState& S, int j, int samples)
+{
+  return j == samples ? S.t_out : S.t_in + (S.t_out - S.t_in) * ((float)j / (float)samples);
+}
+
+// Guess sources for the interior samples' point inversions.
+struct NoGuess {
+  FEMRT_HD bool get(int, float*) const { return false; }
+};
+
+// Samples of the current segment (S.ok).  Brackets found are handed to
+// sink.put(k, bracket) and marked in S.found.  Returns true when a sample
+// has no preimage inside the element on the first pass (the exit is wrong:
+// the caller sets S.redo and finds the exit again).
+template <int NG, int NF, typename Guess, typename Sink>
+FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
+                            MarchState& S, const Guess& guess, Sink& sink, Work& w)
+{
+  const float4* gc = M.geom + (size_t)S.e * (NG * NG * NG);
+  const float* fc = M.field + (size_t)S.e * (NF * NF * NF);
+  const int Ms = P.samples;
+  float rst_in[3];
+  face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
+  float gprev[FEMRT_MAX_ISO];
+  float rst[3] = {rst_in[0], rst_in[1], rst_in[2]}, rprev[3] = {0.f, 0.f, 0.f};
+  float tprev = S.t_in;
+  bool have_prev = false;
+  unsigned found_here = 0;
+  for (int j = 0; j <= Ms; ++j) {
+    const float t = march_sample_t(S, j, Ms);
+    const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
+    float J[9];
+    // the segment ends lie on faces whose (u, v) the face roots give:
+    // evaluate there, never by extrapolating past the face
+    bool known = false;
+    bool zero_retry = false;
+    if (j == 0) {
+      rst[0] = rst_in[0];
+      rst[1] = rst_in[1];
+      rst[2] = rst_in[2];
+      known = true;
+    } else if (j == Ms) {
+      face_to_rst(S.f_out, S.u_out, S.v_out, rst[0], rst[1], rst[2]);
+      known = true;
+    } else if (guess.get(j, rst)) {
+      zero_retry = true;  // plain Newton from 0 is the fallback of a learned guess
+    } else if (P.guess == FEMRT_GUESS_PLAIN) {
+      rst[0] = rst[1] = rst[2] = 0.f;
+    }
+    if (!known && !invert_in_cube_f32<NG>(gc, x, rst, J, w.inv_it, zero_retry)) {
+      // no preimage inside the element: the segment end is wrong
+      if (!S.redo) return true;
+      have_prev = false;
+      rst[0] = rst[1] = rst[2] = 0.f;
+      continue;
+    }
+    const float u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
+    ++w.field;
+    for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
+      if (k >= P.n_iso || ((S.found | found_here) >> k) & 1u) continue;
+      const float g = u - P.iso[k];
+      if (have_prev && gprev[k] * g <= 0.f) {
+        IsoBracket br;
+        br.elem = S.e;
+        br.ta = tprev;
+        br.tb = t;
+        br.ga = gprev[k];
+        // FP32 coupled Newton from the bracket midpoint; the FP64 endgame
+        // starts from its result (or from the midpoint when it leaves the
+        // bracket)
+        float tm = 0.5f * (tprev + t), rm[3], um;
+        for (int c = 0; c < 3; ++c) rm[c] = 0.5f * (rprev[c] + rst[c]);
+        // FP32 resolves rst to ~1e-7 |x| / |J| (~1e-5 on small elements)
+        if (!iso_newton<float, NG, NF>(gc, fc, o, d, P.iso[k], tprev, t, 6, 1e-6f, 1e-3f, tm, rm,
+                                       um, w.inv_it, w.field)) {
+          tm = 0.5f * (tprev + t);
+          for (int c = 0; c < 3; ++c) rm[c] = 0.5f * (rprev[c] + rst[c]);
         }
-        if (redo) {
-          if (!find_exit_search<NG>(gc, bx, o, d, id, f_in, t_lo, P.guess, true, f_out, t_out, u_out,
-                                    v_out, w) &&
-              !find_exit_marching<NG>(gc, bx, o, d, id, rst_in, t_in, t_lo, f_out, t_out, u_out, v_out,
-                                      w)) {
-            ok = false;
-            break;
-          }
-          continue;
-        }
-        for (int k = 0; k < FEMRT_MAX_ISO; ++k)
-          if ((found_here >> k) & 1u) out[k] = br[k];
-        found |= found_here;
-        break;
+        br.tm = tm;
+        for (int c = 0; c < 3; ++c) br.rst[c] = rm[c];
+        sink.put(k, br);
+        found_here |= 1u << k;
       }
+      gprev[k] = g;
     }
-    if (found == all) break;
-    int ne = -1, nfc = -1;
-    if (ok && M.adj) {
-      ne = M.adj[(size_t)e * 12 + f_out * 2];
-      nfc = M.adj[(size_t)e * 12 + f_out * 2 + 1];
-    }
+    rprev[0] = rst[0];
+    rprev[1] = rst[1];
+    rprev[2] = rst[2];
+    tprev = t;
+    have_prev = true;
+  }
+  S.found |= found_here;
+  return false;
+}
+
+// Next segment: the face neighbour of the exit, else (boundary face, no
+// adjacency, or no exit resolved) the next element the ray meets, by the
+// same traversal as the first hit.  Returns false when the ray is done.
+template <int NG>
+FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
+                            MarchState& S, Work& w)
+{
+  const unsigned all = (1u << P.n_iso) - 1u;
+  S.redo = false;
+  if (S.found == all || ++S.step >= P.max_march) return false;
+  if (S.ok && M.adj) {
+    const int ne = M.adj[(size_t)S.e * 12 + S.f_out * 2];
+    const int nfc = M.adj[(size_t)S.e * 12 + S.f_out * 2 + 1];
     if (ne >= 0) {
       // entry point on the neighbour's face: the shared face is parametrised
       // alike on both sides in a conforming mesh of aligned elements; the
       // walk only uses it to seed the next exit prediction
-      e = ne;
-      f_in = nfc;
-      t_in = t_out;
-      u_in = u_out;
-      v_in = v_out;
+      S.e = ne;
+      S.f_in = nfc;
+      S.t_in = S.t_out;
+      S.u_in = S.u_out;
+      S.v_in = S.v_out;
+      return true;
+    }
+  }
+  const float t_from = S.ok ? S.t_out : S.t_in;
+  const HitF32 h2 =
+      first_hit_ray<NG>(M, o, d, P.guess, t_from + 1e-6f * (1.f + fabsf(t_from)), 1e30f, w, S.e);
+  if (h2.elem < 0) return false;
+  S.e = h2.elem;
+  S.f_in = h2.face;
+  S.t_in = h2.t;
+  S.u_in = h2.u;
+  S.v_in = h2.v;
+  return true;
+}
+
+struct BracketArray {
+  IsoBracket* out;
+  FEMRT_HD void put(int k, const IsoBracket& b) { out[k] = b; }
+};
+
+// The whole walk of one ray, one segment after another (host harnesses and
+// single-lane use).  Returns the mask of isovalues found.
+template <int NG, int NF>
+FEMRT_HD unsigned march_ray(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
+                            HitF32 h, IsoBracket* out, Work& w)
+{
+  if (P.n_iso <= 0 || P.max_march <= 0) return 0;
+  float id[3];
+  for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
+  MarchState S;
+  march_begin(S, h);
+  BracketArray sink{out};
+  const NoGuess ng;
+  for (;;) {
+    march_exit<NG>(M, P, o, d, id, S, w);
+    if (S.ok && march_samples<NG, NF>(M, P, o, d, S, ng, sink, w)) {
+      S.redo = true;
       continue;
     }
-    // the ray left the mesh (boundary face, no adjacency, or a segment too
-    // short to resolve): re-enter where it next meets another element, by the
-    // same traversal as the first hit
-    const float t_from = ok ? t_out : t_in;
-    const HitF32 h2 =
-        first_hit_ray<NG>(M, o, d, P.guess, t_from + 1e-6f * (1.f + fabsf(t_from)), 1e30f, w, e);
-    if (h2.elem < 0) break;
-    e = h2.elem;
-    f_in = h2.face;
-    t_in = h2.t;
-    u_in = h2.u;
-    v_in = h2.v;
+    if (!march_advance<NG>(M, P, o, d, S, w)) break;
   }
-  return found;
+  return S.found;
 }
 
 // FP64 endgame of one isovalue bracket: point inversion in FP64 (warm from
