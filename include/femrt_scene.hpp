@@ -1,0 +1,137 @@
+// femrt_scene.hpp -- C++ host layer with the reference's entry points for the
+// hex path, above the C ABI of femrt.h.
+//
+// Mirrors FEM-AABB/raytracer.h (Camera :22-49, Scene :53-309 hex subset,
+// save_image_ppm :334) and FEM-AABB/xml_fld_loader.h:14-16 (load_xml_fld),
+// with the same names, argument meaning and error behaviour (bool returns,
+// messages on stderr, no exceptions).  What differs by design: device memory
+// belongs to C-ABI handles owned by the Scene (never raw pointers patched
+// into host structs, main.cu:362-420), and rendering is a host call,
+// render(), instead of the render_kernel launch (raytracer.h:313-331).
+// Vector types (float3, uint3, uchar4) are HIP's host-side definitions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "femrt.h"
+
+#define FEMRT_CXX_API __attribute__((visibility("default")))
+
+namespace zidingyi {
+// Hex element record (FEM-AABB/fem/HexElement.cuh:8-21): corner vertices in
+// XML/VTK order, modal field and geometry coefficients with their mode counts.
+struct HexElementData {
+  float3 vertices[8];
+  const float* fieldCoefficients;
+  uint3 fieldModes;
+  const float* geomCoefficients[3];
+  uint3 geomModes;
+  float3 aabbMin;
+  float3 aabbMax;
+};
+}  // namespace zidingyi
+
+// Pinhole camera (raytracer.h:22-49); the lens is ignored (the reference
+// renders with aperture 0).
+struct FEMRT_CXX_API Camera {
+  float3 origin;
+  float3 lower_left_corner;
+  float3 horizontal;
+  float3 vertical;
+  float3 u, v, w;
+  float lens_radius;
+
+  Camera() {}
+  Camera(float3 lookfrom, float3 lookat, float3 vup, float vfov, float aspect_ratio, float aperture,
+         float focus_dist);
+};
+
+// Per-pixel results that the reference's HitRecord does not carry: the hit
+// element, face, distance, reference coordinates and field value, and the
+// first crossing of each isovalue (element -1 when none).
+struct HitAOV {
+  int elem, face;
+  double t, rst[3], u;
+  int iso_elem[FEMRT_MAX_ISO];
+  double iso_t[FEMRT_MAX_ISO];
+};
+
+struct FEMRT_CXX_API Scene {
+  struct FieldSlice {
+    std::string name;
+    bool is3D{true};
+    uint3 modes3{0, 0, 0};
+    uint2 modes2{0, 0};
+    size_t count{0};
+    int offset{-1};
+  };
+
+  // pooled host copies, as the reference keeps them (raytracer.h:118-166)
+  std::vector<zidingyi::HexElementData> hexes;
+  std::vector<float> hex_coefficients;
+  std::vector<int> hex_coeff_offsets;
+  std::vector<float> hex_geom_coefficients;
+  std::vector<int> hex_geom_offsets;
+  std::vector<std::vector<FieldSlice>> hex_field_slices;
+
+  Camera camera;
+  int width{0}, height{0};
+  int samples_per_pixel{1};
+  int max_depth{1};
+  float iso_value{0.0f};
+  bool use_iso{true};
+  // further isovalues searched after iso_value (up to FEMRT_MAX_ISO in all)
+  std::vector<double> extra_iso_values;
+
+  Scene() = default;
+  Scene(const Scene&) = delete;
+  Scene& operator=(const Scene&) = delete;
+  ~Scene();
+
+  void add_hex(const zidingyi::HexElementData& h, const float* fieldCoeffs = nullptr,
+               uint3 fieldModes = {0, 0, 0}, size_t field_count = 0,
+               const float* geomX = nullptr, const float* geomY = nullptr,
+               const float* geomZ = nullptr, uint3 geomModes = {0, 0, 0}, size_t geom_count = 0,
+               const std::vector<FieldSlice>* slices = nullptr);
+
+  // Uploads the hexes (uniform mode counts) to the GPU and builds the BVH.
+  // On failure prints to stderr and leaves the scene unbuilt (render then
+  // returns false).
+  void build_bvh();
+
+  // device objects (owned)
+  femrt_ctx* ctx{nullptr};
+  femrt_mesh* mesh{nullptr};
+  femrt_mlp* guess_mlp{nullptr};
+  int device{0};
+};
+
+struct RenderParams {
+  int guess{FEMRT_GUESS_WARM};   // FEMRT_GUESS_*; LEARNED needs Scene::guess_mlp
+  int samples{16};               // iso samples per element segment
+  int max_march{4096};
+};
+
+// Drop-in of load_xml_fld (xml_fld_loader.h:14-16): appends the hexes of a
+// Nektar++ XML mesh with its FLD field to the scene.
+FEMRT_CXX_API bool load_xml_fld(const std::string& xml_path, const std::string& fld_path, Scene& scene);
+
+// Traces one pixel-centre ray per pixel (y = 0 is the bottom row, as the
+// reference's image buffer): geometry first hit, the scene's isovalues,
+// shading into out (width * height).  aov (may be null) receives the
+// per-pixel results.  Synchronous.
+FEMRT_CXX_API bool render(Scene& scene, const Camera& cam, int width, int height, std::vector<uchar4>& out,
+            std::vector<HitAOV>* aov = nullptr, const RenderParams* params = nullptr);
+
+// The rays render() traces: pixel centres (x + 0.5, y + 0.5) of the camera
+// (raytracer.cu:431-443 without the random jitter), unit directions; 6 x
+// (width * height) SoA floats (ox, oy, oz, dx, dy, dz).
+FEMRT_CXX_API void camera_rays(const Camera& cam, int width, int height, std::vector<float>& rays);
+
+// P6 writer (raytracer.cu:480-494).
+FEMRT_CXX_API void save_image_ppm(const std::vector<uchar4>& image, int width, int height,
+                    const std::string& filename);

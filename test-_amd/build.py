@@ -17,6 +17,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 LIB = OUT / "libfemrt.so"
+RENDER = OUT / "femrt_render"
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "_build" / "liboracle.so"
 
@@ -58,6 +59,11 @@ def build_library(verbose: bool = True) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp, "-lz"])
     os.replace(tmp, LIB)
+    # command-line renderer over the C++ host layer (links the library)
+    tmp = RENDER.with_suffix(".tmp")
+    _run([HIPCC, "-x", "c++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+          PKG / "femrt_render.cpp", "-o", tmp, f"-L{OUT}", "-lfemrt", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, RENDER)
     return LIB
 
 

@@ -175,6 +175,54 @@ void tensor_fit(int n, const std::vector<double>& inv, const double* nodal, doub
       }
 }
 
+// ---------------------------------------------------------- face adjacency
+
+// Device form of the face adjacency: per (element, local face) the neighbour
+// element and (neighbour face | orientation << 3), where the orientation o
+// maps the face coordinates (u, v) of this side to the neighbour's:
+// swap when o & 1, then negate u' when o & 2 and v' when o & 4.  It is found
+// from the corner positions (corner c of face f sits at (u, v) = (-1,-1),
+// (1,-1), (1,1), (-1,1) for c = 0..3).  A pair whose corners match under no
+// orientation is dropped (the walk then finds the next element through the
+// BVH).
+void encode_adjacency(int n_elem, const float* vertices, const int* adjacency, int* out)
+{
+  static const int fc[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                               {3, 2, 6, 7}, {0, 3, 7, 4}, {4, 5, 6, 7}};
+  static const int uv[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  for (int e = 0; e < n_elem; ++e)
+    for (int f = 0; f < 6; ++f) {
+      const int ne = adjacency[((size_t)e * 6 + f) * 2], nf = adjacency[((size_t)e * 6 + f) * 2 + 1];
+      out[((size_t)e * 6 + f) * 2] = -1;
+      out[((size_t)e * 6 + f) * 2 + 1] = -1;
+      if (ne < 0 || ne >= n_elem || nf < 0 || nf > 5) continue;
+      const float* P = vertices + (size_t)e * 24;
+      const float* Q = vertices + (size_t)ne * 24;
+      double scale = 0.0;
+      for (int c = 0; c < 4; ++c)
+        for (int k = 0; k < 3; ++k)
+          scale = std::max(scale, std::fabs((double)P[3 * fc[f][c] + k] - P[3 * fc[f][0] + k]));
+      for (int o = 0; o < 8; ++o) {
+        bool all = true;
+        for (int c = 0; c < 4 && all; ++c) {
+          int u = uv[c][0], v = uv[c][1];
+          if (o & 1) std::swap(u, v);
+          if (o & 2) u = -u;
+          if (o & 4) v = -v;
+          int c2 = 0;
+          while (uv[c2][0] != u || uv[c2][1] != v) ++c2;
+          for (int k = 0; k < 3; ++k)
+            if (std::fabs((double)P[3 * fc[f][c] + k] - Q[3 * fc[nf][c2] + k]) > 1e-5 * scale) all = false;
+        }
+        if (all) {
+          out[((size_t)e * 6 + f) * 2] = ne;
+          out[((size_t)e * 6 + f) * 2 + 1] = nf | (o << 3);
+          break;
+        }
+      }
+    }
+}
+
 // ------------------------------------------------------------ random numbers
 
 static inline uint64_t mix64(uint64_t z)

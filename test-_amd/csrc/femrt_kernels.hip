@@ -583,8 +583,11 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
             hipMemcpy(m->d_field, fp.data(), fp.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_verts, vertices, (size_t)n_elem * 24 * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_boxes, m->h_boxes.data(), m->h_boxes.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
-  if (ok && adjacency)
-    ok = hipMemcpy(m->d_adj, adjacency, (size_t)n_elem * 12 * 4, hipMemcpyHostToDevice) == hipSuccess;
+  if (ok && adjacency) {
+    std::vector<int> adj((size_t)n_elem * 12);
+    encode_adjacency(n_elem, vertices, adjacency, adj.data());
+    ok = hipMemcpy(m->d_adj, adj.data(), adj.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+  }
   if (!ok) return cleanup("hipMemcpy mesh");
   *out = m;
   return FEMRT_OK;

@@ -256,7 +256,9 @@ std::string b64_encode(const uint8_t* d, size_t n)
   return out;
 }
 
-std::vector<double> parse_numbers(const std::string& s)
+// Whitespace-separated numbers from the start of s; *used (may be NULL)
+// receives the offset where parsing stopped.
+std::vector<double> parse_numbers(const std::string& s, size_t* used = nullptr)
 {
   std::vector<double> v;
   const char* p = s.c_str();
@@ -267,6 +269,7 @@ std::vector<double> parse_numbers(const std::string& s)
     v.push_back(x);
     p = e;
   }
+  if (used) *used = (size_t)(p - s.c_str());
   return v;
 }
 
@@ -433,7 +436,18 @@ int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_e
       if (!fld_modes[0]) std::copy(bm, bm + 3, fld_modes);
       const int nfield = attr(h, "FIELDS", fields) ? (int)std::count(fields.begin(), fields.end(), ',') + 1 : 1;
       std::vector<double> vals;
-      if (attr(h, "COMPRESSED", comp) && upper(comp).find("B64Z") != std::string::npos) {
+      // ASCII numbers, else base64 + zlib (files of older writers omit the
+      // COMPRESSED attribute)
+      bool ascii = !(attr(h, "COMPRESSED", comp) && upper(comp).find("B64Z") != std::string::npos);
+      if (ascii) {
+        size_t used = 0;
+        vals = parse_numbers(b, &used);
+        if (b.find_first_not_of(" \t\r\n", used) != std::string::npos) {
+          ascii = false;
+          vals.clear();
+        }
+      }
+      if (!ascii) {
         std::vector<uint8_t> raw;
         if (!inflate_all(b64_decode(b), raw)) {
           ok = false;
@@ -452,8 +466,6 @@ int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_e
             vals.push_back(d);
           }
         }
-      } else {
-        vals = parse_numbers(b);
       }
       std::vector<int> idl = attr(h, "ID", ids) ? ranges(ids) : std::vector<int>();
       // first field of the block: element-major, n^3 coefficients each

@@ -825,16 +825,23 @@ FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float*
   if (S.found == all || ++S.step >= P.max_march) return false;
   if (S.ok && M.adj) {
     const int ne = M.adj[(size_t)S.e * 12 + S.f_out * 2];
-    const int nfc = M.adj[(size_t)S.e * 12 + S.f_out * 2 + 1];
+    const int code = M.adj[(size_t)S.e * 12 + S.f_out * 2 + 1];
     if (ne >= 0) {
-      // entry point on the neighbour's face: the shared face is parametrised
-      // alike on both sides in a conforming mesh of aligned elements; the
-      // walk only uses it to seed the next exit prediction
+      // entry point on the neighbour's face: the exit (u, v) in the
+      // neighbour face's own coordinates (orientation in the upper bits)
+      float u = S.u_out, v = S.v_out;
+      if (code & 8) {
+        const float tmp = u;
+        u = v;
+        v = tmp;
+      }
+      if (code & 16) u = -u;
+      if (code & 32) v = -v;
       S.e = ne;
-      S.f_in = nfc;
+      S.f_in = code & 7;
       S.t_in = S.t_out;
-      S.u_in = S.u_out;
-      S.v_in = S.v_out;
+      S.u_in = u;
+      S.v_in = v;
       return true;
     }
   }

@@ -1,0 +1,86 @@
+"""The C++ host layer (include/femrt_scene.hpp) through its command-line
+renderer, the counterpart of the reference's main program: the load path on
+the CPU against the Python binding of the same loader, and on the GPU a
+render of the reference's Hex_CurvedFace fixture and of a generated mesh
+written with the FLD writer, checked against the FP64 oracle on the
+renderer's own rays."""
+from __future__ import annotations
+
+import json
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+GOLD = ROOT / "tests" / "golden"
+RENDER = ROOT / "test-_amd" / "_build" / "femrt_render"
+
+
+def _run(*args, timeout=120):
+    return subprocess.run([str(RENDER), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+
+
+def test_cpp_loader_matches_python_binding(femrt):
+    r = _run(GOLD / "Hex_CurvedFace.xml", GOLD / "Hex_CurvedFace.fld", "unused.ppm", "--load-only")
+    assert r.returncode == 0, r.stderr
+    tok = r.stdout.split()
+    vals = dict(zip(tok[0::2], tok[1::2]))
+    hm = femrt.load_xml_fld(GOLD / "Hex_CurvedFace.xml", GOLD / "Hex_CurvedFace.fld")
+    assert int(vals["hexes"]) == hm.n_elem
+    assert int(vals["geom_modes"]) == hm.ng and int(vals["field_modes"]) == hm.nf
+    assert float(vals["geom_sum"]) == pytest.approx(float(hm.geom.astype(np.float64).sum()), rel=1e-7)
+    assert float(vals["field_sum"]) == pytest.approx(float(hm.field.astype(np.float64).sum()), rel=1e-7)
+    bad = _run(GOLD / "missing.xml", GOLD / "Hex_CurvedFace.fld", "x.ppm", "--load-only")
+    assert bad.returncode == 1 and "cannot read" in bad.stderr
+
+
+AOV = np.dtype([("elem", "<i4"), ("face", "<i4"), ("t", "<f8"), ("rst", "<f8", 3), ("u", "<f8"),
+                ("iso_elem", "<i4", 8), ("iso_t", "<f8", 8)], align=True)
+
+
+def _render_and_compare(femrt, oracle, tmp_path, xml, fld, W, H, isos):
+    import oracle.parity as parity
+    ppm, aov, rays = tmp_path / "o.ppm", tmp_path / "o.aov", tmp_path / "o.rays"
+    r = _run(xml, fld, ppm, W, H, *isos, "--aov", aov, "--rays", rays)
+    assert r.returncode == 0, r.stderr
+    assert ppm.read_bytes().startswith(f"P6\n{W} {H}\n255\n".encode())
+    a = np.fromfile(aov, AOV)
+    ray = np.fromfile(rays, np.float32).reshape(6, W * H)
+    assert a.size == W * H
+    hm = femrt.load_xml_fld(xml, fld)
+    og, oiso = oracle.Oracle.from_mesh(hm).trace(ray, isos)
+    K = len(isos)
+    gh = dict(elem=a["elem"], face=a["face"], t=a["t"], rst=a["rst"].T.copy(), u=a["u"])
+    # the AOV carries the iso element and distance; value and rst are checked
+    # through the Python path elsewhere, here the distance and element
+    giso = dict(elem=a["iso_elem"][:, :K].T.copy(), t=a["iso_t"][:, :K].T.copy(),
+                rst=np.zeros((K, 3, W * H)), u=np.tile(np.array(isos)[:, None], (1, W * H)))
+    rep = parity.compare(gh, giso, og, oiso, isos)
+    print(json.dumps(rep))
+    return rep
+
+
+@pytest.mark.gpu
+def test_cpp_render_hex_curved_face(femrt, oracle, tmp_path):
+    rep = _render_and_compare(femrt, oracle, tmp_path, GOLD / "Hex_CurvedFace.xml",
+                              GOLD / "Hex_CurvedFace.fld", 96, 64, [0.2, 0.5])
+    assert rep["oracle_hits"] > 500
+    for r in rep["iso"]:
+        assert r["found_mismatch"] == 0 and r["t_rel_bad_nontie"] == 0 and r["elem_mismatch"] == 0
+    assert rep["hit_miss_mismatch"] == 0 and rep["id_mismatch"] == 0
+    assert rep["t_rel_max"] <= 1e-6 and rep["rst_abs_max"] <= 1e-7
+
+
+@pytest.mark.gpu
+def test_cpp_render_written_mesh(femrt, oracle, tmp_path):
+    hm = femrt.gen_structured_hex(4, 4, 5, pert=2, amp=0.08, seed=123)
+    xml, fld = tmp_path / "g.xml", tmp_path / "g.fld"
+    femrt.save_xml_fld(hm, xml, fld)
+    rep = _render_and_compare(femrt, oracle, tmp_path, xml, fld, 64, 48, [-0.1, 0.0, 0.1])
+    assert rep["oracle_hits"] > 300
+    for r in rep["iso"]:
+        assert r["found_mismatch"] == 0 and r["t_rel_bad_nontie"] == 0 and r["elem_mismatch"] == 0
+    assert rep["hit_miss_mismatch"] == 0 and rep["id_mismatch"] == 0
+    assert rep["t_rel_max"] <= 1e-6 and rep["rst_abs_max"] <= 1e-7

@@ -95,3 +95,42 @@ def test_trace_three_isovalues_curved_q4(femrt, oracle, gpu_ctx):
     print(json.dumps(rep, indent=1))
     assert rep["oracle_hits"] > 100
     assert rep["ok"], rep
+
+
+def test_walk_across_rotated_face_orientations(femrt, oracle, gpu_ctx, tmp_path):
+    """Elements whose local frames are rotated against their neighbours (as
+    Nektar++ meshes have them): the walk maps the exit face coordinates into
+    the neighbour's face frame.  3 x 1 x 2 trilinear hexes read by the XML
+    loader, every other element's corner list rotated about its t axis."""
+    nx, nz = 3, 2
+    pid = lambda i, j, k: (k * 2 + j) * (nx + 1) + i  # noqa: E731
+    verts = [(i / nx * 1.0, j * 0.5, k * 0.5) for k in range(nz + 1) for j in range(2) for i in range(nx + 1)]
+    hexes = []
+    for k in range(nz):
+        for i in range(nx):
+            h = [pid(i, 0, k), pid(i + 1, 0, k), pid(i + 1, 1, k), pid(i, 1, k),
+                 pid(i, 0, k + 1), pid(i + 1, 0, k + 1), pid(i + 1, 1, k + 1), pid(i, 1, k + 1)]
+            if (i + k) % 2:
+                h = h[1:4] + h[0:1] + h[5:8] + h[4:5]   # rotate the local frame by 90 degrees
+            hexes.append(h)
+    vx = "\n".join(f'<V ID="{n}"> {x} {y} {z} </V>' for n, (x, y, z) in enumerate(verts))
+    el = "\n".join(f'<H ID="{n}"> {" ".join(map(str, h))} </H>' for n, h in enumerate(hexes))
+    xml = tmp_path / "rot.xml"
+    xml.write_text(f'<NEKTAR><GEOMETRY DIM="3" SPACE="3"><VERTEX>{vx}</VERTEX><ELEMENT>{el}</ELEMENT>'
+                   f'</GEOMETRY><EXPANSIONS><E COMPOSITE="C[0]" NUMMODES="4" FIELDS="u"/></EXPANSIONS></NEKTAR>')
+    hm = femrt.load_xml_fld(xml, None)
+    # every shared face seen from both sides
+    assert (hm.adjacency[..., 0] >= 0).sum() == 2 * (nz * (nx - 1) + (nz - 1) * nx)
+    # a smooth field shared across faces: the modal coefficients of
+    # u = X(rst) . (0.7, -0.4, 1.1) per element (linear in the vertices)
+    g = hm.geom.reshape(hm.n_elem, 3, hm.ng ** 3)
+    lin = 0.7 * g[:, 0] - 0.4 * g[:, 1] + 1.1 * g[:, 2]
+    f = np.zeros((hm.n_elem, 4, 4, 4), np.float32)
+    f[:, :2, :2, :2] = lin.reshape(hm.n_elem, 2, 2, 2)
+    hm.field = np.ascontiguousarray(f.reshape(hm.n_elem, -1))
+    isos = [0.35, 0.6, 0.9]
+    rep = _trace_and_compare(femrt, oracle, gpu_ctx, hm, 64, 48, isos, 1)
+    print(json.dumps(rep, indent=1))
+    assert rep["oracle_hits"] > 300
+    assert all(r["gpu_found"] > 50 for r in rep["iso"])
+    assert rep["ok"], rep
