@@ -166,8 +166,11 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
 /* Work counters of the last trace (validation runs): [0] rays, [1] hit rays,
  * [2] face-Newton iterations FP32, [3] point-inversion iterations FP32,
  * [4] field evaluations FP32, [5] FP64 geometry iterations, [6] FP64 field
- * evaluations, [7] slab tests, [8] MLP inferences, [9] elements marched. */
-#define FEMRT_NCOUNTERS 10
+ * evaluations, [7] slab tests, [8] MLP inferences, [9] elements marched,
+ * [10] segment exits found by prediction, [11] by face search, [12] by
+ * marching, [13] BVH re-entries of the walk, [14] redone segments,
+ * [15] isovalue brackets found. */
+#define FEMRT_NCOUNTERS 16
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
 
 /* ------------------------------------------------- learned Newton guess */

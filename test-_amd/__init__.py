@@ -20,9 +20,10 @@ OK = 0
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "ESTATE"}
 GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
 MAX_ISO = 8
-NCOUNTERS = 10
+NCOUNTERS = 16
 COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",
-                 "geo64_it", "field64_evals", "slab_tests", "mlp_infer", "elems_marched"]
+                 "geo64_it", "field64_evals", "slab_tests", "mlp_infer", "elems_marched",
+                 "exit_pred", "exit_search", "exit_march", "reentry", "redo", "brackets"]
 
 
 class FemrtError(RuntimeError):

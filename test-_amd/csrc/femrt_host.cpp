@@ -519,11 +519,18 @@ void bernstein_boxes(int n_elem, int ng, const float* geom, float* out)
         }
       }
     }
+    // Face roots are accepted up to |u|, |v| <= 1 + 1e-4, a band just outside
+    // the face; every box also covers that band: 2e-4 of the element extent
+    // bounds 1e-4 |dx/du| for elements whose faces are not stretched beyond
+    // twice their extent.
+    double eext = 0;
+    for (int c = 0; c < 3; ++c) eext = std::max(eext, hi[0][c] - lo[0][c]);
     for (int b = 0; b < FEMRT_NBOX; ++b) {
       double ext = 0;
       for (int c = 0; c < 3; ++c) ext = std::max(ext, hi[b][c] - lo[b][c]);
       for (int c = 0; c < 3; ++c) {
-        const double pad = 1e-6 * ext + 1e-6 * std::max(std::fabs(lo[b][c]), std::fabs(hi[b][c])) + 1e-9;
+        const double pad = 2e-4 * eext + 1e-6 * ext +
+                           1e-6 * std::max(std::fabs(lo[b][c]), std::fabs(hi[b][c])) + 1e-9;
         out[((size_t)e * FEMRT_NBOX + b) * 6 + c] = std::nextafter((float)(lo[b][c] - pad), -INFINITY);
         out[((size_t)e * FEMRT_NBOX + b) * 6 + 3 + c] = std::nextafter((float)(hi[b][c] + pad), INFINITY);
       }

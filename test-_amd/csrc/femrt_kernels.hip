@@ -402,7 +402,7 @@ __global__ void k_invert(int n, const int* __restrict__ elem, const float* __res
   const float4* gc = geom + (size_t)e * (NG * NG * NG);
   const float xf[3] = {x[p], x[n + p], x[2 * n + p]};
   float rst[3] = {0.f, 0.f, 0.f}, J[9];
-  long long it = 0, it64 = 0;
+  unsigned it = 0, it64 = 0;
   const bool ok = invert_in_cube_f32<NG>(gc, xf, rst, J, it);
   double rd[3] = {rst[0], rst[1], rst[2]}, Jd[9];
   const double xd[3] = {xf[0], xf[1], xf[2]};

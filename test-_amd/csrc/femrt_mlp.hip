@@ -285,7 +285,7 @@ __global__ void k_mlp_samples(TraceMesh M, int n_elem, int n, unsigned long long
   float x[3];
   geom_point<float, NG>(gc, rt[0], rt[1], rt[2], x);
   float rst[3] = {0.f, 0.f, 0.f}, J[9];
-  long long it = 0;
+  unsigned it = 0;
   const bool ok = invert_in_cube_f32<NG>(gc, x, rst, J, it);
   float f[MLP_IN];
   mlp_features(M.boxes + (size_t)e * FEMRT_BOX_STRIDE, M.verts + (size_t)e * 24, x, f);

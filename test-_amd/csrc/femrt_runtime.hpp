@@ -164,6 +164,12 @@ __device__ __forceinline__ void flush_work(long long* work, const Work& w)
   add_work(work, 7, w.slab);
   add_work(work, 8, w.mlp);
   add_work(work, 9, w.marched);
+  add_work(work, 10, w.exit_pred);
+  add_work(work, 11, w.exit_search);
+  add_work(work, 12, w.exit_march);
+  add_work(work, 13, w.reentry);
+  add_work(work, 14, w.redo);
+  add_work(work, 15, w.brackets);
 }
 
 // Append `pred` lanes to a queue: one atomic per wave, ballot + prefix count

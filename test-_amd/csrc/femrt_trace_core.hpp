@@ -38,9 +38,14 @@ struct IsoParams {
 };
 
 // Work counters accumulated per lane.
+// (32-bit per lane: a lane's counts stay far below 2^32; the kernels add them
+// to 64-bit totals)
 struct Work {
-  long long face_it = 0, inv_it = 0, field = 0, geo64 = 0, field64 = 0, slab = 0, mlp = 0,
-            marched = 0;
+  unsigned face_it = 0, inv_it = 0, field = 0, geo64 = 0, field64 = 0, slab = 0, mlp = 0,
+           marched = 0;
+  // element-walk diagnostics: exits found by prediction / face search /
+  // marching, re-entries through the BVH, redone segments, iso brackets
+  unsigned exit_pred = 0, exit_search = 0, exit_march = 0, reentry = 0, redo = 0, brackets = 0;
 };
 
 // FP32 result of the first-hit stage.
@@ -104,7 +109,7 @@ FEMRT_HD void rst_to_face(int f, const float* rst, float& u, float& v)
 template <int NG>
 FEMRT_HD int face_newton(const float4* __restrict__ gc, const float* o, const float* d, int f,
                          float u, float v, float tau, float tmin, float tmax, float& tau_o,
-                         float& u_o, float& v_o, long long& iters)
+                         float& u_o, float& v_o, unsigned& iters)
 {
   float prev = FLT_MAX;
   const float oscale = 1.f + fmaxf(fabsf(o[0]), fmaxf(fabsf(o[1]), fabsf(o[2])));
@@ -170,7 +175,7 @@ FEMRT_HD float seed_tau(const float4* __restrict__ gc, const float* o, const flo
 template <int NG>
 FEMRT_HD bool face_root_grid(const float4* __restrict__ gc, const float* o, const float* d, int f,
                              float tmin, float tmax, float& tau_o, float& u_o, float& v_o,
-                             long long& iters)
+                             unsigned& iters)
 {
   bool found = false;
   float best = FLT_MAX;
@@ -382,7 +387,7 @@ FEMRT_HD void polish_hit(const float4* __restrict__ gc, const double* o, const d
 // the last evaluated iterate.
 template <int NG>
 FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
-                         long long& iters)
+                         unsigned& iters)
 {
   for (int it = 0; it < 50; ++it) {
     float X[3];
@@ -407,7 +412,7 @@ FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* r
 // of a learned guess), then from the 8 octant centres (+-0.5)^3.
 template <int NG>
 FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
-                                 long long& iters, bool zero_retry = false)
+                                 unsigned& iters, bool zero_retry = false)
 {
   if (invert_f32<NG>(gc, x, rst, J, iters) &&
       fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
@@ -434,7 +439,7 @@ FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, 
 // of x(rst) - x divided by |J|).  On success J is evaluated at the result.
 template <int NG>
 FEMRT_HD bool invert_f64(const float4* __restrict__ gc, const double* x, double* rst, double* J,
-                         long long& iters)
+                         unsigned& iters)
 {
   double prev = 1e300;
   for (int it = 0; it < 30; ++it) {
@@ -560,9 +565,10 @@ FEMRT_HD bool find_exit_marching(const float4* __restrict__ gc, const float* __r
 }
 
 // Exit predicted from the entry point: the reference-space line
-// rst_in + s J^{-1}(rst_in) d leaves the cube [-1,1]^3 through face f* at
-// s*; a face Newton on f* seeded there gives the exit.  Returns false when
-// the prediction does not converge to an accepted root.
+// rst_in + s J^{-1}(rst_in) d meets the planes of the cube faces (the entry
+// face excluded) at s_f; a face Newton on each crossed face, nearest crossing
+// first (at most 3), seeded at the predicted point, gives the exit.  Returns
+// false when none converges to an accepted root.
 template <int NG>
 FEMRT_HD bool find_exit_predicted(const float4* __restrict__ gc, const float* o, const float* d,
                                   int f_in, const float* rst_in, float t_in, float t_lo, int& f_out,
@@ -572,36 +578,44 @@ FEMRT_HD bool find_exit_predicted(const float4* __restrict__ gc, const float* o,
   geom_eval<float, NG>(gc, rst_in[0], rst_in[1], rst_in[2], X, J);
   ++w.face_it;
   if (!solve3<float>(J, d, dl)) return false;
-  // first crossing of the cube boundary by rst_in + s dl, s > 0, skipping the
-  // entry face
-  float s_best = FLT_MAX;
-  int f_best = -1;
+  // crossings of the three planes the line heads for
+  float sc[3];
+  int fc[3];
   for (int c = 0; c < 3; ++c) {
+    sc[c] = FLT_MAX;
+    fc[c] = -1;
     if (dl[c] == 0.f) continue;
     const float target = dl[c] > 0.f ? 1.f : -1.f;
-    const float s = (target - rst_in[c]) / dl[c];
+    const float sv = (target - rst_in[c]) / dl[c];
     const int f = c == 0 ? (target > 0 ? 2 : 4) : (c == 1 ? (target > 0 ? 3 : 1) : (target > 0 ? 5 : 0));
-    if (f == f_in || !(s > 0.f)) continue;
-    if (s < s_best) {
-      s_best = s;
-      f_best = f;
+    if (f == f_in || !(sv > 0.f)) continue;
+    sc[c] = sv;
+    fc[c] = f;
+  }
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    int cb = -1;
+    for (int c = 0; c < 3; ++c)
+      if (fc[c] >= 0 && (cb < 0 || sc[c] < sc[cb])) cb = c;
+    if (cb < 0) return false;
+    const int f = fc[cb];
+    const float sv = sc[cb];
+    fc[cb] = -1;
+    float pr[3];
+    for (int c = 0; c < 3; ++c) pr[c] = rst_in[c] + sv * dl[c];
+    float su, sw;
+    rst_to_face(f, pr, su, sw);
+    su = fminf(fmaxf(su, -1.f), 1.f);
+    sw = fminf(fmaxf(sw, -1.f), 1.f);
+    float tt, uu, vv;
+    if (face_newton<NG>(gc, o, d, f, su, sw, t_in + sv, t_lo, 1e30f, tt, uu, vv, w.face_it) == 1) {
+      f_out = f;
+      t_out = tt;
+      u_out = uu;
+      v_out = vv;
+      return true;
     }
   }
-  if (f_best < 0) return false;
-  float pr[3];
-  for (int c = 0; c < 3; ++c) pr[c] = rst_in[c] + s_best * dl[c];
-  float su, sv;
-  rst_to_face(f_best, pr, su, sv);
-  su = fminf(fmaxf(su, -1.5f), 1.5f);
-  sv = fminf(fmaxf(sv, -1.5f), 1.5f);
-  float tt, uu, vv;
-  if (face_newton<NG>(gc, o, d, f_best, su, sv, t_in + s_best, t_lo, 1e30f, tt, uu, vv, w.face_it) != 1)
-    return false;
-  f_out = f_best;
-  t_out = tt;
-  u_out = uu;
-  v_out = vv;
-  return true;
+  return false;
 }
 
 // Coupled Newton for the isosurface point on the ray: unknowns (r, s, t) and
@@ -614,7 +628,7 @@ FEMRT_HD bool find_exit_predicted(const float4* __restrict__ gc, const float* o,
 template <typename T, int NG, int NF>
 FEMRT_HD bool iso_newton(const float4* __restrict__ gc, const float* __restrict__ fc, const T* o,
                          const T* d, T iso, T lo, T hi, int max_it, T tol_t, T tol_r, T& tau, T* rst,
-                         T& u, long long& geo_it, long long& field_it)
+                         T& u, unsigned& geo_it, unsigned& field_it)
 {
   for (int it = 0; it < max_it; ++it) {
     T X[3], J[9], g[3];
@@ -701,19 +715,29 @@ FEMRT_HD void march_exit(const TraceMesh& M, const IsoParams& P, const float* o,
   bool ok = false;
   if (!S.redo) {
     ++w.marched;
-    if (P.guess != FEMRT_GUESS_PLAIN)
+    if (P.guess != FEMRT_GUESS_PLAIN) {
       ok = find_exit_predicted<NG>(gc, o, d, S.f_in, rst_in, S.t_in, t_lo, S.f_out, S.t_out, S.u_out,
                                    S.v_out, w);
-    if (!ok)
+      w.exit_pred += ok;
+    }
+    if (!ok) {
       ok = find_exit_search<NG>(gc, bx, o, d, id, S.f_in, t_lo, P.guess, false, S.f_out, S.t_out,
                                 S.u_out, S.v_out, w);
+      w.exit_search += ok;
+    }
+  } else {
+    ++w.redo;
   }
-  if (!ok)
+  if (!ok) {
     ok = find_exit_search<NG>(gc, bx, o, d, id, S.f_in, t_lo, P.guess, true, S.f_out, S.t_out,
                               S.u_out, S.v_out, w);
-  if (!ok)
+    w.exit_search += ok;
+  }
+  if (!ok) {
     ok = find_exit_marching<NG>(gc, bx, o, d, id, rst_in, S.t_in, t_lo, S.f_out, S.t_out, S.u_out,
                                 S.v_out, w);
+    w.exit_march += ok;
+  }
   S.ok = ok;
 }
 
@@ -785,20 +809,13 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
         br.ta = tprev;
         br.tb = t;
         br.ga = gprev[k];
-        // FP32 coupled Newton from the bracket midpoint; the FP64 endgame
-        // starts from its result (or from the midpoint when it leaves the
-        // bracket)
-        float tm = 0.5f * (tprev + t), rm[3], um;
-        for (int c = 0; c < 3; ++c) rm[c] = 0.5f * (rprev[c] + rst[c]);
-        // FP32 resolves rst to ~1e-7 |x| / |J| (~1e-5 on small elements)
-        if (!iso_newton<float, NG, NF>(gc, fc, o, d, P.iso[k], tprev, t, 6, 1e-6f, 1e-3f, tm, rm,
-                                       um, w.inv_it, w.field)) {
-          tm = 0.5f * (tprev + t);
-          for (int c = 0; c < 3; ++c) rm[c] = 0.5f * (rprev[c] + rst[c]);
-        }
-        br.tm = tm;
-        for (int c = 0; c < 3; ++c) br.rst[c] = rm[c];
+        // the FP64 endgame starts from the bracket midpoint (the root's
+        // refinement runs there on a compact queue, one bracket per lane,
+        // instead of stalling the walk's wave for the few lanes with a bracket)
+        br.tm = 0.5f * (tprev + t);
+        for (int c = 0; c < 3; ++c) br.rst[c] = 0.5f * (rprev[c] + rst[c]);
         sink.put(k, br);
+        ++w.brackets;
         found_here |= 1u << k;
       }
       gprev[k] = g;
@@ -846,6 +863,7 @@ FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float*
     }
   }
   const float t_from = S.ok ? S.t_out : S.t_in;
+  ++w.reentry;
   const HitF32 h2 =
       first_hit_ray<NG>(M, o, d, P.guess, t_from + 1e-6f * (1.f + fabsf(t_from)), 1e30f, w, S.e);
   if (h2.elem < 0) return false;
@@ -900,7 +918,7 @@ FEMRT_HD bool polish_iso(const TraceMesh& M, const double* o, const double* d, d
   const float* fc = M.field + (size_t)br.elem * (NF * NF * NF);
   {
     double tau = br.tm, r4[3] = {br.rst[0], br.rst[1], br.rst[2]}, u4;
-    if (iso_newton<double, NG, NF>(gc, fc, o, d, iso, (double)br.ta, (double)br.tb, 6, 1e-13, 1e-9,
+    if (iso_newton<double, NG, NF>(gc, fc, o, d, iso, (double)br.ta, (double)br.tb, 10, 1e-13, 1e-9,
                                    tau, r4, u4, w.geo64, w.field64)) {
       t_out = tau;
       for (int c = 0; c < 3; ++c) rst_out[c] = r4[c];

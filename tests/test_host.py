@@ -170,6 +170,18 @@ def test_bernstein_boxes_contain_dense_samples(femrt, oracle, pert, amp, q):
                             continue
                         sb = boxes[e, 7 + 9 * f + 3 * b + a]
                         assert np.all(x >= sb[:3]) and np.all(x <= sb[3:]), (e, f, a, b)
+            # face roots are accepted up to |u|, |v| <= 1 + 1e-4: the band just
+            # outside the face lies in the element, face and edge sub-patch boxes
+            acc = 1 + 1e-4
+            for u, v in [(acc, 0.3), (-acc, -0.7), (0.1, acc), (-0.4, -acc), (acc, acc), (-acc, acc)]:
+                rst = np.zeros(3)
+                rst[fix] = 1 if side else -1
+                rst[ua], rst[va] = u, v
+                x, _ = orc.map(e, rst)
+                a = min(2, int((np.clip(u, -1, 1) + 1) * 1.5))
+                b = min(2, int((np.clip(v, -1, 1) + 1) * 1.5))
+                for bx in (boxes[e, 0], boxes[e, 1 + f], boxes[e, 7 + 9 * f + 3 * b + a]):
+                    assert np.all(x >= bx[:3]) and np.all(x <= bx[3:]), (e, f, u, v)
 
 
 def test_bvh_invariants(femrt):
