@@ -121,8 +121,8 @@ def parity_sample(femrt, orc_mod, parity, hm, fr, n_iso, n_check=192):
     return parity.compare(sub(gh), sub(giso), og, oiso, ISOS[:n_iso])
 
 
-def run_variant(femrt, ctx, mesh, fr, guess, steps, warmup):
-    prm = femrt.trace_params(ISOS, guess=guess)
+def run_variant(femrt, ctx, mesh, fr, guess, steps, warmup, flags=1):
+    prm = femrt.trace_params(ISOS, guess=guess, flags=flags)
     for _ in range(warmup):
         fr.trace(mesh, prm)
     ctx.sync()
@@ -156,6 +156,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--persistent-walk", action="store_true",
+                    help="walk each ray to its end in one persistent kernel instead of "
+                         "element-binned rounds")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -197,13 +200,15 @@ def main():
                     "guess_err_p99_on_bench_mesh": float(np.percentile(err, 99))}
         print(json.dumps({"mlp": mlp_info}), file=sys.stderr, flush=True)
     for v in variants:
-        wall, kt, cnt = run_variant(femrt, ctx, mesh, fr, names[v], args.steps, args.warmup)
+        flags = 1 | (2 if args.persistent_walk else 0)
+        wall, kt, cnt = run_variant(femrt, ctx, mesh, fr, names[v], args.steps, args.warmup, flags)
         f32, f64 = kernel_flops(cnt, hm.ng, hm.nf)
         results[v] = dict(wall=wall, kernels=kt, counters=cnt, f32=f32, f64=f64,
                           mrays=args.steps * n_rays / wall / 1e6)
         print(json.dumps({"variant": v, "mrays_s": results[v]["mrays"],
                           "ms_per_step": wall / args.steps * 1e3,
-                          "kernels_ms_per_step": {k: a / max(1, c // 1) for k, (a, c) in kt.items()},
+                          "kernels_ms_per_step": {k: a / args.steps for k, (a, c) in kt.items()},
+                          "launches_per_step": {k: c / args.steps for k, (a, c) in kt.items()},
                           "counters": cnt}), file=sys.stderr, flush=True)
 
     head = "learned" if "learned" in results else ("warm" if "warm" in results else variants[0])
@@ -211,7 +216,10 @@ def main():
     # dominant kernel roofline: the kernel with the largest time share
     kt = R["kernels"]
     dom = max(kt, key=lambda k: kt[k][0])
-    dom_ms = kt[dom][0] / kt[dom][1]
+    # the walk runs as one launch per round: flops of the frame over the
+    # kernel's time per frame (= flops per launch over the mean launch time)
+    dom_ms = kt[dom][0] / args.steps
+    dom_launches = kt[dom][1] / args.steps
     # FP32 flops per launch attributed to the dominant kernel by its counters
     fm = flops_model(hm.ng, hm.nf)
     cnt = R["counters"]
@@ -225,11 +233,12 @@ def main():
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": measured_traffic(dom, head),
             "frac_of_unpacked_fp32": achieved / (PEAK_FP32_TFLOPS / 2),
-            "kernel_ms_per_launch": dom_ms, "flops_per_launch": fl}
+            "kernel_ms_per_step": dom_ms, "launches_per_step": dom_launches,
+            "flops_per_step": fl}
     if cnt.get("mlp_infer"):
         # network inferences of the walk, on the BF16 matrix cores
         mf = cnt["mlp_infer"] * MLP_FLOPS
-        roof["mlp_bf16"] = {"flops_per_launch": mf, "achieved": mf / (dom_ms * 1e-3) / 1e12,
+        roof["mlp_bf16"] = {"flops_per_step": mf, "achieved": mf / (dom_ms * 1e-3) / 1e12,
                             "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s"}
 
     check = None

@@ -136,8 +136,12 @@ typedef struct {
   float t_min;                   /* 0.001 in the reference (raytracer.cu:221)     */
   float t_max;
   int max_march;                 /* element-walk step cap per ray                  */
-  int flags;                     /* bit0: write RGBA                              */
+  int flags;                     /* FEMRT_TRACE_* bits                            */
 } femrt_trace_params;
+
+#define FEMRT_TRACE_RGBA 1             /* shade into the frame's RGBA                 */
+#define FEMRT_TRACE_PERSISTENT_WALK 2  /* one persistent kernel walks each ray to its
+                                          end instead of element-binned rounds     */
 
 FEMRT_API void femrt_trace_params_default(femrt_trace_params* p);
 
@@ -172,6 +176,44 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
  * [15] isovalue brackets found. */
 #define FEMRT_NCOUNTERS 16
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
+
+/* ------------------------------------------------ shading and bounces */
+/* Materials of scatter (raytracer.cu:61-97): type 0 Lambertian (cosine-
+ * weighted), 1 metal (roughness = fuzz), 2 dielectric (ior); color is the
+ * attenuation, emission ends a path with color * emission.  A geometry hit on
+ * local face 0 takes material 1, other faces material 0 (raytracer.cu:248). */
+typedef struct {
+  float color[3];
+  float emission[3];
+  float roughness;
+  float ior;
+  int type;
+} femrt_material;
+
+/* Jittered primary rays of a pinhole camera (render_kernel,
+ * raytracer.cu:431-443): the frame must hold width*height*spp rays; path
+ * p = (y*width + x)*spp + s goes through ((x + u)/width, (y + v)/height)
+ * with u, v drawn from a counter-based generator keyed by (seed, p).
+ * cam12: origin, lower_left_corner, horizontal, vertical (raytracer.h:22-49). */
+FEMRT_API int femrt_frame_camera_rays_jitter(femrt_frame* f, const float* cam12, int width,
+                                             int height, int spp, unsigned long long seed);
+/* Path state: throughput 1 and radiance 0 for every ray of the frame. */
+FEMRT_API int femrt_frame_path_begin(femrt_frame* f);
+/* One bounce after a geometry trace of the frame (femrt_trace, iso stage
+ * off): misses add throughput * sky (raytracer.cu:398-402) and end; hits
+ * scatter off the surface (normal from the hit face's tangents, outward and
+ * front face from the element's orientation) into the frame's rays for the
+ * next trace.  depth numbers the bounce (0 = primary hit); last = 1 ends the
+ * paths still alive with black (the depth cap of ray_color). */
+FEMRT_API int femrt_frame_scatter(femrt_mesh* mesh, femrt_frame* f, const femrt_material* mats,
+                                  int n_mats, unsigned long long seed, int depth, int last);
+/* Mean radiance over each pixel's samples, gamma sqrt, clamp, 8-bit RGBA
+ * (raytracer.cu:460-475); host output width*height. */
+FEMRT_API int femrt_frame_resolve(femrt_frame* f, int width, int height, int spp, uint32_t* rgba);
+/* camera_rays_jitter + path_begin + max_depth x (trace + scatter) + resolve. */
+FEMRT_API int femrt_render_path(femrt_mesh* mesh, femrt_frame* f, const float* cam12, int width,
+                                int height, int spp, int max_depth, const femrt_material* mats,
+                                int n_mats, unsigned long long seed, uint32_t* rgba);
 
 /* ------------------------------------------------- learned Newton guess */
 /* No counterpart in the reference (its Newton solves start from fixed seeds,

@@ -78,8 +78,14 @@ struct FEMRT_CXX_API Scene {
   std::vector<int> hex_geom_offsets;
   std::vector<std::vector<FieldSlice>> hex_field_slices;
 
+  // materials (raytracer.h Material subset as femrt_material); empty means
+  // the reference's defaults, grey and blue Lambertian (main.cu:39-41)
+  std::vector<femrt_material> materials;
+  void add_material(const femrt_material& m) { materials.push_back(m); }
+
   Camera camera;
   int width{0}, height{0};
+  // samples_per_pixel > 1 or max_depth > 1 selects path tracing in render()
   int samples_per_pixel{1};
   int max_depth{1};
   float iso_value{0.0f};
@@ -114,6 +120,7 @@ struct RenderParams {
   int guess{FEMRT_GUESS_WARM};   // FEMRT_GUESS_*; LEARNED needs Scene::guess_mlp
   int samples{16};               // iso samples per element segment
   int max_march{4096};
+  unsigned long long seed{1};    // path tracing: random stream
 };
 
 // Drop-in of load_xml_fld (xml_fld_loader.h:14-16): appends the hexes of a
@@ -123,7 +130,10 @@ FEMRT_CXX_API bool load_xml_fld(const std::string& xml_path, const std::string& 
 // Traces one pixel-centre ray per pixel (y = 0 is the bottom row, as the
 // reference's image buffer): geometry first hit, the scene's isovalues,
 // shading into out (width * height).  aov (may be null) receives the
-// per-pixel results.  Synchronous.
+// per-pixel results.  With scene.samples_per_pixel > 1 or scene.max_depth > 1
+// it path-traces the mesh surface instead (jittered samples, material
+// scatter, sky; render_kernel, raytracer.cu:410-476) and aov must be null.
+// Synchronous.
 FEMRT_CXX_API bool render(Scene& scene, const Camera& cam, int width, int height, std::vector<uchar4>& out,
             std::vector<HitAOV>* aov = nullptr, const RenderParams* params = nullptr);
 

@@ -46,6 +46,50 @@ MLP_NPARAM = 6147
 MLP_IN = 27
 
 
+class Material(C.Structure):
+    _fields_ = [("color", C.c_float * 3), ("emission", C.c_float * 3), ("roughness", C.c_float),
+                ("ior", C.c_float), ("type", C.c_int)]
+
+
+LAMBERT, METAL, DIELECTRIC = 0, 1, 2
+
+
+def materials(specs):
+    """[(type, color, emission, roughness, ior), ...] -> ctypes array."""
+    arr = (Material * len(specs))()
+    for i, (ty, col, em, rough, ior) in enumerate(specs):
+        arr[i].type = ty
+        arr[i].color[:] = list(col)
+        arr[i].emission[:] = list(em)
+        arr[i].roughness = rough
+        arr[i].ior = ior
+    return arr
+
+
+# the reference's default scene materials (main.cu:39-41): grey and blue diffuse
+DEFAULT_MATERIALS = [(LAMBERT, (0.5, 0.5, 0.5), (0, 0, 0), 0.0, 0.0),
+                     (LAMBERT, (0.1, 0.2, 0.8), (0, 0, 0), 0.0, 0.0)]
+
+
+def camera_basis(lookfrom, lookat, vup, vfov, aspect):
+    """Camera of raytracer.h:31-48 (focus distance 1): origin,
+    lower_left_corner, horizontal, vertical as 12 floats (float32 math)."""
+    f = np.float32
+    lf, la, up = (np.asarray(v, f) for v in (lookfrom, lookat, vup))
+    h = f(np.tan(f(vfov) * f(3.14159265) / f(180.0) / f(2.0)))
+    vh = f(2.0) * h
+    vw = f(aspect) * vh
+    w = lf - la
+    w = w / f(np.sqrt(np.dot(w, w)))
+    u = np.cross(up, w).astype(f)
+    u = u / f(np.sqrt(np.dot(u, u)))
+    v = np.cross(w, u).astype(f)
+    hor = vw * u
+    ver = vh * v
+    ll = lf - hor / f(2) - ver / f(2) - w
+    return np.concatenate([lf, ll, hor, ver]).astype(f)
+
+
 class GenParams(C.Structure):
     _fields_ = [("n", C.c_int), ("q", C.c_int), ("p", C.c_int), ("pert", C.c_int),
                 ("amp", C.c_double), ("seed", C.c_uint64)]
@@ -97,6 +141,11 @@ def lib():
                                    C.POINTER(I), P, P, P, P]),
         "femrt_save_xml_fld": (I, [C.c_char_p, C.c_char_p, I, I, I, P, P, P]),
         "femrt_io_last_error": (C.c_char_p, []),
+        "femrt_frame_camera_rays_jitter": (I, [P, P, I, I, I, C.c_ulonglong]),
+        "femrt_frame_path_begin": (I, [P]),
+        "femrt_frame_scatter": (I, [P, P, P, I, C.c_ulonglong, I, I]),
+        "femrt_frame_resolve": (I, [P, I, I, I, P]),
+        "femrt_render_path": (I, [P, P, P, I, I, I, I, P, I, C.c_ulonglong, P]),
         "femrt_mlp_num_params": (I, [I]),
         "femrt_mlp_create": (I, [P, I, C.c_ulonglong, C.POINTER(P)]),
         "femrt_mlp_destroy": (I, [P]),
@@ -129,6 +178,8 @@ def exported_symbols():
             "femrt_frame_get_rgba", "femrt_frame_counters", "femrt_gen_structured_hex",
             "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh",
             "femrt_save_xml_fld", "femrt_io_last_error",
+            "femrt_frame_camera_rays_jitter", "femrt_frame_path_begin", "femrt_frame_scatter",
+            "femrt_frame_resolve", "femrt_render_path",
             "femrt_mlp_num_params", "femrt_mlp_create", "femrt_mlp_destroy",
             "femrt_mlp_get_params", "femrt_mlp_set_params", "femrt_mlp_forward",
             "femrt_mlp_loss_grad", "femrt_mlp_make_samples", "femrt_mlp_train_params_default",
@@ -473,6 +524,30 @@ class Frame:
     def rgba(self):
         out = np.zeros(self.n, np.uint32)
         _check(lib().femrt_frame_get_rgba(self.h, _ptr(out)), self.ctx.h, "get_rgba")
+        return out
+
+    def camera_rays_jitter(self, cam12, W, H, spp, seed):
+        cam = np.ascontiguousarray(cam12, np.float32)
+        _check(lib().femrt_frame_camera_rays_jitter(self.h, _ptr(cam), W, H, spp, seed),
+               self.ctx.h, "camera_rays_jitter")
+
+    def path_begin(self):
+        _check(lib().femrt_frame_path_begin(self.h), self.ctx.h, "path_begin")
+
+    def scatter(self, mesh, mats, seed, depth, last):
+        _check(lib().femrt_frame_scatter(mesh.h, self.h, mats, len(mats), seed, depth, int(last)),
+               self.ctx.h, "scatter")
+
+    def resolve(self, W, H, spp):
+        out = np.zeros(W * H, np.uint32)
+        _check(lib().femrt_frame_resolve(self.h, W, H, spp, _ptr(out)), self.ctx.h, "resolve")
+        return out
+
+    def render_path(self, mesh, cam12, W, H, spp, max_depth, mats, seed):
+        cam = np.ascontiguousarray(cam12, np.float32)
+        out = np.zeros(W * H, np.uint32)
+        _check(lib().femrt_render_path(mesh.h, self.h, _ptr(cam), W, H, spp, max_depth, mats,
+                                       len(mats), seed, _ptr(out)), self.ctx.h, "render_path")
         return out
 
     def counters(self, enable=None):

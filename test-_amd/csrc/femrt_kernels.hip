@@ -161,8 +161,6 @@ __device__ __forceinline__ void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-#define FEMRT_MLP_MAX_SAMPLES 16
-
 // Persistent element walk.  Each round every lane of a wave advances its ray
 // by one element segment (exit, samples, next element); a lane whose ray
 // ended takes the next hit ray from the queue (one atomic per wave), so the
@@ -333,7 +331,7 @@ __global__ void k_clear_frame(int n, int n_iso, int* hit_elem, int* iso_elem, in
     hit_elem[r] = -1;
     for (int k = 0; k < n_iso; ++k) iso_elem[(size_t)k * n + r] = -1;
   }
-  if (r < 4) counts[r] = 0;
+  if (r < 8) counts[r] = 0;
 }
 
 __global__ void k_shade(int n, int n_iso, const float* __restrict__ rays,
@@ -430,22 +428,6 @@ static int pad_nf(int nf)
   if (nf <= 9) return 9;
   return 0;
 }
-
-#define FEMRT_DISPATCH_NG(NGV, BODY)      \
-  switch (NGV) {                          \
-    case 4: { constexpr int NG_ = 4; BODY; } break; \
-    case 5: { constexpr int NG_ = 5; BODY; } break; \
-    case 7: { constexpr int NG_ = 7; BODY; } break; \
-    default: break;                       \
-  }
-#define FEMRT_DISPATCH_NF(NFV, BODY)      \
-  switch (NFV) {                          \
-    case 3: { constexpr int NF_ = 3; BODY; } break; \
-    case 5: { constexpr int NF_ = 5; BODY; } break; \
-    case 7: { constexpr int NF_ = 7; BODY; } break; \
-    case 9: { constexpr int NF_ = 9; BODY; } break; \
-    default: break;                       \
-  }
 
 // Pad mode counts: coefficient (i,j,k) of an n-mode expansion moves to the
 // same (i,j,k) of an N-mode one; higher modes are zero, so the map is
@@ -772,14 +754,14 @@ FEMRT_API int femrt_frame_create(femrt_ctx* c, int n, femrt_frame** out)
             hipMalloc(&f->d_iso_rst32, K * N * 16) == hipSuccess &&
             hipMalloc(&f->d_queue_hit, N * 4) == hipSuccess &&
             hipMalloc(&f->d_queue_iso, K * N * 4) == hipSuccess &&
-            hipMalloc(&f->d_counts, 16) == hipSuccess &&
+            hipMalloc(&f->d_counts, 32) == hipSuccess &&
             hipMalloc(&f->d_rgba, N * 4) == hipSuccess &&
             hipMalloc(&f->d_work, FEMRT_NCOUNTERS * 8) == hipSuccess;
   if (!ok) {
     femrt_frame_destroy(f);
     return fail(c, FEMRT_ENOMEM, "frame_create: hipMalloc failed");
   }
-  hipMemset(f->d_counts, 0, 16);
+  hipMemset(f->d_counts, 0, 32);
   hipMemset(f->d_work, 0, FEMRT_NCOUNTERS * 8);
   hipMemset(f->d_hit_elem, 0xff, N * 4);
   hipMemset(f->d_iso_elem, 0xff, K * N * 4);
@@ -794,7 +776,9 @@ FEMRT_API int femrt_frame_destroy(femrt_frame* f)
   hipStreamSynchronize(f->ctx->stream);
   void* ptrs[] = {f->d_rays, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst,
                   f->d_hit_u, f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, f->d_iso_f32,
-                  f->d_iso_rst32, f->d_queue_hit, f->d_queue_iso, f->d_counts, f->d_rgba, f->d_work};
+                  f->d_iso_rst32, f->d_queue_hit, f->d_queue_iso, f->d_counts, f->d_rgba, f->d_work,
+                  f->d_path_tp, f->d_path_acc, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
+                  f->d_walk_sorted, f->d_walk_reent, f->d_walk_hist};
   for (void* p : ptrs) hipFree(p);
   delete f;
   return FEMRT_OK;
@@ -928,20 +912,24 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
       });
       if (rc) break;
       if (n_iso > 0) {
-        // persistent: lanes refill from the hit queue until it drains
-        const int mblocks = dev_cus * 4;
-        rc = timed_launch(c, "k_march", [&] {
-          if (learned)
-            k_march<NG_, NF_, true><<<mblocks, 256, 0, c->stream>>>(
-                f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
-                f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
-                m->guess_mlp->d_pack);
-          else
-            k_march<NG_, NF_, false><<<mblocks, 256, 0, c->stream>>>(
-                f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
-                f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
-                nullptr);
-        });
+        if (!(p->flags & FEMRT_TRACE_PERSISTENT_WALK)) {
+          rc = walk_run(c, m, f, TM, ip, learned, work);
+        } else {
+          // persistent: lanes refill from the hit queue until it drains
+          const int mblocks = dev_cus * 4;
+          rc = timed_launch(c, "k_march_persistent", [&] {
+            if (learned)
+              k_march<NG_, NF_, true><<<mblocks, 256, 0, c->stream>>>(
+                  f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
+                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
+                  m->guess_mlp->d_pack);
+            else
+              k_march<NG_, NF_, false><<<mblocks, 256, 0, c->stream>>>(
+                  f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
+                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
+                  nullptr);
+          });
+        }
         if (rc) break;
         rc = timed_launch(c, "k_polish_iso", [&] {
           k_polish_iso<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(

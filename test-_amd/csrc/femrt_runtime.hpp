@@ -83,6 +83,17 @@ struct femrt_frame {
   uint32_t* d_rgba = nullptr;
   long long* d_work = nullptr;
   bool counters = false;
+  float4* d_path_tp = nullptr;   // per path: throughput rgb, alive flag (allocated by path_begin)
+  float4* d_path_acc = nullptr;  // per path: accumulated radiance rgb
+  // element-walk rounds (allocated by the first trace with isovalues)
+  int4* d_walk_si = nullptr;     // per ray: element, entry face, step, found | redo << 8
+  float4* d_walk_sf = nullptr;   // per ray: t_in, u_in, v_in
+  int* d_walk_a = nullptr;       // round lists
+  int* d_walk_b = nullptr;
+  int* d_walk_sorted = nullptr;
+  int* d_walk_reent = nullptr;
+  int* d_walk_hist = nullptr;    // n_elem + 1
+  int walk_cap = 0, walk_hist_cap = 0;
 };
 
 static int fail(femrt_ctx* ctx, int code, const std::string& msg)
@@ -186,3 +197,25 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter)
   }
   return base + __popcll(mask & ((1ull << lane) - 1ull));
 }
+
+#define FEMRT_DISPATCH_NG(NGV, BODY)      \
+  switch (NGV) {                          \
+    case 4: { constexpr int NG_ = 4; BODY; } break; \
+    case 5: { constexpr int NG_ = 5; BODY; } break; \
+    case 7: { constexpr int NG_ = 7; BODY; } break; \
+    default: break;                       \
+  }
+#define FEMRT_DISPATCH_NF(NFV, BODY)      \
+  switch (NFV) {                          \
+    case 3: { constexpr int NF_ = 3; BODY; } break; \
+    case 5: { constexpr int NF_ = 5; BODY; } break; \
+    case 7: { constexpr int NF_ = 7; BODY; } break; \
+    case 9: { constexpr int NF_ = 9; BODY; } break; \
+    default: break;                       \
+  }
+
+#define FEMRT_MLP_MAX_SAMPLES 16
+
+// Element walk in element-binned rounds (femrt_walk.hip).
+int walk_run(femrt_ctx* c, femrt_mesh* m, femrt_frame* f, const TraceMesh& TM, const IsoParams& ip,
+             bool learned, long long* work);

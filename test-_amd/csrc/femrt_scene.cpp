@@ -245,6 +245,35 @@ bool render(Scene& scene, const Camera& cam, int width, int height, std::vector<
     return false;
   }
   const RenderParams P = params ? *params : RenderParams();
+  if (scene.samples_per_pixel > 1 || scene.max_depth > 1) {
+    if (aov) {
+      std::cerr << "render: per-pixel AOVs are not produced by path tracing" << std::endl;
+      return false;
+    }
+    std::vector<femrt_material> mats = scene.materials;
+    if (mats.empty()) {
+      femrt_material g{{0.5f, 0.5f, 0.5f}, {0.f, 0.f, 0.f}, 0.f, 0.f, 0};
+      femrt_material b{{0.1f, 0.2f, 0.8f}, {0.f, 0.f, 0.f}, 0.f, 0.f, 0};
+      mats = {g, b};
+    }
+    const int spp = std::max(1, scene.samples_per_pixel);
+    femrt_frame* f = nullptr;
+    if (femrt_frame_create(scene.ctx, width * height * spp, &f) != FEMRT_OK) {
+      std::cerr << "render: " << femrt_last_error(scene.ctx) << std::endl;
+      return false;
+    }
+    const float cam12[12] = {cam.origin.x, cam.origin.y, cam.origin.z,
+                             cam.lower_left_corner.x, cam.lower_left_corner.y, cam.lower_left_corner.z,
+                             cam.horizontal.x, cam.horizontal.y, cam.horizontal.z,
+                             cam.vertical.x, cam.vertical.y, cam.vertical.z};
+    out.resize((size_t)width * height);
+    const bool ok = femrt_render_path(scene.mesh, f, cam12, width, height, spp, std::max(1, scene.max_depth),
+                                      mats.data(), (int)mats.size(), P.seed,
+                                      reinterpret_cast<uint32_t*>(out.data())) == FEMRT_OK;
+    if (!ok) std::cerr << "render: " << femrt_last_error(scene.ctx) << std::endl;
+    femrt_frame_destroy(f);
+    return ok;
+  }
   const size_t n = (size_t)width * height;
   std::vector<float> rays;
   camera_rays(cam, width, height, rays);

@@ -830,16 +830,18 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
   return false;
 }
 
-// Next segment: the face neighbour of the exit, else (boundary face, no
-// adjacency, or no exit resolved) the next element the ray meets, by the
-// same traversal as the first hit.  Returns false when the ray is done.
-template <int NG>
-FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
-                            MarchState& S, Work& w)
+// Next segment through the face adjacency.  Returns MARCH_DONE when the ray
+// is done (every isovalue found or the step cap), MARCH_NEXT when S now holds
+// the neighbour's segment, MARCH_REENTER when the ray left the mesh (boundary
+// face, no adjacency, or no exit resolved) and must find the next element it
+// meets through the BVH (march_reenter).
+enum { MARCH_DONE = 0, MARCH_NEXT = 1, MARCH_REENTER = 2 };
+
+FEMRT_HD int march_next(const TraceMesh& M, const IsoParams& P, MarchState& S)
 {
   const unsigned all = (1u << P.n_iso) - 1u;
   S.redo = false;
-  if (S.found == all || ++S.step >= P.max_march) return false;
+  if (S.found == all || ++S.step >= P.max_march) return MARCH_DONE;
   if (S.ok && M.adj) {
     const int ne = M.adj[(size_t)S.e * 12 + S.f_out * 2];
     const int code = M.adj[(size_t)S.e * 12 + S.f_out * 2 + 1];
@@ -859,9 +861,19 @@ FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float*
       S.t_in = S.t_out;
       S.u_in = u;
       S.v_in = v;
-      return true;
+      return MARCH_NEXT;
     }
   }
+  return MARCH_REENTER;
+}
+
+// Re-entry after march_next returned MARCH_REENTER: the nearest element the
+// ray meets beyond the last resolved point, other than the current one, by
+// the same traversal as the first hit.  Returns false when there is none.
+template <int NG>
+FEMRT_HD bool march_reenter(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
+                            MarchState& S, Work& w)
+{
   const float t_from = S.ok ? S.t_out : S.t_in;
   ++w.reentry;
   const HitF32 h2 =
@@ -872,7 +884,19 @@ FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float*
   S.t_in = h2.t;
   S.u_in = h2.u;
   S.v_in = h2.v;
+  S.ok = false;
   return true;
+}
+
+// march_next, then march_reenter when the ray left the mesh.
+template <int NG>
+FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
+                            MarchState& S, Work& w)
+{
+  const int k = march_next(M, P, S);
+  if (k == MARCH_DONE) return false;
+  if (k == MARCH_NEXT) return true;
+  return march_reenter<NG>(M, P, o, d, S, w);
 }
 
 struct BracketArray {

@@ -5,6 +5,10 @@
 //
 //   femrt_render mesh.xml field.fld out.ppm [width height [iso ...]]
 //                [--guess plain|warm] [--aov file] [--rays file] [--load-only]
+//                [--spp n] [--depth n] [--seed n]
+//
+// --spp / --depth > 1 path-trace the mesh surface (the reference's default
+// grey and blue diffuse materials) instead of the primary-ray AOV render.
 //
 // --aov writes the per-pixel HitAOV records and --rays the 6 x W*H SoA ray
 // floats (raw, host byte order); --load-only stops after loading and prints
@@ -29,6 +33,7 @@ int main(int argc, char** argv)
   std::vector<std::string> pos;
   std::string aov_path, rays_path;
   bool load_only = false;
+  int spp = 1, depth = 1;
   RenderParams rp;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--aov") && i + 1 < argc) {
@@ -37,6 +42,12 @@ int main(int argc, char** argv)
       rays_path = argv[++i];
     } else if (!std::strcmp(argv[i], "--load-only")) {
       load_only = true;
+    } else if (!std::strcmp(argv[i], "--spp") && i + 1 < argc) {
+      spp = std::atoi(argv[++i]);
+    } else if (!std::strcmp(argv[i], "--depth") && i + 1 < argc) {
+      depth = std::atoi(argv[++i]);
+    } else if (!std::strcmp(argv[i], "--seed") && i + 1 < argc) {
+      rp.seed = std::strtoull(argv[++i], nullptr, 10);
     } else if (!std::strcmp(argv[i], "--guess") && i + 1 < argc) {
       const std::string g = argv[++i];
       rp.guess = g == "plain" ? FEMRT_GUESS_PLAIN : FEMRT_GUESS_WARM;
@@ -49,6 +60,8 @@ int main(int argc, char** argv)
   Scene scene;
   scene.width = width;
   scene.height = height;
+  scene.samples_per_pixel = spp;
+  scene.max_depth = depth;
   if (!load_xml_fld(pos[0], pos[1], scene)) return 1;
   if (load_only) {
     double gs = 0.0, fs = 0.0;
@@ -76,7 +89,8 @@ int main(int argc, char** argv)
   const Camera cam(from, c, make_float3(0, 1, 0), 20.0f, (float)width / height, 0.0f, 1.0f);
   std::vector<uchar4> img;
   std::vector<HitAOV> aov;
-  if (!render(scene, cam, width, height, img, aov_path.empty() ? nullptr : &aov, &rp)) return 1;
+  const bool paths = spp > 1 || depth > 1;
+  if (!render(scene, cam, width, height, img, aov_path.empty() || paths ? nullptr : &aov, &rp)) return 1;
   save_image_ppm(img, width, height, pos[2]);
   if (!rays_path.empty()) {
     std::vector<float> rays;

@@ -84,3 +84,26 @@ def test_cpp_render_written_mesh(femrt, oracle, tmp_path):
         assert r["found_mismatch"] == 0 and r["t_rel_bad_nontie"] == 0 and r["elem_mismatch"] == 0
     assert rep["hit_miss_mismatch"] == 0 and rep["id_mismatch"] == 0
     assert rep["t_rel_max"] <= 1e-6 and rep["rst_abs_max"] <= 1e-7
+
+
+@pytest.mark.gpu
+def test_cpp_path_traced_render(tmp_path):
+    """render() path-traces when samples_per_pixel or max_depth exceed 1:
+    the mesh shows the reference's grey (faces other than 0) diffuse shading
+    under the sky, and the same seed gives the same image."""
+    outs = []
+    for k in range(2):
+        ppm = tmp_path / f"p{k}.ppm"
+        r = _run(GOLD / "Hex_CurvedFace.xml", GOLD / "Hex_CurvedFace.fld", ppm, 64, 48,
+                 "--spp", 4, "--depth", 3, "--seed", 9)
+        assert r.returncode == 0, r.stderr
+        data = ppm.read_bytes()
+        header = b"P6\n64 48\n255\n"
+        assert data.startswith(header)
+        outs.append(np.frombuffer(data[len(header):], np.uint8).reshape(48 * 64, 3))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    px = outs[0].astype(int)
+    sky = (px[:, 2] >= 250)                       # sky pixels are saturated blue
+    assert 0.05 < sky.mean() < 0.98
+    grey = ~sky & (np.abs(px[:, 0] - px[:, 1]) < 12) & (np.abs(px[:, 1] - px[:, 2]) < 40)
+    assert grey.sum() > 50
