@@ -156,9 +156,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--persistent-walk", action="store_true",
-                    help="walk each ray to its end in one persistent kernel instead of "
-                         "element-binned rounds")
+    ap.add_argument("--binned-walk", action="store_true",
+                    help="walk in element-binned rounds instead of one persistent kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -200,7 +199,7 @@ def main():
                     "guess_err_p99_on_bench_mesh": float(np.percentile(err, 99))}
         print(json.dumps({"mlp": mlp_info}), file=sys.stderr, flush=True)
     for v in variants:
-        flags = 1 | (2 if args.persistent_walk else 0)
+        flags = 1 | (2 if args.binned_walk else 0)
         wall, kt, cnt = run_variant(femrt, ctx, mesh, fr, names[v], args.steps, args.warmup, flags)
         f32, f64 = kernel_flops(cnt, hm.ng, hm.nf)
         results[v] = dict(wall=wall, kernels=kt, counters=cnt, f32=f32, f64=f64,

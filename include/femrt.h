@@ -140,8 +140,10 @@ typedef struct {
 } femrt_trace_params;
 
 #define FEMRT_TRACE_RGBA 1             /* shade into the frame's RGBA                 */
-#define FEMRT_TRACE_PERSISTENT_WALK 2  /* one persistent kernel walks each ray to its
-                                          end instead of element-binned rounds     */
+#define FEMRT_TRACE_BINNED_WALK 2      /* walk in rounds with the rays binned by
+                                          element, re-entries in their own kernel,
+                                          instead of one persistent kernel that
+                                          walks each ray to its end                 */
 
 FEMRT_API void femrt_trace_params_default(femrt_trace_params* p);
 

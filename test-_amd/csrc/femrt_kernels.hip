@@ -912,12 +912,12 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
       });
       if (rc) break;
       if (n_iso > 0) {
-        if (!(p->flags & FEMRT_TRACE_PERSISTENT_WALK)) {
+        if (p->flags & FEMRT_TRACE_BINNED_WALK) {
           rc = walk_run(c, m, f, TM, ip, learned, work);
         } else {
           // persistent: lanes refill from the hit queue until it drains
           const int mblocks = dev_cus * 4;
-          rc = timed_launch(c, "k_march_persistent", [&] {
+          rc = timed_launch(c, "k_march", [&] {
             if (learned)
               k_march<NG_, NF_, true><<<mblocks, 256, 0, c->stream>>>(
                   f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,

@@ -767,8 +767,10 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
   face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
   float gprev[FEMRT_MAX_ISO];
   float rst[3] = {rst_in[0], rst_in[1], rst_in[2]}, rprev[3] = {0.f, 0.f, 0.f};
+  float rprev2[3] = {0.f, 0.f, 0.f};
   float tprev = S.t_in;
   bool have_prev = false;
+  int n_prev = 0;  // consecutive samples with a preimage
   unsigned found_here = 0;
   for (int j = 0; j <= Ms; ++j) {
     const float t = march_sample_t(S, j, Ms);
@@ -790,11 +792,16 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
       zero_retry = true;  // plain Newton from 0 is the fallback of a learned guess
     } else if (P.guess == FEMRT_GUESS_PLAIN) {
       rst[0] = rst[1] = rst[2] = 0.f;
+    } else if (n_prev >= 2) {
+      // warm start extrapolated from the two previous (uniformly spaced)
+      // samples: O(dt^2) from the preimage instead of O(dt)
+      for (int c = 0; c < 3; ++c) rst[c] = fminf(fmaxf(2.f * rprev[c] - rprev2[c], -1.f), 1.f);
     }
     if (!known && !invert_in_cube_f32<NG>(gc, x, rst, J, w.inv_it, zero_retry)) {
       // no preimage inside the element: the segment end is wrong
       if (!S.redo) return true;
       have_prev = false;
+      n_prev = 0;
       rst[0] = rst[1] = rst[2] = 0.f;
       continue;
     }
@@ -809,22 +816,27 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
         br.ta = tprev;
         br.tb = t;
         br.ga = gprev[k];
-        // the FP64 endgame starts from the bracket midpoint (the root's
-        // refinement runs there on a compact queue, one bracket per lane,
-        // instead of stalling the walk's wave for the few lanes with a bracket)
-        br.tm = 0.5f * (tprev + t);
-        for (int c = 0; c < 3; ++c) br.rst[c] = 0.5f * (rprev[c] + rst[c]);
+        // the FP64 endgame starts from the secant (regula falsi) point of the
+        // bracket, rst interpolated alike; the root's refinement runs there
+        // on a compact queue, one bracket per lane, instead of stalling the
+        // walk's wave for the few lanes with a bracket
+        const float den = gprev[k] - g;
+        const float wt = den != 0.f ? fminf(fmaxf(gprev[k] / den, 0.f), 1.f) : 0.5f;
+        br.tm = tprev + wt * (t - tprev);
+        for (int c = 0; c < 3; ++c) br.rst[c] = rprev[c] + wt * (rst[c] - rprev[c]);
         sink.put(k, br);
         ++w.brackets;
         found_here |= 1u << k;
       }
       gprev[k] = g;
     }
-    rprev[0] = rst[0];
-    rprev[1] = rst[1];
-    rprev[2] = rst[2];
+    for (int c = 0; c < 3; ++c) {
+      rprev2[c] = rprev[c];
+      rprev[c] = rst[c];
+    }
     tprev = t;
     have_prev = true;
+    ++n_prev;
   }
   S.found |= found_here;
   return false;

@@ -350,7 +350,7 @@ int walk_run(femrt_ctx* c, femrt_mesh* m, femrt_frame* f, const TraceMesh& TM, c
     rc = FEMRT_EUNSUP;
     FEMRT_DISPATCH_NG(m->NG, {
       FEMRT_DISPATCH_NF(m->NF, {
-        rc = timed_launch(c, "k_march", [&] {
+        rc = timed_launch(c, "k_walk_segment", [&] {
           if (learned)
             k_walk_segment<NG_, NF_, true><<<ab, 256, 0, c->stream>>>(
                 f->d_rays, n, TM, ip, f->d_walk_sorted, n_act, f->d_walk_si, f->d_walk_sf, nxt,

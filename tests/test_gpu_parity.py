@@ -59,14 +59,14 @@ def test_point_inversion_matches_oracle(femrt, oracle, gpu_ctx):
     mesh.close()
 
 
-def _trace_and_compare(femrt, oracle, ctx, hm, W, H, isos, guess):
+def _trace_and_compare(femrt, oracle, ctx, hm, W, H, isos, guess, flags=1):
     mesh = femrt.Mesh(ctx, hm)
     lo = hm.vertices.reshape(-1, 3).min(0)
     hi = hm.vertices.reshape(-1, 3).max(0)
     cam = femrt.auto_camera(lo, hi)
     fr = femrt.Frame(ctx, W * H)
     fr.camera_rays(cam, W, H)
-    fr.trace(mesh, femrt.trace_params(isos, guess=guess))
+    fr.trace(mesh, femrt.trace_params(isos, guess=guess, flags=flags))
     gh = fr.hits()
     giso = fr.iso(len(isos))
     rays = fr.get_rays()
@@ -88,10 +88,12 @@ def test_trace_config1_geometry_and_iso(femrt, oracle, gpu_ctx, guess):
     assert rep["ok"], rep
 
 
-def test_trace_three_isovalues_curved_q4(femrt, oracle, gpu_ctx):
+@pytest.mark.parametrize("flags", [1, 3], ids=["persistent", "binned"])
+def test_trace_three_isovalues_curved_q4(femrt, oracle, gpu_ctx, flags):
+    """Both walk schedules: one persistent kernel, element-binned rounds."""
     hm = femrt.gen_structured_hex(4, 4, 6, pert=2, amp=0.08, seed=99)
     isos = [-0.1, 0.0, 0.1]
-    rep = _trace_and_compare(femrt, oracle, gpu_ctx, hm, 48, 32, isos, 1)
+    rep = _trace_and_compare(femrt, oracle, gpu_ctx, hm, 48, 32, isos, 1, flags)
     print(json.dumps(rep, indent=1))
     assert rep["oracle_hits"] > 100
     assert rep["ok"], rep

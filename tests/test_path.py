@@ -128,7 +128,10 @@ def test_paths_match_oracle_depth_by_depth(femrt, oracle, gpu_ctx):
         fr.scatter(mesh, mats, seed, depth, depth == depth_max - 1)
         g_next = fr.get_rays()
         ok = alive & ~tie_any
-        assert np.max(np.abs(g_next[:, ok] - nxt[:, ok])) < 2e-4, depth
+        if ok.any():   # (none survive the last depth)
+            assert np.max(np.abs(g_next[:, ok] - nxt[:, ok])) < 2e-4, depth
+        # ended paths parked alike (origin 1e30 in FP32, direction +x)
+        assert np.array_equal(g_next[:, ~alive], nxt[:, ~alive].astype(np.float32))
     assert rehits > 20          # the bounces reach the other block
     img = fr.resolve(W, H, spp)
     ref = po.resolve(acc, W, H, spp)
