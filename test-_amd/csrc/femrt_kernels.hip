@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256, 2) void k_march(
 }
 
 template <int NG, int NF>
-__global__ __launch_bounds__(256) void k_polish_iso(
+__global__ __launch_bounds__(256, 2) void k_polish_iso(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P,
     const int* __restrict__ iso_queue, const int* __restrict__ counts,
     const float4* __restrict__ iso_f32, const float4* __restrict__ iso_rst32,

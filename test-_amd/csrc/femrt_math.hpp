@@ -60,6 +60,26 @@ FEMRT_HD void basis1d(T x, T* __restrict__ phi, T* __restrict__ dphi)
   }
 }
 
+// Recurrence coefficients of P^{(1,1)}_k, k = 2..15, as constants (the rolled
+// loops below would otherwise divide at every step).
+FEMRT_HD double jacobi11_A(int k)
+{
+  constexpr double t[16] = {0.0, 0.0,
+                            5.0 * 3.0 / (2.0 * 4.0), 7.0 * 4.0 / (3.0 * 5.0), 9.0 * 5.0 / (4.0 * 6.0),
+                            11.0 * 6.0 / (5.0 * 7.0), 13.0 * 7.0 / (6.0 * 8.0), 15.0 * 8.0 / (7.0 * 9.0),
+                            17.0 * 9.0 / (8.0 * 10.0), 19.0 * 10.0 / (9.0 * 11.0), 21.0 * 11.0 / (10.0 * 12.0),
+                            23.0 * 12.0 / (11.0 * 13.0), 25.0 * 13.0 / (12.0 * 14.0), 27.0 * 14.0 / (13.0 * 15.0),
+                            29.0 * 15.0 / (14.0 * 16.0), 31.0 * 16.0 / (15.0 * 17.0)};
+  return t[k];
+}
+FEMRT_HD double jacobi11_B(int k)
+{
+  constexpr double t[16] = {0.0, 0.0, 3.0 / 4.0, 4.0 / 5.0, 5.0 / 6.0, 6.0 / 7.0, 7.0 / 8.0, 8.0 / 9.0,
+                            9.0 / 10.0, 10.0 / 11.0, 11.0 / 12.0, 12.0 / 13.0, 13.0 / 14.0, 14.0 / 15.0,
+                            15.0 / 16.0, 16.0 / 17.0};
+  return t[k];
+}
+
 // Modified_A basis generated one mode at a time (phi_p, phi'_p for
 // p = 0, 1, 2, ...), so rolled loops over modes keep only the recurrence state
 // in registers instead of whole tables.
@@ -96,9 +116,7 @@ struct BasisIter {
           Pn = T(2) * x;
           dPn = T(2);
         } else {
-          const T kk = T(k);
-          const T A = (T(2) * kk + T(1)) * (kk + T(1)) / (kk * (kk + T(2)));
-          const T B = (kk + T(1)) / (kk + T(2));
+          const T A = T(jacobi11_A(k)), B = T(jacobi11_B(k));
           Pn = A * x * P1 - B * P0;
           dPn = A * (P1 + x * dP1) - B * dP0;
         }
@@ -125,6 +143,8 @@ FEMRT_HD void geom_eval(const float4* __restrict__ c, T r, T s, T t, T* __restri
 {
   T pr[NG], dpr[NG];
   basis1d<T, NG>(r, pr, dpr);
+  T ps[NG], dps[NG];
+  basis1d<T, NG>(s, ps, dps);
   T x0 = 0, x1 = 0, x2 = 0;
   T jr0 = 0, jr1 = 0, jr2 = 0, js0 = 0, js1 = 0, js2 = 0, jt0 = 0, jt1 = 0, jt2 = 0;
   BasisIter<T> bt;
@@ -136,12 +156,9 @@ FEMRT_HD void geom_eval(const float4* __restrict__ c, T r, T s, T t, T* __restri
     T b0 = 0, b1 = 0, b2 = 0;     // sum_j phi_j(s) a_jk
     T bs0 = 0, bs1 = 0, bs2 = 0;  // sum_j phi'_j(s) a_jk
     T br0 = 0, br1 = 0, br2 = 0;  // sum_j phi_j(s) a'_jk
-    BasisIter<T> bs;
-    bs.init(s);
-#pragma unroll 1
+#pragma unroll
     for (int j = 0; j < NG; ++j) {
-      T sj, dsj;
-      bs.next(sj, dsj);
+      const T sj = ps[j], dsj = dps[j];
       const float4* cj = c + (k * NG + j) * NG;
       T a0 = 0, a1 = 0, a2 = 0, ad0 = 0, ad1 = 0, ad2 = 0;
 #pragma unroll
@@ -278,6 +295,8 @@ FEMRT_HD T field_value(const float* __restrict__ c, T r, T s, T t)
 {
   T pr[NF], dpr[NF];
   basis1d<T, NF>(r, pr, dpr);
+  T ps[NF], dps[NF];
+  basis1d<T, NF>(s, ps, dps);
   T v = 0;
   BasisIter<T> bt;
   bt.init(t);
@@ -286,17 +305,13 @@ FEMRT_HD T field_value(const float* __restrict__ c, T r, T s, T t)
     T tk, dtk;
     bt.next(tk, dtk);
     T b = 0;
-    BasisIter<T> bsi;
-    bsi.init(s);
-#pragma unroll 1
+#pragma unroll
     for (int j = 0; j < NF; ++j) {
-      T sj, dsj;
-      bsi.next(sj, dsj);
       const float* cj = c + (k * NF + j) * NF;
       T a = 0;
 #pragma unroll
       for (int i = 0; i < NF; ++i) a += pr[i] * T(cj[i]);
-      b += sj * a;
+      b += ps[j] * a;
     }
     v += tk * b;
   }
