@@ -201,6 +201,9 @@ FEMRT_API int femrt_frame_camera_rays_jitter(femrt_frame* f, const float* cam12,
                                              int height, int spp, unsigned long long seed);
 /* Path state: throughput 1 and radiance 0 for every ray of the frame. */
 FEMRT_API int femrt_frame_path_begin(femrt_frame* f);
+/* Uploads the frame's material table (at most 64) ahead of scatter; scatter
+ * uploads it itself when the table it is given differs from the frame's. */
+FEMRT_API int femrt_frame_set_materials(femrt_frame* f, const femrt_material* mats, int n_mats);
 /* One bounce after a geometry trace of the frame (femrt_trace, iso stage
  * off): misses add throughput * sky (raytracer.cu:398-402) and end; hits
  * scatter off the surface (normal from the hit face's tangents, outward and
@@ -210,12 +213,45 @@ FEMRT_API int femrt_frame_path_begin(femrt_frame* f);
 FEMRT_API int femrt_frame_scatter(femrt_mesh* mesh, femrt_frame* f, const femrt_material* mats,
                                   int n_mats, unsigned long long seed, int depth, int last);
 /* Mean radiance over each pixel's samples, gamma sqrt, clamp, 8-bit RGBA
- * (raytracer.cu:460-475); host output width*height. */
+ * (raytracer.cu:460-475) into the frame's RGBA buffer; copied to rgba
+ * (width*height) when rgba is not NULL, else read later with
+ * femrt_frame_get_rgba. */
 FEMRT_API int femrt_frame_resolve(femrt_frame* f, int width, int height, int spp, uint32_t* rgba);
 /* camera_rays_jitter + path_begin + max_depth x (trace + scatter) + resolve. */
 FEMRT_API int femrt_render_path(femrt_mesh* mesh, femrt_frame* f, const float* cam12, int width,
                                 int height, int spp, int max_depth, const femrt_material* mats,
                                 int n_mats, unsigned long long seed, uint32_t* rgba);
+
+/* ---------------------------------------------- batches and launch shapes */
+/* No counterpart in the reference (render_kernel renders one mesh per
+ * launch, raytracer.cu:410-476): a batch records traces and path-traced
+ * renders of many meshes of uneven size, each into its own frame, into one
+ * HIP graph that femrt_batch_launch replays with one launch on the context
+ * stream.  Jobs run in the order they were added.  Not capturable: frames
+ * with work counters on, the binned walk (FEMRT_TRACE_BINNED_WALK).  Replays
+ * rewrite each job's frame from scratch; read results after femrt_sync. */
+typedef struct femrt_batch femrt_batch;
+FEMRT_API int femrt_batch_create(femrt_ctx* ctx, femrt_batch** out);
+FEMRT_API int femrt_batch_destroy(femrt_batch* b);
+FEMRT_API int femrt_batch_add_trace(femrt_batch* b, femrt_mesh* mesh, femrt_frame* f,
+                                    const femrt_trace_params* p);
+/* camera_rays_jitter + path_begin + max_depth x (trace + scatter) + resolve,
+ * as femrt_render_path; the RGBA stays in the frame (femrt_frame_get_rgba). */
+FEMRT_API int femrt_batch_add_paths(femrt_batch* b, femrt_mesh* mesh, femrt_frame* f, const float* cam12,
+                                    int width, int height, int spp, int max_depth,
+                                    const femrt_material* mats, int n_mats, unsigned long long seed);
+FEMRT_API int femrt_batch_capture(femrt_batch* b);
+FEMRT_API int femrt_batch_launch(femrt_batch* b);
+
+/* Launch shapes of a mesh's trace kernels: cfg3 = {first-hit block size
+ * (64/128/256), march workgroups per CU, polish workgroups per CU}; default
+ * {256, 4, 8}.  femrt_autotune times candidate shapes on the given trace
+ * (reps timed traces per candidate, one kernel at a time), keeps the fastest
+ * on the mesh and returns it in cfg3 (may be NULL). */
+FEMRT_API int femrt_mesh_get_launch_cfg(const femrt_mesh* mesh, int* cfg3);
+FEMRT_API int femrt_mesh_set_launch_cfg(femrt_mesh* mesh, const int* cfg3);
+FEMRT_API int femrt_autotune(femrt_mesh* mesh, femrt_frame* f, const femrt_trace_params* p, int reps,
+                             int* cfg3);
 
 /* ------------------------------------------------- learned Newton guess */
 /* No counterpart in the reference (its Newton solves start from fixed seeds,

@@ -19,6 +19,7 @@ LIB_PATH = PKG_DIR / "_build" / "libfemrt.so"
 OK = 0
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "ESTATE"}
 GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
+TRACE_RGBA, TRACE_BINNED_WALK = 1, 2     # femrt_trace_params.flags bits
 MAX_ISO = 8
 NCOUNTERS = 16
 COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",
@@ -157,6 +158,16 @@ def lib():
         "femrt_mlp_train_params_default": (None, [P]),
         "femrt_mlp_train": (I, [P, P, P, P]),
         "femrt_mesh_set_guess_mlp": (I, [P, P]),
+        "femrt_frame_set_materials": (I, [P, P, I]),
+        "femrt_batch_create": (I, [P, C.POINTER(P)]),
+        "femrt_batch_destroy": (I, [P]),
+        "femrt_batch_add_trace": (I, [P, P, P, P]),
+        "femrt_batch_add_paths": (I, [P, P, P, P, I, I, I, I, P, I, C.c_ulonglong]),
+        "femrt_batch_capture": (I, [P]),
+        "femrt_batch_launch": (I, [P]),
+        "femrt_mesh_get_launch_cfg": (I, [P, P]),
+        "femrt_mesh_set_launch_cfg": (I, [P, P]),
+        "femrt_autotune": (I, [P, P, P, I, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -183,7 +194,10 @@ def exported_symbols():
             "femrt_mlp_num_params", "femrt_mlp_create", "femrt_mlp_destroy",
             "femrt_mlp_get_params", "femrt_mlp_set_params", "femrt_mlp_forward",
             "femrt_mlp_loss_grad", "femrt_mlp_make_samples", "femrt_mlp_train_params_default",
-            "femrt_mlp_train", "femrt_mesh_set_guess_mlp"]
+            "femrt_mlp_train", "femrt_mesh_set_guess_mlp", "femrt_frame_set_materials",
+            "femrt_batch_create", "femrt_batch_destroy", "femrt_batch_add_trace",
+            "femrt_batch_add_paths", "femrt_batch_capture", "femrt_batch_launch",
+            "femrt_mesh_get_launch_cfg", "femrt_mesh_set_launch_cfg", "femrt_autotune"]
 
 
 def _ptr(a):
@@ -392,6 +406,23 @@ class Mesh:
                self.ctx.h, "set_guess_mlp")
         self._mlp = mlp  # keep the network alive while the mesh uses it
 
+    def launch_cfg(self, cfg=None):
+        """Launch shapes (first-hit block, march and polish workgroups per CU);
+        sets them when cfg is given, returns the current ones."""
+        c3 = np.zeros(3, np.int32)
+        if cfg is not None:
+            c3[:] = cfg
+            _check(lib().femrt_mesh_set_launch_cfg(self.h, _ptr(c3)), self.ctx.h, "set_launch_cfg")
+        _check(lib().femrt_mesh_get_launch_cfg(self.h, _ptr(c3)), self.ctx.h, "get_launch_cfg")
+        return tuple(int(x) for x in c3)
+
+    def autotune(self, frame, params, reps=2):
+        """Times candidate launch shapes on this trace and keeps the fastest."""
+        c3 = np.zeros(3, np.int32)
+        _check(lib().femrt_autotune(self.h, frame.h, C.byref(params), reps, _ptr(c3)),
+               self.ctx.h, "autotune")
+        return tuple(int(x) for x in c3)
+
     def close(self):
         if self.h:
             lib().femrt_mesh_destroy(self.h)
@@ -534,6 +565,9 @@ class Frame:
     def path_begin(self):
         _check(lib().femrt_frame_path_begin(self.h), self.ctx.h, "path_begin")
 
+    def set_materials(self, mats):
+        _check(lib().femrt_frame_set_materials(self.h, mats, len(mats)), self.ctx.h, "set_materials")
+
     def scatter(self, mesh, mats, seed, depth, last):
         _check(lib().femrt_frame_scatter(mesh.h, self.h, mats, len(mats), seed, depth, int(last)),
                self.ctx.h, "scatter")
@@ -559,6 +593,39 @@ class Frame:
     def close(self):
         if self.h:
             lib().femrt_frame_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+class Batch:
+    """Traces and path-traced renders of many meshes recorded into one HIP
+    graph; launch() replays them all with one graph launch."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        self._keep = []
+        _check(lib().femrt_batch_create(ctx.h, C.byref(self.h)), ctx.h, "batch_create")
+
+    def add_trace(self, mesh: Mesh, frame: Frame, params: TraceParams):
+        self._keep.append((mesh, frame, params))
+        _check(lib().femrt_batch_add_trace(self.h, mesh.h, frame.h, C.byref(params)), self.ctx.h,
+               "batch_add_trace")
+
+    def add_paths(self, mesh: Mesh, frame: Frame, cam12, W, H, spp, max_depth, mats, seed):
+        cam = np.ascontiguousarray(cam12, np.float32)
+        self._keep.append((mesh, frame, mats))
+        _check(lib().femrt_batch_add_paths(self.h, mesh.h, frame.h, _ptr(cam), W, H, spp, max_depth,
+                                           mats, len(mats), seed), self.ctx.h, "batch_add_paths")
+
+    def capture(self):
+        _check(lib().femrt_batch_capture(self.h), self.ctx.h, "batch_capture")
+
+    def launch(self):
+        _check(lib().femrt_batch_launch(self.h), self.ctx.h, "batch_launch")
+
+    def close(self):
+        if self.h:
+            lib().femrt_batch_destroy(self.h)
             self.h = C.c_void_p()
 
 

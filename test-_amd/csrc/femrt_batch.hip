@@ -1,0 +1,284 @@
+// femrt_batch.hip -- many traces and path-traced renders of meshes of
+// uneven size recorded into one HIP graph and replayed with one launch, and
+// the autotuner that picks each mesh's kernel launch shapes.
+//
+// A batch records the context stream (stream capture) while it enqueues each
+// job exactly as the eager calls would, so the graph holds the same kernels
+// with the same arguments: replaying it reproduces the eager results.  Jobs
+// must be capture-safe: no work counters, no binned walk (its rounds need the
+// host), materials and path buffers set up beforehand (femrt_batch_add_paths
+// does that).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "femrt_runtime.hpp"
+
+using namespace femrt;
+
+struct BatchJob {
+  int kind = 0;  // 0 trace, 1 path-traced render
+  femrt_mesh* mesh = nullptr;
+  femrt_frame* frame = nullptr;
+  femrt_trace_params tp;
+  float cam12[12];
+  int W = 0, H = 0, spp = 1, depth = 1;
+  std::vector<femrt_material> mats;
+  unsigned long long seed = 0;
+};
+
+struct femrt_batch {
+  femrt_ctx* ctx = nullptr;
+  std::vector<BatchJob> jobs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void batch_drop_graph(femrt_batch* b)
+{
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+static int enqueue_job(const BatchJob& j)
+{
+  if (j.kind == 0) return femrt_trace(j.mesh, j.frame, &j.tp);
+  int rc = femrt_frame_camera_rays_jitter(j.frame, j.cam12, j.W, j.H, j.spp, j.seed);
+  if (rc == FEMRT_OK) rc = femrt_frame_path_begin(j.frame);
+  femrt_trace_params tp;
+  femrt_trace_params_default(&tp);
+  tp.iso_enable = 0;
+  tp.n_iso = 0;
+  tp.flags = 0;
+  tp.guess = FEMRT_GUESS_WARM;
+  for (int d = 0; d < j.depth && rc == FEMRT_OK; ++d) {
+    rc = femrt_trace(j.mesh, j.frame, &tp);
+    if (rc == FEMRT_OK)
+      rc = femrt_frame_scatter(j.mesh, j.frame, j.mats.data(), (int)j.mats.size(), j.seed, d, d == j.depth - 1);
+  }
+  if (rc == FEMRT_OK) rc = femrt_frame_resolve(j.frame, j.W, j.H, j.spp, nullptr);
+  return rc;
+}
+
+extern "C" {
+
+FEMRT_API int femrt_batch_create(femrt_ctx* c, femrt_batch** out)
+{
+  if (!c || !out) return FEMRT_EINVAL;
+  femrt_batch* b = new femrt_batch;
+  b->ctx = c;
+  *out = b;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_batch_destroy(femrt_batch* b)
+{
+  if (!b) return FEMRT_EINVAL;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_drop_graph(b);
+  delete b;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_batch_add_trace(femrt_batch* b, femrt_mesh* m, femrt_frame* f, const femrt_trace_params* p)
+{
+  if (!b || !m || !f || !p) return FEMRT_EINVAL;
+  femrt_ctx* c = b->ctx;
+  if (m->ctx != c || f->ctx != c) return fail(c, FEMRT_EINVAL, "batch_add_trace: objects of another context");
+  if (p->flags & FEMRT_TRACE_BINNED_WALK)
+    return fail(c, FEMRT_EUNSUP, "batch_add_trace: the binned walk needs the host between rounds");
+  if (f->counters) return fail(c, FEMRT_EUNSUP, "batch_add_trace: switch the frame's work counters off");
+  BatchJob j;
+  j.kind = 0;
+  j.mesh = m;
+  j.frame = f;
+  j.tp = *p;
+  b->jobs.push_back(j);
+  batch_drop_graph(b);
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_batch_add_paths(femrt_batch* b, femrt_mesh* m, femrt_frame* f, const float* cam12, int W,
+                                    int H, int spp, int max_depth, const femrt_material* mats, int n_mats,
+                                    unsigned long long seed)
+{
+  if (!b || !m || !f || !cam12 || !mats || n_mats < 1 || n_mats > 64 || max_depth < 1 || W <= 0 || H <= 0 ||
+      spp <= 0)
+    return FEMRT_EINVAL;
+  femrt_ctx* c = b->ctx;
+  if (m->ctx != c || f->ctx != c) return fail(c, FEMRT_EINVAL, "batch_add_paths: objects of another context");
+  if ((long long)W * H * spp != f->n) return fail(c, FEMRT_EINVAL, "batch_add_paths: frame must hold W*H*spp rays");
+  if (f->counters) return fail(c, FEMRT_EUNSUP, "batch_add_paths: switch the frame's work counters off");
+  BatchJob j;
+  j.kind = 1;
+  j.mesh = m;
+  j.frame = f;
+  std::memcpy(j.cam12, cam12, sizeof(j.cam12));
+  j.W = W;
+  j.H = H;
+  j.spp = spp;
+  j.depth = max_depth;
+  j.mats.assign(mats, mats + n_mats);
+  j.seed = seed;
+  // path buffers and the material table exist before any capture
+  int rc = femrt_frame_path_begin(f);
+  if (rc == FEMRT_OK) rc = femrt_frame_set_materials(f, mats, n_mats);
+  if (rc) return rc;
+  b->jobs.push_back(j);
+  batch_drop_graph(b);
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_batch_capture(femrt_batch* b)
+{
+  if (!b) return FEMRT_EINVAL;
+  femrt_ctx* c = b->ctx;
+  if (b->jobs.empty()) return fail(c, FEMRT_ESTATE, "batch_capture: no jobs");
+  batch_drop_graph(b);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  int rc = FEMRT_OK;
+  for (const BatchJob& j : b->jobs) {
+    rc = enqueue_job(j);
+    if (rc) break;
+  }
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess || !g) return fail(c, FEMRT_EHIP, std::string("batch_capture: ") + hipGetErrorString(e));
+  b->graph = g;
+  if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    batch_drop_graph(b);
+    return fail(c, FEMRT_EHIP, "batch_capture: graph instantiation failed");
+  }
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_batch_launch(femrt_batch* b)
+{
+  if (!b) return FEMRT_EINVAL;
+  femrt_ctx* c = b->ctx;
+  if (!b->exec) return fail(c, FEMRT_ESTATE, "batch_launch: capture the batch first");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipGraphLaunch(b->exec, c->stream));
+  return FEMRT_OK;
+}
+
+// ------------------------------------------------------------- autotuner
+
+FEMRT_API int femrt_mesh_get_launch_cfg(const femrt_mesh* m, int* cfg3)
+{
+  if (!m || !cfg3) return FEMRT_EINVAL;
+  cfg3[0] = m->cfg.hit_block;
+  cfg3[1] = m->cfg.march_bpc;
+  cfg3[2] = m->cfg.polish_bpc;
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_mesh_set_launch_cfg(femrt_mesh* m, const int* cfg3)
+{
+  if (!m || !cfg3) return FEMRT_EINVAL;
+  if ((cfg3[0] != 64 && cfg3[0] != 128 && cfg3[0] != 256) || cfg3[1] < 1 || cfg3[1] > 64 || cfg3[2] < 1 ||
+      cfg3[2] > 64)
+    return fail(m->ctx, FEMRT_EINVAL, "set_launch_cfg: block 64/128/256, blocks per CU 1..64");
+  m->cfg.hit_block = cfg3[0];
+  m->cfg.march_bpc = cfg3[1];
+  m->cfg.polish_bpc = cfg3[2];
+  return FEMRT_OK;
+}
+
+// Times each trace kernel under candidate launch shapes on the given job
+// (one shape at a time, the others held) and keeps the fastest on the mesh.
+FEMRT_API int femrt_autotune(femrt_mesh* m, femrt_frame* f, const femrt_trace_params* p, int reps, int* cfg3)
+{
+  if (!m || !f || !p || reps < 1) return FEMRT_EINVAL;
+  femrt_ctx* c = m->ctx;
+  if (c->capturing) return fail(c, FEMRT_ESTATE, "autotune: not while a batch records");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  const bool was_timing = c->timing;
+  const bool was_counting = f->counters;
+  const auto saved_stats = c->stats;
+  f->counters = false;
+  // total event time of `kernel` over reps traces, after one untimed trace
+  auto measure = [&](const char* kernel, double& ms, double* iso_ms) -> int {
+    c->timing = false;
+    int rc = femrt_trace(m, f, p);
+    if (rc) return rc;
+    c->timing = true;
+    c->stats.clear();
+    for (int r = 0; r < reps && rc == FEMRT_OK; ++r) rc = femrt_trace(m, f, p);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    if (rc) return rc;
+    ms = c->stats.count(kernel) ? c->stats[kernel].first : 1e30;
+    if (iso_ms) *iso_ms = c->stats.count("k_polish_iso") ? c->stats["k_polish_iso"].first : 0.0;
+    return FEMRT_OK;
+  };
+  LaunchCfg best = m->cfg;
+  int rc = FEMRT_OK;
+  {
+    double bt = 1e300;
+    for (int hb : {64, 128, 256}) {
+      m->cfg = best;
+      m->cfg.hit_block = hb;
+      double ms = 0;
+      if ((rc = measure("k_first_hit", ms, nullptr))) break;
+      if (ms < bt) {
+        bt = ms;
+        best.hit_block = hb;
+      }
+    }
+  }
+  if (rc == FEMRT_OK && p->iso_enable && p->n_iso > 0) {
+    double bt = 1e300;
+    for (int bpc : {2, 4, 8, 16}) {
+      m->cfg = best;
+      m->cfg.march_bpc = bpc;
+      double ms = 0;
+      if ((rc = measure("k_march", ms, nullptr))) break;
+      if (ms < bt) {
+        bt = ms;
+        best.march_bpc = bpc;
+      }
+    }
+  }
+  if (rc == FEMRT_OK) {
+    double bt = 1e300;
+    for (int bpc : {4, 8, 16}) {
+      m->cfg = best;
+      m->cfg.polish_bpc = bpc;
+      double ms = 0, ms2 = 0;
+      if ((rc = measure("k_polish_hit", ms, &ms2))) break;
+      if (ms + ms2 < bt) {
+        bt = ms + ms2;
+        best.polish_bpc = bpc;
+      }
+    }
+  }
+  m->cfg = best;
+  c->stats = saved_stats;
+  c->timing = was_timing;
+  f->counters = was_counting;
+  if (rc) return rc;
+  if (cfg3) {
+    cfg3[0] = best.hit_block;
+    cfg3[1] = best.march_bpc;
+    cfg3[2] = best.polish_bpc;
+  }
+  return FEMRT_OK;
+}
+
+}  // extern "C"

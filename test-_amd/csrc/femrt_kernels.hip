@@ -464,6 +464,9 @@ FEMRT_API int femrt_ctx_create(int device, femrt_ctx** out)
     delete c;
     return FEMRT_EHIP;
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->cus = cus;
   *out = c;
   return FEMRT_OK;
 }
@@ -777,7 +780,7 @@ FEMRT_API int femrt_frame_destroy(femrt_frame* f)
   void* ptrs[] = {f->d_rays, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst,
                   f->d_hit_u, f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, f->d_iso_f32,
                   f->d_iso_rst32, f->d_queue_hit, f->d_queue_iso, f->d_counts, f->d_rgba, f->d_work,
-                  f->d_path_tp, f->d_path_acc, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
+                  f->d_path_tp, f->d_path_acc, f->d_mats, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
                   f->d_walk_sorted, f->d_walk_reent, f->d_walk_hist};
   for (void* p : ptrs) hipFree(p);
   delete f;
@@ -864,7 +867,9 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
     return fail(c, FEMRT_ESTATE, "trace: learned guesses need a network (femrt_mesh_set_guess_mlp)");
   if (learned && p->samples > FEMRT_MLP_MAX_SAMPLES)
     return fail(c, FEMRT_EUNSUP, "trace: learned guesses support at most 16 samples per segment");
-  HIPCHK(c, hipSetDevice(c->device));
+  if (c->capturing && (f->counters || (p->flags & FEMRT_TRACE_BINNED_WALK)))
+    return fail(c, FEMRT_EUNSUP, "trace: work counters and the binned walk cannot be captured in a batch");
+  if (!c->capturing) HIPCHK(c, hipSetDevice(c->device));
   const int n = f->n;
   const int n_iso = p->iso_enable ? p->n_iso : 0;
   long long* work = f->counters ? f->d_work : nullptr;
@@ -874,9 +879,10 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
     k_clear_frame<<<blocks, 256, 0, c->stream>>>(n, FEMRT_MAX_ISO, f->d_hit_elem, f->d_iso_elem, f->d_counts);
   });
   if (rc) return rc;
-  int dev_cus = 256;
-  hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, c->device);
-  const int pblocks = dev_cus * 8;
+  const int dev_cus = c->cus;
+  const LaunchCfg& cfg = m->cfg;
+  const int pblocks = dev_cus * cfg.polish_bpc;
+  const int hblock = cfg.hit_block;
   IsoParams ip;
   std::memset(&ip, 0, sizeof(ip));
   for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
@@ -899,7 +905,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   rc = FEMRT_EUNSUP;
   FEMRT_DISPATCH_NG(m->NG, {
     rc = timed_launch(c, "k_first_hit", [&] {
-      k_first_hit<NG_><<<blocks, 256, 0, c->stream>>>(
+      k_first_hit<NG_><<<(n + hblock - 1) / hblock, hblock, 0, c->stream>>>(
           f->d_rays, n, TM, p->guess, p->t_min, p->t_max, f->d_hit_elem, f->d_hit_face,
           f->d_hit_f32, f->d_queue_hit, f->d_counts, work);
     });
@@ -916,7 +922,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
           rc = walk_run(c, m, f, TM, ip, learned, work);
         } else {
           // persistent: lanes refill from the hit queue until it drains
-          const int mblocks = dev_cus * 4;
+          const int mblocks = dev_cus * cfg.march_bpc;
           rc = timed_launch(c, "k_march", [&] {
             if (learned)
               k_march<NG_, NF_, true><<<mblocks, 256, 0, c->stream>>>(

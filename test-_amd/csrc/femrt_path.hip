@@ -252,7 +252,6 @@ FEMRT_API int femrt_frame_camera_rays_jitter(femrt_frame* f, const float* cam12,
   femrt_ctx* c = f->ctx;
   if ((long long)W * H * spp != f->n)
     return fail(c, FEMRT_EINVAL, "camera_rays_jitter: frame must hold width*height*spp rays");
-  HIPCHK(c, hipSetDevice(c->device));
   const float4 o = make_float4(cam12[0], cam12[1], cam12[2], 0.f);
   const float4 ll = make_float4(cam12[3], cam12[4], cam12[5], 0.f);
   const float4 h = make_float4(cam12[6], cam12[7], cam12[8], 0.f);
@@ -266,8 +265,9 @@ FEMRT_API int femrt_frame_path_begin(femrt_frame* f)
 {
   if (!f) return FEMRT_EINVAL;
   femrt_ctx* c = f->ctx;
-  HIPCHK(c, hipSetDevice(c->device));
   if (!f->d_path_tp) {
+    if (c->capturing) return fail(c, FEMRT_ESTATE, "path_begin: path buffers must exist before a batch records");
+    HIPCHK(c, hipSetDevice(c->device));
     if (hipMalloc(&f->d_path_tp, (size_t)f->n * 16) != hipSuccess ||
         hipMalloc(&f->d_path_acc, (size_t)f->n * 16) != hipSuccess)
       return fail(c, FEMRT_ENOMEM, "path_begin: hipMalloc failed");
@@ -277,60 +277,77 @@ FEMRT_API int femrt_frame_path_begin(femrt_frame* f)
   });
 }
 
+// The material table lives in the frame and is uploaded when it changes
+// (never while a batch records the stream).
+static int set_materials(femrt_ctx* c, femrt_frame* f, const femrt_material* mats, int n_mats)
+{
+  std::vector<Mat> hm(n_mats);
+  for (int i = 0; i < n_mats; ++i) {
+    std::memset(&hm[i], 0, sizeof(Mat));
+    std::memcpy(hm[i].color, mats[i].color, 12);
+    std::memcpy(hm[i].emission, mats[i].emission, 12);
+    hm[i].roughness = mats[i].roughness;
+    hm[i].ior = mats[i].ior;
+    hm[i].type = mats[i].type;
+    if (hm[i].type < 0 || hm[i].type > 2) return fail(c, FEMRT_EINVAL, "materials: type must be 0..2");
+  }
+  const unsigned char* bytes = reinterpret_cast<const unsigned char*>(hm.data());
+  const size_t nb = n_mats * sizeof(Mat);
+  if (f->h_mats.size() == nb && std::memcmp(f->h_mats.data(), bytes, nb) == 0) return FEMRT_OK;
+  if (c->capturing) return fail(c, FEMRT_ESTATE, "materials: set them before a batch is captured");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!f->d_mats && hipMalloc(&f->d_mats, 64 * sizeof(Mat)) != hipSuccess)
+    return fail(c, FEMRT_ENOMEM, "materials: hipMalloc failed");
+  HIPCHK(c, hipMemcpyAsync(f->d_mats, bytes, nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  f->h_mats.assign(bytes, bytes + nb);
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_frame_set_materials(femrt_frame* f, const femrt_material* mats, int n_mats)
+{
+  if (!f || !mats || n_mats < 1 || n_mats > 64) return FEMRT_EINVAL;
+  return set_materials(f->ctx, f, mats, n_mats);
+}
+
 FEMRT_API int femrt_frame_scatter(femrt_mesh* m, femrt_frame* f, const femrt_material* mats, int n_mats,
                                   unsigned long long seed, int depth, int last)
 {
   if (!m || !f || !mats || n_mats < 1 || n_mats > 64 || depth < 0) return FEMRT_EINVAL;
   femrt_ctx* c = f->ctx;
   if (!f->d_path_tp) return fail(c, FEMRT_ESTATE, "scatter: call femrt_frame_path_begin first");
-  HIPCHK(c, hipSetDevice(c->device));
-  std::vector<Mat> hm(n_mats);
-  for (int i = 0; i < n_mats; ++i) {
-    std::memcpy(hm[i].color, mats[i].color, 12);
-    std::memcpy(hm[i].emission, mats[i].emission, 12);
-    hm[i].roughness = mats[i].roughness;
-    hm[i].ior = mats[i].ior;
-    hm[i].type = mats[i].type;
-    if (hm[i].type < 0 || hm[i].type > 2) return fail(c, FEMRT_EINVAL, "scatter: material type must be 0..2");
-  }
-  Mat* dm = nullptr;
-  HIPCHK(c, hipMalloc(&dm, n_mats * sizeof(Mat)));
-  int rc = FEMRT_OK;
-  if (hipMemcpyAsync(dm, hm.data(), n_mats * sizeof(Mat), hipMemcpyHostToDevice, c->stream) != hipSuccess)
-    rc = fail(c, FEMRT_EHIP, "scatter: upload failed");
+  const int mrc = set_materials(c, f, mats, n_mats);
+  if (mrc) return mrc;
+  const Mat* dm = static_cast<const Mat*>(f->d_mats);
   TraceMesh TM{m->d_geom, m->d_field, m->d_boxes, m->d_verts, m->d_adj, m->d_nodes, m->d_prims};
-  if (rc == FEMRT_OK) {
-    rc = FEMRT_EUNSUP;
-    auto go = [&](auto tag) {
-      constexpr int NG_ = decltype(tag)::value;
-      rc = timed_launch(c, "k_scatter", [&] {
-        k_scatter<NG_><<<(f->n + 255) / 256, 256, 0, c->stream>>>(
-            f->n, f->d_rays, TM, f->d_hit_elem, f->d_hit_face, f->d_hit_t, f->d_hit_rst, dm, n_mats, seed,
-            depth, last, f->d_path_tp, f->d_path_acc);
-      });
-    };
-    switch (m->NG) {
-      case 4: go(std::integral_constant<int, 4>()); break;
-      case 5: go(std::integral_constant<int, 5>()); break;
-      case 7: go(std::integral_constant<int, 7>()); break;
-    }
+  int rc = FEMRT_EUNSUP;
+  auto go = [&](auto tag) {
+    constexpr int NG_ = decltype(tag)::value;
+    rc = timed_launch(c, "k_scatter", [&] {
+      k_scatter<NG_><<<(f->n + 255) / 256, 256, 0, c->stream>>>(
+          f->n, f->d_rays, TM, f->d_hit_elem, f->d_hit_face, f->d_hit_t, f->d_hit_rst, dm, n_mats, seed,
+          depth, last, f->d_path_tp, f->d_path_acc);
+    });
+  };
+  switch (m->NG) {
+    case 4: go(std::integral_constant<int, 4>()); break;
+    case 5: go(std::integral_constant<int, 5>()); break;
+    case 7: go(std::integral_constant<int, 7>()); break;
   }
-  (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(dm);
   return rc;
 }
 
 FEMRT_API int femrt_frame_resolve(femrt_frame* f, int W, int H, int spp, uint32_t* rgba)
 {
-  if (!f || !rgba || W <= 0 || H <= 0 || spp <= 0) return FEMRT_EINVAL;
+  if (!f || W <= 0 || H <= 0 || spp <= 0) return FEMRT_EINVAL;
   femrt_ctx* c = f->ctx;
   if ((long long)W * H * spp != f->n) return fail(c, FEMRT_EINVAL, "resolve: frame must hold width*height*spp paths");
   if (!f->d_path_acc) return fail(c, FEMRT_ESTATE, "resolve: no paths traced");
-  HIPCHK(c, hipSetDevice(c->device));
+  if (rgba && c->capturing) return fail(c, FEMRT_EINVAL, "resolve: no host copy while a batch records");
   const int rc = timed_launch(c, "k_resolve", [&] {
     k_resolve<<<(W * H + 255) / 256, 256, 0, c->stream>>>(W, H, spp, f->d_path_acc, f->d_rgba);
   });
-  if (rc) return rc;
+  if (rc || !rgba) return rc;
   HIPCHK(c, hipMemcpyAsync(rgba, f->d_rgba, (size_t)W * H * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FEMRT_OK;

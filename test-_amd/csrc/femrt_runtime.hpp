@@ -22,9 +22,11 @@ struct TimedLaunch {
 
 struct femrt_ctx {
   int device = 0;
+  int cus = 256;               // compute units of the device
   hipStream_t stream = nullptr;
   std::string err;
   bool timing = false;
+  bool capturing = false;      // a batch is recording the stream into a graph
   std::vector<TimedLaunch> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> stats;
@@ -43,8 +45,16 @@ struct femrt_mlp {
   long long adam_t = 0;
 };
 
+// Launch shapes of the trace kernels (chosen per mesh by the autotuner).
+struct LaunchCfg {
+  int hit_block = 256;     // k_first_hit / k_raygen threads per block
+  int march_bpc = 4;       // persistent walk: blocks of 256 per compute unit
+  int polish_bpc = 8;      // polish kernels: blocks of 256 per compute unit
+};
+
 struct femrt_mesh {
   femrt_ctx* ctx = nullptr;
+  LaunchCfg cfg;
   const femrt_mlp* guess_mlp = nullptr;  // network used by FEMRT_GUESS_LEARNED
   int n_elem = 0, ng = 0, nf = 0;      // true mode counts
   int NG = 0, NF = 0;                  // kernel mode counts (padded)
@@ -85,6 +95,8 @@ struct femrt_frame {
   bool counters = false;
   float4* d_path_tp = nullptr;   // per path: throughput rgb, alive flag (allocated by path_begin)
   float4* d_path_acc = nullptr;  // per path: accumulated radiance rgb
+  void* d_mats = nullptr;        // scatter materials (64 slots), uploaded when they change
+  std::vector<unsigned char> h_mats;
   // element-walk rounds (allocated by the first trace with isovalues)
   int4* d_walk_si = nullptr;     // per ray: element, entry face, step, found | redo << 8
   float4* d_walk_sf = nullptr;   // per ray: t_in, u_in, v_in
@@ -142,7 +154,7 @@ template <typename F>
 static int timed_launch(femrt_ctx* c, const char* name, F&& launch)
 {
   hipEvent_t a = nullptr, b = nullptr;
-  if (c->timing) {
+  if (c->timing && !c->capturing) {
     a = take_event(c);
     b = take_event(c);
     if (a) (void)hipEventRecord(a, c->stream);
@@ -150,7 +162,7 @@ static int timed_launch(femrt_ctx* c, const char* name, F&& launch)
   launch();
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, FEMRT_EHIP, std::string("launch ") + name + ": " + hipGetErrorString(e));
-  if (c->timing && a && b) {
+  if (c->timing && !c->capturing && a && b) {
     (void)hipEventRecord(b, c->stream);
     c->pending.push_back({name, a, b});
   }
