@@ -36,6 +36,40 @@ double modA(int p, double x)
   return b * pk;
 }
 
+// Per element, an upper bound of |u(a) - u(b)| / |a - b|_inf over the
+// reference cube: sum_ijk |c_ijk| (D_i B_j B_k + B_i D_j B_k + B_i B_j D_k)
+// with B_p = max |phi_p| and D_p = max |phi_p'| on [-1, 1], taken on a dense
+// grid (central differences) with a 5% margin.
+void field_lipschitz(int n_elem, int nf, const float* field, float* out)
+{
+  std::vector<double> B(nf, 0.0), D(nf, 0.0);
+  const int G = 4000;
+  const double h = 1e-6;
+  for (int p = 0; p < nf; ++p)
+    for (int g = 0; g <= G; ++g) {
+      const double x = -1.0 + 2.0 * g / G;
+      B[p] = std::max(B[p], std::fabs(modA(p, x)));
+      const double xa = std::max(-1.0, x - h), xb = std::min(1.0, x + h);
+      D[p] = std::max(D[p], std::fabs((modA(p, xb) - modA(p, xa)) / (xb - xa)));
+    }
+  for (int p = 0; p < nf; ++p) {
+    B[p] *= 1.05;
+    D[p] *= 1.05;
+  }
+  const size_t F3 = (size_t)nf * nf * nf;
+  for (int e = 0; e < n_elem; ++e) {
+    const float* c = field + (size_t)e * F3;
+    double L = 0.0;
+    for (int k = 0; k < nf; ++k)
+      for (int j = 0; j < nf; ++j)
+        for (int i = 0; i < nf; ++i) {
+          const double a = std::fabs((double)c[((size_t)k * nf + j) * nf + i]);
+          L += a * (D[i] * B[j] * B[k] + B[i] * D[j] * B[k] + B[i] * B[j] * D[k]);
+        }
+    out[e] = (float)(L * (1.0 + 1e-6));
+  }
+}
+
 static double legendre(int n, double x, double* pnm1)
 {
   double p0 = 1.0, p1 = x;

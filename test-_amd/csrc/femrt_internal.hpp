@@ -29,6 +29,7 @@ void build_bvh_sah(int n_elem, const float* boxes /* n x FEMRT_NBOX x 6, element
 int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom, float* field,
                        int* adjacency);
 void encode_adjacency(int n_elem, const float* vertices, const int* adjacency, int* out);
+void field_lipschitz(int n_elem, int nf, const float* field, float* out /* n */);
 std::vector<double> gll_nodes(int n);
 bool modal_fit_1d_matrix(int n, const std::vector<double>& nodes, std::vector<double>& inv);
 double modA(int p, double x);

@@ -63,15 +63,40 @@ __global__ void k_raygen(float* __restrict__ rays, int W, int H, float ox, float
   rays[5 * n + p] = d2 * il;
 }
 
+// Thread -> ray order of the first-hit pass.  With an image layout (W > 0)
+// consecutive threads cover 8x8-pixel tiles (the spp samples of a pixel
+// adjacent), so a wave traces one tile and the hit queue, which the walk
+// consumes in order, stays tile-coherent; otherwise ray order.
+struct TileOrder {
+  int W, H, spp, tiles_x;
+  __host__ __device__ int threads(int n) const
+  {
+    if (W <= 0) return n;
+    return tiles_x * ((H + 7) >> 3) * 64 * spp;
+  }
+  __device__ bool ray(int t, int n, int& r) const
+  {
+    if (W <= 0) {
+      r = t;
+      return t < n;
+    }
+    const int s = t % spp, pix = t / spp;
+    const int tile = pix >> 6, in = pix & 63;
+    const int x = (tile % tiles_x) * 8 + (in & 7), y = (tile / tiles_x) * 8 + (in >> 3);
+    r = (y * W + x) * spp + s;
+    return x < W && y < H;
+  }
+};
+
 template <int NG>
 __global__ __launch_bounds__(256) void k_first_hit(const float* __restrict__ rays, int n, TraceMesh M,
-                                                   int guess, float tmin, float tmax,
+                                                   int guess, float tmin, float tmax, TileOrder order,
                                                    int* __restrict__ hit_elem, int* __restrict__ hit_face,
                                                    float4* __restrict__ hit_f32, int* __restrict__ queue,
                                                    int* __restrict__ counts, long long* __restrict__ work)
 {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = r < n;
+  int r = 0;
+  const bool active = order.ray(blockIdx.x * blockDim.x + threadIdx.x, n, r);
   Work w;
   HitF32 h;
   if (active) {
@@ -554,8 +579,11 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   pad_coeffs(field, nf, NF, 1, n_elem, fp, 1);
   m->h_boxes.resize((size_t)n_elem * FEMRT_BOX_STRIDE);
   bernstein_boxes(n_elem, ng, geom, m->h_boxes.data());
+  std::vector<float> lip(n_elem);
+  field_lipschitz(n_elem, nf, field, lip.data());
   auto cleanup = [&](const std::string& msg) {
     hipFree(m->d_geom); hipFree(m->d_field); hipFree(m->d_verts); hipFree(m->d_boxes); hipFree(m->d_adj);
+    hipFree(m->d_lip);
     delete m;
     return fail(c, FEMRT_ENOMEM, msg);
   };
@@ -564,10 +592,12 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   if (hipMalloc(&m->d_verts, (size_t)n_elem * 24 * 4) != hipSuccess) return cleanup("hipMalloc verts");
   if (hipMalloc(&m->d_boxes, m->h_boxes.size() * 4) != hipSuccess) return cleanup("hipMalloc boxes");
   if (adjacency && hipMalloc(&m->d_adj, (size_t)n_elem * 12 * 4) != hipSuccess) return cleanup("hipMalloc adj");
+  if (hipMalloc(&m->d_lip, (size_t)n_elem * 4) != hipSuccess) return cleanup("hipMalloc lip");
   bool ok = hipMemcpy(m->d_geom, g4.data(), g4.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_field, fp.data(), fp.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_verts, vertices, (size_t)n_elem * 24 * 4, hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(m->d_boxes, m->h_boxes.data(), m->h_boxes.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+            hipMemcpy(m->d_boxes, m->h_boxes.data(), m->h_boxes.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(m->d_lip, lip.data(), lip.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
   if (ok && adjacency) {
     std::vector<int> adj((size_t)n_elem * 12);
     encode_adjacency(n_elem, vertices, adjacency, adj.data());
@@ -588,6 +618,7 @@ FEMRT_API int femrt_mesh_destroy(femrt_mesh* m)
   hipFree(m->d_verts);
   hipFree(m->d_boxes);
   hipFree(m->d_adj);
+  hipFree(m->d_lip);
   hipFree(m->d_nodes);
   hipFree(m->d_prims);
   delete m;
@@ -794,6 +825,8 @@ FEMRT_API int femrt_frame_set_rays(femrt_frame* f, const float* rays)
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(f->d_rays, rays, 6 * (size_t)f->n * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  f->img_w = f->img_h = 0;
+  f->img_spp = 1;
   return FEMRT_OK;
 }
 
@@ -832,6 +865,9 @@ FEMRT_API int femrt_frame_camera_rays(femrt_frame* f, const float* cam, int W, i
   }
   HIPCHK(c, hipSetDevice(c->device));
   const int n = W * H;
+  f->img_w = W;
+  f->img_h = H;
+  f->img_spp = 1;
   return timed_launch(c, "k_raygen", [&] {
     k_raygen<<<(n + 255) / 256, 256, 0, c->stream>>>(f->d_rays, W, H, cam[0], cam[1], cam[2], llc[0],
                                                     llc[1], llc[2], hor[0], hor[1], hor[2], ver[0],
@@ -881,6 +917,9 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   if (rc) return rc;
   const int dev_cus = c->cus;
   const LaunchCfg& cfg = m->cfg;
+  TileOrder order{0, 0, 1, 0};
+  if (f->img_w > 0 && (long long)f->img_w * f->img_h * f->img_spp == n)
+    order = TileOrder{f->img_w, f->img_h, f->img_spp, (f->img_w + 7) >> 3};
   const int pblocks = dev_cus * cfg.polish_bpc;
   const int hblock = cfg.hit_block;
   IsoParams ip;
@@ -902,11 +941,12 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   TM.adj = m->d_adj;
   TM.nodes = m->d_nodes;
   TM.prims = m->d_prims;
+  TM.lip = m->d_lip;
   rc = FEMRT_EUNSUP;
   FEMRT_DISPATCH_NG(m->NG, {
     rc = timed_launch(c, "k_first_hit", [&] {
-      k_first_hit<NG_><<<(n + hblock - 1) / hblock, hblock, 0, c->stream>>>(
-          f->d_rays, n, TM, p->guess, p->t_min, p->t_max, f->d_hit_elem, f->d_hit_face,
+      k_first_hit<NG_><<<(order.threads(n) + hblock - 1) / hblock, hblock, 0, c->stream>>>(
+          f->d_rays, n, TM, p->guess, p->t_min, p->t_max, order, f->d_hit_elem, f->d_hit_face,
           f->d_hit_f32, f->d_queue_hit, f->d_counts, work);
     });
     if (rc) break;

@@ -252,6 +252,9 @@ FEMRT_API int femrt_frame_camera_rays_jitter(femrt_frame* f, const float* cam12,
   femrt_ctx* c = f->ctx;
   if ((long long)W * H * spp != f->n)
     return fail(c, FEMRT_EINVAL, "camera_rays_jitter: frame must hold width*height*spp rays");
+  f->img_w = W;
+  f->img_h = H;
+  f->img_spp = spp;
   const float4 o = make_float4(cam12[0], cam12[1], cam12[2], 0.f);
   const float4 ll = make_float4(cam12[3], cam12[4], cam12[5], 0.f);
   const float4 h = make_float4(cam12[6], cam12[7], cam12[8], 0.f);

@@ -63,6 +63,7 @@ struct femrt_mesh {
   float* d_verts = nullptr;            // n_elem x 24
   float* d_boxes = nullptr;            // n_elem x FEMRT_BOX_STRIDE
   int* d_adj = nullptr;                // n_elem x 12
+  float* d_lip = nullptr;              // n_elem: Lipschitz bound of the field in rst
   float4* d_nodes = nullptr;           // n_nodes x 2
   int* d_prims = nullptr;
   int n_nodes = 0;
@@ -74,6 +75,10 @@ struct femrt_mesh {
 struct femrt_frame {
   femrt_ctx* ctx = nullptr;
   int n = 0;
+  // image layout of the rays when a camera made them (ray (y*img_w + x)*img_spp
+  // + s); 0 when the caller set them.  Traces then visit the rays in 8x8-pixel
+  // tiles so that a wave's lanes walk neighbouring elements.
+  int img_w = 0, img_h = 0, img_spp = 1;
   float* d_rays = nullptr;
   int* d_hit_elem = nullptr;
   int* d_hit_face = nullptr;

@@ -25,6 +25,7 @@ struct TraceMesh {
   const int* adj;       // n_elem x 12 (neighbour, neighbour face) or null
   const float4* nodes;  // BVH nodes x 2
   const int* prims;
+  const float* lip;     // n_elem: bound of |u(a) - u(b)| / |a - b|_inf, or null
 };
 
 struct IsoParams {
@@ -385,10 +386,20 @@ FEMRT_HD void polish_hit(const float4* __restrict__ gc, const double* o, const d
 // Newton x -> rst in FP32 from the guess in rst (HexElement.cu:227-272:
 // undamped, |delta|_inf < 1e-5, <= 50 iterations).  J holds the Jacobian at
 // the last evaluated iterate.
-template <int NG>
+//
+// `cert(rst, step)` may end the iteration early: it is called once a step is
+// below 1e-2 (and not yet below the tolerance) and returns true to accept
+// the iterate (see SampleCert).
+struct NoCert {
+  FEMRT_HD void begin() {}
+  FEMRT_HD bool operator()(const float*, float) { return false; }
+};
+
+template <int NG, typename Cert>
 FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
-                         unsigned& iters)
+                         unsigned& iters, Cert& cert)
 {
+  cert.begin();
   for (int it = 0; it < 50; ++it) {
     float X[3];
     geom_eval<float, NG>(gc, rst[0], rst[1], rst[2], X, J);
@@ -401,8 +412,18 @@ FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* r
     rst[2] -= dl[2];
     if (!(fabsf(rst[0]) < 1e4f && fabsf(rst[1]) < 1e4f && fabsf(rst[2]) < 1e4f)) return false;
     if (fabsf(dl[0]) < 1e-5f && fabsf(dl[1]) < 1e-5f && fabsf(dl[2]) < 1e-5f) return true;
+    const float st = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
+    if (st < 1e-2f && cert(rst, st)) return true;
   }
   return false;
+}
+
+template <int NG>
+FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
+                         unsigned& iters)
+{
+  NoCert nc;
+  return invert_f32<NG>(gc, x, rst, J, iters, nc);
 }
 
 // The preimage of x inside the element (unique for a valid element): Newton
@@ -410,16 +431,16 @@ FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* r
 // rst = 0 finds spurious preimages of the polynomial map for points near a
 // face) or fails, restart from rst = 0 (when zero_retry: the plain fallback
 // of a learned guess), then from the 8 octant centres (+-0.5)^3.
-template <int NG>
+template <int NG, typename Cert>
 FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
-                                 unsigned& iters, bool zero_retry = false)
+                                 unsigned& iters, bool zero_retry, Cert& cert)
 {
-  if (invert_f32<NG>(gc, x, rst, J, iters) &&
+  if (invert_f32<NG>(gc, x, rst, J, iters, cert) &&
       fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
     return true;
   if (zero_retry) {
     rst[0] = rst[1] = rst[2] = 0.f;
-    if (invert_f32<NG>(gc, x, rst, J, iters) &&
+    if (invert_f32<NG>(gc, x, rst, J, iters, cert) &&
         fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
       return true;
   }
@@ -427,12 +448,55 @@ FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, 
     rst[0] = (s & 1) ? 0.5f : -0.5f;
     rst[1] = (s & 2) ? 0.5f : -0.5f;
     rst[2] = (s & 4) ? 0.5f : -0.5f;
-    if (invert_f32<NG>(gc, x, rst, J, iters) &&
+    if (invert_f32<NG>(gc, x, rst, J, iters, cert) &&
         fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
       return true;
   }
   return false;
 }
+
+template <int NG>
+FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
+                                 unsigned& iters, bool zero_retry = false)
+{
+  NoCert nc;
+  return invert_in_cube_f32<NG>(gc, x, rst, J, iters, zero_retry, nc);
+}
+
+// Early acceptance of a scan sample's inversion.  The scan needs only the
+// sign of g_k = u - iso_k at each sample.  Once a Newton step s is below 1e-2
+// the iterate is within about C s^2 << s of the preimage (quadratic
+// convergence), so |u(iterate) - u(preimage)| < lip * s; when every open
+// isovalue has |g_k| > 2 lip s the signs are those of the converged sample
+// and the iteration stops there.  One attempt per inversion (a failed one
+// costs a field evaluation); u of an accepted iterate is kept.
+template <int NF>
+struct SampleCert {
+  const float* fc;
+  const IsoParams* P;
+  float lip;
+  unsigned open;      // isovalues still searched (bit k)
+  unsigned* fevals;
+  bool tried = false, accepted = false;
+  float u = 0.f;
+  FEMRT_HD void begin()
+  {
+    tried = false;
+    accepted = false;
+  }
+  FEMRT_HD bool operator()(const float* rst, float st)
+  {
+    if (tried || !(lip > 0.f)) return false;
+    tried = true;
+    u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
+    ++*fevals;
+    const float margin = 2.f * lip * st;
+    for (int k = 0; k < FEMRT_MAX_ISO; ++k)
+      if (k < P->n_iso && ((open >> k) & 1u) && !(fabsf(u - P->iso[k]) > margin)) return false;
+    accepted = true;
+    return true;
+  }
+};
 
 // FP64 Newton x -> rst from a guess.  Converged when the step drops below
 // 1e-13, or once it is below 1e-9 and stops halving (the FP64 rounding floor
@@ -762,6 +826,8 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
 {
   const float4* gc = M.geom + (size_t)S.e * (NG * NG * NG);
   const float* fc = M.field + (size_t)S.e * (NF * NF * NF);
+  const float lip = M.lip ? M.lip[S.e] : 0.f;
+  const unsigned open_all = (1u << P.n_iso) - 1u;
   const int Ms = P.samples;
   float rst_in[3];
   face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
@@ -797,7 +863,8 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
       // samples: O(dt^2) from the preimage instead of O(dt)
       for (int c = 0; c < 3; ++c) rst[c] = fminf(fmaxf(2.f * rprev[c] - rprev2[c], -1.f), 1.f);
     }
-    if (!known && !invert_in_cube_f32<NG>(gc, x, rst, J, w.inv_it, zero_retry)) {
+    SampleCert<NF> cert{fc, &P, lip, open_all & ~(S.found | found_here), &w.field};
+    if (!known && !invert_in_cube_f32<NG>(gc, x, rst, J, w.inv_it, zero_retry, cert)) {
       // no preimage inside the element: the segment end is wrong
       if (!S.redo) return true;
       have_prev = false;
@@ -805,8 +872,13 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
       rst[0] = rst[1] = rst[2] = 0.f;
       continue;
     }
-    const float u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
-    ++w.field;
+    float u;
+    if (!known && cert.accepted) {
+      u = cert.u;
+    } else {
+      u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
+      ++w.field;
+    }
     for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
       if (k >= P.n_iso || ((S.found | found_here) >> k) & 1u) continue;
       const float g = u - P.iso[k];
