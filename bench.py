@@ -152,7 +152,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variants", default="learned,warm,plain")
+    ap.add_argument("--variants", default="learned,learned_every,warm,plain")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -175,11 +175,12 @@ def main():
     fr = femrt.Frame(ctx, n_rays)
     fr.camera_rays(CAM, W, H)
 
-    names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED}
+    names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED,
+             "learned_every": femrt.GUESS_LEARNED}
     variants = [v for v in args.variants.split(",") if v]
     results = {}
     mlp_info = None
-    if "learned" in variants:
+    if any(v.startswith("learned") for v in variants):
         # the guess network is trained on another mesh of the same family
         # (different seed, 16^3) by the library's own GPU kernels, untimed
         t0 = time.perf_counter()
@@ -199,7 +200,9 @@ def main():
                     "guess_err_p99_on_bench_mesh": float(np.percentile(err, 99))}
         print(json.dumps({"mlp": mlp_info}), file=sys.stderr, flush=True)
     for v in variants:
-        flags = 1 | (2 if args.binned_walk else 0)
+        flags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
+        if v == "learned_every":
+            flags |= femrt.TRACE_LEARNED_EVERY_SAMPLE
         wall, kt, cnt = run_variant(femrt, ctx, mesh, fr, names[v], args.steps, args.warmup, flags)
         f32, f64 = kernel_flops(cnt, hm.ng, hm.nf)
         results[v] = dict(wall=wall, kernels=kt, counters=cnt, f32=f32, f64=f64,

@@ -121,7 +121,10 @@ enum {
   FEMRT_GUESS_PLAIN = 0,   /* reference initial guesses: face seeds {-0.8,0,0.8}^2,
                               point inversion from rst = 0 */
   FEMRT_GUESS_WARM = 1,    /* engineered: warm starts along the ray + bilinear seeds */
-  FEMRT_GUESS_LEARNED = 2  /* MLP initial guesses (femrt_mlp_*), plain fallback */
+  FEMRT_GUESS_LEARNED = 2  /* MLP initial guesses (femrt_mlp_*) for the point
+                              inversions with no earlier sample to start from
+                              (all of them with FEMRT_TRACE_LEARNED_EVERY_SAMPLE),
+                              warm starts elsewhere, plain fallback */
 };
 
 #define FEMRT_MAX_ISO 8
@@ -144,6 +147,11 @@ typedef struct {
                                           element, re-entries in their own kernel,
                                           instead of one persistent kernel that
                                           walks each ray to its end                 */
+#define FEMRT_TRACE_LEARNED_EVERY_SAMPLE 4 /* GUESS_LEARNED: network guesses for every
+                                         interior scan sample; by default only
+                                         the first of each segment takes one (no
+                                         earlier sample to extrapolate from) and
+                                         the rest extrapolated warm starts */
 
 FEMRT_API void femrt_trace_params_default(femrt_trace_params* p);
 

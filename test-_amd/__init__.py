@@ -19,7 +19,7 @@ LIB_PATH = PKG_DIR / "_build" / "libfemrt.so"
 OK = 0
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "ESTATE"}
 GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
-TRACE_RGBA, TRACE_BINNED_WALK = 1, 2     # femrt_trace_params.flags bits
+TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE = 1, 2, 4   # trace flag bits
 MAX_ISO = 8
 NCOUNTERS = 16
 COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",

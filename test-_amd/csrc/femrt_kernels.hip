@@ -168,8 +168,10 @@ struct GlobalSink {
 // Network guesses of the interior samples of the lane's current segment.
 struct LdsGuess {
   const _Float16* g;  // (samples - 1) x 3
+  int lead;           // samples 1..lead carry a guess
   __device__ bool get(int j, float* rst) const
   {
+    if (j > lead) return false;
     rst[0] = (float)g[(j - 1) * 3 + 0];
     rst[1] = (float)g[(j - 1) * 3 + 1];
     rst[2] = (float)g[(j - 1) * 3 + 2];
@@ -190,8 +192,9 @@ __device__ __forceinline__ void wave_lds_sync()
 // by one element segment (exit, samples, next element); a lane whose ray
 // ended takes the next hit ray from the queue (one atomic per wave), so the
 // lanes stay busy until the queue drains.  With LEARNED, the point-inversion
-// guesses of the round's interior samples come from the network, evaluated
-// on the matrix cores for the whole wave between the exit and sample steps.
+// guesses of the first P.mlp_lead interior samples of the round's segments
+// come from the network, evaluated on the matrix cores for the whole wave
+// between the exit and sample steps (later samples extrapolate).
 template <int NG, int NF, bool LEARNED>
 __global__ __launch_bounds__(256, 2) void k_march(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ queue,
@@ -256,7 +259,8 @@ __global__ __launch_bounds__(256, 2) void k_march(
       const bool seg = active && S.ok;
       const float* bx = M.boxes + (size_t)(seg ? S.e : 0) * FEMRT_BOX_STRIDE;
       const float* vt = M.verts + (size_t)(seg ? S.e : 0) * 24;
-      for (int j = 1; j < P.samples; ++j) {
+      const int jmax = min(P.mlp_lead, P.samples - 1);
+      for (int j = 1; j <= jmax; ++j) {
         float f[MLP_IN];
         if (seg) {
           const float t = march_sample_t(S, j, P.samples);
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(256, 2) void k_march(
     if (active) {
       bool redo;
       if constexpr (LEARNED)
-        redo = S.ok && march_samples<NG, NF>(M, P, o, d, S, LdsGuess{G}, sink, w);
+        redo = S.ok && march_samples<NG, NF>(M, P, o, d, S, LdsGuess{G, P.mlp_lead}, sink, w);
       else
         redo = S.ok && march_samples<NG, NF>(M, P, o, d, S, NoGuess(), sink, w);
       if (redo) {
@@ -932,6 +936,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   ip.samples = p->samples;
   ip.guess = p->guess;
   ip.max_march = p->max_march;
+  ip.mlp_lead = (p->flags & FEMRT_TRACE_LEARNED_EVERY_SAMPLE) ? p->samples - 1 : 1;
   ip.tmin = p->t_min;
   TraceMesh TM;
   TM.geom = m->d_geom;

@@ -36,6 +36,7 @@ struct IsoParams {
   int guess;
   int max_march;
   float tmin;
+  int mlp_lead;   // leading interior samples of a segment guessed by the network
 };
 
 // Work counters accumulated per lane.

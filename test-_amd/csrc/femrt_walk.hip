@@ -48,8 +48,10 @@ struct GlobalSink {
 
 struct LdsGuess {
   const _Float16* g;  // (samples - 1) x 3
+  int lead;           // samples 1..lead carry a guess
   __device__ bool get(int j, float* rst) const
   {
+    if (j > lead) return false;
     rst[0] = (float)g[(j - 1) * 3 + 0];
     rst[1] = (float)g[(j - 1) * 3 + 1];
     rst[2] = (float)g[(j - 1) * 3 + 2];
@@ -206,7 +208,8 @@ __global__ __launch_bounds__(256, 2) void k_walk_segment(
     const bool seg = active && S.ok;
     const float* bx = M.boxes + (size_t)(seg ? S.e : 0) * FEMRT_BOX_STRIDE;
     const float* vt = M.verts + (size_t)(seg ? S.e : 0) * 24;
-    for (int j = 1; j < P.samples; ++j) {
+    const int jmax = min(P.mlp_lead, P.samples - 1);
+    for (int j = 1; j <= jmax; ++j) {
       float f[MLP_IN];
       if (seg) {
         const float t = march_sample_t(S, j, P.samples);
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(256, 2) void k_walk_segment(
       }
       wave_lds_sync();
     }
-    if (seg) redo = march_samples<NG, NF>(M, P, o, d, S, LdsGuess{G}, sink, w);
+    if (seg) redo = march_samples<NG, NF>(M, P, o, d, S, LdsGuess{G, P.mlp_lead}, sink, w);
   } else {
     if (active && S.ok) redo = march_samples<NG, NF>(M, P, o, d, S, NoGuess(), sink, w);
   }

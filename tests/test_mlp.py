@@ -188,16 +188,22 @@ def test_learned_guess_keeps_parity_and_saves_newton_iterations(femrt, oracle, g
     isos = [-0.1, 0.0, 0.1]
     og, oiso = oracle.Oracle.from_mesh(hm).trace(fr.get_rays(), isos)
     work = {}
-    for name, net in (("trained", mlp), ("zero", zero)):
-        mesh.set_guess_mlp(net)
-        fr.counters(True)
-        fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_LEARNED))
-        rep = parity.compare(fr.hits(), fr.iso(len(isos)), og, oiso, isos)
-        work[name] = fr.counters()
-        print(name, json.dumps(rep), work[name])
-        assert rep["ok"], (name, rep)
-        assert work[name]["mlp_infer"] > 0
-    assert work["trained"]["invert_it"] < 0.8 * work["zero"]["invert_it"]
+    # every interior sample guessed by the network, then the default (only the
+    # first sample of each segment, the rest extrapolated)
+    for every in (True, False):
+        flags = 1 | (femrt.TRACE_LEARNED_EVERY_SAMPLE if every else 0)
+        for name, net in (("trained", mlp), ("zero", zero)):
+            mesh.set_guess_mlp(net)
+            fr.counters(True)
+            fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_LEARNED, flags=flags))
+            rep = parity.compare(fr.hits(), fr.iso(len(isos)), og, oiso, isos)
+            work[name, every] = fr.counters()
+            print(name, every, json.dumps(rep), work[name, every])
+            assert rep["ok"], (name, every, rep)
+            assert work[name, every]["mlp_infer"] > 0
+    assert work["trained", True]["invert_it"] < 0.8 * work["zero", True]["invert_it"]
+    assert work["trained", False]["invert_it"] < work["zero", False]["invert_it"]
+    assert work["trained", False]["mlp_infer"] < 0.25 * work["trained", True]["mlp_infer"]
     with pytest.raises(femrt.FemrtError):
         mesh.set_guess_mlp(None)
         fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_LEARNED))
