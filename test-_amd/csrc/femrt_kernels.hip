@@ -196,12 +196,12 @@ __device__ __forceinline__ void wave_lds_sync()
 // come from the network, evaluated on the matrix cores for the whole wave
 // between the exit and sample steps (later samples extrapolate).
 template <int NG, int NF, bool LEARNED>
-__global__ __launch_bounds__(256, 2) void k_march(
+__global__ __launch_bounds__(256, 3) void k_march(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ queue,
     int* __restrict__ counts, const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
     const float4* __restrict__ hit_f32, int* __restrict__ iso_elem, float4* __restrict__ iso_f32,
     float4* __restrict__ iso_rst32, int* __restrict__ iso_queue, long long* __restrict__ work,
-    const void* __restrict__ mlp_pack)
+    const void* __restrict__ mlp_pack, _Float16* __restrict__ mlp_g)
 {
   const int count = counts[0];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -210,12 +210,13 @@ __global__ __launch_bounds__(256, 2) void k_march(
   bf16x8* ops = nullptr;
   float* bias = nullptr;
   bf16x8* xs = nullptr;
-  _Float16* G = nullptr;
+  _Float16* G = nullptr;  // this lane's guesses, gs halfs per lane
+  int gs = 3;
   if constexpr (LEARNED) {
     __shared__ bf16x8 s_ops[PK_INFER_BF16 / 8];
     __shared__ float s_bias[PK_NF32];
     __shared__ bf16x8 s_xs[4][64 * 4];
-    __shared__ _Float16 s_g[4][64][(FEMRT_MLP_MAX_SAMPLES - 1) * 3];
+    __shared__ _Float16 s_g[4][64 * 3];
     const bf16x8* gp = (const bf16x8*)mlp_pack;
     for (int t = threadIdx.x; t < PK_INFER_BF16 / 8; t += blockDim.x) s_ops[t] = gp[t];
     const float* gf = (const float*)((const __bf16*)mlp_pack + PK_NBF16);
@@ -224,7 +225,14 @@ __global__ __launch_bounds__(256, 2) void k_march(
     ops = s_ops;
     bias = s_bias;
     xs = s_xs[wave];
-    G = &s_g[wave][lane][0];
+    if (P.mlp_lead > 1) {
+      // every interior sample guessed: per-lane rows of the global buffer
+      // (the LDS holds one guess per lane, which keeps three blocks per CU)
+      gs = (FEMRT_MLP_MAX_SAMPLES - 1) * 3;
+      G = mlp_g + ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * gs;
+    } else {
+      G = &s_g[wave][lane * 3];
+    }
   }
   bool active = false, exhausted = false;
   float o[3] = {0.f, 0.f, 0.f}, d[3] = {1.f, 1.f, 1.f}, id[3] = {1.f, 1.f, 1.f};
@@ -276,7 +284,7 @@ __global__ __launch_bounds__(256, 2) void k_march(
         wave_mlp_forward(xs, ops, bias, bias + 64, y0, y1);
         if (lane < 32) {
           _Float16* g0 = &G[(j - 1) * 3];
-          _Float16* g1 = g0 + 32 * (FEMRT_MLP_MAX_SAMPLES - 1) * 3;
+          _Float16* g1 = g0 + 32 * gs;
           for (int c = 0; c < 3; ++c) {
             g0[c] = (_Float16)y0[c];
             g1[c] = (_Float16)y1[c];
@@ -815,7 +823,7 @@ FEMRT_API int femrt_frame_destroy(femrt_frame* f)
   void* ptrs[] = {f->d_rays, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst,
                   f->d_hit_u, f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, f->d_iso_f32,
                   f->d_iso_rst32, f->d_queue_hit, f->d_queue_iso, f->d_counts, f->d_rgba, f->d_work,
-                  f->d_path_tp, f->d_path_acc, f->d_mats, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
+                  f->d_path_tp, f->d_path_acc, f->d_mats, f->d_mlp_g, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
                   f->d_walk_sorted, f->d_walk_reent, f->d_walk_hist};
   for (void* p : ptrs) hipFree(p);
   delete f;
@@ -968,17 +976,35 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
         } else {
           // persistent: lanes refill from the hit queue until it drains
           const int mblocks = dev_cus * cfg.march_bpc;
+          if (learned && ip.mlp_lead > 1) {
+            // per-lane rows of every-sample network guesses
+            const size_t need = (size_t)mblocks * 256 * (FEMRT_MLP_MAX_SAMPLES - 1) * 3;
+            if (f->mlp_g_cap < need) {
+              if (c->capturing) {
+                rc = fail(c, FEMRT_ESTATE, "trace: run every-sample learned traces once before a batch records");
+                break;
+              }
+              (void)hipFree(f->d_mlp_g);
+              f->d_mlp_g = nullptr;
+              f->mlp_g_cap = 0;
+              if (hipMalloc(&f->d_mlp_g, need * sizeof(_Float16)) != hipSuccess) {
+                rc = fail(c, FEMRT_ENOMEM, "trace: hipMalloc of the guess rows failed");
+                break;
+              }
+              f->mlp_g_cap = need;
+            }
+          }
           rc = timed_launch(c, "k_march", [&] {
             if (learned)
               k_march<NG_, NF_, true><<<mblocks, 256, 0, c->stream>>>(
                   f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
                   f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
-                  m->guess_mlp->d_pack);
+                  m->guess_mlp->d_pack, f->d_mlp_g);
             else
               k_march<NG_, NF_, false><<<mblocks, 256, 0, c->stream>>>(
                   f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
                   f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
-                  nullptr);
+                  nullptr, nullptr);
           });
         }
         if (rc) break;

@@ -101,6 +101,8 @@ struct femrt_frame {
   float4* d_path_tp = nullptr;   // per path: throughput rgb, alive flag (allocated by path_begin)
   float4* d_path_acc = nullptr;  // per path: accumulated radiance rgb
   void* d_mats = nullptr;        // scatter materials (64 slots), uploaded when they change
+  _Float16* d_mlp_g = nullptr;   // every-sample network guesses, per walk lane
+  size_t mlp_g_cap = 0;          // halfs
   std::vector<unsigned char> h_mats;
   // element-walk rounds (allocated by the first trace with isovalues)
   int4* d_walk_si = nullptr;     // per ray: element, entry face, step, found | redo << 8
