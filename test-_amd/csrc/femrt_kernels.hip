@@ -234,15 +234,24 @@ __global__ __launch_bounds__(256, 3) void k_march(
       G = &s_g[wave][lane * 3];
     }
   }
-  bool active = false, exhausted = false;
+  bool active = false;
   float o[3] = {0.f, 0.f, 0.f}, d[3] = {1.f, 1.f, 1.f}, id[3] = {1.f, 1.f, 1.f};
   MarchState S;
   march_begin(S, HitF32());
+  // The hit queue is split into FEMRT_QSEG contiguous segments (tile order,
+  // so each covers one band of the image) with a counter each.  A block
+  // starts on the segment of its XCD (workgroups are dispatched round-robin
+  // over the XCDs) so that each XCD's L2 holds the elements of one band; a
+  // wave that finds its segment drained moves on to the next one.
+  int seg = blockIdx.x % FEMRT_QSEG, drained = 0;  // wave-uniform
   for (;;) {
-    const bool need = !active && !exhausted;
-    const int q = wave_append(need, &counts[2]);
-    if (need) {
-      if (q < count) {
+    bool need = !active;
+    while (drained < FEMRT_QSEG && __any(need)) {
+      const int lo = (int)((long long)count * seg / FEMRT_QSEG);
+      const int hi = (int)((long long)count * (seg + 1) / FEMRT_QSEG);
+      const int q = lo + wave_append(need, &counts[FEMRT_QSEG_CTR + seg]);
+      const bool got = need && q < hi;
+      if (got) {
         const int r = queue[q];
         sink.r = r;
         load_ray(rays, n, r, o, d);
@@ -256,9 +265,12 @@ __global__ __launch_bounds__(256, 3) void k_march(
         h.t = hf.z;
         march_begin(S, h);
         active = true;
-      } else {
-        exhausted = true;
       }
+      if (__any(need && !got)) {
+        seg = seg + 1 == FEMRT_QSEG ? 0 : seg + 1;
+        ++drained;
+      }
+      need = need && !got;
     }
     if (!__any(active)) break;
     if (active) march_exit<NG>(M, P, o, d, id, S, w);
@@ -368,7 +380,7 @@ __global__ void k_clear_frame(int n, int n_iso, int* hit_elem, int* iso_elem, in
     hit_elem[r] = -1;
     for (int k = 0; k < n_iso; ++k) iso_elem[(size_t)k * n + r] = -1;
   }
-  if (r < 8) counts[r] = 0;
+  if (r < FEMRT_NCOUNTS) counts[r] = 0;
 }
 
 __global__ void k_shade(int n, int n_iso, const float* __restrict__ rays,
@@ -800,14 +812,14 @@ FEMRT_API int femrt_frame_create(femrt_ctx* c, int n, femrt_frame** out)
             hipMalloc(&f->d_iso_rst32, K * N * 16) == hipSuccess &&
             hipMalloc(&f->d_queue_hit, N * 4) == hipSuccess &&
             hipMalloc(&f->d_queue_iso, K * N * 4) == hipSuccess &&
-            hipMalloc(&f->d_counts, 32) == hipSuccess &&
+            hipMalloc(&f->d_counts, FEMRT_NCOUNTS * 4) == hipSuccess &&
             hipMalloc(&f->d_rgba, N * 4) == hipSuccess &&
             hipMalloc(&f->d_work, FEMRT_NCOUNTERS * 8) == hipSuccess;
   if (!ok) {
     femrt_frame_destroy(f);
     return fail(c, FEMRT_ENOMEM, "frame_create: hipMalloc failed");
   }
-  hipMemset(f->d_counts, 0, 32);
+  hipMemset(f->d_counts, 0, FEMRT_NCOUNTS * 4);
   hipMemset(f->d_work, 0, FEMRT_NCOUNTERS * 8);
   hipMemset(f->d_hit_elem, 0xff, N * 4);
   hipMemset(f->d_iso_elem, 0xff, K * N * 4);

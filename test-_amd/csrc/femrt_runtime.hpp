@@ -94,7 +94,8 @@ struct femrt_frame {
   float4* d_iso_rst32 = nullptr;  // MAX_ISO x n: rst* (FP32)
   int* d_queue_hit = nullptr;     // n
   int* d_queue_iso = nullptr;     // MAX_ISO x n, entries r*MAX_ISO+k
-  int* d_counts = nullptr;        // [0] hits [1] iso items
+  int* d_counts = nullptr;        // FEMRT_NCOUNTS: [0] hits [1] iso items, [2..3] walk
+                                  // rounds, [FEMRT_QSEG_CTR..] walk queue segments
   uint32_t* d_rgba = nullptr;
   long long* d_work = nullptr;
   bool counters = false;
@@ -234,6 +235,11 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter)
   }
 
 #define FEMRT_MLP_MAX_SAMPLES 16
+// Frame queue counters: [0] hits, [1] iso items, [2..3] binned-walk lists,
+// [FEMRT_QSEG_CTR + s] progress in segment s of the hit queue (persistent walk).
+#define FEMRT_QSEG 8
+#define FEMRT_QSEG_CTR 16
+#define FEMRT_NCOUNTS 32
 
 // Element walk in element-binned rounds (femrt_walk.hip).
 int walk_run(femrt_ctx* c, femrt_mesh* m, femrt_frame* f, const TraceMesh& TM, const IsoParams& ip,

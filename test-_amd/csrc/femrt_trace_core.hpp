@@ -833,53 +833,13 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
   float rst_in[3];
   face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
   float gprev[FEMRT_MAX_ISO];
-  float rst[3] = {rst_in[0], rst_in[1], rst_in[2]}, rprev[3] = {0.f, 0.f, 0.f};
-  float rprev2[3] = {0.f, 0.f, 0.f};
+  float rprev[3] = {0.f, 0.f, 0.f}, rprev2[3] = {0.f, 0.f, 0.f};
   float tprev = S.t_in;
   bool have_prev = false;
   int n_prev = 0;  // consecutive samples with a preimage
   unsigned found_here = 0;
-  for (int j = 0; j <= Ms; ++j) {
-    const float t = march_sample_t(S, j, Ms);
-    const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
-    float J[9];
-    // the segment ends lie on faces whose (u, v) the face roots give:
-    // evaluate there, never by extrapolating past the face
-    bool known = false;
-    bool zero_retry = false;
-    if (j == 0) {
-      rst[0] = rst_in[0];
-      rst[1] = rst_in[1];
-      rst[2] = rst_in[2];
-      known = true;
-    } else if (j == Ms) {
-      face_to_rst(S.f_out, S.u_out, S.v_out, rst[0], rst[1], rst[2]);
-      known = true;
-    } else if (guess.get(j, rst)) {
-      zero_retry = true;  // plain Newton from 0 is the fallback of a learned guess
-    } else if (P.guess == FEMRT_GUESS_PLAIN) {
-      rst[0] = rst[1] = rst[2] = 0.f;
-    } else if (n_prev >= 2) {
-      // warm start extrapolated from the two previous (uniformly spaced)
-      // samples: O(dt^2) from the preimage instead of O(dt)
-      for (int c = 0; c < 3; ++c) rst[c] = fminf(fmaxf(2.f * rprev[c] - rprev2[c], -1.f), 1.f);
-    }
-    SampleCert<NF> cert{fc, &P, lip, open_all & ~(S.found | found_here), &w.field};
-    if (!known && !invert_in_cube_f32<NG>(gc, x, rst, J, w.inv_it, zero_retry, cert)) {
-      // no preimage inside the element: the segment end is wrong
-      if (!S.redo) return true;
-      have_prev = false;
-      n_prev = 0;
-      rst[0] = rst[1] = rst[2] = 0.f;
-      continue;
-    }
-    float u;
-    if (!known && cert.accepted) {
-      u = cert.u;
-    } else {
-      u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
-      ++w.field;
-    }
+  // sign checks of a resolved sample against the previous one; brackets
+  auto take = [&](float t, const float* r, float u) {
     for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
       if (k >= P.n_iso || ((S.found | found_here) >> k) & 1u) continue;
       const float g = u - P.iso[k];
@@ -896,7 +856,7 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
         const float den = gprev[k] - g;
         const float wt = den != 0.f ? fminf(fmaxf(gprev[k] / den, 0.f), 1.f) : 0.5f;
         br.tm = tprev + wt * (t - tprev);
-        for (int c = 0; c < 3; ++c) br.rst[c] = rprev[c] + wt * (rst[c] - rprev[c]);
+        for (int c = 0; c < 3; ++c) br.rst[c] = rprev[c] + wt * (r[c] - rprev[c]);
         sink.put(k, br);
         ++w.brackets;
         found_here |= 1u << k;
@@ -905,11 +865,119 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
     }
     for (int c = 0; c < 3; ++c) {
       rprev2[c] = rprev[c];
-      rprev[c] = rst[c];
+      rprev[c] = r[c];
     }
     tprev = t;
     have_prev = true;
     ++n_prev;
+  };
+  // the segment ends lie on faces whose (u, v) the face roots give:
+  // evaluate there, never by extrapolating past the face
+  {
+    const float u = field_value<float, NF>(fc, rst_in[0], rst_in[1], rst_in[2]);
+    ++w.field;
+    take(S.t_in, rst_in, u);
+  }
+  // Interior samples 1 .. Ms-1 (point inversion, then the field).  Each trip
+  // of the loop is one Newton iteration; a lane moves on to its next sample
+  // as soon as the current one is resolved, so the lanes of a wave stay busy
+  // while they need different iteration counts.  Per sample, the Newton runs
+  // of invert_in_cube_f32 in its order: from the guess, from rst = 0 (after a
+  // network guess), from the octant centres; accepted inside the cube only.
+  float rst[3] = {rst_in[0], rst_in[1], rst_in[2]};
+  int j = 1, stage = 0, its = 0;
+  bool zero_retry = false;
+  SampleCert<NF> cert{fc, &P, lip, 0u, &w.field};
+  auto start_sample = [&]() {
+    zero_retry = false;
+    if (guess.get(j, rst)) {
+      zero_retry = true;  // plain Newton from 0 is the fallback of a learned guess
+    } else if (P.guess == FEMRT_GUESS_PLAIN) {
+      rst[0] = rst[1] = rst[2] = 0.f;
+    } else if (n_prev >= 2) {
+      // warm start extrapolated from the two previous (uniformly spaced)
+      // samples: O(dt^2) from the preimage instead of O(dt)
+      for (int c = 0; c < 3; ++c) rst[c] = fminf(fmaxf(2.f * rprev[c] - rprev2[c], -1.f), 1.f);
+    }
+    stage = 0;
+    its = 0;
+    cert.open = open_all & ~(S.found | found_here);
+    cert.begin();
+  };
+  if (j < Ms) start_sample();
+  while (j < Ms) {
+    const float t = march_sample_t(S, j, Ms);
+    const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
+    float X[3], J[9], dl[3];
+    geom_eval<float, NG>(gc, rst[0], rst[1], rst[2], X, J);
+    ++w.inv_it;
+    ++its;
+    const float bv[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
+    bool fail_run = !solve3<float>(J, bv, dl), done = false;
+    if (!fail_run) {
+      rst[0] -= dl[0];
+      rst[1] -= dl[1];
+      rst[2] -= dl[2];
+      if (!(fabsf(rst[0]) < 1e4f && fabsf(rst[1]) < 1e4f && fabsf(rst[2]) < 1e4f)) {
+        fail_run = true;
+      } else if (fabsf(dl[0]) < 1e-5f && fabsf(dl[1]) < 1e-5f && fabsf(dl[2]) < 1e-5f) {
+        done = true;
+      } else {
+        const float st = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
+        if (st < 1e-2f && cert(rst, st))
+          done = true;
+        else if (its >= 50)
+          fail_run = true;
+      }
+    }
+    if (done && !(fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)) {
+      done = false;  // a preimage outside the element: next seed
+      fail_run = true;
+    }
+    if (fail_run) {
+      ++stage;
+      if (stage == 1 && !zero_retry) stage = 2;
+      if (stage <= 9) {
+        if (stage == 1) {
+          rst[0] = rst[1] = rst[2] = 0.f;
+        } else {
+          const int sd = stage - 2;
+          rst[0] = (sd & 1) ? 0.5f : -0.5f;
+          rst[1] = (sd & 2) ? 0.5f : -0.5f;
+          rst[2] = (sd & 4) ? 0.5f : -0.5f;
+        }
+        its = 0;
+        cert.begin();
+        continue;
+      }
+      // no preimage inside the element: the segment end is wrong
+      if (!S.redo) return true;
+      have_prev = false;
+      n_prev = 0;
+      rst[0] = rst[1] = rst[2] = 0.f;
+      ++j;
+      if (j < Ms) start_sample();
+      continue;
+    }
+    if (done) {
+      float u;
+      if (cert.accepted) {
+        u = cert.u;
+      } else {
+        u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
+        ++w.field;
+      }
+      take(t, rst, u);
+      ++j;
+      if (j < Ms) start_sample();
+    }
+  }
+  {
+    float r[3];
+    face_to_rst(S.f_out, S.u_out, S.v_out, r[0], r[1], r[2]);
+    const float u = field_value<float, NF>(fc, r[0], r[1], r[2]);
+    ++w.field;
+    take(S.t_out, r, u);
   }
   S.found |= found_here;
   return false;
