@@ -145,6 +145,79 @@ def run_variant(femrt, ctx, mesh, fr, guess, steps, warmup, flags=1):
 
 CAM = None
 
+BATCH_METRIC = "Mrays/s (camera rays), 64 curved-hex meshes 512..32768 hex, P 2..8, 512x512 8 spp, depth 3, one graph"
+
+
+def run_batch(femrt, args):
+    """The batched workload: 64 meshes of seeded sizes (n^3 hexes, n in
+    8..32) and field orders P in 2..8, Q = 3, each rendered 512x512 at 8 spp
+    with diffuse bounces to depth 3.  Launch shapes are autotuned per mesh;
+    every render is recorded into one HIP graph and a step is one replay."""
+    rng = np.random.default_rng(SEED + 7)
+    Wb, Hb, spp, depth, n_mesh = 512, 512, 8, 3, args.batch_meshes
+    ctx = femrt.Context(0)
+    mats = femrt.materials(femrt.DEFAULT_MATERIALS)
+    meshes, frames, cams, info = [], [], [], []
+    t0 = time.perf_counter()
+    for i in range(n_mesh):
+        n = int(rng.integers(8, 33))
+        p = int(rng.integers(2, 9))
+        hm = femrt.gen_structured_hex(n, 3, p, pert=2, amp=AMP, seed=SEED + 100 + i)
+        m = femrt.Mesh(ctx, hm)
+        fr = femrt.Frame(ctx, Wb * Hb * spp)
+        cam = femrt.auto_camera([0, 0, 0], [1, 1, 1])
+        cam12 = femrt.camera_basis(cam[:3], cam[3:6], cam[6:9], float(cam[9]), Wb / Hb)
+        fr.camera_rays_jitter(cam12, Wb, Hb, spp, SEED + i)
+        prm = femrt.trace_params([], guess=femrt.GUESS_WARM, iso_enable=False, flags=0)
+        cfg = m.autotune(fr, prm, reps=1)
+        meshes.append(m)
+        frames.append(fr)
+        cams.append(cam12)
+        info.append({"n_hex": n ** 3, "P": p, "launch_cfg": list(cfg)})
+    setup_s = time.perf_counter() - t0
+    b = femrt.Batch(ctx)
+    for i in range(n_mesh):
+        b.add_paths(meshes[i], frames[i], cams[i], Wb, Hb, spp, depth, mats, SEED + i)
+    b.capture()
+    for _ in range(args.warmup):
+        b.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.launch()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    # the replayed image of mesh 0 equals an eager render of it
+    img_graph = frames[0].rgba()[:Wb * Hb].copy()
+    check = femrt.Frame(ctx, Wb * Hb * spp)
+    img_eager = check.render_path(meshes[0], cams[0], Wb, Hb, spp, depth, mats, SEED)
+    check.close()
+    cam_rays = n_mesh * Wb * Hb * spp
+    line = {
+        "metric": BATCH_METRIC,
+        "value": args.steps * cam_rays / wall / 1e6,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded curved-hex meshes and modal fields)",
+        "config": {"workload": f"{n_mesh} meshes, n^3 hexes n in 8..32, Q=3, P in 2..8, "
+                               f"{Wb}x{Hb} {spp} spp, Lambertian bounces to depth {depth}, one HIP graph",
+                   "camera_rays_per_step": cam_rays, "traces_per_step": n_mesh * depth,
+                   "meshes": info, "setup_s": setup_s,
+                   "graph_equals_eager": bool(np.array_equal(img_graph, img_eager))},
+    }
+    print(json.dumps(line), flush=True)
+    b.close()
+    for o in frames + meshes:
+        o.close()
+    ctx.close()
+
 
 def main():
     global CAM
@@ -158,6 +231,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--binned-walk", action="store_true",
                     help="walk in element-binned rounds instead of one persistent kernel")
+    ap.add_argument("--workload", default="frame", choices=["frame", "batch"],
+                    help="frame: the 32^3 P=6 1080p frame (default); batch: 64 meshes in one graph")
+    ap.add_argument("--batch-meshes", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -167,6 +244,9 @@ def main():
 
     femrt = importlib.import_module("test-_amd")
     femrt.lib()
+    if args.workload == "batch":
+        run_batch(femrt, args)
+        return
     hm = femrt.gen_structured_hex(N_EL, Q, P, pert=2, amp=AMP, seed=SEED)
     CAM = femrt.auto_camera([0, 0, 0], [1, 1, 1])
     ctx = femrt.Context(0)
@@ -199,6 +279,15 @@ def main():
                     "guess_err_median_on_bench_mesh": float(np.median(err)),
                     "guess_err_p99_on_bench_mesh": float(np.percentile(err, 99))}
         print(json.dumps({"mlp": mlp_info}), file=sys.stderr, flush=True)
+    launch_cfg = list(mesh.launch_cfg())
+    if not args.no_autotune:
+        # launch shapes picked by the autotuner on the headline variant's trace,
+        # then used by every variant (untimed)
+        tv = "learned" if "learned" in variants else variants[0]
+        tflags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
+        fr.camera_rays(CAM, W, H)
+        launch_cfg = list(mesh.autotune(fr, femrt.trace_params(ISOS, guess=names[tv], flags=tflags), reps=2))
+        print(json.dumps({"autotuned_launch_cfg": launch_cfg}), file=sys.stderr, flush=True)
     for v in variants:
         flags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
         if v == "learned_every":
@@ -272,7 +361,9 @@ def main():
         "config": {"workload": "32^3 curved hex Q=4 P=6, 1920x1080 1spp, isos {-0.1,0,0.1}",
                    "variant": head, "rays": n_rays, "hit_fraction": hits / n_rays,
                    "mrays_s_hit_rays": R["mrays"] * hits / n_rays,
-                   "variants_mrays_s": {k: v["mrays"] for k, v in results.items()}},
+                   "variants_mrays_s": {k: v["mrays"] for k, v in results.items()},
+                   "launch_cfg": {"first_hit_block": launch_cfg[0], "walk_blocks_per_cu": launch_cfg[1],
+                                  "polish_blocks_per_cu": launch_cfg[2]}},
         "roofline": roof,
         "cpu_baseline": cpu,
         "parity_sample": check,
