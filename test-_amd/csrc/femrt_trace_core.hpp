@@ -101,63 +101,96 @@ FEMRT_HD void rst_to_face(int f, const float* rst, float& u, float& v)
   v = (f == 0 || f == 5) ? rst[1] : rst[2];
 }
 
-// Face-constrained Newton on (u, v, tau) from one seed, FP32, with the
-// reference's update rules (high_order.h:213-300): step capped to 0.25 in the
-// inf-norm, tau clamped to [tmin, tmax], u, v clamped to [-1.5, 1.5],
-// divergence break after 5 iterations when the residual grows 1.5x, at most
-// 50 iterations.  Returns 1 for an accepted root (max(|u|,|v|) <= 1 + 1e-4,
-// tmin <= tau <= tmax), 0 for a converged root outside the face, -1 when the
-// iteration did not converge.
+// Face-constrained Newton on (u, v, tau), FP32, with the reference's update
+// rules (high_order.h:213-300): step capped to 0.25 in the inf-norm, tau
+// clamped to [tmin, tmax], u, v clamped to [-1.5, 1.5], divergence break
+// after 5 iterations when the residual grows 1.5x, at most 50 iterations.
+// FaceIter is the iteration's state; face_newton_step is one iteration with
+// x(rst) and J already evaluated at the current (u, v).
+struct FaceIter {
+  int f;
+  float u, v, tau, prev, tmin, tmax;
+  int it;
+};
+
+enum { FN_CONTINUE = 0, FN_ACCEPT = 1, FN_REJECT = 2, FN_FAIL = 3 };
+
+FEMRT_HD void face_iter_begin(FaceIter& F, int f, float u, float v, float tau, float tmin, float tmax)
+{
+  F.f = f;
+  F.u = u;
+  F.v = v;
+  F.tmin = tmin;
+  F.tmax = tmax;
+  F.tau = fminf(fmaxf(tau, tmin), tmax);
+  F.prev = FLT_MAX;
+  F.it = 0;
+}
+
+FEMRT_HD int face_newton_step(FaceIter& F, const float* o, const float* d, const float* X,
+                              const float* J)
+{
+  const float oscale = 1.f + fmaxf(fabsf(o[0]), fmaxf(fabsf(o[1]), fabsf(o[2])));
+  const bool us = (F.f == 2 || F.f == 4);
+  const bool vs = (F.f == 0 || F.f == 5);
+  const float R0 = X[0] - fmaf(F.tau, d[0], o[0]);
+  const float R1 = X[1] - fmaf(F.tau, d[1], o[1]);
+  const float R2 = X[2] - fmaf(F.tau, d[2], o[2]);
+  const float res = fmaxf(fabsf(R0), fmaxf(fabsf(R1), fabsf(R2)));
+  const float tol = 2e-6f * (oscale + fabsf(F.tau));
+  const float A[9] = {us ? J[1] : J[0], vs ? J[1] : J[2], -d[0],
+                      us ? J[4] : J[3], vs ? J[4] : J[5], -d[1],
+                      us ? J[7] : J[6], vs ? J[7] : J[8], -d[2]};
+  const float b[3] = {-R0, -R1, -R2};
+  float dl[3];
+  const bool solved = solve3<float>(A, b, dl);
+  if (res < tol) {
+    // converged: one more (free) Newton step with the Jacobian at hand takes
+    // the root to the FP32 floor
+    if (solved && fmaxf(fabsf(dl[0]), fabsf(dl[1])) < 0.01f) {
+      F.u += dl[0];
+      F.v += dl[1];
+      F.tau += dl[2];
+    }
+    return (fmaxf(fabsf(F.u), fabsf(F.v)) <= 1.f + FEMRT_FACE_ACC && F.tau >= F.tmin && F.tau <= F.tmax)
+               ? FN_ACCEPT
+               : FN_REJECT;
+  }
+  if (F.it > 5 && res > F.prev * 1.5f) return FN_FAIL;
+  F.prev = res;
+  if (!solved) return FN_FAIL;
+  const float mx = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
+  const float sc = mx > 0.25f ? 0.25f / mx : 1.f;
+  F.u = fminf(fmaxf(F.u + dl[0] * sc, -1.5f), 1.5f);
+  F.v = fminf(fmaxf(F.v + dl[1] * sc, -1.5f), 1.5f);
+  F.tau = fminf(fmaxf(F.tau + dl[2] * sc, F.tmin), F.tmax);
+  if (++F.it >= 50) return FN_FAIL;
+  return FN_CONTINUE;
+}
+
+// Face Newton from one seed.  Returns 1 for an accepted root (max(|u|,|v|)
+// <= 1 + 1e-4, tmin <= tau <= tmax), 0 for a converged root outside the
+// face, -1 when the iteration did not converge.
 template <int NG>
 FEMRT_HD int face_newton(const float4* __restrict__ gc, const float* o, const float* d, int f,
                          float u, float v, float tau, float tmin, float tmax, float& tau_o,
                          float& u_o, float& v_o, unsigned& iters)
 {
-  float prev = FLT_MAX;
-  const float oscale = 1.f + fmaxf(fabsf(o[0]), fmaxf(fabsf(o[1]), fabsf(o[2])));
-  tau = fminf(fmaxf(tau, tmin), tmax);
-  const bool us = (f == 2 || f == 4);
-  const bool vs = (f == 0 || f == 5);
-  for (int it = 0; it < 50; ++it) {
+  FaceIter F;
+  face_iter_begin(F, f, u, v, tau, tmin, tmax);
+  for (;;) {
     float r, s, t;
-    face_to_rst(f, u, v, r, s, t);
+    face_to_rst(F.f, F.u, F.v, r, s, t);
     float X[3], J[9];
     geom_eval<float, NG>(gc, r, s, t, X, J);
     ++iters;
-    const float R0 = X[0] - fmaf(tau, d[0], o[0]);
-    const float R1 = X[1] - fmaf(tau, d[1], o[1]);
-    const float R2 = X[2] - fmaf(tau, d[2], o[2]);
-    const float res = fmaxf(fabsf(R0), fmaxf(fabsf(R1), fabsf(R2)));
-    const float tol = 2e-6f * (oscale + fabsf(tau));
-    const float A[9] = {us ? J[1] : J[0], vs ? J[1] : J[2], -d[0],
-                        us ? J[4] : J[3], vs ? J[4] : J[5], -d[1],
-                        us ? J[7] : J[6], vs ? J[7] : J[8], -d[2]};
-    const float b[3] = {-R0, -R1, -R2};
-    float dl[3];
-    const bool solved = solve3<float>(A, b, dl);
-    if (res < tol) {
-      // converged: one more (free) Newton step with the Jacobian at hand takes
-      // the root to the FP32 floor
-      if (solved && fmaxf(fabsf(dl[0]), fabsf(dl[1])) < 0.01f) {
-        u += dl[0];
-        v += dl[1];
-        tau += dl[2];
-      }
-      tau_o = tau;
-      u_o = u;
-      v_o = v;
-      return (fmaxf(fabsf(u), fabsf(v)) <= 1.f + FEMRT_FACE_ACC && tau >= tmin && tau <= tmax) ? 1 : 0;
-    }
-    if (it > 5 && res > prev * 1.5f) return -1;
-    prev = res;
-    if (!solved) return -1;
-    const float mx = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
-    const float sc = mx > 0.25f ? 0.25f / mx : 1.f;
-    u = fminf(fmaxf(u + dl[0] * sc, -1.5f), 1.5f);
-    v = fminf(fmaxf(v + dl[1] * sc, -1.5f), 1.5f);
-    tau = fminf(fmaxf(tau + dl[2] * sc, tmin), tmax);
+    const int st = face_newton_step(F, o, d, X, J);
+    if (st == FN_CONTINUE) continue;
+    tau_o = F.tau;
+    u_o = F.u;
+    v_o = F.v;
+    return st == FN_ACCEPT ? 1 : (st == FN_REJECT ? 0 : -1);
   }
-  return -1;
 }
 
 // Projection of X(seed) onto the ray (high_order.h:154-161, 221-225).
@@ -657,7 +690,10 @@ FEMRT_HD bool find_exit_predicted(const float4* __restrict__ gc, const float* o,
     sc[c] = sv;
     fc[c] = f;
   }
-  for (int attempt = 0; attempt < 3; ++attempt) {
+  // the crossed faces nearest first; one face-Newton iteration per trip of
+  // the loop, so a lane moves to its next face as soon as one fails instead
+  // of the wave finishing each attempt together
+  auto next_face = [&](FaceIter& F) -> bool {
     int cb = -1;
     for (int c = 0; c < 3; ++c)
       if (fc[c] >= 0 && (cb < 0 || sc[c] < sc[cb])) cb = c;
@@ -669,16 +705,25 @@ FEMRT_HD bool find_exit_predicted(const float4* __restrict__ gc, const float* o,
     for (int c = 0; c < 3; ++c) pr[c] = rst_in[c] + sv * dl[c];
     float su, sw;
     rst_to_face(f, pr, su, sw);
-    su = fminf(fmaxf(su, -1.f), 1.f);
-    sw = fminf(fmaxf(sw, -1.f), 1.f);
-    float tt, uu, vv;
-    if (face_newton<NG>(gc, o, d, f, su, sw, t_in + sv, t_lo, 1e30f, tt, uu, vv, w.face_it) == 1) {
-      f_out = f;
-      t_out = tt;
-      u_out = uu;
-      v_out = vv;
+    face_iter_begin(F, f, fminf(fmaxf(su, -1.f), 1.f), fminf(fmaxf(sw, -1.f), 1.f), t_in + sv, t_lo, 1e30f);
+    return true;
+  };
+  FaceIter F;
+  bool have = next_face(F);
+  while (have) {
+    float r, s, t;
+    face_to_rst(F.f, F.u, F.v, r, s, t);
+    geom_eval<float, NG>(gc, r, s, t, X, J);
+    ++w.face_it;
+    const int st = face_newton_step(F, o, d, X, J);
+    if (st == FN_ACCEPT) {
+      f_out = F.f;
+      t_out = F.tau;
+      u_out = F.u;
+      v_out = F.v;
       return true;
     }
+    if (st != FN_CONTINUE) have = next_face(F);
   }
   return false;
 }
