@@ -169,7 +169,7 @@ def run_batch(femrt, args):
         cam12 = femrt.camera_basis(cam[:3], cam[3:6], cam[6:9], float(cam[9]), Wb / Hb)
         fr.camera_rays_jitter(cam12, Wb, Hb, spp, SEED + i)
         prm = femrt.trace_params([], guess=femrt.GUESS_WARM, iso_enable=False, flags=0)
-        cfg = m.autotune(fr, prm, reps=1)
+        cfg = m.autotune(fr, prm, reps=2)
         meshes.append(m)
         frames.append(fr)
         cams.append(cam12)
@@ -286,7 +286,7 @@ def main():
         tv = "learned" if "learned" in variants else variants[0]
         tflags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
         fr.camera_rays(CAM, W, H)
-        launch_cfg = list(mesh.autotune(fr, femrt.trace_params(ISOS, guess=names[tv], flags=tflags), reps=2))
+        launch_cfg = list(mesh.autotune(fr, femrt.trace_params(ISOS, guess=names[tv], flags=tflags), reps=3))
         print(json.dumps({"autotuned_launch_cfg": launch_cfg}), file=sys.stderr, flush=True)
     for v in variants:
         flags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)

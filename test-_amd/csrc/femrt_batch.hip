@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -199,7 +200,8 @@ FEMRT_API int femrt_mesh_set_launch_cfg(femrt_mesh* m, const int* cfg3)
 }
 
 // Times each trace kernel under candidate launch shapes on the given job
-// (one shape at a time, the others held) and keeps the fastest on the mesh.
+// (one dimension at a time, the others held) and keeps the fastest on the
+// mesh.
 FEMRT_API int femrt_autotune(femrt_mesh* m, femrt_frame* f, const femrt_trace_params* p, int reps, int* cfg3)
 {
   if (!m || !f || !p || reps < 1) return FEMRT_EINVAL;
@@ -227,47 +229,32 @@ FEMRT_API int femrt_autotune(femrt_mesh* m, femrt_frame* f, const femrt_trace_pa
     if (iso_ms) *iso_ms = c->stats.count("k_polish_iso") ? c->stats["k_polish_iso"].first : 0.0;
     return FEMRT_OK;
   };
+  // one dimension at a time, the current shape measured first; another
+  // shape replaces it only when at least 3% faster (timing noise)
   LaunchCfg best = m->cfg;
   int rc = FEMRT_OK;
-  {
-    double bt = 1e300;
-    for (int hb : {64, 128, 256}) {
+  auto tune = [&](int LaunchCfg::*field, std::initializer_list<int> cands, const char* kernel, bool with_iso) {
+    double bt = 0.0;
+    std::vector<int> order{best.*field};
+    for (int v : cands)
+      if (v != best.*field) order.push_back(v);
+    for (size_t i = 0; i < order.size() && rc == FEMRT_OK; ++i) {
       m->cfg = best;
-      m->cfg.hit_block = hb;
-      double ms = 0;
-      if ((rc = measure("k_first_hit", ms, nullptr))) break;
-      if (ms < bt) {
+      m->cfg.*field = order[i];
+      double ms = 0, ms_iso = 0;
+      if ((rc = measure(kernel, ms, with_iso ? &ms_iso : nullptr))) break;
+      ms += ms_iso;
+      if (i == 0) {
         bt = ms;
-        best.hit_block = hb;
-      }
-    }
-  }
-  if (rc == FEMRT_OK && p->iso_enable && p->n_iso > 0) {
-    double bt = 1e300;
-    for (int bpc : {2, 4, 8, 16}) {
-      m->cfg = best;
-      m->cfg.march_bpc = bpc;
-      double ms = 0;
-      if ((rc = measure("k_march", ms, nullptr))) break;
-      if (ms < bt) {
+      } else if (ms < 0.97 * bt) {
         bt = ms;
-        best.march_bpc = bpc;
+        best.*field = order[i];
       }
     }
-  }
-  if (rc == FEMRT_OK) {
-    double bt = 1e300;
-    for (int bpc : {4, 8, 16}) {
-      m->cfg = best;
-      m->cfg.polish_bpc = bpc;
-      double ms = 0, ms2 = 0;
-      if ((rc = measure("k_polish_hit", ms, &ms2))) break;
-      if (ms + ms2 < bt) {
-        bt = ms + ms2;
-        best.polish_bpc = bpc;
-      }
-    }
-  }
+  };
+  tune(&LaunchCfg::hit_block, {64, 128, 256}, "k_first_hit", false);
+  if (rc == FEMRT_OK && p->iso_enable && p->n_iso > 0) tune(&LaunchCfg::march_bpc, {2, 4, 8, 16}, "k_march", false);
+  if (rc == FEMRT_OK) tune(&LaunchCfg::polish_bpc, {4, 8, 16}, "k_polish_hit", p->iso_enable && p->n_iso > 0);
   m->cfg = best;
   c->stats = saved_stats;
   c->timing = was_timing;

@@ -421,7 +421,7 @@ __global__ void k_shade(int n, int n_iso, const float* __restrict__ rays,
 // Point kernels: field value + world gradient; inversion.
 template <typename T, int NG, int NF>
 __global__ void k_eval_field(int n, const int* __restrict__ elem, const float* __restrict__ rst,
-                             const float4* __restrict__ geom, const float* __restrict__ field,
+                             const GeoMode* __restrict__ geom, const float* __restrict__ field,
                              T* __restrict__ out)
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -440,13 +440,13 @@ __global__ void k_eval_field(int n, const int* __restrict__ elem, const float* _
 
 template <int NG>
 __global__ void k_invert(int n, const int* __restrict__ elem, const float* __restrict__ x,
-                         const float4* __restrict__ geom, double* __restrict__ rst_out,
+                         const GeoMode* __restrict__ geom, double* __restrict__ rst_out,
                          int* __restrict__ iters_out)
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int e = elem[p];
-  const float4* gc = geom + (size_t)e * (NG * NG * NG);
+  const GeoMode* gc = geom + (size_t)e * (NG * NG * NG);
   const float xf[3] = {x[p], x[n + p], x[2 * n + p]};
   float rst[3] = {0.f, 0.f, 0.f}, J[9];
   unsigned it = 0, it64 = 0;
@@ -598,8 +598,8 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   m->NG = NG;
   m->NF = NF;
   const size_t N3 = (size_t)NG * NG * NG, F3 = (size_t)NF * NF * NF;
-  std::vector<float> g4((size_t)n_elem * N3 * 4, 0.f), fp((size_t)n_elem * F3, 0.f);
-  pad_coeffs(geom, ng, NG, 3, n_elem, g4, 4);
+  std::vector<float> g4((size_t)n_elem * N3 * 3, 0.f), fp((size_t)n_elem * F3, 0.f);
+  pad_coeffs(geom, ng, NG, 3, n_elem, g4, 3);
   pad_coeffs(field, nf, NF, 1, n_elem, fp, 1);
   m->h_boxes.resize((size_t)n_elem * FEMRT_BOX_STRIDE);
   bernstein_boxes(n_elem, ng, geom, m->h_boxes.data());

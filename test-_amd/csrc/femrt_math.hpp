@@ -132,13 +132,17 @@ struct BasisIter {
   }
 };
 
+// One geometry mode: the x, y and z coefficients (12 bytes, one dwordx3 load).
+struct GeoMode {
+  float x, y, z;
+};
+
 // x(r,s,t) and J (row-major, rows x,y,z; columns d/dr, d/ds, d/dt).
-// c: NG^3 float4 (x, y, z, unused) for one element, mode (i,j,k) at
-// (k*NG+j)*NG+i.  1035 FMA at NG=5.  The k and j loops stay rolled (the s and
-// t basis is generated by the recurrence as they advance) so only NG float4
-// coefficients are live at a time.
+// c: NG^3 GeoMode for one element, mode (i,j,k) at (k*NG+j)*NG+i.  1035 FMA at NG=5.  The k and j loops stay rolled (the s and
+// t basis is generated by the recurrence as they advance) so only NG
+// coefficient triples are live at a time.
 template <typename T, int NG>
-FEMRT_HD void geom_eval(const float4* __restrict__ c, T r, T s, T t, T* __restrict__ X,
+FEMRT_HD void geom_eval(const GeoMode* __restrict__ c, T r, T s, T t, T* __restrict__ X,
                         T* __restrict__ J)
 {
   T pr[NG], dpr[NG];
@@ -159,11 +163,11 @@ FEMRT_HD void geom_eval(const float4* __restrict__ c, T r, T s, T t, T* __restri
 #pragma unroll
     for (int j = 0; j < NG; ++j) {
       const T sj = ps[j], dsj = dps[j];
-      const float4* cj = c + (k * NG + j) * NG;
+      const GeoMode* cj = c + (k * NG + j) * NG;
       T a0 = 0, a1 = 0, a2 = 0, ad0 = 0, ad1 = 0, ad2 = 0;
 #pragma unroll
       for (int i = 0; i < NG; ++i) {
-        const float4 q = cj[i];
+        const GeoMode q = cj[i];
         a0 += pr[i] * T(q.x);
         a1 += pr[i] * T(q.y);
         a2 += pr[i] * T(q.z);
@@ -204,7 +208,7 @@ FEMRT_HD void geom_eval(const float4* __restrict__ c, T r, T s, T t, T* __restri
 
 // x(r,s,t) only (no Jacobian): 3 NG^3 + 3 NG^2 + 3 NG FMA.
 template <typename T, int NG>
-FEMRT_HD void geom_point(const float4* __restrict__ c, T r, T s, T t, T* __restrict__ X)
+FEMRT_HD void geom_point(const GeoMode* __restrict__ c, T r, T s, T t, T* __restrict__ X)
 {
   T pr[NG], dpr[NG];
   basis1d<T, NG>(r, pr, dpr);
@@ -222,11 +226,11 @@ FEMRT_HD void geom_point(const float4* __restrict__ c, T r, T s, T t, T* __restr
     for (int j = 0; j < NG; ++j) {
       T sj, dsj;
       bs.next(sj, dsj);
-      const float4* cj = c + (k * NG + j) * NG;
+      const GeoMode* cj = c + (k * NG + j) * NG;
       T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
       for (int i = 0; i < NG; ++i) {
-        const float4 q = cj[i];
+        const GeoMode q = cj[i];
         a0 += pr[i] * T(q.x);
         a1 += pr[i] * T(q.y);
         a2 += pr[i] * T(q.z);

@@ -281,7 +281,7 @@ __global__ void k_mlp_samples(TraceMesh M, int n_elem, int n, unsigned long long
   const int e = (int)(mix64(h0 ^ 1) % (unsigned long long)n_elem);
   float rt[3];
   for (int c = 0; c < 3; ++c) rt[c] = 2.f * unit_float(mix64(h0 + 2 + c)) - 1.f;
-  const float4* gc = M.geom + (size_t)e * (NG * NG * NG);
+  const GeoMode* gc = M.geom + (size_t)e * (NG * NG * NG);
   float x[3];
   geom_point<float, NG>(gc, rt[0], rt[1], rt[2], x);
   float rst[3] = {0.f, 0.f, 0.f}, J[9];

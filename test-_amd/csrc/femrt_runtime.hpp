@@ -58,7 +58,7 @@ struct femrt_mesh {
   const femrt_mlp* guess_mlp = nullptr;  // network used by FEMRT_GUESS_LEARNED
   int n_elem = 0, ng = 0, nf = 0;      // true mode counts
   int NG = 0, NF = 0;                  // kernel mode counts (padded)
-  float4* d_geom = nullptr;            // n_elem x NG^3
+  GeoMode* d_geom = nullptr;           // n_elem x NG^3 (x, y, z)
   float* d_field = nullptr;            // n_elem x NF^3
   float* d_verts = nullptr;            // n_elem x 24
   float* d_boxes = nullptr;            // n_elem x FEMRT_BOX_STRIDE

@@ -18,7 +18,7 @@ namespace femrt {
 
 // Device-resident mesh as the trace kernels see it.
 struct TraceMesh {
-  const float4* geom;   // n_elem x NG^3 (x, y, z, unused)
+  const GeoMode* geom;  // n_elem x NG^3 (x, y, z)
   const float* field;   // n_elem x NF^3
   const float* boxes;   // n_elem x FEMRT_BOX_STRIDE: element, face, face sub-patch boxes
   const float* verts;   // n_elem x 24 corner vertices, XML order
@@ -172,7 +172,7 @@ FEMRT_HD int face_newton_step(FaceIter& F, const float* o, const float* d, const
 // <= 1 + 1e-4, tmin <= tau <= tmax), 0 for a converged root outside the
 // face, -1 when the iteration did not converge.
 template <int NG>
-FEMRT_HD int face_newton(const float4* __restrict__ gc, const float* o, const float* d, int f,
+FEMRT_HD int face_newton(const GeoMode* __restrict__ gc, const float* o, const float* d, int f,
                          float u, float v, float tau, float tmin, float tmax, float& tau_o,
                          float& u_o, float& v_o, unsigned& iters)
 {
@@ -195,7 +195,7 @@ FEMRT_HD int face_newton(const float4* __restrict__ gc, const float* o, const fl
 
 // Projection of X(seed) onto the ray (high_order.h:154-161, 221-225).
 template <int NG>
-FEMRT_HD float seed_tau(const float4* __restrict__ gc, const float* o, const float* d, int f,
+FEMRT_HD float seed_tau(const GeoMode* __restrict__ gc, const float* o, const float* d, int f,
                         float u, float v)
 {
   float r, s, t, X[3];
@@ -208,7 +208,7 @@ FEMRT_HD float seed_tau(const float4* __restrict__ gc, const float* o, const flo
 // The reference's seed grid {-0.8,0,0.8}^2, every seed run, nearest accepted
 // root kept.
 template <int NG>
-FEMRT_HD bool face_root_grid(const float4* __restrict__ gc, const float* o, const float* d, int f,
+FEMRT_HD bool face_root_grid(const GeoMode* __restrict__ gc, const float* o, const float* d, int f,
                              float tmin, float tmax, float& tau_o, float& u_o, float& v_o,
                              unsigned& iters)
 {
@@ -239,7 +239,7 @@ FEMRT_HD bool face_root_grid(const float4* __restrict__ gc, const float* o, cons
 // face's polynomial extension outside the element while a crossing inside
 // exists) is avoided.
 template <int NG>
-FEMRT_HD bool face_root_patches(const float4* __restrict__ gc, const float* __restrict__ bx,
+FEMRT_HD bool face_root_patches(const GeoMode* __restrict__ gc, const float* __restrict__ bx,
                                 const float* o, const float* d, const float* id, int f, float tmin,
                                 float tcull, float& tau_o, float& u_o, float& v_o, Work& w)
 {
@@ -273,7 +273,7 @@ FEMRT_HD bool face_root_patches(const float4* __restrict__ gc, const float* __re
 // seed grid, then the sub-patch seeds when the grid finds nothing.  WARM and
 // LEARNED: the sub-patch seeds.
 template <int NG>
-FEMRT_HD bool face_root(const float4* __restrict__ gc, const float* __restrict__ bx, const float* o,
+FEMRT_HD bool face_root(const GeoMode* __restrict__ gc, const float* __restrict__ bx, const float* o,
                         const float* d, const float* id, int f, float tmin, float tcull, int guess,
                         float& tau_o, float& u_o, float& v_o, Work& w)
 {
@@ -336,7 +336,7 @@ FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d
         float e0, e1;
         ++w.slab;
         if (!slab(o, id, bx, bx + 3, tmin, best_t, e0, e1)) continue;
-        const float4* gc = M.geom + (size_t)e * (NG * NG * NG);
+        const GeoMode* gc = M.geom + (size_t)e * (NG * NG * NG);
         for (int f = 0; f < 6; ++f) {
           const float* fb = bx + 6 * (1 + f);
           float f0, f1;
@@ -391,7 +391,7 @@ FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d
 // FP64 face Newton polish of a hit (unconstrained, quadratic convergence from
 // the FP32 root).
 template <int NG>
-FEMRT_HD void polish_hit(const float4* __restrict__ gc, const double* o, const double* d, int f,
+FEMRT_HD void polish_hit(const GeoMode* __restrict__ gc, const double* o, const double* d, int f,
                          double& u, double& v, double& tau, Work& w)
 {
   const bool us = (f == 2 || f == 4), vs = (f == 0 || f == 5);
@@ -430,7 +430,7 @@ struct NoCert {
 };
 
 template <int NG, typename Cert>
-FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
+FEMRT_HD bool invert_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
                          unsigned& iters, Cert& cert)
 {
   cert.begin();
@@ -453,7 +453,7 @@ FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* r
 }
 
 template <int NG>
-FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
+FEMRT_HD bool invert_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
                          unsigned& iters)
 {
   NoCert nc;
@@ -466,7 +466,7 @@ FEMRT_HD bool invert_f32(const float4* __restrict__ gc, const float* x, float* r
 // face) or fails, restart from rst = 0 (when zero_retry: the plain fallback
 // of a learned guess), then from the 8 octant centres (+-0.5)^3.
 template <int NG, typename Cert>
-FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
+FEMRT_HD bool invert_in_cube_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
                                  unsigned& iters, bool zero_retry, Cert& cert)
 {
   if (invert_f32<NG>(gc, x, rst, J, iters, cert) &&
@@ -490,7 +490,7 @@ FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, 
 }
 
 template <int NG>
-FEMRT_HD bool invert_in_cube_f32(const float4* __restrict__ gc, const float* x, float* rst, float* J,
+FEMRT_HD bool invert_in_cube_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
                                  unsigned& iters, bool zero_retry = false)
 {
   NoCert nc;
@@ -536,7 +536,7 @@ struct SampleCert {
 // 1e-13, or once it is below 1e-9 and stops halving (the FP64 rounding floor
 // of x(rst) - x divided by |J|).  On success J is evaluated at the result.
 template <int NG>
-FEMRT_HD bool invert_f64(const float4* __restrict__ gc, const double* x, double* rst, double* J,
+FEMRT_HD bool invert_f64(const GeoMode* __restrict__ gc, const double* x, double* rst, double* J,
                          unsigned& iters)
 {
   double prev = 1e300;
@@ -567,7 +567,7 @@ FEMRT_HD bool invert_f64(const float4* __restrict__ gc, const double* x, double*
 // t > t_lo on the faces other than the entry face (and on the entry face too
 // when include_entry is set).
 template <int NG>
-FEMRT_HD bool find_exit_search(const float4* __restrict__ gc, const float* __restrict__ bx,
+FEMRT_HD bool find_exit_search(const GeoMode* __restrict__ gc, const float* __restrict__ bx,
                                const float* o, const float* d, const float* id, int f_in,
                                float t_lo, int guess, bool include_entry, int& f_out, float& t_out,
                                float& u_out, float& v_out, Work& w)
@@ -599,7 +599,7 @@ FEMRT_HD bool find_exit_search(const float4* __restrict__ gc, const float* __res
 // coordinate was crossed, seeded at the bisected point.  Used when the face
 // searches find no exit (two crossings of one face inside one sub-patch).
 template <int NG>
-FEMRT_HD bool find_exit_marching(const float4* __restrict__ gc, const float* __restrict__ bx,
+FEMRT_HD bool find_exit_marching(const GeoMode* __restrict__ gc, const float* __restrict__ bx,
                                  const float* o, const float* d, const float* id,
                                  const float* rst_in, float t_in, float t_lo, int& f_out,
                                  float& t_out, float& u_out, float& v_out, Work& w)
@@ -668,7 +668,7 @@ FEMRT_HD bool find_exit_marching(const float4* __restrict__ gc, const float* __r
 // first (at most 3), seeded at the predicted point, gives the exit.  Returns
 // false when none converges to an accepted root.
 template <int NG>
-FEMRT_HD bool find_exit_predicted(const float4* __restrict__ gc, const float* o, const float* d,
+FEMRT_HD bool find_exit_predicted(const GeoMode* __restrict__ gc, const float* o, const float* d,
                                   int f_in, const float* rst_in, float t_in, float t_lo, int& f_out,
                                   float& t_out, float& u_out, float& v_out, Work& w)
 {
@@ -736,7 +736,7 @@ FEMRT_HD bool find_exit_predicted(const float4* __restrict__ gc, const float* o,
 // derivative of u along the ray).  tau stays inside [lo, hi]; returns false
 // when an iterate leaves it or the system is singular.
 template <typename T, int NG, int NF>
-FEMRT_HD bool iso_newton(const float4* __restrict__ gc, const float* __restrict__ fc, const T* o,
+FEMRT_HD bool iso_newton(const GeoMode* __restrict__ gc, const float* __restrict__ fc, const T* o,
                          const T* d, T iso, T lo, T hi, int max_it, T tol_t, T tol_r, T& tau, T* rst,
                          T& u, unsigned& geo_it, unsigned& field_it)
 {
@@ -815,7 +815,7 @@ template <int NG>
 FEMRT_HD void march_exit(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
                          const float* id, MarchState& S, Work& w)
 {
-  const float4* gc = M.geom + (size_t)S.e * (NG * NG * NG);
+  const GeoMode* gc = M.geom + (size_t)S.e * (NG * NG * NG);
   const float* bx = M.boxes + (size_t)S.e * FEMRT_BOX_STRIDE;
   const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
   // exits closer than eps to the entry are not resolved in FP32
@@ -870,7 +870,7 @@ template <int NG, int NF, typename Guess, typename Sink>
 FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
                             MarchState& S, const Guess& guess, Sink& sink, Work& w)
 {
-  const float4* gc = M.geom + (size_t)S.e * (NG * NG * NG);
+  const GeoMode* gc = M.geom + (size_t)S.e * (NG * NG * NG);
   const float* fc = M.field + (size_t)S.e * (NF * NF * NF);
   const float lip = M.lip ? M.lip[S.e] : 0.f;
   const unsigned open_all = (1u << P.n_iso) - 1u;
@@ -1136,7 +1136,7 @@ FEMRT_HD bool polish_iso(const TraceMesh& M, const double* o, const double* d, d
                          const IsoBracket& br, double& t_out, double* rst_out, double& u_out,
                          Work& w)
 {
-  const float4* gc = M.geom + (size_t)br.elem * (NG * NG * NG);
+  const GeoMode* gc = M.geom + (size_t)br.elem * (NG * NG * NG);
   const float* fc = M.field + (size_t)br.elem * (NF * NF * NF);
   {
     double tau = br.tm, r4[3] = {br.rst[0], br.rst[1], br.rst[2]}, u4;
