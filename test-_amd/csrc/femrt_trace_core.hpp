@@ -420,20 +420,10 @@ FEMRT_HD void polish_hit(const GeoMode* __restrict__ gc, const double* o, const 
 // Newton x -> rst in FP32 from the guess in rst (HexElement.cu:227-272:
 // undamped, |delta|_inf < 1e-5, <= 50 iterations).  J holds the Jacobian at
 // the last evaluated iterate.
-//
-// `cert(rst, step)` may end the iteration early: it is called once a step is
-// below 1e-2 (and not yet below the tolerance) and returns true to accept
-// the iterate (see SampleCert).
-struct NoCert {
-  FEMRT_HD void begin() {}
-  FEMRT_HD bool operator()(const float*, float) { return false; }
-};
-
-template <int NG, typename Cert>
+template <int NG>
 FEMRT_HD bool invert_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
-                         unsigned& iters, Cert& cert)
+                         unsigned& iters)
 {
-  cert.begin();
   for (int it = 0; it < 50; ++it) {
     float X[3];
     geom_eval<float, NG>(gc, rst[0], rst[1], rst[2], X, J);
@@ -446,18 +436,8 @@ FEMRT_HD bool invert_f32(const GeoMode* __restrict__ gc, const float* x, float* 
     rst[2] -= dl[2];
     if (!(fabsf(rst[0]) < 1e4f && fabsf(rst[1]) < 1e4f && fabsf(rst[2]) < 1e4f)) return false;
     if (fabsf(dl[0]) < 1e-5f && fabsf(dl[1]) < 1e-5f && fabsf(dl[2]) < 1e-5f) return true;
-    const float st = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
-    if (st < 1e-2f && cert(rst, st)) return true;
   }
   return false;
-}
-
-template <int NG>
-FEMRT_HD bool invert_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
-                         unsigned& iters)
-{
-  NoCert nc;
-  return invert_f32<NG>(gc, x, rst, J, iters, nc);
 }
 
 // The preimage of x inside the element (unique for a valid element): Newton
@@ -465,16 +445,16 @@ FEMRT_HD bool invert_f32(const GeoMode* __restrict__ gc, const float* x, float* 
 // rst = 0 finds spurious preimages of the polynomial map for points near a
 // face) or fails, restart from rst = 0 (when zero_retry: the plain fallback
 // of a learned guess), then from the 8 octant centres (+-0.5)^3.
-template <int NG, typename Cert>
+template <int NG>
 FEMRT_HD bool invert_in_cube_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
-                                 unsigned& iters, bool zero_retry, Cert& cert)
+                                 unsigned& iters, bool zero_retry = false)
 {
-  if (invert_f32<NG>(gc, x, rst, J, iters, cert) &&
+  if (invert_f32<NG>(gc, x, rst, J, iters) &&
       fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
     return true;
   if (zero_retry) {
     rst[0] = rst[1] = rst[2] = 0.f;
-    if (invert_f32<NG>(gc, x, rst, J, iters, cert) &&
+    if (invert_f32<NG>(gc, x, rst, J, iters) &&
         fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
       return true;
   }
@@ -482,55 +462,12 @@ FEMRT_HD bool invert_in_cube_f32(const GeoMode* __restrict__ gc, const float* x,
     rst[0] = (s & 1) ? 0.5f : -0.5f;
     rst[1] = (s & 2) ? 0.5f : -0.5f;
     rst[2] = (s & 4) ? 0.5f : -0.5f;
-    if (invert_f32<NG>(gc, x, rst, J, iters, cert) &&
+    if (invert_f32<NG>(gc, x, rst, J, iters) &&
         fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
       return true;
   }
   return false;
 }
-
-template <int NG>
-FEMRT_HD bool invert_in_cube_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
-                                 unsigned& iters, bool zero_retry = false)
-{
-  NoCert nc;
-  return invert_in_cube_f32<NG>(gc, x, rst, J, iters, zero_retry, nc);
-}
-
-// Early acceptance of a scan sample's inversion.  The scan needs only the
-// sign of g_k = u - iso_k at each sample.  Once a Newton step s is below 1e-2
-// the iterate is within about C s^2 << s of the preimage (quadratic
-// convergence), so |u(iterate) - u(preimage)| < lip * s; when every open
-// isovalue has |g_k| > 2 lip s the signs are those of the converged sample
-// and the iteration stops there.  One attempt per inversion (a failed one
-// costs a field evaluation); u of an accepted iterate is kept.
-template <int NF>
-struct SampleCert {
-  const float* fc;
-  const IsoParams* P;
-  float lip;
-  unsigned open;      // isovalues still searched (bit k)
-  unsigned* fevals;
-  bool tried = false, accepted = false;
-  float u = 0.f;
-  FEMRT_HD void begin()
-  {
-    tried = false;
-    accepted = false;
-  }
-  FEMRT_HD bool operator()(const float* rst, float st)
-  {
-    if (tried || !(lip > 0.f)) return false;
-    tried = true;
-    u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
-    ++*fevals;
-    const float margin = 2.f * lip * st;
-    for (int k = 0; k < FEMRT_MAX_ISO; ++k)
-      if (k < P->n_iso && ((open >> k) & 1u) && !(fabsf(u - P->iso[k]) > margin)) return false;
-    accepted = true;
-    return true;
-  }
-};
 
 // FP64 Newton x -> rst from a guess.  Converged when the step drops below
 // 1e-13, or once it is below 1e-9 and stops halving (the FP64 rounding floor
@@ -923,6 +860,14 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
     ++w.field;
     take(S.t_in, rst_in, u);
   }
+  // Early sign acceptance: the scan needs only the sign of g_k = u - iso_k
+  // at each sample.  Once a Newton step st is below 1e-2 the iterate is
+  // within about C st^2 << st of the preimage (quadratic convergence), so
+  // |u(iterate) - u(preimage)| < lip * st, lip bounding |u(a) - u(b)| /
+  // |a - b|_inf on the element; when every open isovalue has |g_k| > 2 lip st
+  // the signs are those of the converged sample and the iterate is taken.
+  // One attempt per Newton run (a failed one costs a field value).
+  //
   // Interior samples 1 .. Ms-1 (point inversion, then the field).  Each trip
   // of the loop is one Newton iteration; a lane moves on to its next sample
   // as soon as the current one is resolved, so the lanes of a wave stay busy
@@ -931,8 +876,7 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
   // network guess), from the octant centres; accepted inside the cube only.
   float rst[3] = {rst_in[0], rst_in[1], rst_in[2]};
   int j = 1, stage = 0, its = 0;
-  bool zero_retry = false;
-  SampleCert<NF> cert{fc, &P, lip, 0u, &w.field};
+  bool zero_retry = false, cert_tried = false;
   auto start_sample = [&]() {
     zero_retry = false;
     if (guess.get(j, rst)) {
@@ -946,8 +890,7 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
     }
     stage = 0;
     its = 0;
-    cert.open = open_all & ~(S.found | found_here);
-    cert.begin();
+    cert_tried = false;
   };
   if (j < Ms) start_sample();
   while (j < Ms) {
@@ -958,7 +901,8 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
     ++w.inv_it;
     ++its;
     const float bv[3] = {X[0] - x[0], X[1] - x[1], X[2] - x[2]};
-    bool fail_run = !solve3<float>(J, bv, dl), done = false;
+    bool fail_run = !solve3<float>(J, bv, dl), conv = false, try_cert = false;
+    float st = 0.f;
     if (!fail_run) {
       rst[0] -= dl[0];
       rst[1] -= dl[1];
@@ -966,16 +910,33 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
       if (!(fabsf(rst[0]) < 1e4f && fabsf(rst[1]) < 1e4f && fabsf(rst[2]) < 1e4f)) {
         fail_run = true;
       } else if (fabsf(dl[0]) < 1e-5f && fabsf(dl[1]) < 1e-5f && fabsf(dl[2]) < 1e-5f) {
-        done = true;
+        conv = true;
       } else {
-        const float st = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
-        if (st < 1e-2f && cert(rst, st))
-          done = true;
-        else if (its >= 50)
-          fail_run = true;
+        st = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
+        // early sign acceptance: one attempt per Newton run
+        try_cert = st < 1e-2f && !cert_tried && lip > 0.f;
+        cert_tried = cert_tried || try_cert;
       }
     }
-    if (done && !(fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)) {
+    const bool inside = fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f;
+    // one field value per trip (a single call site keeps the lanes together):
+    // the converged sample inside the element, or the iterate of an attempt
+    float u = 0.f;
+    if ((conv && inside) || try_cert) {
+      u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
+      ++w.field;
+    }
+    bool done = conv;
+    if (try_cert) {
+      // accepted when every open isovalue is clear of the margin 2 lip st
+      const float margin = 2.f * lip * st;
+      const unsigned open = open_all & ~(S.found | found_here);
+      done = true;
+      for (int k = 0; k < FEMRT_MAX_ISO; ++k)
+        if (k < P.n_iso && ((open >> k) & 1u) && !(fabsf(u - P.iso[k]) > margin)) done = false;
+    }
+    if (!fail_run && !done && its >= 50) fail_run = true;
+    if (done && !inside) {
       done = false;  // a preimage outside the element: next seed
       fail_run = true;
     }
@@ -992,7 +953,7 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
           rst[2] = (sd & 4) ? 0.5f : -0.5f;
         }
         its = 0;
-        cert.begin();
+        cert_tried = false;
         continue;
       }
       // no preimage inside the element: the segment end is wrong
@@ -1005,13 +966,6 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
       continue;
     }
     if (done) {
-      float u;
-      if (cert.accepted) {
-        u = cert.u;
-      } else {
-        u = field_value<float, NF>(fc, rst[0], rst[1], rst[2]);
-        ++w.field;
-      }
       take(t, rst, u);
       ++j;
       if (j < Ms) start_sample();
