@@ -12,9 +12,9 @@
 //                        in each element the exit face root (FP32), uniform
 //                        samples of g = u - iso over [t_in, t_out] with point
 //                        inversion + field evaluation (FP32), first sign
-//                        change per isovalue refined in FP32; brackets queued.
+//                        change per isovalue kept as a bracket in the frame.
 //   k_polish_iso<NG,NF>  FP64 safeguarded Newton/bisection endgame per
-//                        (ray, isovalue) bracket.
+//                        (hit ray, isovalue) bracket, in hit-queue order.
 //   k_shade              RGBA from the AOVs.
 // Reference: raytracer.cu:102-254 (traversal + hex branch), high_order.h:
 // 189-304, 409-465 (face Newton, nearest face), fem/HexElement.cu:227-272
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256, 3) void k_march(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ queue,
     int* __restrict__ counts, const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
     const float4* __restrict__ hit_f32, int* __restrict__ iso_elem, float4* __restrict__ iso_f32,
-    float4* __restrict__ iso_rst32, int* __restrict__ iso_queue, long long* __restrict__ work,
+    float4* __restrict__ iso_rst32, long long* __restrict__ work,
     const void* __restrict__ mlp_pack, _Float16* __restrict__ mlp_g)
 {
   const int count = counts[0];
@@ -305,7 +305,6 @@ __global__ __launch_bounds__(256, 3) void k_march(
         wave_lds_sync();
       }
     }
-    bool finished = false;
     if (active) {
       bool redo;
       if constexpr (LEARNED)
@@ -316,35 +315,30 @@ __global__ __launch_bounds__(256, 3) void k_march(
         S.redo = true;
       } else if (!march_advance<NG>(M, P, o, d, S, w)) {
         active = false;
-        finished = true;
         // a bracket written by a pass that was redone is not a result
         for (int k = 0; k < P.n_iso; ++k)
           if (!((S.found >> k) & 1u)) iso_elem[(size_t)k * n + sink.r] = -1;
       }
-    }
-    for (int k = 0; k < P.n_iso; ++k) {
-      const bool pred = finished && ((S.found >> k) & 1u);
-      const int slot = wave_append(pred, &counts[1]);
-      if (pred) iso_queue[slot] = sink.r * FEMRT_MAX_ISO + k;
     }
   }
   flush_work(work, w);
 }
 
 template <int NG, int NF>
-__global__ __launch_bounds__(256, 2) void k_polish_iso(
-    const float* __restrict__ rays, int n, TraceMesh M, IsoParams P,
-    const int* __restrict__ iso_queue, const int* __restrict__ counts,
-    const float4* __restrict__ iso_f32, const float4* __restrict__ iso_rst32,
+__global__ __launch_bounds__(256, 3) void k_polish_iso(
+    const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ hit_queue,
+    const int* __restrict__ counts, const float4* __restrict__ iso_f32, const float4* __restrict__ iso_rst32,
     int* __restrict__ iso_elem, double* __restrict__ iso_t, double* __restrict__ iso_rst,
     double* __restrict__ iso_u, long long* __restrict__ work)
 {
-  const int count = counts[1];
+  // (hit ray, isovalue) pairs in hit-queue order, which is tile order, so
+  // neighbouring lanes polish roots in the same elements
+  const int count = counts[0] * P.n_iso;
   Work w;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
-    const int item = iso_queue[q];
-    const int r = item / FEMRT_MAX_ISO, k = item % FEMRT_MAX_ISO;
+    const int r = hit_queue[q / P.n_iso], k = q % P.n_iso;
     const size_t slot = (size_t)k * n + r;
+    if (iso_elem[slot] < 0) continue;
     float of[3], df[3];
     load_ray(rays, n, r, of, df);
     const double o[3] = {of[0], of[1], of[2]}, d[3] = {df[0], df[1], df[2]};
@@ -811,7 +805,6 @@ FEMRT_API int femrt_frame_create(femrt_ctx* c, int n, femrt_frame** out)
             hipMalloc(&f->d_iso_f32, K * N * 16) == hipSuccess &&
             hipMalloc(&f->d_iso_rst32, K * N * 16) == hipSuccess &&
             hipMalloc(&f->d_queue_hit, N * 4) == hipSuccess &&
-            hipMalloc(&f->d_queue_iso, K * N * 4) == hipSuccess &&
             hipMalloc(&f->d_counts, FEMRT_NCOUNTS * 4) == hipSuccess &&
             hipMalloc(&f->d_rgba, N * 4) == hipSuccess &&
             hipMalloc(&f->d_work, FEMRT_NCOUNTERS * 8) == hipSuccess;
@@ -834,7 +827,7 @@ FEMRT_API int femrt_frame_destroy(femrt_frame* f)
   hipStreamSynchronize(f->ctx->stream);
   void* ptrs[] = {f->d_rays, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst,
                   f->d_hit_u, f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, f->d_iso_f32,
-                  f->d_iso_rst32, f->d_queue_hit, f->d_queue_iso, f->d_counts, f->d_rgba, f->d_work,
+                  f->d_iso_rst32, f->d_queue_hit, f->d_counts, f->d_rgba, f->d_work,
                   f->d_path_tp, f->d_path_acc, f->d_mats, f->d_mlp_g, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
                   f->d_walk_sorted, f->d_walk_reent, f->d_walk_hist};
   for (void* p : ptrs) hipFree(p);
@@ -1010,19 +1003,19 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
             if (learned)
               k_march<NG_, NF_, true><<<mblocks, 256, 0, c->stream>>>(
                   f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
-                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
+                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work,
                   m->guess_mlp->d_pack, f->d_mlp_g);
             else
               k_march<NG_, NF_, false><<<mblocks, 256, 0, c->stream>>>(
                   f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
-                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, f->d_queue_iso, work,
+                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work,
                   nullptr, nullptr);
           });
         }
         if (rc) break;
         rc = timed_launch(c, "k_polish_iso", [&] {
           k_polish_iso<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
-              f->d_rays, n, TM, ip, f->d_queue_iso, f->d_counts, f->d_iso_f32, f->d_iso_rst32,
+              f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_iso_f32, f->d_iso_rst32,
               f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, work);
         });
         if (rc) break;

@@ -93,8 +93,7 @@ struct femrt_frame {
   float4* d_iso_f32 = nullptr;  // MAX_ISO x n: (t_a, t_b, t*, packed signs)
   float4* d_iso_rst32 = nullptr;  // MAX_ISO x n: rst* (FP32)
   int* d_queue_hit = nullptr;     // n
-  int* d_queue_iso = nullptr;     // MAX_ISO x n, entries r*MAX_ISO+k
-  int* d_counts = nullptr;        // FEMRT_NCOUNTS: [0] hits [1] iso items, [2..3] walk
+  int* d_counts = nullptr;        // FEMRT_NCOUNTS: [0] hits, [2..3] walk
                                   // rounds, [FEMRT_QSEG_CTR..] walk queue segments
   uint32_t* d_rgba = nullptr;
   long long* d_work = nullptr;
@@ -235,7 +234,7 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter)
   }
 
 #define FEMRT_MLP_MAX_SAMPLES 16
-// Frame queue counters: [0] hits, [1] iso items, [2..3] binned-walk lists,
+// Frame queue counters: [0] hits, [2..3] binned-walk lists,
 // [FEMRT_QSEG_CTR + s] progress in segment s of the hit queue (persistent walk).
 #define FEMRT_QSEG 8
 #define FEMRT_QSEG_CTR 16

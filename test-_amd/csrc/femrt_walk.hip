@@ -16,8 +16,7 @@
 //   k_walk_scan     exclusive prefix sum of the histogram
 //   k_walk_sort     ray ids in element order
 //   k_walk_segment  one segment per ray; continuing rays to the next list,
-//                   rays that left the mesh to the re-entry list, finished
-//                   rays to the isovalue queue
+//                   rays that left the mesh to the re-entry list
 //   k_walk_reenter  BVH re-entry; rays that meet another element continue
 #include <hip/hip_runtime.h>
 
@@ -89,18 +88,12 @@ __device__ __forceinline__ void store_state(int4* si, float4* sf, int r, const M
   sf[r] = make_float4(S.t_in, S.u_in, S.v_in, 0.f);
 }
 
-// Finished ray: its isovalue items to the endgame queue (whole waves call
-// this), and brackets of isovalues not found cleared (a pass that was redone
-// may have written one).
-__device__ __forceinline__ void finish_ray(bool fin, int r, unsigned found, int n_iso, int n,
-                                           int* iso_elem, int* counts, int* iso_queue)
+// Finished ray: brackets of isovalues not found cleared (a pass that was
+// redone may have written one); the endgame takes the others.
+__device__ __forceinline__ void finish_ray(bool fin, int r, unsigned found, int n_iso, int n, int* iso_elem)
 {
-  for (int k = 0; k < n_iso; ++k) {
-    const bool pred = fin && ((found >> k) & 1u);
-    const int slot = wave_append(pred, &counts[1]);
-    if (pred) iso_queue[slot] = r * FEMRT_MAX_ISO + k;
+  for (int k = 0; k < n_iso; ++k)
     if (fin && !((found >> k) & 1u)) iso_elem[(size_t)k * n + r] = -1;
-  }
 }
 
 }  // namespace
@@ -174,8 +167,8 @@ __global__ __launch_bounds__(256, 2) void k_walk_segment(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ sorted,
     int n_active, int4* __restrict__ si, float4* __restrict__ sf, int* __restrict__ next,
     int* __restrict__ reent, int* __restrict__ counts, int* __restrict__ iso_elem,
-    float4* __restrict__ iso_f32, float4* __restrict__ iso_rst32, int* __restrict__ iso_queue,
-    long long* __restrict__ work, const void* __restrict__ mlp_pack)
+    float4* __restrict__ iso_f32, float4* __restrict__ iso_rst32, long long* __restrict__ work,
+    const void* __restrict__ mlp_pack)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,7 +249,7 @@ __global__ __launch_bounds__(256, 2) void k_walk_segment(
   if (active && k == MARCH_NEXT) next[sn] = r;
   const int sr = wave_append(active && k == MARCH_REENTER, &counts[3]);
   if (active && k == MARCH_REENTER) reent[sr] = r;
-  finish_ray(active && k == MARCH_DONE, r, S.found, P.n_iso, n, iso_elem, counts, iso_queue);
+  finish_ray(active && k == MARCH_DONE, r, S.found, P.n_iso, n, iso_elem);
   flush_work(work, w);
 }
 
@@ -264,7 +257,7 @@ template <int NG>
 __global__ __launch_bounds__(256) void k_walk_reenter(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ reent,
     int4* __restrict__ si, float4* __restrict__ sf, int* __restrict__ next, int* __restrict__ counts,
-    int* __restrict__ iso_elem, int* __restrict__ iso_queue, long long* __restrict__ work)
+    int* __restrict__ iso_elem, long long* __restrict__ work)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < counts[3];
@@ -283,7 +276,7 @@ __global__ __launch_bounds__(256) void k_walk_reenter(
   }
   const int sn = wave_append(active && cont, &counts[2]);
   if (active && cont) next[sn] = r;
-  finish_ray(active && !cont, r, S.found, P.n_iso, n, iso_elem, counts, iso_queue);
+  finish_ray(active && !cont, r, S.found, P.n_iso, n, iso_elem);
   flush_work(work, w);
 }
 
@@ -357,20 +350,20 @@ int walk_run(femrt_ctx* c, femrt_mesh* m, femrt_frame* f, const TraceMesh& TM, c
           if (learned)
             k_walk_segment<NG_, NF_, true><<<ab, 256, 0, c->stream>>>(
                 f->d_rays, n, TM, ip, f->d_walk_sorted, n_act, f->d_walk_si, f->d_walk_sf, nxt,
-                f->d_walk_reent, f->d_counts, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32,
-                f->d_queue_iso, work, m->guess_mlp->d_pack);
+                f->d_walk_reent, f->d_counts, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work,
+                m->guess_mlp->d_pack);
           else
             k_walk_segment<NG_, NF_, false><<<ab, 256, 0, c->stream>>>(
                 f->d_rays, n, TM, ip, f->d_walk_sorted, n_act, f->d_walk_si, f->d_walk_sf, nxt,
-                f->d_walk_reent, f->d_counts, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32,
-                f->d_queue_iso, work, nullptr);
+                f->d_walk_reent, f->d_counts, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work,
+                nullptr);
         });
       });
       if (rc) break;
       rc = timed_launch(c, "k_walk_reenter", [&] {
         k_walk_reenter<NG_><<<ab, 256, 0, c->stream>>>(f->d_rays, n, TM, ip, f->d_walk_reent, f->d_walk_si,
                                                         f->d_walk_sf, nxt, f->d_counts, f->d_iso_elem,
-                                                        f->d_queue_iso, work);
+                                                        work);
       });
     });
     if (rc) return rc;
