@@ -183,8 +183,12 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
  * evaluations, [7] slab tests, [8] MLP inferences, [9] elements marched,
  * [10] segment exits found by prediction, [11] by face search, [12] by
  * marching, [13] BVH re-entries of the walk, [14] redone segments,
- * [15] isovalue brackets found. */
-#define FEMRT_NCOUNTERS 16
+ * [15] isovalue brackets found, [16..18] timeline of the persistent walk on
+ * the constant 100 MHz clock: start, the moment its hit queue first ran dry
+ * (after that lanes only finish the rays they hold), end; [19..26] rays
+ * by segments walked, log2 bins 1, 2-3, 4-7, ..., 128+; [27] segments whose
+ * scan the element's field bounds ruled out. */
+#define FEMRT_NCOUNTERS 28
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
 
 /* ------------------------------------------------ shading and bounces */
@@ -318,6 +322,10 @@ FEMRT_API int femrt_mesh_set_guess_mlp(femrt_mesh* mesh, const femrt_mlp* m);
 FEMRT_API int femrt_host_boxes(int n_elem, int ng, const float* geom, float* boxes);
 FEMRT_API int femrt_host_build_bvh(int n_elem, const float* boxes, int* n_nodes, float* nodes8,
                                    int* prims);
+/* Per element, bounds (lo, hi) of the field over the reference cube, as the
+ * walk uses them to skip scans that can cross no open isovalue: the extreme
+ * tensor-Bernstein coefficients, widened by 1e-5 of sum |c|.  out: 2 n_elem. */
+FEMRT_API int femrt_host_field_range(int n_elem, int nf, const float* field, float* out);
 
 /* ------------------------------------------------------- synthetic meshes */
 typedef struct {

@@ -21,10 +21,13 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "
 GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
 TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE = 1, 2, 4   # trace flag bits
 MAX_ISO = 8
-NCOUNTERS = 16
+NCOUNTERS = 28
 COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",
                  "geo64_it", "field64_evals", "slab_tests", "mlp_infer", "elems_marched",
-                 "exit_pred", "exit_search", "exit_march", "reentry", "redo", "brackets"]
+                 "exit_pred", "exit_search", "exit_march", "reentry", "redo", "brackets",
+                 "walk_t_start", "walk_t_drained", "walk_t_end",
+                 "len_1", "len_2_3", "len_4_7", "len_8_15", "len_16_31", "len_32_63", "len_64_127",
+                 "len_128_up", "scan_skipped"]
 
 
 class FemrtError(RuntimeError):
@@ -137,6 +140,7 @@ def lib():
         "femrt_frame_counters": (I, [P, I, P]),
         "femrt_gen_structured_hex": (I, [P, P, P, P, P]),
         "femrt_host_boxes": (I, [I, I, P, P]),
+        "femrt_host_field_range": (I, [I, I, P, P]),
         "femrt_host_build_bvh": (I, [I, P, C.POINTER(I), P, P]),
         "femrt_load_xml_fld": (I, [C.c_char_p, C.c_char_p, I, C.POINTER(I), C.POINTER(I),
                                    C.POINTER(I), P, P, P, P]),
@@ -187,7 +191,7 @@ def exported_symbols():
             "femrt_frame_destroy", "femrt_frame_set_rays", "femrt_frame_camera_rays",
             "femrt_frame_get_rays", "femrt_trace", "femrt_frame_get_hits", "femrt_frame_get_iso",
             "femrt_frame_get_rgba", "femrt_frame_counters", "femrt_gen_structured_hex",
-            "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh",
+            "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh", "femrt_host_field_range",
             "femrt_save_xml_fld", "femrt_io_last_error",
             "femrt_frame_camera_rays_jitter", "femrt_frame_path_begin", "femrt_frame_scatter",
             "femrt_frame_resolve", "femrt_render_path",
@@ -248,6 +252,13 @@ NBOX = 61  # element box, 6 face boxes, 6 x 9 face sub-patch boxes
 def host_boxes(hm: HexMesh):
     out = np.zeros((hm.n_elem, NBOX, 6), np.float32)
     _check(lib().femrt_host_boxes(hm.n_elem, hm.ng, _ptr(hm.geom), _ptr(out)), what="host_boxes")
+    return out
+
+
+def host_field_range(hm: HexMesh):
+    """Per element (lo, hi) bounds of the field over the reference cube."""
+    out = np.zeros((hm.n_elem, 2), np.float32)
+    _check(lib().femrt_host_field_range(hm.n_elem, hm.nf, _ptr(hm.field), _ptr(out)), what="host_field_range")
     return out
 
 

@@ -476,6 +476,52 @@ static std::vector<double> modal_to_bernstein(int n, double a, double b)
   return T;
 }
 
+// Per element, bounds of the field u over the reference cube: the extreme
+// coefficients of u in the tensor Bernstein basis (a polynomial lies in the
+// convex hull of its Bernstein coefficients), widened by 1e-5 of sum |c| for
+// the FP32 evaluation error of u, rounded outward.  out: lo, hi per element.
+void field_range(int n_elem, int nf, const float* field, float* out)
+{
+  const int n = nf;
+  const std::vector<double> T = modal_to_bernstein(n, -1.0, 1.0);  // T[q][m]: modal q -> Bernstein m
+  const size_t n3 = (size_t)n * n * n;
+  std::vector<double> c(n3), t1(n3), t2(n3);
+  for (int e = 0; e < n_elem; ++e) {
+    double asum = 0.0;
+    for (size_t m = 0; m < n3; ++m) {
+      c[m] = field[(size_t)e * n3 + m];
+      asum += std::fabs(c[m]);
+    }
+    // along i (r), then j (s), then k (t); index (k*n + j)*n + i
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int mi = 0; mi < n; ++mi) {
+          double acc = 0.0;
+          for (int i = 0; i < n; ++i) acc += T[(size_t)i * n + mi] * c[((size_t)k * n + j) * n + i];
+          t1[((size_t)k * n + j) * n + mi] = acc;
+        }
+    for (int k = 0; k < n; ++k)
+      for (int mj = 0; mj < n; ++mj)
+        for (int mi = 0; mi < n; ++mi) {
+          double acc = 0.0;
+          for (int j = 0; j < n; ++j) acc += T[(size_t)j * n + mj] * t1[((size_t)k * n + j) * n + mi];
+          t2[((size_t)k * n + mj) * n + mi] = acc;
+        }
+    double lo = 1e300, hi = -1e300;
+    for (int mk = 0; mk < n; ++mk)
+      for (int mj = 0; mj < n; ++mj)
+        for (int mi = 0; mi < n; ++mi) {
+          double acc = 0.0;
+          for (int k = 0; k < n; ++k) acc += T[(size_t)k * n + mk] * t2[((size_t)k * n + mj) * n + mi];
+          lo = std::min(lo, acc);
+          hi = std::max(hi, acc);
+        }
+    const double pad = 1e-5 * asum + 1e-30;
+    out[2 * (size_t)e] = std::nextafter((float)(lo - pad), -INFINITY);
+    out[2 * (size_t)e + 1] = std::nextafter((float)(hi + pad), INFINITY);
+  }
+}
+
 static const int kBoxSplit = 3;
 static_assert(kBoxSplit == 3, "face sub-patch layout assumes 3x3 sub-patches per face");
 

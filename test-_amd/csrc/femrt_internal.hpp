@@ -30,6 +30,7 @@ int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
                        int* adjacency);
 void encode_adjacency(int n_elem, const float* vertices, const int* adjacency, int* out);
 void field_lipschitz(int n_elem, int nf, const float* field, float* out /* n */);
+void field_range(int n_elem, int nf, const float* field, float* out /* n x (lo, hi) */);
 std::vector<double> gll_nodes(int n);
 bool modal_fit_1d_matrix(int n, const std::vector<double>& nodes, std::vector<double>& inv);
 double modA(int p, double x);

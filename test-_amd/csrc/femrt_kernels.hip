@@ -244,6 +244,10 @@ __global__ __launch_bounds__(256, 3) void k_march(
   // over the XCDs) so that each XCD's L2 holds the elements of one band; a
   // wave that finds its segment drained moves on to the next one.
   int seg = blockIdx.x % FEMRT_QSEG, drained = 0;  // wave-uniform
+  // timeline with the work counters: start (min), queue dry (min), end (max)
+  const bool timeline = work != nullptr && lane == 0;
+  if (timeline) atomicMin((unsigned long long*)&work[16], (unsigned long long)wall_clock64());
+  bool saw_dry = false;
   for (;;) {
     bool need = !active;
     while (drained < FEMRT_QSEG && __any(need)) {
@@ -271,6 +275,10 @@ __global__ __launch_bounds__(256, 3) void k_march(
         ++drained;
       }
       need = need && !got;
+    }
+    if (timeline && drained == FEMRT_QSEG && !saw_dry) {
+      saw_dry = true;
+      atomicMin((unsigned long long*)&work[17], (unsigned long long)wall_clock64());
     }
     if (!__any(active)) break;
     if (active) march_exit<NG>(M, P, o, d, id, S, w);
@@ -315,12 +323,17 @@ __global__ __launch_bounds__(256, 3) void k_march(
         S.redo = true;
       } else if (!march_advance<NG>(M, P, o, d, S, w)) {
         active = false;
+        if (work) {
+          const int len = max(1, S.step);
+          add_work(work, 19 + min(7, 31 - __clz(len)), 1);
+        }
         // a bracket written by a pass that was redone is not a result
         for (int k = 0; k < P.n_iso; ++k)
           if (!((S.found >> k) & 1u)) iso_elem[(size_t)k * n + sink.r] = -1;
       }
     }
   }
+  if (timeline) atomicMax((unsigned long long*)&work[18], (unsigned long long)wall_clock64());
   flush_work(work, w);
 }
 
@@ -597,11 +610,13 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   pad_coeffs(field, nf, NF, 1, n_elem, fp, 1);
   m->h_boxes.resize((size_t)n_elem * FEMRT_BOX_STRIDE);
   bernstein_boxes(n_elem, ng, geom, m->h_boxes.data());
-  std::vector<float> lip(n_elem);
+  std::vector<float> lip(n_elem), frange(2 * (size_t)n_elem);
   field_lipschitz(n_elem, nf, field, lip.data());
+  field_range(n_elem, nf, field, frange.data());
   auto cleanup = [&](const std::string& msg) {
     hipFree(m->d_geom); hipFree(m->d_field); hipFree(m->d_verts); hipFree(m->d_boxes); hipFree(m->d_adj);
     hipFree(m->d_lip);
+    hipFree(m->d_frange);
     delete m;
     return fail(c, FEMRT_ENOMEM, msg);
   };
@@ -611,11 +626,13 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   if (hipMalloc(&m->d_boxes, m->h_boxes.size() * 4) != hipSuccess) return cleanup("hipMalloc boxes");
   if (adjacency && hipMalloc(&m->d_adj, (size_t)n_elem * 12 * 4) != hipSuccess) return cleanup("hipMalloc adj");
   if (hipMalloc(&m->d_lip, (size_t)n_elem * 4) != hipSuccess) return cleanup("hipMalloc lip");
+  if (hipMalloc(&m->d_frange, (size_t)n_elem * 8) != hipSuccess) return cleanup("hipMalloc field range");
   bool ok = hipMemcpy(m->d_geom, g4.data(), g4.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_field, fp.data(), fp.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_verts, vertices, (size_t)n_elem * 24 * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_boxes, m->h_boxes.data(), m->h_boxes.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(m->d_lip, lip.data(), lip.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+            hipMemcpy(m->d_lip, lip.data(), lip.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(m->d_frange, frange.data(), frange.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
   if (ok && adjacency) {
     std::vector<int> adj((size_t)n_elem * 12);
     encode_adjacency(n_elem, vertices, adjacency, adj.data());
@@ -637,6 +654,7 @@ FEMRT_API int femrt_mesh_destroy(femrt_mesh* m)
   hipFree(m->d_boxes);
   hipFree(m->d_adj);
   hipFree(m->d_lip);
+  hipFree(m->d_frange);
   hipFree(m->d_nodes);
   hipFree(m->d_prims);
   delete m;
@@ -926,7 +944,10 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   const int n = f->n;
   const int n_iso = p->iso_enable ? p->n_iso : 0;
   long long* work = f->counters ? f->d_work : nullptr;
-  if (work) HIPCHK(c, hipMemsetAsync(f->d_work, 0, FEMRT_NCOUNTERS * 8, c->stream));
+  if (work) {
+    HIPCHK(c, hipMemsetAsync(f->d_work, 0, FEMRT_NCOUNTERS * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(f->d_work + 16, 0xff, 2 * 8, c->stream));  // minima start at the top
+  }
   const int blocks = (n + 255) / 256;
   int rc = timed_launch(c, "k_clear_frame", [&] {
     k_clear_frame<<<blocks, 256, 0, c->stream>>>(n, FEMRT_MAX_ISO, f->d_hit_elem, f->d_iso_elem, f->d_counts);
@@ -960,6 +981,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   TM.nodes = m->d_nodes;
   TM.prims = m->d_prims;
   TM.lip = m->d_lip;
+  TM.frange = m->d_frange;
   rc = FEMRT_EUNSUP;
   FEMRT_DISPATCH_NG(m->NG, {
     rc = timed_launch(c, "k_first_hit", [&] {
@@ -1075,6 +1097,13 @@ FEMRT_API int femrt_host_boxes(int n_elem, int ng, const float* geom, float* box
 {
   if (n_elem <= 0 || ng < 1 || ng > 16 || !geom || !boxes) return FEMRT_EINVAL;
   bernstein_boxes(n_elem, ng, geom, boxes);
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_host_field_range(int n_elem, int nf, const float* field, float* out)
+{
+  if (n_elem <= 0 || nf < 1 || nf > 16 || !field || !out) return FEMRT_EINVAL;
+  field_range(n_elem, nf, field, out);
   return FEMRT_OK;
 }
 

@@ -26,6 +26,7 @@ struct TraceMesh {
   const float4* nodes;  // BVH nodes x 2
   const int* prims;
   const float* lip;     // n_elem: bound of |u(a) - u(b)| / |a - b|_inf, or null
+  const float* frange;  // n_elem x 2: bounds of u on the element (widened), or null
 };
 
 struct IsoParams {
@@ -48,6 +49,7 @@ struct Work {
   // element-walk diagnostics: exits found by prediction / face search /
   // marching, re-entries through the BVH, redone segments, iso brackets
   unsigned exit_pred = 0, exit_search = 0, exit_march = 0, reentry = 0, redo = 0, brackets = 0;
+  unsigned skipped = 0;  // segments whose samples the field bounds rule out
 };
 
 // FP32 result of the first-hit stage.
@@ -812,6 +814,19 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
   const float lip = M.lip ? M.lip[S.e] : 0.f;
   const unsigned open_all = (1u << P.n_iso) - 1u;
   const int Ms = P.samples;
+  if (M.frange) {
+    // no isovalue still open lies within the element's bounds of u: every
+    // sample of the segment has the same sign for each of them, so the scan
+    // can find no crossing here and is skipped
+    const float lo = M.frange[2 * (size_t)S.e], hi = M.frange[2 * (size_t)S.e + 1];
+    bool any = false;
+    for (int k = 0; k < FEMRT_MAX_ISO; ++k)
+      if (k < P.n_iso && !((S.found >> k) & 1u) && P.iso[k] >= lo && P.iso[k] <= hi) any = true;
+    if (!any) {
+      ++w.skipped;
+      return false;
+    }
+  }
   float rst_in[3];
   face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
   float gprev[FEMRT_MAX_ISO];

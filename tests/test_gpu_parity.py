@@ -136,3 +136,29 @@ def test_walk_across_rotated_face_orientations(femrt, oracle, gpu_ctx, tmp_path)
     assert rep["oracle_hits"] > 300
     assert all(r["gpu_found"] > 50 for r in rep["iso"])
     assert rep["ok"], rep
+
+
+def test_scan_skip_by_field_bounds_keeps_parity(femrt, oracle, gpu_ctx):
+    """Field u = z (the geometry's own z coefficients, so the isosurfaces are
+    the planes z = 0.3 and z = 0.55 through a curved mesh): most elements'
+    field bounds exclude the open isovalues and their scans are skipped; the
+    results still match the oracle."""
+    hm = femrt.gen_structured_hex(4, 4, 4, pert=2, amp=0.08, seed=31)
+    assert hm.ng == hm.nf
+    hm.field = np.ascontiguousarray(hm.geom_blocks()[:, 2, :])
+    isos = [0.3, 0.55]
+    mesh = femrt.Mesh(gpu_ctx, hm)
+    W, H = 64, 48
+    fr = femrt.Frame(gpu_ctx, W * H)
+    fr.camera_rays(femrt.auto_camera([0, 0, 0], [1, 1, 1]), W, H)
+    fr.counters(True)
+    fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_WARM))
+    cnt = fr.counters(False)
+    og, oiso = oracle.Oracle.from_mesh(hm).trace(fr.get_rays(), isos)
+    rep = _parity().compare(fr.hits(), fr.iso(len(isos)), og, oiso, isos)
+    print(json.dumps(rep), cnt["scan_skipped"], cnt["elems_marched"])
+    assert rep["ok"], rep
+    assert rep["oracle_hits"] > 500 and all(r["oracle_found"] > 100 for r in rep["iso"])
+    assert cnt["scan_skipped"] > 0.3 * cnt["elems_marched"]
+    fr.close()
+    mesh.close()

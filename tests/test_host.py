@@ -209,3 +209,38 @@ def test_bvh_invariants(femrt):
     depth = visit(0, 0)
     assert np.all(seen == 1)
     assert depth < 40
+
+
+@pytest.mark.parametrize("p", [2, 4, 6])
+def test_field_bounds_contain_dense_samples(femrt, oracle, p):
+    """The per-element field bounds the walk uses to skip scans contain u at
+    a dense grid and random points of every element, and are not vacuous."""
+    hm = femrt.gen_structured_hex(2, 3, p, pert=1, amp=0.05, seed=4)
+    fr = femrt.host_field_range(hm)
+    nf = hm.nf
+    field = hm.field.reshape(hm.n_elem, -1)
+    g = np.linspace(-1, 1, 9)
+    rng = np.random.default_rng(3)
+    for e in range(hm.n_elem):
+        pts = list(itertools.product(g, g, g)) + [tuple(rng.uniform(-1, 1, 3)) for _ in range(100)]
+        u = np.array([oracle.eval_modal(field[e], nf, np.array(r))[0] for r in pts])
+        assert fr[e, 0] <= u.min() and u.max() <= fr[e, 1], e
+        assert fr[e, 1] - fr[e, 0] <= 3.0 * (u.max() - u.min()) + 1e-4, e
+
+
+def test_field_bounds_closed_forms(femrt):
+    """Bernstein bounds are exact for a constant and for a linear field
+    (u = 0.7 from the vertex modes; u = r from phi_1 - phi_0), up to the pad
+    of 1e-5 sum |c|."""
+    nf = 5
+    c = np.zeros((2, nf, nf, nf), np.float32)   # [elem][k][j][i]
+    c[0, :2, :2, :2] = 0.7
+    c[1, :2, :2, 1] = 1.0
+    c[1, :2, :2, 0] = -1.0
+    hm = femrt.HexMesh(np.zeros((2, 24), np.float32), np.zeros((2, 3 * 64), np.float32),
+                       c.reshape(2, -1), np.full((2, 12), -1, np.int32), 4, nf)
+    fr = femrt.host_field_range(hm)
+    np.testing.assert_allclose(fr[0], [0.7, 0.7], atol=1e-4)
+    assert fr[0, 0] < 0.7 < fr[0, 1]
+    np.testing.assert_allclose(fr[1], [-1.0, 1.0], atol=1e-4)
+    assert fr[1, 0] < -1.0 and fr[1, 1] > 1.0
