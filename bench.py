@@ -8,6 +8,10 @@ first hit plus isosurfaces at {-0.1, 0, 0.1}.  A step is one trace of the
 whole frame: ray generation, BVH traversal + face Newton (FP32), FP64 hit
 polish, element walk + iso sampling (FP32), FP64 iso endgame, shading.  The
 mesh, its BVH and the frame buffers are resident in HBM before timing starts.
+By default two frames are in flight, each on its own HIP stream (context,
+resident mesh copy and frame buffers), so one frame's first hit and walk run
+while the previous frame's walk finishes its longest rays; --inflight 1 traces
+one frame at a time.
 
 Prints one JSON line (see the field docs in main()).
 """
@@ -121,22 +125,35 @@ def parity_sample(femrt, orc_mod, parity, hm, fr, n_iso, n_check=192):
     return parity.compare(sub(gh), sub(giso), og, oiso, ISOS[:n_iso])
 
 
-def run_variant(femrt, ctx, mesh, fr, guess, steps, warmup, flags=1):
+def run_variant(femrt, lanes, guess, steps, warmup, flags=1):
+    """Time `steps` frames.  `lanes` is a list of (context, mesh, frame), one
+    per HIP stream; frame i goes to lane i mod len(lanes), so with two lanes
+    a frame's first hit and walk start while the previous frame's walk is
+    finishing its longest rays."""
     prm = femrt.trace_params(ISOS, guess=guess, flags=flags)
     for _ in range(warmup):
-        fr.trace(mesh, prm)
-    ctx.sync()
-    ctx.timing(True)
-    ctx.timing_reset()
+        for ctx, mesh, fr in lanes:
+            fr.trace(mesh, prm)
+    for ctx, _, _ in lanes:
+        ctx.sync()
+        ctx.timing(True)
+        ctx.timing_reset()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for s in range(steps):
+        ctx, mesh, fr = lanes[s % len(lanes)]
         fr.camera_rays(CAM, W, H)
         fr.trace(mesh, prm)
-    ctx.sync()
+    for ctx, _, _ in lanes:
+        ctx.sync()
     wall = time.perf_counter() - t0
-    kt = ctx.timing_read()
-    ctx.timing(False)
+    kt = {}
+    for ctx, _, _ in lanes:
+        for k, (ms, n) in ctx.timing_read().items():
+            a, b = kt.get(k, (0.0, 0))
+            kt[k] = (a + ms, b + n)
+        ctx.timing(False)
     # one validation trace with work counters on (not timed)
+    ctx, mesh, fr = lanes[0]
     fr.counters(enable=True)
     fr.trace(mesh, prm)
     cnt = fr.counters(enable=False)
@@ -235,6 +252,8 @@ def main():
                     help="frame: the 32^3 P=6 1080p frame (default); batch: 64 meshes in one graph")
     ap.add_argument("--batch-meshes", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="frames in flight, each on its own HIP stream (1 = one frame at a time)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -254,6 +273,12 @@ def main():
     n_rays = W * H
     fr = femrt.Frame(ctx, n_rays)
     fr.camera_rays(CAM, W, H)
+    # frames in flight: one context (HIP stream), resident mesh copy and frame
+    # per stream
+    lanes = [(ctx, mesh, fr)]
+    for _ in range(1, args.inflight):
+        c2 = femrt.Context(0)
+        lanes.append((c2, femrt.Mesh(c2, hm), femrt.Frame(c2, n_rays)))
 
     names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED,
              "learned_every": femrt.GUESS_LEARNED}
@@ -272,6 +297,11 @@ def main():
         feats, tgt = femrt.mlp_samples(mesh, 1 << 16, seed=SEED + 3)
         err = np.abs(mlp.forward(feats) - tgt).max(1)
         mesh.set_guess_mlp(mlp)
+        for lane_ctx, lane_mesh, _ in lanes[1:]:
+            # the same trained parameters in each stream's context
+            m2 = femrt.Mlp(lane_ctx, seed=SEED)
+            m2.set_params(mlp.params())
+            lane_mesh.set_guess_mlp(m2)
         tmesh.close()
         mlp_info = {"train_mesh": "16^3 Q=4 P=6 amp 0.08 h, seed+1", "train_pairs": 1 << 21,
                     "epochs": 4, "epoch_loss": [float(x) for x in losses],
@@ -287,12 +317,14 @@ def main():
         tflags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
         fr.camera_rays(CAM, W, H)
         launch_cfg = list(mesh.autotune(fr, femrt.trace_params(ISOS, guess=names[tv], flags=tflags), reps=3))
+        for _, lane_mesh, _ in lanes[1:]:
+            lane_mesh.launch_cfg(launch_cfg)
         print(json.dumps({"autotuned_launch_cfg": launch_cfg}), file=sys.stderr, flush=True)
     for v in variants:
         flags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
         if v == "learned_every":
             flags |= femrt.TRACE_LEARNED_EVERY_SAMPLE
-        wall, kt, cnt = run_variant(femrt, ctx, mesh, fr, names[v], args.steps, args.warmup, flags)
+        wall, kt, cnt = run_variant(femrt, lanes, names[v], args.steps, args.warmup, flags)
         f32, f64 = kernel_flops(cnt, hm.ng, hm.nf)
         results[v] = dict(wall=wall, kernels=kt, counters=cnt, f32=f32, f64=f64,
                           mrays=args.steps * n_rays / wall / 1e6)
@@ -360,6 +392,7 @@ def main():
         "data": "synthetic (seeded 32^3 curved-hex mesh and modal field)",
         "config": {"workload": "32^3 curved hex Q=4 P=6, 1920x1080 1spp, isos {-0.1,0,0.1}",
                    "variant": head, "rays": n_rays, "hit_fraction": hits / n_rays,
+                   "frames_in_flight": args.inflight,
                    "mrays_s_hit_rays": R["mrays"] * hits / n_rays,
                    "variants_mrays_s": {k: v["mrays"] for k, v in results.items()},
                    "launch_cfg": {"first_hit_block": launch_cfg[0], "walk_blocks_per_cu": launch_cfg[1],
@@ -373,9 +406,10 @@ def main():
     if "plain" in results and head != "plain":
         line["config"]["speedup_vs_plain_newton"] = R["mrays"] / results["plain"]["mrays"]
     print(json.dumps(line), flush=True)
-    fr.close()
-    mesh.close()
-    ctx.close()
+    for c, m, f in reversed(lanes):
+        f.close()
+        m.close()
+        c.close()
 
 
 if __name__ == "__main__":
