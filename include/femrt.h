@@ -335,7 +335,8 @@ typedef struct {
   int pert;       /* 0 none; 1 A sin(2 pi a) sin(2 pi b) of the node's two
                      in-face world coordinates; 2 seeded bubble per face
                      A h xi_f (1-u^2)(1-v^2), xi_f ~ N(0,1) clipped to +-2,
-                     (u, v) the node's face coordinates                     */
+                     (u, v) the node's face coordinates; 3 the same bubble
+                     with xi_f uniform on (-1, 1) (peak amp h)               */
   double amp;
   uint64_t seed;
 } femrt_gen_params;
@@ -347,6 +348,18 @@ typedef struct {
  * (seed, element, mode).  Output arrays sized as for femrt_mesh_upload_hex. */
 FEMRT_API int femrt_gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
                                        float* field, int* adjacency);
+
+/* Seeded multi-frequency field on a given geometry (n_elem x 3 x ng^3 modal
+ * coefficients): u(x) = sum of n_waves (1..64) plane waves sin(k.x + phi)
+ * with 1, 2, 4, 8 periods over the unit cube in turn (times a seeded factor
+ * in [1, 1.5)), amplitude 1/frequency, seeded directions and phases,
+ * normalised to unit RMS; sampled at each element's nf^3 GLL nodes and
+ * fitted to nf modes per direction.  field: n_elem x nf^3.  The reference
+ * reads fields from FLD files only (xml_fld_field_parser.cpp:165-289); this
+ * generator provides smooth fields with isosurfaces that span many elements
+ * for the same loader-shaped arrays. */
+FEMRT_API int femrt_gen_field_waves(int n_elem, int ng, const float* geom, int nf, int n_waves,
+                                    uint64_t seed, float* field);
 
 /* Drop-in loader: Nektar++ XML mesh + FLD field (xml_fld_loader.h:14-16).
  * Two-phase: call with NULL arrays to get sizes, then with arrays.  fld_path

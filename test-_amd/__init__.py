@@ -139,6 +139,7 @@ def lib():
         "femrt_frame_get_rgba": (I, [P, P]),
         "femrt_frame_counters": (I, [P, I, P]),
         "femrt_gen_structured_hex": (I, [P, P, P, P, P]),
+        "femrt_gen_field_waves": (I, [I, I, P, I, I, C.c_ulonglong, P]),
         "femrt_host_boxes": (I, [I, I, P, P]),
         "femrt_host_field_range": (I, [I, I, P, P]),
         "femrt_host_build_bvh": (I, [I, P, C.POINTER(I), P, P]),
@@ -191,6 +192,7 @@ def exported_symbols():
             "femrt_frame_destroy", "femrt_frame_set_rays", "femrt_frame_camera_rays",
             "femrt_frame_get_rays", "femrt_trace", "femrt_frame_get_hits", "femrt_frame_get_iso",
             "femrt_frame_get_rgba", "femrt_frame_counters", "femrt_gen_structured_hex",
+            "femrt_gen_field_waves",
             "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh", "femrt_host_field_range",
             "femrt_save_xml_fld", "femrt_io_last_error",
             "femrt_frame_camera_rays_jitter", "femrt_frame_path_begin", "femrt_frame_scatter",
@@ -244,6 +246,17 @@ def gen_structured_hex(n, q, p, pert=0, amp=0.0, seed=0) -> HexMesh:
     _check(lib().femrt_gen_structured_hex(C.byref(gp), _ptr(v), _ptr(g), _ptr(f), _ptr(a)),
            what="gen_structured_hex")
     return HexMesh(v, g, f, a, ng, nf)
+
+
+def gen_field_waves(hm: HexMesh, p, n_waves=8, seed=0) -> HexMesh:
+    """The mesh `hm` with its field replaced by a seeded multi-frequency
+    field (a sum of `n_waves` plane waves of 1, 2, 4 and 8 periods over the
+    unit cube, unit RMS) fitted at field order `p` on each element."""
+    nf = p + 1
+    f = np.zeros((hm.n_elem, nf ** 3), np.float32)
+    _check(lib().femrt_gen_field_waves(hm.n_elem, hm.ng, _ptr(hm.geom), nf, n_waves, seed, _ptr(f)),
+           what="gen_field_waves")
+    return HexMesh(hm.vertices, hm.geom, f, hm.adjacency, hm.ng, nf)
 
 
 NBOX = 61  # element box, 6 face boxes, 6 x 9 face sub-patch boxes

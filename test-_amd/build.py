@@ -48,7 +48,7 @@ def build_library(verbose: bool = True) -> Path:
         if src.suffix == ".hip":
             cmd = [HIPCC, *HIP_FLAGS, "-c", src, "-o", obj]
         else:
-            cmd = [HIPCC, "-x", "c++", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
+            cmd = [HIPCC, "-x", "c++", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-pthread",
                    "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", src, "-o", obj]
         print("+", " ".join(str(c) for c in cmd), flush=True)
         jobs.append((src, subprocess.Popen([str(c) for c in cmd])))
@@ -57,7 +57,7 @@ def build_library(verbose: bool = True) -> Path:
     if failed:
         raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp, "-lz"])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp, "-lz", "-pthread"])
     os.replace(tmp, LIB)
     # command-line renderer over the C++ host layer (links the library)
     tmp = RENDER.with_suffix(".tmp")

@@ -6,11 +6,31 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "femrt_internal.hpp"
 
 namespace femrt {
+
+// Runs body(e0, e1) on contiguous chunks of [0, n), one host thread per chunk
+// (at most 16).  Used for per-element work whose result does not depend on
+// the split.
+template <typename F>
+static void parallel_chunks(int n, F body)
+{
+  unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (n < 512) nt = 1;
+  if (nt <= 1) {
+    body(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back(body, (int)((long long)n * t / nt), (int)((long long)n * (t + 1) / nt));
+  for (std::thread& x : th) x.join();
+}
 
 // ---------------------------------------------------------------- 1-D tools
 
@@ -57,17 +77,19 @@ void field_lipschitz(int n_elem, int nf, const float* field, float* out)
     D[p] *= 1.05;
   }
   const size_t F3 = (size_t)nf * nf * nf;
-  for (int e = 0; e < n_elem; ++e) {
-    const float* c = field + (size_t)e * F3;
-    double L = 0.0;
-    for (int k = 0; k < nf; ++k)
-      for (int j = 0; j < nf; ++j)
-        for (int i = 0; i < nf; ++i) {
-          const double a = std::fabs((double)c[((size_t)k * nf + j) * nf + i]);
-          L += a * (D[i] * B[j] * B[k] + B[i] * D[j] * B[k] + B[i] * B[j] * D[k]);
-        }
-    out[e] = (float)(L * (1.0 + 1e-6));
-  }
+  parallel_chunks(n_elem, [&](int e0, int e1) {
+    for (int e = e0; e < e1; ++e) {
+      const float* c = field + (size_t)e * F3;
+      double L = 0.0;
+      for (int k = 0; k < nf; ++k)
+        for (int j = 0; j < nf; ++j)
+          for (int i = 0; i < nf; ++i) {
+            const double a = std::fabs((double)c[((size_t)k * nf + j) * nf + i]);
+            L += a * (D[i] * B[j] * B[k] + B[i] * D[j] * B[k] + B[i] * B[j] * D[k]);
+          }
+      out[e] = (float)(L * (1.0 + 1e-6));
+    }
+  });
 }
 
 static double legendre(int n, double x, double* pnm1)
@@ -308,15 +330,17 @@ int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
       const int a0 = axis == 0 ? 1 : 0, a1 = axis == 2 ? 1 : 2;
       return gp->amp * std::sin(2.0 * M_PI * w[a0]) * std::sin(2.0 * M_PI * w[a1]);
     }
-    if (gp->pert == 2) {
+    if (gp->pert == 2 || gp->pert == 3) {
       // seeded bubble per face: amp h xi_f (1-u^2)(1-v^2), xi_f ~ N(0,1)
-      // clipped to +-2, keyed by the face's axis and lattice position so both
-      // elements of an interior face draw the same xi_f
+      // clipped to +-2 (pert 2) or uniform on (-1, 1) (pert 3, the peak
+      // displacement is then at most amp h), keyed by the face's axis and
+      // lattice position so both elements of an interior face draw the same
+      // xi_f
       const int a0 = axis == 0 ? 1 : 0, a1 = axis == 2 ? 1 : 2;
       const long fq[3] = {lat[axis] / q, lat[a0] / q, lat[a1] / q};
       double xi = normal_keyed(gp->seed, (uint64_t)axis + 1,
                                ((uint64_t)fq[0] << 21) ^ (uint64_t)fq[1], (uint64_t)fq[2]);
-      xi = std::max(-2.0, std::min(2.0, xi));
+      xi = gp->pert == 3 ? std::erf(xi / std::sqrt(2.0)) : std::max(-2.0, std::min(2.0, xi));
       const double u = -1.0 + 2.0 * (double)(lat[a0] % q) / q;
       const double v = -1.0 + 2.0 * (double)(lat[a1] % q) / q;
       return gp->amp * h * xi * (1.0 - u * u) * (1.0 - v * v);
@@ -324,11 +348,11 @@ int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
     return 0.0;
   };
 
+  parallel_chunks(N * N * N, [&](int e0, int e1) {
   std::vector<double> nodal(3 * n3), coef(n3);
-  for (int k = 0; k < N; ++k)
-    for (int j = 0; j < N; ++j)
-      for (int i = 0; i < N; ++i) {
-        const size_t e = ((size_t)k * N + j) * N + i;
+  for (int ee = e0; ee < e1; ++ee) {
+        const int i = ee % N, j = (ee / N) % N, k = ee / (N * N);
+        const size_t e = (size_t)ee;
         double V[8][3];
         const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
@@ -434,7 +458,89 @@ int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
           }
         }
       }
+  });
   (void)kFaceCorners;
+  return FEMRT_OK;
+}
+
+// -------------------------------------------------- multi-frequency fields
+
+// Field u(x) = sum_w a_w sin(k_w . x + phi_w) / sqrt(sum_w a_w^2 / 2) (unit
+// RMS): wave w has frequency f_w = 2^(w mod 4) (1, 2, 4, 8 periods over the
+// unit cube) times a seeded factor in [1, 1.5), a seeded direction, amplitude
+// 1 / f_w and a seeded phase.  Per element it is sampled at the nf^3 GLL
+// nodes of the element's geometry map and fitted to nf modes per direction
+// (the tensor modal fit used for geometry), so neighbouring elements agree
+// up to the fit error.
+int gen_field_waves(int n_elem, int ng, const float* geom, int nf, int n_waves, uint64_t seed,
+                    float* field)
+{
+  if (n_elem < 0 || ng < 2 || ng > 16 || nf < 2 || nf > 16 || n_waves < 1 || n_waves > 64 || !geom || !field)
+    return FEMRT_EINVAL;
+  std::vector<double> kv(3 * (size_t)n_waves), ph(n_waves), am(n_waves);
+  double norm = 0.0;
+  for (int w = 0; w < n_waves; ++w) {
+    const double u1 = 0.5 + 0.5 * std::erf(normal_keyed(seed, 11, (uint64_t)w, 0) / std::sqrt(2.0));
+    const double f = (double)(1 << (w % 4)) * (1.0 + 0.5 * u1);
+    double dir[3], dn = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      dir[c] = normal_keyed(seed, 12, (uint64_t)w, (uint64_t)c);
+      dn += dir[c] * dir[c];
+    }
+    dn = std::sqrt(std::max(dn, 1e-30));
+    for (int c = 0; c < 3; ++c) kv[3 * (size_t)w + c] = 2.0 * M_PI * f * dir[c] / dn;
+    ph[w] = 2.0 * M_PI * (0.5 + 0.5 * std::erf(normal_keyed(seed, 13, (uint64_t)w, 0) / std::sqrt(2.0)));
+    am[w] = 1.0 / f;
+    norm += 0.5 * am[w] * am[w];
+  }
+  norm = 1.0 / std::sqrt(norm);
+  const std::vector<double> gll = gll_nodes(nf);
+  std::vector<double> inv;
+  if (!modal_fit_1d_matrix(nf, gll, inv)) return FEMRT_EINVAL;
+  // geometry basis at the field's GLL nodes: Bm[a * ng + p] = phi_p(gll_a)
+  std::vector<double> Bm((size_t)nf * ng);
+  for (int a = 0; a < nf; ++a)
+    for (int q = 0; q < ng; ++q) Bm[(size_t)a * ng + q] = modA(q, gll[a]);
+  const size_t g3 = (size_t)ng * ng * ng, f3 = (size_t)nf * nf * nf;
+  parallel_chunks(n_elem, [&](int e0, int e1) {
+    std::vector<double> t1((size_t)ng * ng * nf), t2((size_t)ng * nf * nf), X(3 * f3), nodal(f3), coef(f3);
+    for (int e = e0; e < e1; ++e) {
+      // x at the GLL grid, sum-factorised over r, then s, then t
+      for (int c = 0; c < 3; ++c) {
+        const float* gc = geom + ((size_t)e * 3 + c) * g3;
+        for (int r = 0; r < ng; ++r)
+          for (int q = 0; q < ng; ++q)
+            for (int i = 0; i < nf; ++i) {
+              double acc = 0.0;
+              for (int p0 = 0; p0 < ng; ++p0) acc += Bm[(size_t)i * ng + p0] * gc[((size_t)r * ng + q) * ng + p0];
+              t1[((size_t)r * ng + q) * nf + i] = acc;
+            }
+        for (int r = 0; r < ng; ++r)
+          for (int j = 0; j < nf; ++j)
+            for (int i = 0; i < nf; ++i) {
+              double acc = 0.0;
+              for (int q = 0; q < ng; ++q) acc += Bm[(size_t)j * ng + q] * t1[((size_t)r * ng + q) * nf + i];
+              t2[((size_t)r * nf + j) * nf + i] = acc;
+            }
+        for (int k = 0; k < nf; ++k)
+          for (int j = 0; j < nf; ++j)
+            for (int i = 0; i < nf; ++i) {
+              double acc = 0.0;
+              for (int r = 0; r < ng; ++r) acc += Bm[(size_t)k * ng + r] * t2[((size_t)r * nf + j) * nf + i];
+              X[(size_t)c * f3 + ((size_t)k * nf + j) * nf + i] = acc;
+            }
+      }
+      for (size_t m = 0; m < f3; ++m) {
+        double u = 0.0;
+        for (int w = 0; w < n_waves; ++w)
+          u += am[w] * std::sin(kv[3 * (size_t)w] * X[m] + kv[3 * (size_t)w + 1] * X[f3 + m] +
+                                kv[3 * (size_t)w + 2] * X[2 * f3 + m] + ph[w]);
+        nodal[m] = u * norm;
+      }
+      tensor_fit(nf, inv, nodal.data(), coef.data());
+      for (size_t m = 0; m < f3; ++m) field[(size_t)e * f3 + m] = (float)coef[m];
+    }
+  });
   return FEMRT_OK;
 }
 
@@ -485,8 +591,9 @@ void field_range(int n_elem, int nf, const float* field, float* out)
   const int n = nf;
   const std::vector<double> T = modal_to_bernstein(n, -1.0, 1.0);  // T[q][m]: modal q -> Bernstein m
   const size_t n3 = (size_t)n * n * n;
+  parallel_chunks(n_elem, [&](int e0, int e1) {
   std::vector<double> c(n3), t1(n3), t2(n3);
-  for (int e = 0; e < n_elem; ++e) {
+  for (int e = e0; e < e1; ++e) {
     double asum = 0.0;
     for (size_t m = 0; m < n3; ++m) {
       c[m] = field[(size_t)e * n3 + m];
@@ -520,6 +627,7 @@ void field_range(int n_elem, int nf, const float* field, float* out)
     out[2 * (size_t)e] = std::nextafter((float)(lo - pad), -INFINITY);
     out[2 * (size_t)e + 1] = std::nextafter((float)(hi + pad), INFINITY);
   }
+  });
 }
 
 static const int kBoxSplit = 3;
@@ -531,8 +639,9 @@ void bernstein_boxes(int n_elem, int ng, const float* geom, float* out)
   const size_t n3 = (size_t)n * n * n;
   std::vector<std::vector<double>> T(S);
   for (int s = 0; s < S; ++s) T[s] = modal_to_bernstein(n, -1.0 + 2.0 * s / S, -1.0 + 2.0 * (s + 1) / S);
+  parallel_chunks(n_elem, [&](int e0, int e1) {
   std::vector<double> cc(3 * n3), t1(3 * n3), t2(3 * n3), bz(3 * n3);
-  for (int e = 0; e < n_elem; ++e) {
+  for (int e = e0; e < e1; ++e) {
     double lo[FEMRT_NBOX][3], hi[FEMRT_NBOX][3];
     for (int b = 0; b < FEMRT_NBOX; ++b)
       for (int c = 0; c < 3; ++c) {
@@ -616,6 +725,7 @@ void bernstein_boxes(int n_elem, int ng, const float* geom, float* out)
       }
     }
   }
+  });
 }
 
 // ------------------------------------------------------------ binned SAH BVH

@@ -1128,6 +1128,12 @@ FEMRT_API int femrt_gen_structured_hex(const femrt_gen_params* gp, float* vertic
   return gen_structured_hex(gp, vertices, geom, field, adjacency);
 }
 
+FEMRT_API int femrt_gen_field_waves(int n_elem, int ng, const float* geom, int nf, int n_waves,
+                                    uint64_t seed, float* field)
+{
+  return gen_field_waves(n_elem, ng, geom, nf, n_waves, seed, field);
+}
+
 static thread_local std::string g_io_err;
 
 FEMRT_API const char* femrt_io_last_error(void) { return g_io_err.c_str(); }

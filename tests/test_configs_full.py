@@ -1,0 +1,143 @@
+"""Parity at full size on the GPU for the three single-frame workloads:
+
+* 32^3 curved hex, Q=4, amplitude 0.08 h, P=6, 1920x1080, isovalues
+  {-0.1, 0, 0.1} (the benchmark frame);
+* 64^3 curved hex, Q=4, face bubbles of peak 0.15 h, P=6, 3840x2160
+  (learned guesses on a larger, more curved mesh);
+* 64^3 curved hex, Q=4, face bubbles of peak 0.15 h, P=8 multi-frequency field with
+  four isovalues, 1920x1080 at 16 jittered samples per pixel.
+
+Each frame is traced whole on the GPU.  A seeded random sample of hit and
+miss rays is checked against the FP64 oracle at the acceptance bar (the
+oracle would take hours over the whole frame), and the whole frame is
+checked by a size-independent property: the learned, warm-start and plain
+Newton variants start their FP32 iterations from different points but must
+converge to the same FP64 results on every ray, ties aside.
+"""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N_SAMPLE_HIT, N_SAMPLE_MISS = 240, 30
+# GPU-vs-GPU agreement over the whole frame: the same FP64 root to 1e-9
+# relative; a ray may differ only where an FP32 sign or id decision is a tie
+# (a sample of u - iso within about 1e-5 of zero, or two faces within 1e-6 t)
+SAME_T_REL = 1e-9
+MAX_GEOM_DIFF_FRAC = 1e-4
+MAX_ISO_DIFF_FRAC = 2e-3
+
+
+def _sample_check(oracle, hm, fr, isos, seed):
+    import oracle.parity as parity
+    gh, giso, rays = fr.hits(), fr.iso(len(isos)), fr.get_rays()
+    rng = np.random.default_rng(seed)
+    hit = np.flatnonzero(gh["elem"] >= 0)
+    miss = np.flatnonzero(gh["elem"] < 0)
+    sel = np.concatenate([rng.choice(hit, min(N_SAMPLE_HIT, hit.size), replace=False),
+                          rng.choice(miss, min(N_SAMPLE_MISS, miss.size), replace=False)])
+
+    def sub(d):
+        return {k: (v[..., sel] if isinstance(v, np.ndarray) else v) for k, v in d.items()}
+    og, oiso = oracle.Oracle.from_mesh(hm).trace(np.ascontiguousarray(rays[:, sel]), isos)
+    rep = parity.compare(sub(gh), sub(giso), og, oiso, isos)
+    return rep, gh, giso
+
+
+def _frame_agreement(a_hits, a_iso, b_hits, b_iso):
+    """Whole-frame comparison of two GPU traces of the same rays."""
+    ha, hb = a_hits["elem"] >= 0, b_hits["elem"] >= 0
+    n_hit = max(1, int(ha.sum()))
+    both = ha & hb
+    same = both & (a_hits["elem"] == b_hits["elem"]) & (a_hits["face"] == b_hits["face"])
+    trel = np.abs(a_hits["t"] - b_hits["t"]) / np.maximum(np.abs(b_hits["t"]), 1e-300)
+    geom_diff = int((ha != hb).sum() + (both & ~same).sum() + (same & (trel > SAME_T_REL)).sum())
+    out = {"hits": int(ha.sum()), "geom_diff": geom_diff, "geom_diff_frac": geom_diff / n_hit,
+           "t_rel_max_same": float(trel[same].max()) if same.any() else 0.0, "iso": []}
+    for k in range(a_iso["elem"].shape[0]):
+        fa, fb = a_iso["elem"][k] >= 0, b_iso["elem"][k] >= 0
+        b = fa & fb
+        tr = np.abs(a_iso["t"][k] - b_iso["t"][k]) / np.maximum(np.abs(b_iso["t"][k]), 1e-300)
+        se = b & (a_iso["elem"][k] == b_iso["elem"][k]) & (tr <= SAME_T_REL)
+        diff = int((fa != fb).sum() + (b & ~se).sum())
+        out["iso"].append({"found": int(fa.sum()), "diff": diff,
+                           "diff_frac": diff / max(1, int(fa.sum()))})
+    return out
+
+
+def _trained_mlp(femrt, ctx, pert, amp, seed):
+    """Guess network trained by the library's kernels on a 16^3 mesh of the
+    same family (another seed)."""
+    thm = femrt.gen_structured_hex(16, 4, 2, pert=pert, amp=amp, seed=seed)
+    tmesh = femrt.Mesh(ctx, thm, build_bvh=False)
+    mlp = femrt.Mlp(ctx, seed=seed)
+    losses = mlp.train(tmesh, n_samples=1 << 20, batch=4096, epochs=3, lr=3e-3, lr_final=1e-4,
+                       seed=seed + 1)
+    tmesh.close()
+    assert losses[-1] < 0.5 * losses[0]
+    return mlp
+
+
+def _run(femrt, oracle, ctx, hm, rays_fn, n_rays, isos, pert, amp, seed, variants):
+    mesh = femrt.Mesh(ctx, hm)
+    mlp = _trained_mlp(femrt, ctx, pert, amp, seed)
+    mesh.set_guess_mlp(mlp)
+    fr = femrt.Frame(ctx, n_rays)
+    rays_fn(fr)
+    fr.counters(True)
+    fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_LEARNED))
+    cnt = fr.counters(False)
+    rep, gh, giso = _sample_check(oracle, hm, fr, isos, seed)
+    print(json.dumps({"learned_vs_oracle": rep, "counters": {k: cnt[k] for k in ("hit_rays", "mlp_infer",
+                                                                                     "brackets")}}),
+          flush=True)
+    assert rep["ok"], rep
+    assert rep["oracle_hits"] >= N_SAMPLE_HIT - 5
+    assert cnt["mlp_infer"] > 0
+    for name in variants:
+        fr.trace(mesh, femrt.trace_params(isos, guess=name))
+        agree = _frame_agreement(gh, giso, fr.hits(), fr.iso(len(isos)))
+        print(json.dumps({"learned_vs_variant": name, **agree}), flush=True)
+        assert agree["geom_diff_frac"] <= MAX_GEOM_DIFF_FRAC, agree
+        assert all(r["diff_frac"] <= MAX_ISO_DIFF_FRAC for r in agree["iso"]), agree
+    for o in (fr, mesh, mlp):
+        o.close()
+    return rep, cnt
+
+
+def test_bench_frame_32cube_p6_1080p(femrt, oracle, gpu_ctx):
+    hm = femrt.gen_structured_hex(32, 4, 6, pert=2, amp=0.08, seed=20240611)
+    cam = femrt.auto_camera([0, 0, 0], [1, 1, 1])
+    W, H = 1920, 1080
+    isos = [-0.1, 0.0, 0.1]
+    rep, cnt = _run(femrt, oracle, gpu_ctx, hm, lambda fr: fr.camera_rays(cam, W, H), W * H, isos,
+                    2, 0.08, 101, [femrt.GUESS_WARM, femrt.GUESS_PLAIN])
+    assert all(r["oracle_found"] > 100 for r in rep["iso"])
+
+
+def test_learned_64cube_amp015_4k(femrt, oracle, gpu_ctx):
+    hm = femrt.gen_structured_hex(64, 4, 6, pert=3, amp=0.15, seed=404)
+    cam = femrt.auto_camera([0, 0, 0], [1, 1, 1])
+    W, H = 3840, 2160
+    isos = [0.0]
+    rep, cnt = _run(femrt, oracle, gpu_ctx, hm, lambda fr: fr.camera_rays(cam, W, H), W * H, isos,
+                    3, 0.15, 202, [femrt.GUESS_WARM])
+    assert cnt["hit_rays"] > 0.3 * W * H
+    assert rep["iso"][0]["oracle_found"] > 150
+
+
+def test_multifrequency_64cube_p8_four_isos_16spp(femrt, oracle, gpu_ctx):
+    base = femrt.gen_structured_hex(64, 4, 2, pert=3, amp=0.15, seed=505)
+    hm = femrt.gen_field_waves(base, 8, n_waves=8, seed=506)
+    assert hm.nf == 9
+    cam = femrt.auto_camera([0, 0, 0], [1, 1, 1])
+    W, H, spp = 1920, 1080, 16
+    cam12 = femrt.camera_basis(cam[:3], cam[3:6], cam[6:9], float(cam[9]), W / H)
+    isos = [-0.6, -0.2, 0.2, 0.6]
+    rep, cnt = _run(femrt, oracle, gpu_ctx, hm, lambda fr: fr.camera_rays_jitter(cam12, W, H, spp, 507),
+                    W * H * spp, isos, 3, 0.15, 303, [femrt.GUESS_WARM])
+    assert all(r["oracle_found"] > 60 for r in rep["iso"])

@@ -112,7 +112,8 @@ def test_generator_corners_adjacency_and_conformity(femrt, oracle):
 
 
 @pytest.mark.parametrize("N,q,pert,amp,seed,n_check", [(4, 3, 1, 0.05, 1, 64), (6, 4, 2, 0.08, 5, 216),
-                                                      (32, 4, 2, 0.08, 20240611, 150)])
+                                                      (32, 4, 2, 0.08, 20240611, 150),
+                                                      (64, 4, 3, 0.15, 404, 150)])
 def test_generator_elements_are_valid(femrt, oracle, N, q, pert, amp, seed, n_check):
     """Every checked element has a positive Jacobian determinant on a 5^3 grid
     (the map is one-to-one, so point inversion and face roots are defined)."""
@@ -244,3 +245,39 @@ def test_field_bounds_closed_forms(femrt):
     assert fr[0, 0] < 0.7 < fr[0, 1]
     np.testing.assert_allclose(fr[1], [-1.0, 1.0], atol=1e-4)
     assert fr[1, 0] < -1.0 and fr[1, 1] > 1.0
+
+
+def test_field_waves_is_continuous_seeded_and_unit_rms(femrt, oracle):
+    """The multi-frequency field interpolates one smooth function at each
+    element's GLL nodes: both sides of every checked interior face give the
+    same u (the face trace of the interpolant depends on the face nodes
+    only), it is seeded, and its RMS over the cube is about 1."""
+    hm = femrt.gen_field_waves(femrt.gen_structured_hex(8, 4, 2, pert=2, amp=0.15, seed=3), 8,
+                               n_waves=8, seed=5)
+    assert hm.nf == 9 and hm.field.shape == (512, 729)
+    again = femrt.gen_field_waves(hm, 8, n_waves=8, seed=5)
+    other = femrt.gen_field_waves(hm, 8, n_waves=8, seed=6)
+    assert np.array_equal(hm.field, again.field) and not np.array_equal(hm.field, other.field)
+    orc = oracle.Oracle.from_mesh(hm)
+    rng = np.random.default_rng(0)
+    # faces 2 (r = +1) and 4 (r = -1) of x-neighbours, 1 (s = -1) / 3 (s = +1)
+    pairs = [(2, 0, 1.0, -1.0), (3, 1, 1.0, -1.0)]
+    n_checked = 0
+    for _ in range(150):
+        e = int(rng.integers(0, hm.n_elem))
+        for f, axis, side_a, side_b in pairs:
+            ne = hm.adjacency[e, f, 0]
+            if ne < 0:
+                continue
+            a, b = rng.uniform(-1, 1, 2)
+            pa = np.insert(np.array([a, b]), axis, side_a)
+            pb = np.insert(np.array([a, b]), axis, side_b)
+            xa = orc.map(e, pa)[0]
+            xb = orc.map(ne, pb)[0]
+            np.testing.assert_allclose(xa, xb, atol=1e-6)
+            assert abs(orc.field_at(e, pa)[0] - orc.field_at(ne, pb)[0]) < 1e-5
+            n_checked += 1
+    assert n_checked > 100
+    us = np.array([orc.field_at(int(rng.integers(0, hm.n_elem)), rng.uniform(-1, 1, 3))[0]
+                   for _ in range(3000)])
+    assert 0.7 < np.sqrt(np.mean(us ** 2)) < 1.3
