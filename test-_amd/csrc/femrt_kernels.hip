@@ -7,7 +7,9 @@
 //                        hit rays are compacted into a queue with a wave
 //                        ballot + prefix and one atomic per wave.
 //   k_polish_hit<NG,NF>  FP64 Newton polish of each hit on its face, FP64
-//                        field value at the hit.
+//                        field value at the hit; chord bin of each hit.
+//   k_walk_order         per queue segment, hits sorted by descending chord
+//                        bin (stable): the walk starts the longest rays first.
 //   k_march<NG,NF>       per hit ray: element walk through face adjacency;
 //                        in each element the exit face root (FP32), uniform
 //                        samples of g = u - iso over [t_in, t_out] with point
@@ -118,13 +120,35 @@ __global__ __launch_bounds__(256) void k_first_hit(const float* __restrict__ ray
 }
 
 // FP64 face Newton polish of each queued hit; FP64 field value at the hit.
+// Chord bins of the walk order: the ray's length from its hit to where it
+// leaves the mesh's bounding box, in FEMRT_QBINS steps of the box diagonal.
+// A ray can walk no more elements than its chord crosses, so long chords
+// are the candidates for long walks.
+#define FEMRT_QBINS 8
+
+__device__ __forceinline__ int chord_bin(const TraceMesh& M, const float* o, const float* d, float t_hit)
+{
+  if (!M.nodes) return 0;
+  const float4 lo = M.nodes[0], hi = M.nodes[1];
+  const float l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+  float t_exit = 3.4e38f, dn = 0.f, diag = 0.f;
+  for (int c = 0; c < 3; ++c) {
+    if (d[c] != 0.f) t_exit = fminf(t_exit, fmaxf((l[c] - o[c]) / d[c], (h[c] - o[c]) / d[c]));
+    dn += d[c] * d[c];
+    diag += (h[c] - l[c]) * (h[c] - l[c]);
+  }
+  const float len = fmaxf(t_exit - t_hit, 0.f) * sqrtf(dn);
+  const float x = diag > 0.f ? len * rsqrtf(diag) * FEMRT_QBINS : 0.f;
+  return x >= FEMRT_QBINS - 1 ? FEMRT_QBINS - 1 : (int)x;
+}
+
 template <int NG, int NF>
 __global__ __launch_bounds__(256) void k_polish_hit(
     const float* __restrict__ rays, int n, TraceMesh M, const int* __restrict__ queue,
     const int* __restrict__ counts, const int* __restrict__ hit_elem,
     const int* __restrict__ hit_face, const float4* __restrict__ hit_f32,
     double* __restrict__ hit_t, double* __restrict__ hit_rst, double* __restrict__ hit_u,
-    long long* __restrict__ work)
+    unsigned char* __restrict__ qbin, long long* __restrict__ work)
 {
   const int count = counts[0];
   Work w;
@@ -132,6 +156,7 @@ __global__ __launch_bounds__(256) void k_polish_hit(
     const int r = queue[q];
     float of[3], df[3];
     load_ray(rays, n, r, of, df);
+    qbin[q] = (unsigned char)chord_bin(M, of, df, hit_f32[r].z);
     const double o[3] = {of[0], of[1], of[2]}, d[3] = {df[0], df[1], df[2]};
     const int e = hit_elem[r], f = hit_face[r];
     const float4 h = hit_f32[r];
@@ -147,6 +172,66 @@ __global__ __launch_bounds__(256) void k_polish_hit(
     ++w.field64;
   }
   flush_work(work, w);
+}
+
+// Walk order of the hit queue.  The queue is split into FEMRT_QSEG
+// contiguous segments (tile order: each covers one band of the image), as
+// the walk reads it; within each segment the hits are put in descending
+// chord bin, tile order kept within a bin (a stable counting sort, one block
+// per segment, each thread a contiguous chunk).  The walk then starts the
+// rays that may walk longest first, and the rays still walking when the
+// queue drains are short ones.
+#define FEMRT_ORDER_THREADS 1024
+
+__global__ __launch_bounds__(FEMRT_ORDER_THREADS) void k_walk_order(
+    const int* __restrict__ queue, const unsigned char* __restrict__ qbin,
+    const int* __restrict__ counts, int* __restrict__ out)
+{
+  __shared__ int s_cnt[FEMRT_QBINS][FEMRT_ORDER_THREADS];
+  __shared__ int s_base[FEMRT_QBINS];
+  const int count = counts[0], seg = blockIdx.x, t = threadIdx.x;
+  const int lo = (int)((long long)count * seg / FEMRT_QSEG);
+  const int hi = (int)((long long)count * (seg + 1) / FEMRT_QSEG);
+  const int c0 = lo + (int)((long long)(hi - lo) * t / FEMRT_ORDER_THREADS);
+  const int c1 = lo + (int)((long long)(hi - lo) * (t + 1) / FEMRT_ORDER_THREADS);
+  int cnt[FEMRT_QBINS];
+  for (int b = 0; b < FEMRT_QBINS; ++b) cnt[b] = 0;
+  for (int q = c0; q < c1; ++q) {
+    const int b = qbin[q];
+    for (int k = 0; k < FEMRT_QBINS; ++k) cnt[k] += (k == b) ? 1 : 0;
+  }
+  for (int b = 0; b < FEMRT_QBINS; ++b) s_cnt[b][t] = cnt[b];
+  __syncthreads();
+  if (t < FEMRT_QBINS) {
+    // exclusive prefix over the threads' chunks for bin t; s_base[t] = total
+    int acc = 0;
+    for (int i = 0; i < FEMRT_ORDER_THREADS; ++i) {
+      const int v = s_cnt[t][i];
+      s_cnt[t][i] = acc;
+      acc += v;
+    }
+    s_base[t] = acc;
+  }
+  __syncthreads();
+  if (t == 0) {
+    // bins in descending order: the longest chords first
+    int acc = 0;
+    for (int b = FEMRT_QBINS - 1; b >= 0; --b) {
+      const int v = s_base[b];
+      s_base[b] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  int off[FEMRT_QBINS];
+  for (int b = 0; b < FEMRT_QBINS; ++b) off[b] = lo + s_base[b] + s_cnt[b][t];
+  for (int q = c0; q < c1; ++q) {
+    const int b = qbin[q];
+    int dst = 0;
+    for (int k = 0; k < FEMRT_QBINS; ++k)
+      if (k == b) dst = off[k]++;
+    out[dst] = queue[q];
+  }
 }
 
 // Bracket output of the walk: FP32 bracket and estimate per (isovalue, ray).
@@ -823,6 +908,8 @@ FEMRT_API int femrt_frame_create(femrt_ctx* c, int n, femrt_frame** out)
             hipMalloc(&f->d_iso_f32, K * N * 16) == hipSuccess &&
             hipMalloc(&f->d_iso_rst32, K * N * 16) == hipSuccess &&
             hipMalloc(&f->d_queue_hit, N * 4) == hipSuccess &&
+            hipMalloc(&f->d_queue_walk, N * 4) == hipSuccess &&
+            hipMalloc(&f->d_qbin, N) == hipSuccess &&
             hipMalloc(&f->d_counts, FEMRT_NCOUNTS * 4) == hipSuccess &&
             hipMalloc(&f->d_rgba, N * 4) == hipSuccess &&
             hipMalloc(&f->d_work, FEMRT_NCOUNTERS * 8) == hipSuccess;
@@ -845,7 +932,7 @@ FEMRT_API int femrt_frame_destroy(femrt_frame* f)
   hipStreamSynchronize(f->ctx->stream);
   void* ptrs[] = {f->d_rays, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst,
                   f->d_hit_u, f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, f->d_iso_f32,
-                  f->d_iso_rst32, f->d_queue_hit, f->d_counts, f->d_rgba, f->d_work,
+                  f->d_iso_rst32, f->d_queue_hit, f->d_queue_walk, f->d_qbin, f->d_counts, f->d_rgba, f->d_work,
                   f->d_path_tp, f->d_path_acc, f->d_mats, f->d_mlp_g, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
                   f->d_walk_sorted, f->d_walk_reent, f->d_walk_hist};
   for (void* p : ptrs) hipFree(p);
@@ -994,9 +1081,16 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
       rc = timed_launch(c, "k_polish_hit", [&] {
         k_polish_hit<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
             f->d_rays, n, TM, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
-            f->d_hit_f32, f->d_hit_t, f->d_hit_rst, f->d_hit_u, work);
+            f->d_hit_f32, f->d_hit_t, f->d_hit_rst, f->d_hit_u, f->d_qbin, work);
       });
       if (rc) break;
+      if (n_iso > 0 && !(p->flags & FEMRT_TRACE_BINNED_WALK)) {
+        rc = timed_launch(c, "k_walk_order", [&] {
+          k_walk_order<<<FEMRT_QSEG, FEMRT_ORDER_THREADS, 0, c->stream>>>(f->d_queue_hit, f->d_qbin,
+                                                                          f->d_counts, f->d_queue_walk);
+        });
+        if (rc) break;
+      }
       if (n_iso > 0) {
         if (p->flags & FEMRT_TRACE_BINNED_WALK) {
           rc = walk_run(c, m, f, TM, ip, learned, work);
@@ -1024,12 +1118,12 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
           rc = timed_launch(c, "k_march", [&] {
             if (learned)
               k_march<NG_, NF_, true><<<mblocks, 256, 0, c->stream>>>(
-                  f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
+                  f->d_rays, n, TM, ip, f->d_queue_walk, f->d_counts, f->d_hit_elem, f->d_hit_face,
                   f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work,
                   m->guess_mlp->d_pack, f->d_mlp_g);
             else
               k_march<NG_, NF_, false><<<mblocks, 256, 0, c->stream>>>(
-                  f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
+                  f->d_rays, n, TM, ip, f->d_queue_walk, f->d_counts, f->d_hit_elem, f->d_hit_face,
                   f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work,
                   nullptr, nullptr);
           });

@@ -94,6 +94,8 @@ struct femrt_frame {
   float4* d_iso_f32 = nullptr;  // MAX_ISO x n: (t_a, t_b, t*, packed signs)
   float4* d_iso_rst32 = nullptr;  // MAX_ISO x n: rst* (FP32)
   int* d_queue_hit = nullptr;     // n
+  int* d_queue_walk = nullptr;    // n: the hit queue in walk order (longest chords first per segment)
+  unsigned char* d_qbin = nullptr;  // n: chord bin per hit-queue position
   int* d_counts = nullptr;        // FEMRT_NCOUNTS: [0] hits, [2..3] walk
                                   // rounds, [FEMRT_QSEG_CTR..] walk queue segments
   uint32_t* d_rgba = nullptr;
