@@ -30,6 +30,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "femrt_internal.hpp"
@@ -280,7 +281,8 @@ __device__ __forceinline__ void wave_lds_sync()
 // guesses of the first P.mlp_lead interior samples of the round's segments
 // come from the network, evaluated on the matrix cores for the whole wave
 // between the exit and sample steps (later samples extrapolate).
-template <int NG, int NF, bool LEARNED>
+// COUNT selects the work counters (validation runs) or none (timed runs).
+template <int NG, int NF, bool LEARNED, bool COUNT>
 __global__ __launch_bounds__(256, 3) void k_march(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ queue,
     int* __restrict__ counts, const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(256, 3) void k_march(
 {
   const int count = counts[0];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  Work w;
+  typename std::conditional<COUNT, Work, NoWork>::type w;
   GlobalSink sink{iso_elem, iso_f32, iso_rst32, (size_t)n, 0};
   bf16x8* ops = nullptr;
   float* bias = nullptr;
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(256, 3) void k_march(
     }
   }
   if (timeline) atomicMax((unsigned long long*)&work[18], (unsigned long long)wall_clock64());
-  flush_work(work, w);
+  if constexpr (COUNT) flush_work(work, w);
 }
 
 template <int NG, int NF>
@@ -1116,16 +1118,19 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
             }
           }
           rc = timed_launch(c, "k_march", [&] {
-            if (learned)
-              k_march<NG_, NF_, true><<<mblocks, 256, 0, c->stream>>>(
+            const void* pack = learned ? m->guess_mlp->d_pack : nullptr;
+            _Float16* g = learned ? f->d_mlp_g : nullptr;
+            auto go = [&](auto L, auto CNT) {
+              k_march<NG_, NF_, decltype(L)::value, decltype(CNT)::value><<<mblocks, 256, 0, c->stream>>>(
                   f->d_rays, n, TM, ip, f->d_queue_walk, f->d_counts, f->d_hit_elem, f->d_hit_face,
-                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work,
-                  m->guess_mlp->d_pack, f->d_mlp_g);
-            else
-              k_march<NG_, NF_, false><<<mblocks, 256, 0, c->stream>>>(
-                  f->d_rays, n, TM, ip, f->d_queue_walk, f->d_counts, f->d_hit_elem, f->d_hit_face,
-                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work,
-                  nullptr, nullptr);
+                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work, pack, g);
+            };
+            using T1 = std::true_type;
+            using F0 = std::false_type;
+            if (learned && work) go(T1{}, T1{});
+            else if (learned) go(T1{}, F0{});
+            else if (work) go(F0{}, T1{});
+            else go(F0{}, F0{});
           });
         }
         if (rc) break;

@@ -52,6 +52,18 @@ struct Work {
   unsigned skipped = 0;  // segments whose samples the field bounds rule out
 };
 
+// The same counters compiled away: the kernels of timed runs take NoWork,
+// so the counts cost neither registers nor instructions there.  The code
+// that counts only ever increments them.
+struct NoCount {
+  FEMRT_HD NoCount& operator++() { return *this; }
+  FEMRT_HD NoCount& operator+=(unsigned) { return *this; }
+};
+struct NoWork {
+  NoCount face_it, inv_it, field, geo64, field64, slab, mlp, marched;
+  NoCount exit_pred, exit_search, exit_march, reentry, redo, brackets, skipped;
+};
+
 // FP32 result of the first-hit stage.
 struct HitF32 {
   int elem = -1, face = -1;
@@ -173,10 +185,10 @@ FEMRT_HD int face_newton_step(FaceIter& F, const float* o, const float* d, const
 // Face Newton from one seed.  Returns 1 for an accepted root (max(|u|,|v|)
 // <= 1 + 1e-4, tmin <= tau <= tmax), 0 for a converged root outside the
 // face, -1 when the iteration did not converge.
-template <int NG>
+template <int NG, typename C>
 FEMRT_HD int face_newton(const GeoMode* __restrict__ gc, const float* o, const float* d, int f,
                          float u, float v, float tau, float tmin, float tmax, float& tau_o,
-                         float& u_o, float& v_o, unsigned& iters)
+                         float& u_o, float& v_o, C& iters)
 {
   FaceIter F;
   face_iter_begin(F, f, u, v, tau, tmin, tmax);
@@ -209,10 +221,10 @@ FEMRT_HD float seed_tau(const GeoMode* __restrict__ gc, const float* o, const fl
 
 // The reference's seed grid {-0.8,0,0.8}^2, every seed run, nearest accepted
 // root kept.
-template <int NG>
+template <int NG, typename C>
 FEMRT_HD bool face_root_grid(const GeoMode* __restrict__ gc, const float* o, const float* d, int f,
                              float tmin, float tmax, float& tau_o, float& u_o, float& v_o,
-                             unsigned& iters)
+                             C& iters)
 {
   bool found = false;
   float best = FLT_MAX;
@@ -240,10 +252,10 @@ FEMRT_HD bool face_root_grid(const GeoMode* __restrict__ gc, const float* o, con
 // and a single flat seed's failure mode (converging to a crossing of the
 // face's polynomial extension outside the element while a crossing inside
 // exists) is avoided.
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD bool face_root_patches(const GeoMode* __restrict__ gc, const float* __restrict__ bx,
                                 const float* o, const float* d, const float* id, int f, float tmin,
-                                float tcull, float& tau_o, float& u_o, float& v_o, Work& w)
+                                float tcull, float& tau_o, float& u_o, float& v_o, W& w)
 {
   const float* sb = bx + 6 * (7 + 9 * f);
   bool found = false;
@@ -274,10 +286,10 @@ FEMRT_HD bool face_root_patches(const GeoMode* __restrict__ gc, const float* __r
 // Nearest accepted root of one face below `tcull`.  PLAIN: the reference's
 // seed grid, then the sub-patch seeds when the grid finds nothing.  WARM and
 // LEARNED: the sub-patch seeds.
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD bool face_root(const GeoMode* __restrict__ gc, const float* __restrict__ bx, const float* o,
                         const float* d, const float* id, int f, float tmin, float tcull, int guess,
-                        float& tau_o, float& u_o, float& v_o, Work& w)
+                        float& tau_o, float& u_o, float& v_o, W& w)
 {
   if (guess == FEMRT_GUESS_PLAIN) {
     float tt, uu, vv;
@@ -308,9 +320,9 @@ FEMRT_HD void load_ray(const float* __restrict__ rays, int n, int r, float* o, f
 // element: element box, then per face its box and the face root.  The
 // nearest accepted root over all elements (but `exclude`) and faces is the
 // hit.
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d, int guess,
-                              float tmin, float tmax, Work& w, int exclude = -1)
+                              float tmin, float tmax, W& w, int exclude = -1)
 {
   HitF32 h;
   float id[3];
@@ -392,9 +404,9 @@ FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d
 
 // FP64 face Newton polish of a hit (unconstrained, quadratic convergence from
 // the FP32 root).
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD void polish_hit(const GeoMode* __restrict__ gc, const double* o, const double* d, int f,
-                         double& u, double& v, double& tau, Work& w)
+                         double& u, double& v, double& tau, W& w)
 {
   const bool us = (f == 2 || f == 4), vs = (f == 0 || f == 5);
   for (int it = 0; it < 4; ++it) {
@@ -422,9 +434,9 @@ FEMRT_HD void polish_hit(const GeoMode* __restrict__ gc, const double* o, const 
 // Newton x -> rst in FP32 from the guess in rst (HexElement.cu:227-272:
 // undamped, |delta|_inf < 1e-5, <= 50 iterations).  J holds the Jacobian at
 // the last evaluated iterate.
-template <int NG>
+template <int NG, typename C>
 FEMRT_HD bool invert_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
-                         unsigned& iters)
+                         C& iters)
 {
   for (int it = 0; it < 50; ++it) {
     float X[3];
@@ -447,9 +459,9 @@ FEMRT_HD bool invert_f32(const GeoMode* __restrict__ gc, const float* x, float* 
 // rst = 0 finds spurious preimages of the polynomial map for points near a
 // face) or fails, restart from rst = 0 (when zero_retry: the plain fallback
 // of a learned guess), then from the 8 octant centres (+-0.5)^3.
-template <int NG>
+template <int NG, typename C>
 FEMRT_HD bool invert_in_cube_f32(const GeoMode* __restrict__ gc, const float* x, float* rst, float* J,
-                                 unsigned& iters, bool zero_retry = false)
+                                 C& iters, bool zero_retry = false)
 {
   if (invert_f32<NG>(gc, x, rst, J, iters) &&
       fmaxf(fabsf(rst[0]), fmaxf(fabsf(rst[1]), fabsf(rst[2]))) <= 1.001f)
@@ -474,9 +486,9 @@ FEMRT_HD bool invert_in_cube_f32(const GeoMode* __restrict__ gc, const float* x,
 // FP64 Newton x -> rst from a guess.  Converged when the step drops below
 // 1e-13, or once it is below 1e-9 and stops halving (the FP64 rounding floor
 // of x(rst) - x divided by |J|).  On success J is evaluated at the result.
-template <int NG>
+template <int NG, typename C>
 FEMRT_HD bool invert_f64(const GeoMode* __restrict__ gc, const double* x, double* rst, double* J,
-                         unsigned& iters)
+                         C& iters)
 {
   double prev = 1e300;
   for (int it = 0; it < 30; ++it) {
@@ -505,11 +517,11 @@ FEMRT_HD bool invert_f64(const GeoMode* __restrict__ gc, const double* x, double
 // Exit of the ray from element e: nearest accepted face root with
 // t > t_lo on the faces other than the entry face (and on the entry face too
 // when include_entry is set).
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD bool find_exit_search(const GeoMode* __restrict__ gc, const float* __restrict__ bx,
                                const float* o, const float* d, const float* id, int f_in,
                                float t_lo, int guess, bool include_entry, int& f_out, float& t_out,
-                               float& u_out, float& v_out, Work& w)
+                               float& u_out, float& v_out, W& w)
 {
   bool found = false;
   float best = FLT_MAX;
@@ -537,11 +549,11 @@ FEMRT_HD bool find_exit_search(const GeoMode* __restrict__ gc, const float* __re
 // of that inside/outside predicate, then a face Newton on the face whose
 // coordinate was crossed, seeded at the bisected point.  Used when the face
 // searches find no exit (two crossings of one face inside one sub-patch).
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD bool find_exit_marching(const GeoMode* __restrict__ gc, const float* __restrict__ bx,
                                  const float* o, const float* d, const float* id,
                                  const float* rst_in, float t_in, float t_lo, int& f_out,
-                                 float& t_out, float& u_out, float& v_out, Work& w)
+                                 float& t_out, float& u_out, float& v_out, W& w)
 {
   float b0, b1;
   ++w.slab;
@@ -606,10 +618,10 @@ FEMRT_HD bool find_exit_marching(const GeoMode* __restrict__ gc, const float* __
 // face excluded) at s_f; a face Newton on each crossed face, nearest crossing
 // first (at most 3), seeded at the predicted point, gives the exit.  Returns
 // false when none converges to an accepted root.
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD bool find_exit_predicted(const GeoMode* __restrict__ gc, const float* o, const float* d,
                                   int f_in, const float* rst_in, float t_in, float t_lo, int& f_out,
-                                  float& t_out, float& u_out, float& v_out, Work& w)
+                                  float& t_out, float& u_out, float& v_out, W& w)
 {
   float X[3], J[9], dl[3];
   geom_eval<float, NG>(gc, rst_in[0], rst_in[1], rst_in[2], X, J);
@@ -674,10 +686,10 @@ FEMRT_HD bool find_exit_predicted(const GeoMode* __restrict__ gc, const float* o
 // dtau = (F_u - g.a) / (g.b), drst = a + b dtau (g.b = grad u . d is the
 // derivative of u along the ray).  tau stays inside [lo, hi]; returns false
 // when an iterate leaves it or the system is singular.
-template <typename T, int NG, int NF>
+template <typename T, int NG, int NF, typename C>
 FEMRT_HD bool iso_newton(const GeoMode* __restrict__ gc, const float* __restrict__ fc, const T* o,
                          const T* d, T iso, T lo, T hi, int max_it, T tol_t, T tol_r, T& tau, T* rst,
-                         T& u, unsigned& geo_it, unsigned& field_it)
+                         T& u, C& geo_it, C& field_it)
 {
   for (int it = 0; it < max_it; ++it) {
     T X[3], J[9], g[3];
@@ -750,9 +762,9 @@ FEMRT_HD void march_begin(MarchState& S, const HitF32& h)
 // ray can leave through the face it entered by: a curved boundary face seen
 // at a grazing angle), then marching.  After a redo: the search including the
 // entry face, then marching.
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD void march_exit(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
-                         const float* id, MarchState& S, Work& w)
+                         const float* id, MarchState& S, W& w)
 {
   const GeoMode* gc = M.geom + (size_t)S.e * (NG * NG * NG);
   const float* bx = M.boxes + (size_t)S.e * FEMRT_BOX_STRIDE;
@@ -805,9 +817,9 @@ struct NoGuess {
 // sink.put(k, bracket) and marked in S.found.  Returns true when a sample
 // has no preimage inside the element on the first pass (the exit is wrong:
 // the caller sets S.redo and finds the exit again).
-template <int NG, int NF, typename Guess, typename Sink>
+template <int NG, int NF, typename Guess, typename Sink, typename W>
 FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
-                            MarchState& S, const Guess& guess, Sink& sink, Work& w)
+                            MarchState& S, const Guess& guess, Sink& sink, W& w)
 {
   const GeoMode* gc = M.geom + (size_t)S.e * (NG * NG * NG);
   const float* fc = M.field + (size_t)S.e * (NF * NF * NF);
@@ -1037,9 +1049,9 @@ FEMRT_HD int march_next(const TraceMesh& M, const IsoParams& P, MarchState& S)
 // Re-entry after march_next returned MARCH_REENTER: the nearest element the
 // ray meets beyond the last resolved point, other than the current one, by
 // the same traversal as the first hit.  Returns false when there is none.
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD bool march_reenter(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
-                            MarchState& S, Work& w)
+                            MarchState& S, W& w)
 {
   const float t_from = S.ok ? S.t_out : S.t_in;
   ++w.reentry;
@@ -1056,9 +1068,9 @@ FEMRT_HD bool march_reenter(const TraceMesh& M, const IsoParams& P, const float*
 }
 
 // march_next, then march_reenter when the ray left the mesh.
-template <int NG>
+template <int NG, typename W>
 FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
-                            MarchState& S, Work& w)
+                            MarchState& S, W& w)
 {
   const int k = march_next(M, P, S);
   if (k == MARCH_DONE) return false;
@@ -1073,9 +1085,9 @@ struct BracketArray {
 
 // The whole walk of one ray, one segment after another (host harnesses and
 // single-lane use).  Returns the mask of isovalues found.
-template <int NG, int NF>
+template <int NG, int NF, typename W>
 FEMRT_HD unsigned march_ray(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
-                            HitF32 h, IsoBracket* out, Work& w)
+                            HitF32 h, IsoBracket* out, W& w)
 {
   if (P.n_iso <= 0 || P.max_march <= 0) return 0;
   float id[3];
@@ -1100,10 +1112,10 @@ FEMRT_HD unsigned march_ray(const TraceMesh& M, const IsoParams& P, const float*
 // g'(t) = grad_world(u) . d, kept inside the sign-change bracket; bisection
 // when the Newton step leaves it.  Converged when the step or the bracket is
 // below 1e-13 (1 + |t|).
-template <int NG, int NF>
+template <int NG, int NF, typename W>
 FEMRT_HD bool polish_iso(const TraceMesh& M, const double* o, const double* d, double iso,
                          const IsoBracket& br, double& t_out, double* rst_out, double& u_out,
-                         Work& w)
+                         W& w)
 {
   const GeoMode* gc = M.geom + (size_t)br.elem * (NG * NG * NG);
   const float* fc = M.field + (size_t)br.elem * (NF * NF * NF);
