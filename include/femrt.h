@@ -326,6 +326,13 @@ FEMRT_API int femrt_host_build_bvh(int n_elem, const float* boxes, int* n_nodes,
  * walk uses them to skip scans that can cross no open isovalue: the extreme
  * tensor-Bernstein coefficients, widened by 1e-5 of sum |c|.  out: 2 n_elem. */
 FEMRT_API int femrt_host_field_range(int n_elem, int nf, const float* field, float* out);
+/* Per element, a bound L of |u(a) - u(b)| / max_c |a_c - b_c| over the
+ * reference cube, as the walk uses it to accept a sample's signs after one
+ * Newton step: over the 8 half-cubes, the largest sum over r, s, t of the
+ * largest |Bernstein coefficient| of the partial derivative, plus 5%.
+ * out: n_elem.  (No counterpart in the
+ * reference, whose scan converges every sample: fem/RayIsosurface.cu:57.) */
+FEMRT_API int femrt_host_field_lipschitz(int n_elem, int nf, const float* field, float* out);
 
 /* ------------------------------------------------------- synthetic meshes */
 typedef struct {

@@ -142,6 +142,7 @@ def lib():
         "femrt_gen_field_waves": (I, [I, I, P, I, I, C.c_ulonglong, P]),
         "femrt_host_boxes": (I, [I, I, P, P]),
         "femrt_host_field_range": (I, [I, I, P, P]),
+        "femrt_host_field_lipschitz": (I, [I, I, P, P]),
         "femrt_host_build_bvh": (I, [I, P, C.POINTER(I), P, P]),
         "femrt_load_xml_fld": (I, [C.c_char_p, C.c_char_p, I, C.POINTER(I), C.POINTER(I),
                                    C.POINTER(I), P, P, P, P]),
@@ -194,6 +195,7 @@ def exported_symbols():
             "femrt_frame_get_rgba", "femrt_frame_counters", "femrt_gen_structured_hex",
             "femrt_gen_field_waves",
             "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh", "femrt_host_field_range",
+            "femrt_host_field_lipschitz",
             "femrt_save_xml_fld", "femrt_io_last_error",
             "femrt_frame_camera_rays_jitter", "femrt_frame_path_begin", "femrt_frame_scatter",
             "femrt_frame_resolve", "femrt_render_path",
@@ -272,6 +274,15 @@ def host_field_range(hm: HexMesh):
     """Per element (lo, hi) bounds of the field over the reference cube."""
     out = np.zeros((hm.n_elem, 2), np.float32)
     _check(lib().femrt_host_field_range(hm.n_elem, hm.nf, _ptr(hm.field), _ptr(out)), what="host_field_range")
+    return out
+
+
+def host_field_lipschitz(hm: HexMesh):
+    """Per element, the bound of |u(a) - u(b)| / |a - b|_inf on the
+    reference cube that the walk's early sign acceptance uses."""
+    out = np.zeros(hm.n_elem, np.float32)
+    _check(lib().femrt_host_field_lipschitz(hm.n_elem, hm.nf, _ptr(hm.field), _ptr(out)),
+           what="host_field_lipschitz")
     return out
 
 

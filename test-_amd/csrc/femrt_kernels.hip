@@ -1199,6 +1199,13 @@ FEMRT_API int femrt_host_boxes(int n_elem, int ng, const float* geom, float* box
   return FEMRT_OK;
 }
 
+FEMRT_API int femrt_host_field_lipschitz(int n_elem, int nf, const float* field, float* out)
+{
+  if (n_elem < 0 || nf < 1 || nf > 16 || !field || !out) return FEMRT_EINVAL;
+  field_lipschitz(n_elem, nf, field, out);
+  return FEMRT_OK;
+}
+
 FEMRT_API int femrt_host_field_range(int n_elem, int nf, const float* field, float* out)
 {
   if (n_elem <= 0 || nf < 1 || nf > 16 || !field || !out) return FEMRT_EINVAL;

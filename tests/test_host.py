@@ -247,6 +247,49 @@ def test_field_bounds_closed_forms(femrt):
     assert fr[1, 0] < -1.0 and fr[1, 1] > 1.0
 
 
+@pytest.mark.parametrize("p", [2, 4, 6, 8])
+def test_field_lipschitz_bounds_the_gradient(femrt, oracle, p):
+    """The bound the walk's early sign acceptance uses: |u(a) - u(b)| <=
+    L |a - b|_inf on the reference cube, i.e. L >= |du/dr| + |du/ds| +
+    |du/dt| everywhere (checked on a dense grid and random points), and L is
+    within 3 times that maximum (the half-cube Bernstein bound)."""
+    hm = femrt.gen_structured_hex(2, 3, p, pert=1, amp=0.05, seed=8)
+    L = femrt.host_field_lipschitz(hm)
+    nf = hm.nf
+    field = hm.field.reshape(hm.n_elem, -1)
+    g = np.linspace(-1, 1, 9)
+    rng = np.random.default_rng(5)
+    for e in range(hm.n_elem):
+        pts = list(itertools.product(g, g, g)) + [tuple(rng.uniform(-1, 1, 3)) for _ in range(200)]
+        g1 = max(np.abs(oracle.eval_modal(field[e], nf, np.array(r))[1:]).sum() for r in pts)
+        assert g1 <= L[e], (e, g1, L[e])
+        assert L[e] <= 3.0 * g1, (e, g1, L[e])
+        # pairs of points: the Lipschitz statement itself
+        for _ in range(50):
+            a, b = rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)
+            du = abs(oracle.eval_modal(field[e], nf, a)[0] - oracle.eval_modal(field[e], nf, b)[0])
+            assert du <= L[e] * np.abs(a - b).max() * (1 + 1e-6) + 1e-12
+
+
+def test_field_lipschitz_closed_forms(femrt):
+    """u = 0.7 (vertex modes): L = 0; u = r (phi_1 - phi_0): L = 1.05 (the
+    gradient is exactly (1, 0, 0), its Bernstein coefficients are all 1, plus
+    the 5% margin); u = r + 2 s: L = 3.15."""
+    nf = 5
+    c = np.zeros((3, nf, nf, nf), np.float32)   # [elem][k][j][i]
+    c[0, :2, :2, :2] = 0.7
+    c[1, :2, :2, 1] = 1.0
+    c[1, :2, :2, 0] = -1.0
+    c[2, :2, :2, 1] += 1.0
+    c[2, :2, :2, 0] -= 1.0
+    c[2, :2, 1, :2] += 2.0
+    c[2, :2, 0, :2] -= 2.0
+    hm = femrt.HexMesh(np.zeros((3, 24), np.float32), np.zeros((3, 3 * 64), np.float32),
+                       c.reshape(3, -1), np.full((3, 12), -1, np.int32), 4, nf)
+    L = femrt.host_field_lipschitz(hm)
+    np.testing.assert_allclose(L, [0.0, 1.05, 3.15], atol=1e-5)
+
+
 def test_field_waves_is_continuous_seeded_and_unit_rms(femrt, oracle):
     """The multi-frequency field interpolates one smooth function at each
     element's GLL nodes: both sides of every checked interior face give the
