@@ -324,7 +324,8 @@ FEMRT_API int femrt_host_build_bvh(int n_elem, const float* boxes, int* n_nodes,
                                    int* prims);
 /* Per element, bounds (lo, hi) of the field over the reference cube, as the
  * walk uses them to skip scans that can cross no open isovalue: the extreme
- * tensor-Bernstein coefficients, widened by 1e-5 of sum |c|.  out: 2 n_elem. */
+ * tensor-Bernstein coefficients over the 8 half-cubes, widened by 1e-5 of
+ * sum |c|.  out: 2 n_elem. */
 FEMRT_API int femrt_host_field_range(int n_elem, int nf, const float* field, float* out);
 /* Per element, a bound L of |u(a) - u(b)| / max_c |a_c - b_c| over the
  * reference cube, as the walk uses it to accept a sample's signs after one

@@ -226,7 +226,7 @@ def test_field_bounds_contain_dense_samples(femrt, oracle, p):
         pts = list(itertools.product(g, g, g)) + [tuple(rng.uniform(-1, 1, 3)) for _ in range(100)]
         u = np.array([oracle.eval_modal(field[e], nf, np.array(r))[0] for r in pts])
         assert fr[e, 0] <= u.min() and u.max() <= fr[e, 1], e
-        assert fr[e, 1] - fr[e, 0] <= 3.0 * (u.max() - u.min()) + 1e-4, e
+        assert fr[e, 1] - fr[e, 0] <= 1.5 * (u.max() - u.min()) + 1e-4, e
 
 
 def test_field_bounds_closed_forms(femrt):
