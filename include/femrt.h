@@ -152,6 +152,9 @@ typedef struct {
                                          the first of each segment takes one (no
                                          earlier sample to extrapolate from) and
                                          the rest extrapolated warm starts */
+#define FEMRT_TRACE_NO_SAMPLE_SKIP 8   /* evaluate every scan sample, even where the
+                                          element's bounds certify its signs from
+                                          an earlier sample (same results)          */
 
 FEMRT_API void femrt_trace_params_default(femrt_trace_params* p);
 
@@ -187,8 +190,9 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
  * the constant 100 MHz clock: start, the moment its hit queue first ran dry
  * (after that lanes only finish the rays they hold), end; [19..26] rays
  * by segments walked, log2 bins 1, 2-3, 4-7, ..., 128+; [27] segments whose
- * scan the element's field bounds ruled out. */
-#define FEMRT_NCOUNTERS 28
+ * scan the element's field bounds ruled out; [28] scan samples whose signs
+ * the field and geometry bounds certified without evaluating them. */
+#define FEMRT_NCOUNTERS 29
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
 
 /* ------------------------------------------------ shading and bounces */
@@ -334,6 +338,13 @@ FEMRT_API int femrt_host_field_range(int n_elem, int nf, const float* field, flo
  * out: n_elem.  (No counterpart in the
  * reference, whose scan converges every sample: fem/RayIsosurface.cu:57.) */
 FEMRT_API int femrt_host_field_lipschitz(int n_elem, int nf, const float* field, float* out);
+/* Per element, K >= |J^-1|_inf (max row sum, J = dx/drst) everywhere in the
+ * reference cube, or 0 when none is found: interval bounds of J on the 8
+ * half-cubes (Bernstein coefficients of the geometry's partial derivatives)
+ * and a Neumann-series bound of the interval matrix's inverse, plus 5%.  With
+ * the field bound it certifies the signs of scan samples the walk then skips.
+ * out: n_elem.  (No counterpart in the reference.) */
+FEMRT_API int femrt_host_geom_jinv_bound(int n_elem, int ng, const float* geom, float* out);
 
 /* ------------------------------------------------------- synthetic meshes */
 typedef struct {

@@ -19,15 +19,15 @@ LIB_PATH = PKG_DIR / "_build" / "libfemrt.so"
 OK = 0
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "ESTATE"}
 GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
-TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE = 1, 2, 4   # trace flag bits
+TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE, TRACE_NO_SAMPLE_SKIP = 1, 2, 4, 8   # trace flag bits
 MAX_ISO = 8
-NCOUNTERS = 28
+NCOUNTERS = 29
 COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",
                  "geo64_it", "field64_evals", "slab_tests", "mlp_infer", "elems_marched",
                  "exit_pred", "exit_search", "exit_march", "reentry", "redo", "brackets",
                  "walk_t_start", "walk_t_drained", "walk_t_end",
                  "len_1", "len_2_3", "len_4_7", "len_8_15", "len_16_31", "len_32_63", "len_64_127",
-                 "len_128_up", "scan_skipped"]
+                 "len_128_up", "scan_skipped", "samples_skipped"]
 
 
 class FemrtError(RuntimeError):
@@ -143,6 +143,7 @@ def lib():
         "femrt_host_boxes": (I, [I, I, P, P]),
         "femrt_host_field_range": (I, [I, I, P, P]),
         "femrt_host_field_lipschitz": (I, [I, I, P, P]),
+        "femrt_host_geom_jinv_bound": (I, [I, I, P, P]),
         "femrt_host_build_bvh": (I, [I, P, C.POINTER(I), P, P]),
         "femrt_load_xml_fld": (I, [C.c_char_p, C.c_char_p, I, C.POINTER(I), C.POINTER(I),
                                    C.POINTER(I), P, P, P, P]),
@@ -195,7 +196,7 @@ def exported_symbols():
             "femrt_frame_get_rgba", "femrt_frame_counters", "femrt_gen_structured_hex",
             "femrt_gen_field_waves",
             "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh", "femrt_host_field_range",
-            "femrt_host_field_lipschitz",
+            "femrt_host_field_lipschitz", "femrt_host_geom_jinv_bound",
             "femrt_save_xml_fld", "femrt_io_last_error",
             "femrt_frame_camera_rays_jitter", "femrt_frame_path_begin", "femrt_frame_scatter",
             "femrt_frame_resolve", "femrt_render_path",
@@ -283,6 +284,14 @@ def host_field_lipschitz(hm: HexMesh):
     out = np.zeros(hm.n_elem, np.float32)
     _check(lib().femrt_host_field_lipschitz(hm.n_elem, hm.nf, _ptr(hm.field), _ptr(out)),
            what="host_field_lipschitz")
+    return out
+
+
+def host_geom_jinv_bound(hm: HexMesh):
+    """Per element, a bound of |J^-1|_inf over the reference cube (0: none)."""
+    out = np.zeros(hm.n_elem, np.float32)
+    _check(lib().femrt_host_geom_jinv_bound(hm.n_elem, hm.ng, _ptr(hm.geom), _ptr(out)),
+           what="host_geom_jinv_bound")
     return out
 
 

@@ -609,6 +609,147 @@ void field_range(int n_elem, int nf, const float* field, float* out)
   });
 }
 
+// y = x transformed along one axis by A (A[q][m], q < d[axis], m < mo);
+// index i + d0 (j + d1 k), i along r.  d is updated to y's dimensions.
+static void along(const std::vector<double>& x, int* d, int axis, const double* A, int mo,
+                  std::vector<double>& y)
+{
+  int od[3] = {d[0], d[1], d[2]};
+  od[axis] = mo;
+  y.assign((size_t)od[0] * od[1] * od[2], 0.0);
+  for (int k = 0; k < od[2]; ++k)
+    for (int j = 0; j < od[1]; ++j)
+      for (int i = 0; i < od[0]; ++i) {
+        const int o[3] = {i, j, k};
+        double acc = 0.0;
+        for (int q = 0; q < d[axis]; ++q) {
+          int src[3] = {i, j, k};
+          src[axis] = q;
+          acc += A[(size_t)q * mo + o[axis]] * x[src[0] + (size_t)d[0] * (src[1] + (size_t)d[1] * src[2])];
+        }
+        y[i + (size_t)od[0] * (j + (size_t)od[1] * k)] = acc;
+      }
+  d[axis] = mo;
+}
+
+// Bernstein forms of the Modified_A basis and of its derivative on the S
+// equal pieces of [-1, 1]: T[h][q][m] (stride n) and D[h][q][m] (stride
+// n - 1).  On [a, b] the derivative of a degree-N Bernstein form has the
+// degree-(N-1) coefficients N (beta_{m+1} - beta_m) / (b - a).
+static void piece_bernstein(int n, int S, std::vector<std::vector<double>>& T,
+                            std::vector<std::vector<double>>& D)
+{
+  const int N = n - 1;
+  T.assign(S, {});
+  D.assign(S, {});
+  for (int h = 0; h < S; ++h) {
+    const double a = -1.0 + 2.0 * h / S, b = -1.0 + 2.0 * (h + 1) / S;
+    T[h] = modal_to_bernstein(n, a, b);
+    D[h].assign((size_t)n * std::max(N, 1), 0.0);
+    for (int q = 0; q < n; ++q)
+      for (int m = 0; m < N; ++m)
+        D[h][(size_t)q * N + m] = N * (T[h][(size_t)q * n + m + 1] - T[h][(size_t)q * n + m]) / (b - a);
+  }
+}
+
+// Per element, K >= |J^-1|_inf (max row sum) everywhere in the reference
+// cube, J = dx/drst, or 0 when no bound is found.  On each of the 4 x 4 x 4
+// sub-cubes the 9 entries of J lie in intervals [J0 - R, J0 + R] (the
+// extreme Bernstein coefficients of the geometry's partial derivatives).
+// With M = J0^-1 and beta = |(|M| R)|_inf < 0.9, every J of the interval
+// matrix has |J^-1|_inf <= |M|_inf / (1 - beta) (J^-1 = (I + M E)^-1 M,
+// |M E| <= beta, a Neumann series).  K is the largest bound over the
+// sub-cubes, plus 5%.  (Face bubbles shear J by up to ~60% of its diagonal
+// across an element: on half-cubes beta came near 1 and K ~5x the true
+// maximum; on quarter-cubes K is ~1.7x it on the 32^3 benchmark mesh.)
+// Along a ray segment inside the element, |rst(t) - rst(t')|_inf <=
+// K |d|_inf |t - t'|.
+void geom_jinv_bound(int n_elem, int ng, const float* geom, float* out)
+{
+  const int n = ng, N = ng - 1, S = 4;
+  std::vector<std::vector<double>> T, D;
+  piece_bernstein(n, S, T, D);
+  const size_t n3 = (size_t)n * n * n;
+  const int S3 = S * S * S;
+  parallel_chunks(n_elem, [&](int e0, int e1) {
+    std::vector<double> c(n3), y1, y2, y3;
+    std::vector<double> jlo((size_t)S3 * 9), jhi((size_t)S3 * 9);
+    for (int e = e0; e < e1; ++e) {
+      // intervals of J[a][ax] = dx_a/dr_ax on every sub-cube, by a tree of
+      // transforms: along r per piece, then s, then t
+      bool ok = N > 0;
+      for (int a = 0; a < 3 && ok; ++a) {
+        for (size_t m = 0; m < n3; ++m) c[m] = geom[((size_t)e * 3 + a) * n3 + m];
+        for (int ax = 0; ax < 3; ++ax)
+          for (int h0 = 0; h0 < S; ++h0) {
+            int d1[3] = {n, n, n};
+            along(c, d1, 0, ax == 0 ? D[h0].data() : T[h0].data(), ax == 0 ? N : n, y1);
+            for (int h1 = 0; h1 < S; ++h1) {
+              int d2[3] = {d1[0], d1[1], d1[2]};
+              along(y1, d2, 1, ax == 1 ? D[h1].data() : T[h1].data(), ax == 1 ? N : n, y2);
+              for (int h2 = 0; h2 < S; ++h2) {
+                int d3[3] = {d2[0], d2[1], d2[2]};
+                along(y2, d3, 2, ax == 2 ? D[h2].data() : T[h2].data(), ax == 2 ? N : n, y3);
+                double lo = 1e300, hi = -1e300;
+                for (double v : y3) {
+                  lo = std::min(lo, v);
+                  hi = std::max(hi, v);
+                }
+                const size_t h = ((size_t)h2 * S + h1) * S + h0;
+                jlo[h * 9 + a * 3 + ax] = lo;
+                jhi[h * 9 + a * 3 + ax] = hi;
+              }
+            }
+          }
+      }
+      double K = 0.0;
+      for (int h = 0; h < S3 && ok; ++h) {
+        double J0[3][3], R[3][3];
+        for (int a = 0; a < 3; ++a)
+          for (int ax = 0; ax < 3; ++ax) {
+            const double lo = jlo[(size_t)h * 9 + a * 3 + ax], hi = jhi[(size_t)h * 9 + a * 3 + ax];
+            J0[a][ax] = 0.5 * (lo + hi);
+            R[a][ax] = 0.5 * (hi - lo) * (1.0 + 1e-9) + 1e-12 * std::fabs(J0[a][ax]);
+          }
+        const double c00 = J0[1][1] * J0[2][2] - J0[1][2] * J0[2][1];
+        const double c01 = J0[1][2] * J0[2][0] - J0[1][0] * J0[2][2];
+        const double c02 = J0[1][0] * J0[2][1] - J0[1][1] * J0[2][0];
+        const double det = J0[0][0] * c00 + J0[0][1] * c01 + J0[0][2] * c02;
+        if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) {
+          ok = false;
+          break;
+        }
+        double M[3][3];
+        M[0][0] = c00 / det;
+        M[1][0] = c01 / det;
+        M[2][0] = c02 / det;
+        M[0][1] = (J0[0][2] * J0[2][1] - J0[0][1] * J0[2][2]) / det;
+        M[1][1] = (J0[0][0] * J0[2][2] - J0[0][2] * J0[2][0]) / det;
+        M[2][1] = (J0[0][1] * J0[2][0] - J0[0][0] * J0[2][1]) / det;
+        M[0][2] = (J0[0][1] * J0[1][2] - J0[0][2] * J0[1][1]) / det;
+        M[1][2] = (J0[0][2] * J0[1][0] - J0[0][0] * J0[1][2]) / det;
+        M[2][2] = (J0[0][0] * J0[1][1] - J0[0][1] * J0[1][0]) / det;
+        double mnorm = 0.0, beta = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          double rs = 0.0, bs = 0.0;
+          for (int b = 0; b < 3; ++b) {
+            rs += std::fabs(M[a][b]);
+            for (int k = 0; k < 3; ++k) bs += std::fabs(M[a][k]) * R[k][b];
+          }
+          mnorm = std::max(mnorm, rs);
+          beta = std::max(beta, bs);
+        }
+        if (!(beta < 0.9)) {
+          ok = false;
+          break;
+        }
+        K = std::max(K, mnorm / (1.0 - beta));
+      }
+      out[e] = ok ? (float)(K * 1.05) : 0.f;
+    }
+  });
+}
+
 // Per element, a bound L of |u(a) - u(b)| / |a - b|_inf over the reference
 // cube.  On each of the 2 x 2 x 2 half-cubes, G_c bounds |du/dc| (c = r, s,
 // t) by the largest |Bernstein coefficient| of that partial derivative on
@@ -623,37 +764,9 @@ void field_range(int n_elem, int nf, const float* field, float* out)
 void field_lipschitz(int n_elem, int nf, const float* field, float* out)
 {
   const int n = nf, N = nf - 1, S = 2;
-  std::vector<std::vector<double>> T(S), D(S);
-  for (int h = 0; h < S; ++h) {
-    const double a = -1.0 + 2.0 * h / S, b = -1.0 + 2.0 * (h + 1) / S;
-    T[h] = modal_to_bernstein(n, a, b);  // T[q][m], stride n
-    D[h].assign((size_t)n * std::max(N, 1), 0.0);  // D[q][m], stride N
-    for (int q = 0; q < n; ++q)
-      for (int m = 0; m < N; ++m)
-        D[h][(size_t)q * N + m] = N * (T[h][(size_t)q * n + m + 1] - T[h][(size_t)q * n + m]) / (b - a);
-  }
+  std::vector<std::vector<double>> T, D;
+  piece_bernstein(n, S, T, D);
   const size_t n3 = (size_t)n * n * n;
-  // y = x transformed along one axis by A (A[q][m], q < d[axis], m < mo);
-  // index i + d0 (j + d1 k), i along r
-  auto along = [](const std::vector<double>& x, int* d, int axis, const double* A, int mo,
-                  std::vector<double>& y) {
-    int od[3] = {d[0], d[1], d[2]};
-    od[axis] = mo;
-    y.assign((size_t)od[0] * od[1] * od[2], 0.0);
-    for (int k = 0; k < od[2]; ++k)
-      for (int j = 0; j < od[1]; ++j)
-        for (int i = 0; i < od[0]; ++i) {
-          const int o[3] = {i, j, k};
-          double acc = 0.0;
-          for (int q = 0; q < d[axis]; ++q) {
-            int src[3] = {i, j, k};
-            src[axis] = q;
-            acc += A[(size_t)q * mo + o[axis]] * x[src[0] + (size_t)d[0] * (src[1] + (size_t)d[1] * src[2])];
-          }
-          y[i + (size_t)od[0] * (j + (size_t)od[1] * k)] = acc;
-        }
-    d[axis] = mo;
-  };
   parallel_chunks(n_elem, [&](int e0, int e1) {
     std::vector<double> c(n3), y1, y2, y3;
     for (int e = e0; e < e1; ++e) {

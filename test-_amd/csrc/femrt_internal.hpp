@@ -31,6 +31,7 @@ int gen_structured_hex(const femrt_gen_params* gp, float* vertices, float* geom,
 void encode_adjacency(int n_elem, const float* vertices, const int* adjacency, int* out);
 int gen_field_waves(int n_elem, int ng, const float* geom, int nf, int n_waves, uint64_t seed,
                     float* field);
+void geom_jinv_bound(int n_elem, int ng, const float* geom, float* out /* n */);
 void field_lipschitz(int n_elem, int nf, const float* field, float* out /* n */);
 void field_range(int n_elem, int nf, const float* field, float* out /* n x (lo, hi) */);
 std::vector<double> gll_nodes(int n);

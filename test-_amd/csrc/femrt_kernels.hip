@@ -697,12 +697,14 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   pad_coeffs(field, nf, NF, 1, n_elem, fp, 1);
   m->h_boxes.resize((size_t)n_elem * FEMRT_BOX_STRIDE);
   bernstein_boxes(n_elem, ng, geom, m->h_boxes.data());
-  std::vector<float> lip(n_elem), frange(2 * (size_t)n_elem);
+  std::vector<float> lip(n_elem), frange(2 * (size_t)n_elem), jinv(n_elem);
   field_lipschitz(n_elem, nf, field, lip.data());
+  geom_jinv_bound(n_elem, ng, geom, jinv.data());
   field_range(n_elem, nf, field, frange.data());
   auto cleanup = [&](const std::string& msg) {
     hipFree(m->d_geom); hipFree(m->d_field); hipFree(m->d_verts); hipFree(m->d_boxes); hipFree(m->d_adj);
     hipFree(m->d_lip);
+    hipFree(m->d_jinv);
     hipFree(m->d_frange);
     delete m;
     return fail(c, FEMRT_ENOMEM, msg);
@@ -713,12 +715,14 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   if (hipMalloc(&m->d_boxes, m->h_boxes.size() * 4) != hipSuccess) return cleanup("hipMalloc boxes");
   if (adjacency && hipMalloc(&m->d_adj, (size_t)n_elem * 12 * 4) != hipSuccess) return cleanup("hipMalloc adj");
   if (hipMalloc(&m->d_lip, (size_t)n_elem * 4) != hipSuccess) return cleanup("hipMalloc lip");
+  if (hipMalloc(&m->d_jinv, (size_t)n_elem * 4) != hipSuccess) return cleanup("hipMalloc jinv");
   if (hipMalloc(&m->d_frange, (size_t)n_elem * 8) != hipSuccess) return cleanup("hipMalloc field range");
   bool ok = hipMemcpy(m->d_geom, g4.data(), g4.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_field, fp.data(), fp.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_verts, vertices, (size_t)n_elem * 24 * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_boxes, m->h_boxes.data(), m->h_boxes.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_lip, lip.data(), lip.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(m->d_jinv, jinv.data(), jinv.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->d_frange, frange.data(), frange.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
   if (ok && adjacency) {
     std::vector<int> adj((size_t)n_elem * 12);
@@ -741,6 +745,7 @@ FEMRT_API int femrt_mesh_destroy(femrt_mesh* m)
   hipFree(m->d_boxes);
   hipFree(m->d_adj);
   hipFree(m->d_lip);
+  hipFree(m->d_jinv);
   hipFree(m->d_frange);
   hipFree(m->d_nodes);
   hipFree(m->d_prims);
@@ -1061,6 +1066,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   ip.max_march = p->max_march;
   ip.mlp_lead = (p->flags & FEMRT_TRACE_LEARNED_EVERY_SAMPLE) ? p->samples - 1 : 1;
   ip.tmin = p->t_min;
+  ip.skip = (p->flags & FEMRT_TRACE_NO_SAMPLE_SKIP) ? 0 : 1;
   TraceMesh TM;
   TM.geom = m->d_geom;
   TM.field = m->d_field;
@@ -1071,6 +1077,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   TM.prims = m->d_prims;
   TM.lip = m->d_lip;
   TM.frange = m->d_frange;
+  TM.jinv = m->d_jinv;
   rc = FEMRT_EUNSUP;
   FEMRT_DISPATCH_NG(m->NG, {
     rc = timed_launch(c, "k_first_hit", [&] {
@@ -1203,6 +1210,13 @@ FEMRT_API int femrt_host_field_lipschitz(int n_elem, int nf, const float* field,
 {
   if (n_elem < 0 || nf < 1 || nf > 16 || !field || !out) return FEMRT_EINVAL;
   field_lipschitz(n_elem, nf, field, out);
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_host_geom_jinv_bound(int n_elem, int ng, const float* geom, float* out)
+{
+  if (n_elem < 0 || ng < 1 || ng > 16 || !geom || !out) return FEMRT_EINVAL;
+  geom_jinv_bound(n_elem, ng, geom, out);
   return FEMRT_OK;
 }
 

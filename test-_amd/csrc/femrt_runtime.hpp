@@ -64,6 +64,7 @@ struct femrt_mesh {
   float* d_boxes = nullptr;            // n_elem x FEMRT_BOX_STRIDE
   int* d_adj = nullptr;                // n_elem x 12
   float* d_lip = nullptr;              // n_elem: Lipschitz bound of the field in rst
+  float* d_jinv = nullptr;             // n_elem: bound of |J^-1|_inf over the element (0: none)
   float* d_frange = nullptr;           // n_elem x 2: bounds of the field
   float4* d_nodes = nullptr;           // n_nodes x 2
   int* d_prims = nullptr;
@@ -204,6 +205,7 @@ __device__ __forceinline__ void flush_work(long long* work, const Work& w)
   add_work(work, 14, w.redo);
   add_work(work, 15, w.brackets);
   add_work(work, 27, w.skipped);
+  add_work(work, 28, w.samples_skipped);
 }
 
 // Append `pred` lanes to a queue: one atomic per wave, ballot + prefix count

@@ -27,6 +27,7 @@ struct TraceMesh {
   const int* prims;
   const float* lip;     // n_elem: bound of |u(a) - u(b)| / |a - b|_inf, or null
   const float* frange;  // n_elem x 2: bounds of u on the element (widened), or null
+  const float* jinv;    // n_elem: bound of |J^-1|_inf over the element (0: none), or null
 };
 
 struct IsoParams {
@@ -38,6 +39,7 @@ struct IsoParams {
   int max_march;
   float tmin;
   int mlp_lead;   // leading interior samples of a segment guessed by the network
+  int skip;       // certified sample skipping on (see march_samples)
 };
 
 // Work counters accumulated per lane.
@@ -50,6 +52,7 @@ struct Work {
   // marching, re-entries through the BVH, redone segments, iso brackets
   unsigned exit_pred = 0, exit_search = 0, exit_march = 0, reentry = 0, redo = 0, brackets = 0;
   unsigned skipped = 0;  // segments whose samples the field bounds rule out
+  unsigned samples_skipped = 0;  // scan samples whose signs the bounds certified
 };
 
 // The same counters compiled away: the kernels of timed runs take NoWork,
@@ -61,7 +64,7 @@ struct NoCount {
 };
 struct NoWork {
   NoCount face_it, inv_it, field, geo64, field64, slab, mlp, marched;
-  NoCount exit_pred, exit_search, exit_march, reentry, redo, brackets, skipped;
+  NoCount exit_pred, exit_search, exit_march, reentry, redo, brackets, skipped, samples_skipped;
 };
 
 // FP32 result of the first-hit stage.
@@ -841,9 +844,26 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
   }
   float rst_in[3];
   face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
+  // Certified skipping.  Inside the element |rst(t) - rst(t')|_inf <=
+  // jinv |d|_inf |t - t'| (jinv bounds |J^-1|_inf there) and |u(a) - u(b)| <=
+  // lip |a - b|_inf, so from an interior sample j with |g_k| > lip err + eps
+  // for every open isovalue (err bounding the sample's distance to its
+  // preimage, eps the FP32 evaluation error of u), the next
+  // R = floor((min_k |g_k| - lip err - eps) / cstep) samples keep the signs
+  // of sample j, cstep = lip jinv |d|_inf dt.  The scan moves on to the last
+  // of them (never past the last interior sample) instead of the next one:
+  // every sample where a sign can change is still evaluated next to its
+  // predecessor, so the brackets are those of the full scan.
+  float cstep = 0.f;
+  if (P.skip && M.jinv && lip > 0.f) {
+    const float kj = M.jinv[S.e];
+    const float dmax = fmaxf(fabsf(d[0]), fmaxf(fabsf(d[1]), fabsf(d[2])));
+    if (kj > 0.f) cstep = lip * kj * dmax * ((S.t_out - S.t_in) / (float)Ms) * (1.f + 1e-5f);
+  }
   float gprev[FEMRT_MAX_ISO];
   float rprev[3] = {0.f, 0.f, 0.f}, rprev2[3] = {0.f, 0.f, 0.f};
-  float tprev = S.t_in;
+  float tprev = S.t_in, tprev2 = S.t_in;
+  bool jumped = false, jumped_prev = false;  // the steps out of and into the last sample skipped samples
   bool have_prev = false;
   int n_prev = 0;  // consecutive samples with a preimage
   unsigned found_here = 0;
@@ -876,6 +896,7 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
       rprev2[c] = rprev[c];
       rprev[c] = r[c];
     }
+    tprev2 = tprev;
     tprev = t;
     have_prev = true;
     ++n_prev;
@@ -910,10 +931,15 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
       zero_retry = true;  // plain Newton from 0 is the fallback of a learned guess
     } else if (P.guess == FEMRT_GUESS_PLAIN) {
       rst[0] = rst[1] = rst[2] = 0.f;
-    } else if (n_prev >= 2) {
+    } else if (n_prev >= 2 && !jumped && !jumped_prev) {
       // warm start extrapolated from the two previous (uniformly spaced)
       // samples: O(dt^2) from the preimage instead of O(dt)
       for (int c = 0; c < 3; ++c) rst[c] = fminf(fmaxf(2.f * rprev[c] - rprev2[c], -1.f), 1.f);
+    } else if (n_prev >= 2) {
+      // after a certified skip the spacing is uneven: linear in t
+      const float tn = march_sample_t(S, j, Ms);
+      const float q = (tn - tprev) / (tprev - tprev2);
+      for (int c = 0; c < 3; ++c) rst[c] = fminf(fmaxf(rprev[c] + q * (rprev[c] - rprev2[c]), -1.f), 1.f);
     }
     stage = 0;
     its = 0;
@@ -987,6 +1013,7 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
       if (!S.redo) return true;
       have_prev = false;
       n_prev = 0;
+      jumped = jumped_prev = false;
       rst[0] = rst[1] = rst[2] = 0.f;
       ++j;
       if (j < Ms) start_sample();
@@ -994,7 +1021,25 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
     }
     if (done) {
       take(t, rst, u);
-      ++j;
+      int nj = j + 1;
+      if (cstep > 0.f) {
+        const unsigned open = open_all & ~(S.found | found_here);
+        float gl = 3.4e38f;
+        for (int k = 0; k < FEMRT_MAX_ISO; ++k)
+          if (k < P.n_iso && ((open >> k) & 1u)) gl = fminf(gl, fabsf(u - P.iso[k]));
+        // distance to the preimage: 2 st after an accepted step, else within
+        // the convergence test's 1e-5 (doubled); FP32 evaluation error of u
+        gl -= lip * (conv ? 2e-5f : 2.f * st) + 2e-4f + 1e-4f * lip;
+        if (u == u && gl >= 2.f * cstep) {
+          nj = j + (int)fminf(gl / cstep, (float)Ms);
+          nj = nj < Ms - 1 ? nj : Ms - 1;
+          if (nj <= j) nj = j + 1;
+          w.samples_skipped += (unsigned)(nj - j - 1);
+        }
+      }
+      jumped_prev = jumped;
+      jumped = nj > j + 1;
+      j = nj;
       if (j < Ms) start_sample();
     }
   }

@@ -290,6 +290,33 @@ def test_field_lipschitz_closed_forms(femrt):
     np.testing.assert_allclose(L, [0.0, 1.05, 3.15], atol=1e-5)
 
 
+@pytest.mark.parametrize("N,q,pert,amp,found", [(4, 3, 1, 0.05, 0.0), (6, 4, 2, 0.08, 0.95),
+                                                 (8, 4, 3, 0.15, 0.95)])
+def test_geom_jinv_bound_holds(femrt, oracle, N, q, pert, amp, found):
+    """The bound K of |J^-1|_inf (max row sum) that certifies skipped scan
+    samples: where one is found (K > 0), K >= |J^-1|_inf on a dense grid and
+    random points of the element, and within 6x of the sampled maximum; it
+    is found for most elements of the benchmark-like meshes (the strongly
+    sheared 4^3 mesh with sinusoidal faces of 0.2 h gets none, and its walk
+    simply skips nothing).  On the unit cube split in N^3 straight hexes
+    J^-1 = (2N) I exactly, so K is then 2N (+5%)."""
+    hm = femrt.gen_structured_hex(N, q, 2, pert=pert, amp=amp, seed=12)
+    K = femrt.host_geom_jinv_bound(hm)
+    assert np.mean(K > 0) >= found
+    orc = oracle.Oracle.from_mesh(hm)
+    rng = np.random.default_rng(2)
+    g = np.linspace(-1, 1, 6)
+    for e in rng.choice(hm.n_elem, min(hm.n_elem, 24), replace=False):
+        if K[e] == 0:
+            continue
+        pts = list(itertools.product(g, g, g)) + [tuple(rng.uniform(-1, 1, 3)) for _ in range(100)]
+        worst = max(np.abs(np.linalg.inv(orc.map(e, np.array(r))[1])).sum(1).max() for r in pts)
+        assert worst <= K[e], (e, worst, K[e])
+        assert K[e] <= 6.0 * worst, (e, worst, K[e])
+    flat = femrt.gen_structured_hex(3, 3, 2, pert=0, seed=1)
+    np.testing.assert_allclose(femrt.host_geom_jinv_bound(flat), 6.0 * 1.05, rtol=1e-5)
+
+
 def test_field_waves_is_continuous_seeded_and_unit_rms(femrt, oracle):
     """The multi-frequency field interpolates one smooth function at each
     element's GLL nodes: both sides of every checked interior face give the
