@@ -12,7 +12,8 @@ miss rays is checked against the FP64 oracle at the acceptance bar (the
 oracle would take hours over the whole frame), and the whole frame is
 checked by a size-independent property: the learned, warm-start and plain
 Newton variants start their FP32 iterations from different points but must
-converge to the same FP64 results on every ray, ties aside.
+converge to the same FP64 results on every ray, ties aside; so must the
+scan that evaluates every sample (no certified skipping).
 """
 from __future__ import annotations
 
@@ -99,9 +100,10 @@ def _run(femrt, oracle, ctx, hm, rays_fn, n_rays, isos, pert, amp, seed, variant
     assert rep["oracle_hits"] >= N_SAMPLE_HIT - 5
     assert cnt["mlp_infer"] > 0
     for name in variants:
-        fr.trace(mesh, femrt.trace_params(isos, guess=name))
+        guess, flags = name if isinstance(name, tuple) else (name, 1)
+        fr.trace(mesh, femrt.trace_params(isos, guess=guess, flags=flags))
         agree = _frame_agreement(gh, giso, fr.hits(), fr.iso(len(isos)))
-        print(json.dumps({"learned_vs_variant": name, **agree}), flush=True)
+        print(json.dumps({"learned_vs_variant": [guess, flags], **agree}), flush=True)
         assert agree["geom_diff_frac"] <= MAX_GEOM_DIFF_FRAC, agree
         assert all(r["diff_frac"] <= MAX_ISO_DIFF_FRAC for r in agree["iso"]), agree
     for o in (fr, mesh, mlp):
@@ -115,7 +117,8 @@ def test_bench_frame_32cube_p6_1080p(femrt, oracle, gpu_ctx):
     W, H = 1920, 1080
     isos = [-0.1, 0.0, 0.1]
     rep, cnt = _run(femrt, oracle, gpu_ctx, hm, lambda fr: fr.camera_rays(cam, W, H), W * H, isos,
-                    2, 0.08, 101, [femrt.GUESS_WARM, femrt.GUESS_PLAIN])
+                    2, 0.08, 101, [femrt.GUESS_WARM, femrt.GUESS_PLAIN,
+                                   (femrt.GUESS_LEARNED, 1 | femrt.TRACE_NO_SAMPLE_SKIP)])
     assert all(r["oracle_found"] > 100 for r in rep["iso"])
 
 
