@@ -35,6 +35,36 @@ METRIC = "Mrays/s, 32³ curved hex P=6 @1080p: learned-guess vs plain Newton at 
 N_EL, Q, P, AMP, SEED = 32, 4, 6, 0.08, 20240611
 W, H = 1920, 1080
 ISOS = [-0.1, 0.0, 0.1]
+PERT, SPP, WAVES = 2, 1, 0
+
+# Single-frame workloads.  "frame" is the one the headline metric is quoted
+# on; the two 64^3 frames are the larger learned-guess and multi-frequency
+# cases (face bubbles with xi uniform on (-1, 1): peak 0.15 h; the gaussian
+# bubbles fold elements at that amplitude).
+WORKLOADS = {
+    "frame": dict(n=32, q=4, p=6, pert=2, amp=0.08, w=1920, h=1080, spp=1, isos=[-0.1, 0.0, 0.1],
+                  waves=0, metric=None,
+                  text="32^3 curved hex Q=4 P=6, 1920x1080 1spp, isos {-0.1,0,0.1}"),
+    "64cube_4k": dict(n=64, q=4, p=6, pert=3, amp=0.15, w=3840, h=2160, spp=1, isos=[0.0], waves=0,
+                      metric="Mrays/s, 64³ curved hex Q=4 P=6 amp 0.15 @4K 1spp, iso 0: learned guess",
+                      text="64^3 curved hex Q=4 P=6, face bubbles peak 0.15 h, 3840x2160 1spp, iso {0}"),
+    "64cube_p8_16spp": dict(n=64, q=4, p=8, pert=3, amp=0.15, w=1920, h=1080, spp=16,
+                            isos=[-0.6, -0.2, 0.2, 0.6], waves=8,
+                            metric="Mrays/s, 64³ curved hex Q=4 P=8 multi-frequency field, 4 isos @1080p 16spp",
+                            text="64^3 curved hex Q=4 P=8 (8-wave multi-frequency field), peak bubbles 0.15 h, "
+                                 "1920x1080 16spp jittered, isos {-0.6,-0.2,0.2,0.6}"),
+}
+
+
+def gen_rays(fr):
+    """The workload's camera rays into frame `fr`: pixel centres at 1 spp,
+    seeded jitter otherwise."""
+    if SPP == 1:
+        fr.camera_rays(CAM, W, H)
+    else:
+        fr.camera_rays_jitter(CAM12, W, H, SPP, SEED + 5)
+
+
 PEAK_FP32_TFLOPS = 157.3   # MI355X published vector/matrix FP32 (packed); 78.6 unpacked
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
@@ -91,7 +121,7 @@ def cpu_baseline(femrt, orc_mod, hm, rays, budget_s=15.0):
     n = rays.shape[1]
     # stratified pixel sample: every 16th pixel in x and y, in a shuffled order
     ys, xs = np.meshgrid(np.arange(8, H, 16), np.arange(8, W, 16), indexing="ij")
-    idx = (ys * W + xs).ravel().astype(np.int32)
+    idx = ((ys * W + xs) * SPP).ravel().astype(np.int32)   # the first sample of each pixel
     np.random.default_rng(0).shuffle(idx)
     isos = np.array(ISOS, np.float32)
     done, t_used, chunk = 0, 0.0, 64
@@ -108,7 +138,7 @@ def cpu_baseline(femrt, orc_mod, hm, rays, budget_s=15.0):
     mrays = done / t_used / 1e6
     return {"value": mrays, "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": (f"{done} of the {idx.size} rays at every 16th pixel in x and y "
-                       f"({done / n * 100:.3f}% of the frame), 3 isovalue passes per ray, "
+                       f"({done / n * 100:.3f}% of the frame), {len(ISOS)} isovalue passes per ray, "
                        f"{t_used:.1f} s on {cores} threads, {platform.processor() or platform.machine()}")}
 
 
@@ -141,7 +171,7 @@ def run_variant(femrt, lanes, guess, steps, warmup, flags=1):
     t0 = time.perf_counter()
     for s in range(steps):
         ctx, mesh, fr = lanes[s % len(lanes)]
-        fr.camera_rays(CAM, W, H)
+        gen_rays(fr)
         fr.trace(mesh, prm)
     for ctx, _, _ in lanes:
         ctx.sync()
@@ -161,6 +191,7 @@ def run_variant(femrt, lanes, guess, steps, warmup, flags=1):
 
 
 CAM = None
+CAM12 = None
 
 BATCH_METRIC = "Mrays/s (camera rays), 64 curved-hex meshes 512..32768 hex, P 2..8, 512x512 8 spp, depth 3, one graph"
 
@@ -237,7 +268,7 @@ def run_batch(femrt, args):
 
 
 def main():
-    global CAM
+    global CAM, CAM12, N_EL, Q, P, AMP, PERT, W, H, SPP, ISOS, WAVES
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -248,8 +279,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--binned-walk", action="store_true",
                     help="walk in element-binned rounds instead of one persistent kernel")
-    ap.add_argument("--workload", default="frame", choices=["frame", "batch"],
-                    help="frame: the 32^3 P=6 1080p frame (default); batch: 64 meshes in one graph")
+    ap.add_argument("--workload", default="frame", choices=["frame", "batch", "64cube_4k", "64cube_p8_16spp"],
+                    help="frame: the 32^3 P=6 1080p frame (default); batch: 64 meshes in one graph; "
+                         "64cube_4k, 64cube_p8_16spp: the larger single frames")
     ap.add_argument("--batch-meshes", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--inflight", type=int, default=2,
@@ -266,13 +298,19 @@ def main():
     if args.workload == "batch":
         run_batch(femrt, args)
         return
-    hm = femrt.gen_structured_hex(N_EL, Q, P, pert=2, amp=AMP, seed=SEED)
+    wl = WORKLOADS[args.workload]
+    N_EL, Q, P, PERT, AMP = wl["n"], wl["q"], wl["p"], wl["pert"], wl["amp"]
+    W, H, SPP, ISOS, WAVES = wl["w"], wl["h"], wl["spp"], wl["isos"], wl["waves"]
+    hm = femrt.gen_structured_hex(N_EL, Q, P if not WAVES else 2, pert=PERT, amp=AMP, seed=SEED)
+    if WAVES:
+        hm = femrt.gen_field_waves(hm, P, n_waves=WAVES, seed=SEED + 4)
     CAM = femrt.auto_camera([0, 0, 0], [1, 1, 1])
+    CAM12 = femrt.camera_basis(CAM[:3], CAM[3:6], CAM[6:9], float(CAM[9]), W / H)
     ctx = femrt.Context(0)
     mesh = femrt.Mesh(ctx, hm)
-    n_rays = W * H
+    n_rays = W * H * SPP
     fr = femrt.Frame(ctx, n_rays)
-    fr.camera_rays(CAM, W, H)
+    gen_rays(fr)
     # frames in flight: one context (HIP stream), resident mesh copy and frame
     # per stream
     lanes = [(ctx, mesh, fr)]
@@ -289,7 +327,7 @@ def main():
         # the guess network is trained on another mesh of the same family
         # (different seed, 16^3) by the library's own GPU kernels, untimed
         t0 = time.perf_counter()
-        thm = femrt.gen_structured_hex(16, Q, P, pert=2, amp=AMP, seed=SEED + 1)
+        thm = femrt.gen_structured_hex(16, Q, P if not WAVES else 2, pert=PERT, amp=AMP, seed=SEED + 1)
         tmesh = femrt.Mesh(ctx, thm, build_bvh=False)
         mlp = femrt.Mlp(ctx, seed=SEED)
         losses = mlp.train(tmesh, n_samples=1 << 21, batch=4096, epochs=4, lr=3e-3, lr_final=1e-4,
@@ -303,7 +341,7 @@ def main():
             m2.set_params(mlp.params())
             lane_mesh.set_guess_mlp(m2)
         tmesh.close()
-        mlp_info = {"train_mesh": "16^3 Q=4 P=6 amp 0.08 h, seed+1", "train_pairs": 1 << 21,
+        mlp_info = {"train_mesh": f"16^3 Q={Q} bubbles {AMP} h (kind {PERT}), seed+1", "train_pairs": 1 << 21,
                     "epochs": 4, "epoch_loss": [float(x) for x in losses],
                     "train_s": time.perf_counter() - t0,
                     "guess_err_median_on_bench_mesh": float(np.median(err)),
@@ -315,7 +353,7 @@ def main():
         # then used by every variant (untimed)
         tv = "learned" if "learned" in variants else variants[0]
         tflags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
-        fr.camera_rays(CAM, W, H)
+        gen_rays(fr)
         launch_cfg = list(mesh.autotune(fr, femrt.trace_params(ISOS, guess=names[tv], flags=tflags), reps=3))
         for _, lane_mesh, _ in lanes[1:]:
             lane_mesh.launch_cfg(launch_cfg)
@@ -378,7 +416,7 @@ def main():
 
     hits = R["counters"]["hit_rays"]
     line = {
-        "metric": METRIC,
+        "metric": wl["metric"] or METRIC,
         "value": R["mrays"],
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -389,8 +427,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic (seeded 32^3 curved-hex mesh and modal field)",
-        "config": {"workload": "32^3 curved hex Q=4 P=6, 1920x1080 1spp, isos {-0.1,0,0.1}",
+        "data": f"synthetic (seeded {N_EL}^3 curved-hex mesh and modal field)",
+        "config": {"workload": wl["text"],
                    "variant": head, "rays": n_rays, "hit_fraction": hits / n_rays,
                    "frames_in_flight": args.inflight,
                    "mrays_s_hit_rays": R["mrays"] * hits / n_rays,

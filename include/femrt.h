@@ -138,7 +138,7 @@ typedef struct {
   int guess;                     /* FEMRT_GUESS_*                                 */
   float t_min;                   /* 0.001 in the reference (raytracer.cu:221)     */
   float t_max;
-  int max_march;                 /* element-walk step cap per ray                  */
+  int max_march;                 /* element-walk step cap per ray (at most 65535)  */
   int flags;                     /* FEMRT_TRACE_* bits                            */
 } femrt_trace_params;
 
@@ -191,8 +191,9 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
  * (after that lanes only finish the rays they hold), end; [19..26] rays
  * by segments walked, log2 bins 1, 2-3, 4-7, ..., 128+; [27] segments whose
  * scan the element's field bounds ruled out; [28] scan samples whose signs
- * the field and geometry bounds certified without evaluating them. */
-#define FEMRT_NCOUNTERS 29
+ * the field and geometry bounds certified without evaluating them; [29] walks
+ * ended as stalled (more than 16 exits found by the marching fallback). */
+#define FEMRT_NCOUNTERS 30
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
 
 /* ------------------------------------------------ shading and bounces */

@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256, 3) void k_march(
       } else if (!march_advance<NG>(M, P, o, d, S, w)) {
         active = false;
         if (work) {
-          const int len = max(1, S.step);
+          const int len = max(1, S.step & 0xFFFF);
           add_work(work, 19 + min(7, 31 - __clz(len)), 1);
         }
         // a bracket written by a pass that was redone is not a result
@@ -1063,7 +1063,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   ip.n_iso = n_iso;
   ip.samples = p->samples;
   ip.guess = p->guess;
-  ip.max_march = p->max_march;
+  ip.max_march = std::min(p->max_march, 0xFFFF);
   ip.mlp_lead = (p->flags & FEMRT_TRACE_LEARNED_EVERY_SAMPLE) ? p->samples - 1 : 1;
   ip.tmin = p->t_min;
   ip.skip = (p->flags & FEMRT_TRACE_NO_SAMPLE_SKIP) ? 0 : 1;

@@ -206,6 +206,7 @@ __device__ __forceinline__ void flush_work(long long* work, const Work& w)
   add_work(work, 15, w.brackets);
   add_work(work, 27, w.skipped);
   add_work(work, 28, w.samples_skipped);
+  add_work(work, 29, w.stalled);
 }
 
 // Append `pred` lanes to a queue: one atomic per wave, ballot + prefix count

@@ -53,6 +53,7 @@ struct Work {
   unsigned exit_pred = 0, exit_search = 0, exit_march = 0, reentry = 0, redo = 0, brackets = 0;
   unsigned skipped = 0;  // segments whose samples the field bounds rule out
   unsigned samples_skipped = 0;  // scan samples whose signs the bounds certified
+  unsigned stalled = 0;  // walks ended after FEMRT_MAX_MARCHED_EXITS marched exits
 };
 
 // The same counters compiled away: the kernels of timed runs take NoWork,
@@ -64,7 +65,7 @@ struct NoCount {
 };
 struct NoWork {
   NoCount face_it, inv_it, field, geo64, field64, slab, mlp, marched;
-  NoCount exit_pred, exit_search, exit_march, reentry, redo, brackets, skipped, samples_skipped;
+  NoCount exit_pred, exit_search, exit_march, reentry, redo, brackets, skipped, samples_skipped, stalled;
 };
 
 // FP32 result of the first-hit stage.
@@ -82,6 +83,8 @@ struct IsoBracket {
 };
 
 #define FEMRT_STACK 64
+// exits a walk may find by the marching fallback before it counts as stalled
+#define FEMRT_MAX_MARCHED_EXITS 16
 #define FEMRT_FACE_ACC 1e-4f
 // Boxes per element: [0] element, [1..6] faces (Nektar order), [7 + 9 f +
 // 3 b + a] sub-patch (a, b) of face f on the 3 x 3 split of its (u, v) square.
@@ -739,7 +742,7 @@ struct MarchState {
   float t_in, u_in, v_in;
   int f_out;              // exit of the current segment (valid when ok)
   float t_out, u_out, v_out;
-  int step;
+  int step;               // segments walked (low 16 bits), exits found by marching (bits 16+)
   unsigned found;         // isovalues whose bracket has been recorded
   bool ok;
   bool redo;              // samples showed a wrong exit: search again, entry face included
@@ -801,6 +804,10 @@ FEMRT_HD void march_exit(const TraceMesh& M, const IsoParams& P, const float* o,
     ok = find_exit_marching<NG>(gc, bx, o, d, id, rst_in, S.t_in, t_lo, S.f_out, S.t_out, S.u_out,
                                 S.v_out, w);
     w.exit_march += ok;
+    if (ok) {
+      S.step += 1 << 16;
+      if ((S.step >> 16) == FEMRT_MAX_MARCHED_EXITS + 1) ++w.stalled;
+    }
   }
   S.ok = ok;
 }
@@ -1065,7 +1072,13 @@ FEMRT_HD int march_next(const TraceMesh& M, const IsoParams& P, MarchState& S)
 {
   const unsigned all = (1u << P.n_iso) - 1u;
   S.redo = false;
-  if (S.found == all || ++S.step >= P.max_march) return MARCH_DONE;
+  // a walk that keeps needing the marching fallback for its exits has
+  // stalled between elements (each such exit advances the ray by as little
+  // as 1e-4 of the element and costs thousands of inversion iterations): it
+  // ends there
+  const int seg = (S.step & 0xFFFF) + 1;
+  S.step = (S.step & ~0xFFFF) | seg;
+  if (S.found == all || seg >= P.max_march || (S.step >> 16) > FEMRT_MAX_MARCHED_EXITS) return MARCH_DONE;
   if (S.ok && M.adj) {
     const int ne = M.adj[(size_t)S.e * 12 + S.f_out * 2];
     const int code = M.adj[(size_t)S.e * 12 + S.f_out * 2 + 1];
