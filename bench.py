@@ -11,7 +11,9 @@ mesh, its BVH and the frame buffers are resident in HBM before timing starts.
 By default two frames are in flight, each on its own HIP stream (context,
 resident mesh copy and frame buffers), so one frame's first hit and walk run
 while the previous frame's walk finishes its longest rays; --inflight 1 traces
-one frame at a time.
+one frame at a time.  --workload selects the larger single frames (64^3 at 4K;
+64^3 P=8 multi-frequency field at 1080p x 16 spp, four isovalues) or the
+batch of 64 meshes in one graph; each prints its own metric.
 
 Prints one JSON line (see the field docs in main()).
 """
