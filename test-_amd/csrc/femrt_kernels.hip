@@ -617,11 +617,11 @@ FEMRT_API int femrt_ctx_create(int device, femrt_ctx** out)
 FEMRT_API int femrt_ctx_destroy(femrt_ctx* c)
 {
   if (!c) return FEMRT_EINVAL;
-  hipSetDevice(c->device);
-  hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   collect_timing(c);
-  for (auto e : c->pool) hipEventDestroy(e);
-  hipStreamDestroy(c->stream);
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
   delete c;
   return FEMRT_OK;
 }
@@ -702,10 +702,10 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   geom_jinv_bound(n_elem, ng, geom, jinv.data());
   field_range(n_elem, nf, field, frange.data());
   auto cleanup = [&](const std::string& msg) {
-    hipFree(m->d_geom); hipFree(m->d_field); hipFree(m->d_verts); hipFree(m->d_boxes); hipFree(m->d_adj);
-    hipFree(m->d_lip);
-    hipFree(m->d_jinv);
-    hipFree(m->d_frange);
+    (void)hipFree(m->d_geom); (void)hipFree(m->d_field); (void)hipFree(m->d_verts); (void)hipFree(m->d_boxes); (void)hipFree(m->d_adj);
+    (void)hipFree(m->d_lip);
+    (void)hipFree(m->d_jinv);
+    (void)hipFree(m->d_frange);
     delete m;
     return fail(c, FEMRT_ENOMEM, msg);
   };
@@ -737,18 +737,18 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
 FEMRT_API int femrt_mesh_destroy(femrt_mesh* m)
 {
   if (!m) return FEMRT_EINVAL;
-  hipSetDevice(m->ctx->device);
-  hipStreamSynchronize(m->ctx->stream);
-  hipFree(m->d_geom);
-  hipFree(m->d_field);
-  hipFree(m->d_verts);
-  hipFree(m->d_boxes);
-  hipFree(m->d_adj);
-  hipFree(m->d_lip);
-  hipFree(m->d_jinv);
-  hipFree(m->d_frange);
-  hipFree(m->d_nodes);
-  hipFree(m->d_prims);
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  (void)hipFree(m->d_geom);
+  (void)hipFree(m->d_field);
+  (void)hipFree(m->d_verts);
+  (void)hipFree(m->d_boxes);
+  (void)hipFree(m->d_adj);
+  (void)hipFree(m->d_lip);
+  (void)hipFree(m->d_jinv);
+  (void)hipFree(m->d_frange);
+  (void)hipFree(m->d_nodes);
+  (void)hipFree(m->d_prims);
   delete m;
   return FEMRT_OK;
 }
@@ -780,8 +780,8 @@ FEMRT_API int femrt_mesh_build_bvh(femrt_mesh* m, int builder, int* n_nodes)
   if (m->bvh_depth >= FEMRT_STACK)
     return fail(c, FEMRT_EUNSUP, "build_bvh: tree deeper than the traversal stack");
   HIPCHK(c, hipSetDevice(c->device));
-  hipFree(m->d_nodes);
-  hipFree(m->d_prims);
+  (void)hipFree(m->d_nodes);
+  (void)hipFree(m->d_prims);
   m->d_nodes = nullptr;
   m->d_prims = nullptr;
   HIPCHK(c, hipMalloc(&m->d_nodes, m->bvh.nodes.size() * 4));
@@ -835,9 +835,9 @@ FEMRT_API int femrt_eval_field(femrt_mesh* m, int n, const int* elem, const floa
     HIPCHK(c, hipMemcpyAsync(out, d_o, (size_t)n * 4 * osz, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
-  hipFree(d_e);
-  hipFree(d_r);
-  hipFree(d_o);
+  (void)hipFree(d_e);
+  (void)hipFree(d_r);
+  (void)hipFree(d_o);
   return rc;
 }
 
@@ -871,10 +871,10 @@ FEMRT_API int femrt_invert_points(femrt_mesh* m, int n, const int* elem, const f
     if (iters_out) HIPCHK(c, hipMemcpyAsync(iters_out, d_it, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
-  hipFree(d_e);
-  hipFree(d_x);
-  hipFree(d_r);
-  hipFree(d_it);
+  (void)hipFree(d_e);
+  (void)hipFree(d_x);
+  (void)hipFree(d_r);
+  (void)hipFree(d_it);
   return rc;
 }
 
@@ -924,10 +924,13 @@ FEMRT_API int femrt_frame_create(femrt_ctx* c, int n, femrt_frame** out)
     femrt_frame_destroy(f);
     return fail(c, FEMRT_ENOMEM, "frame_create: hipMalloc failed");
   }
-  hipMemset(f->d_counts, 0, FEMRT_NCOUNTS * 4);
-  hipMemset(f->d_work, 0, FEMRT_NCOUNTERS * 8);
-  hipMemset(f->d_hit_elem, 0xff, N * 4);
-  hipMemset(f->d_iso_elem, 0xff, K * N * 4);
+  if (hipMemset(f->d_counts, 0, FEMRT_NCOUNTS * 4) != hipSuccess ||
+      hipMemset(f->d_work, 0, FEMRT_NCOUNTERS * 8) != hipSuccess ||
+      hipMemset(f->d_hit_elem, 0xff, N * 4) != hipSuccess ||
+      hipMemset(f->d_iso_elem, 0xff, K * N * 4) != hipSuccess) {
+    femrt_frame_destroy(f);
+    return fail(c, FEMRT_EHIP, "frame_create: hipMemset failed");
+  }
   *out = f;
   return FEMRT_OK;
 }
@@ -935,14 +938,14 @@ FEMRT_API int femrt_frame_create(femrt_ctx* c, int n, femrt_frame** out)
 FEMRT_API int femrt_frame_destroy(femrt_frame* f)
 {
   if (!f) return FEMRT_EINVAL;
-  hipSetDevice(f->ctx->device);
-  hipStreamSynchronize(f->ctx->stream);
+  (void)hipSetDevice(f->ctx->device);
+  (void)hipStreamSynchronize(f->ctx->stream);
   void* ptrs[] = {f->d_rays, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst,
                   f->d_hit_u, f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, f->d_iso_f32,
                   f->d_iso_rst32, f->d_queue_hit, f->d_queue_walk, f->d_qbin, f->d_counts, f->d_rgba, f->d_work,
                   f->d_path_tp, f->d_path_acc, f->d_mats, f->d_mlp_g, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
                   f->d_walk_sorted, f->d_walk_reent, f->d_walk_hist};
-  for (void* p : ptrs) hipFree(p);
+  for (void* p : ptrs) (void)hipFree(p);
   delete f;
   return FEMRT_OK;
 }
