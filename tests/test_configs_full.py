@@ -144,3 +144,41 @@ def test_multifrequency_64cube_p8_four_isos_16spp(femrt, oracle, gpu_ctx):
     rep, cnt = _run(femrt, oracle, gpu_ctx, hm, lambda fr: fr.camera_rays_jitter(cam12, W, H, spp, 507),
                     W * H * spp, isos, 3, 0.15, 303, [femrt.GUESS_WARM])
     assert all(r["oracle_found"] > 60 for r in rep["iso"])
+
+
+def test_stalled_walks_end_without_changing_results(femrt, gpu_ctx):
+    """Regression: on this 64^3 P=8 frame one ray's walk stalled between
+    elements (every exit from the marching fallback, advancing by 1e-4 of an
+    element) and walked to the 4096-segment cap in about a minute.  Such walks
+    now end after 16 marched exits: the frame is counted with stalled walks,
+    takes seconds, and gives the same crossings as with the walk capped at
+    256 segments."""
+    import time
+    base = femrt.gen_structured_hex(64, 4, 2, pert=3, amp=0.15, seed=20240611)
+    hm = femrt.gen_field_waves(base, 8, n_waves=8, seed=20240611 + 4)
+    mesh = femrt.Mesh(gpu_ctx, hm)
+    cam = femrt.auto_camera([0, 0, 0], [1, 1, 1])
+    W, H, spp = 960, 540, 16
+    cam12 = femrt.camera_basis(cam[:3], cam[3:6], cam[6:9], float(cam[9]), W / H)
+    fr = femrt.Frame(gpu_ctx, W * H * spp)
+    isos = [-0.6, -0.2, 0.2, 0.6]
+    out = {}
+    for mm in (4096, 256):
+        fr.camera_rays_jitter(cam12, W, H, spp, 7)
+        fr.counters(True)
+        gpu_ctx.sync()
+        t0 = time.perf_counter()
+        fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_WARM, max_march=mm))
+        gpu_ctx.sync()
+        dt = time.perf_counter() - t0
+        cnt = fr.counters(False)
+        out[mm] = fr.iso(len(isos))
+        print(json.dumps({"max_march": mm, "s": dt, "walks_stalled": cnt["walks_stalled"]}), flush=True)
+        if mm == 4096:
+            assert cnt["walks_stalled"] >= 1
+            assert dt < 15.0
+    for k in range(len(isos)):
+        np.testing.assert_array_equal(out[4096]["elem"][k], out[256]["elem"][k])
+        np.testing.assert_array_equal(out[4096]["t"][k], out[256]["t"][k])
+    fr.close()
+    mesh.close()
