@@ -283,7 +283,7 @@ __device__ __forceinline__ void wave_lds_sync()
 // between the exit and sample steps (later samples extrapolate).
 // COUNT selects the work counters (validation runs) or none (timed runs).
 template <int NG, int NF, bool LEARNED, bool COUNT>
-__global__ __launch_bounds__(256, 3) void k_march(
+__global__ __launch_bounds__(256, 2) void k_march(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ queue,
     int* __restrict__ counts, const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
     const float4* __restrict__ hit_f32, int* __restrict__ iso_elem, float4* __restrict__ iso_f32,
