@@ -101,10 +101,23 @@ def flops_model(ng, nf):
 
 def kernel_flops(counters, ng, nf):
     fm = flops_model(ng, nf)
-    f32 = ((counters["face_newton_it"] + counters["invert_it"]) * fm["geo"] +
-           counters["field_evals"] * fm["field_val"] + counters["slab_tests"] * fm["slab"])
+    f32 = ((counters["face_newton_it"] + counters["first_hit_face_it"] + counters["invert_it"]) * fm["geo"] +
+           counters["field_evals"] * fm["field_val"] +
+           (counters["slab_tests"] + counters["first_hit_slab_tests"]) * fm["slab"])
     f64 = counters["geo64_it"] * fm["geo"] + counters["field64_evals"] * fm["field_grad"]
     return f32, f64
+
+
+def kernel_flops_of(kernel, cnt, fm):
+    """FP32 flops of one kernel's work in a frame from its own counters: the
+    first-hit pass counts its face-Newton iterations and slab tests apart
+    from the walk's (which include the walk's BVH re-entries)."""
+    if kernel == "k_march":
+        return ((cnt["invert_it"] + cnt["face_newton_it"]) * fm["geo"] +
+                cnt["field_evals"] * fm["field_val"] + cnt["slab_tests"] * fm["slab"])
+    if kernel == "k_first_hit":
+        return cnt["first_hit_face_it"] * fm["geo"] + cnt["first_hit_slab_tests"] * fm["slab"]
+    return None
 
 
 def cpu_baseline(femrt, orc_mod, hm, rays, budget_s=15.0):
@@ -386,10 +399,8 @@ def main():
     # FP32 flops per launch attributed to the dominant kernel by its counters
     fm = flops_model(hm.ng, hm.nf)
     cnt = R["counters"]
-    if dom == "k_march":
-        fl = ((cnt["invert_it"] + cnt["face_newton_it"]) * fm["geo"] +
-              cnt["field_evals"] * fm["field_val"] + cnt["slab_tests"] * fm["slab"])
-    else:
+    fl = kernel_flops_of(dom, cnt, fm)
+    if fl is None:
         fl = R["f32"]
     achieved = fl / (dom_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,

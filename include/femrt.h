@@ -174,14 +174,17 @@ FEMRT_API int femrt_frame_get_rays(femrt_frame* f, float* rays);
 /* Queue the trace of every ray of the frame on the context stream (async). */
 FEMRT_API int femrt_trace(femrt_mesh* mesh, femrt_frame* f, const femrt_trace_params* p);
 /* Copy results to host (synchronises).  Any pointer may be NULL.  iso arrays
- * are n_iso x n_rays. */
+ * are n_iso x n_rays.  A ray without a hit reports elem -1, face -1 and NaN
+ * t, rst and u; a (ray, isovalue) pair without a crossing reports elem -1 and
+ * NaN t, rst and u. */
 FEMRT_API int femrt_frame_get_hits(femrt_frame* f, int* hit_elem, int* hit_face, double* hit_t,
                                    double* hit_rst, double* hit_u);
 FEMRT_API int femrt_frame_get_iso(femrt_frame* f, int n_iso, int* iso_elem, double* iso_t,
                                   double* iso_rst, double* iso_u);
 FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
 /* Work counters of the last trace (validation runs): [0] rays, [1] hit rays,
- * [2] face-Newton iterations FP32, [3] point-inversion iterations FP32,
+ * [2] face-Newton iterations FP32 of the element walk (exits and BVH
+ * re-entries), [3] point-inversion iterations FP32,
  * [4] field evaluations FP32, [5] FP64 geometry iterations, [6] FP64 field
  * evaluations, [7] slab tests, [8] MLP inferences, [9] elements marched,
  * [10] segment exits found by prediction, [11] by face search, [12] by
@@ -192,8 +195,10 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
  * by segments walked, log2 bins 1, 2-3, 4-7, ..., 128+; [27] segments whose
  * scan the element's field bounds ruled out; [28] scan samples whose signs
  * the field and geometry bounds certified without evaluating them; [29] walks
- * ended as stalled (more than 16 exits found by the marching fallback). */
-#define FEMRT_NCOUNTERS 30
+ * ended as stalled (more than 16 exits found by the marching fallback);
+ * [30] face-Newton iterations FP32 of the first-hit pass, [31] its slab
+ * tests ([7] counts the slab tests of the other passes). */
+#define FEMRT_NCOUNTERS 32
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
 
 /* ------------------------------------------------ shading and bounces */

@@ -78,18 +78,24 @@ def compare(gh, giso, og, oiso, isos):
             r["t_rel_max"] = float(tr.max())
             r["t_rel_bad"] = int((tr > T_REL).sum())
             r["t_rel_bad_nontie"] = int(((tr > T_REL) & ~itie[b]).sum())
-            du = np.abs(giso["u"][k] - iso)
+            # u of rays without a crossing is NaN (not reported): compare found rays only
+            du = np.zeros(fg.shape)
+            du[fg] = np.abs(giso["u"][k][fg] - iso)
             r["u_minus_iso_max"] = float(du[fg & ~itie].max()) if (fg & ~itie).any() else 0.0
             # near-tangent crossings (tie class) are reported, not gated
             r["u_minus_iso_max_ties"] = float(du[fg & itie].max()) if (fg & itie).any() else 0.0
             se = b & (giso["elem"][k] == oiso["elem"][k])
             r["elem_mismatch"] = int((b & ~se & ~itie).sum())
-            r["rst_abs_max"] = float(np.abs(giso["rst"][k][:, se] - oiso["rst"][k][:, se]).max()) if se.any() else 0.0
+            dr = np.abs(giso["rst"][k] - oiso["rst"][k]).max(axis=0)
+            sn = se & ~itie
+            r["rst_abs_max"] = float(dr[sn].max()) if sn.any() else 0.0
+            r["rst_abs_max_ties"] = float(dr[se & itie].max()) if (se & itie).any() else 0.0
         else:
             r.update(t_rel_max=0.0, t_rel_bad=0, t_rel_bad_nontie=0, u_minus_iso_max=0.0,
-                     elem_mismatch=0, rst_abs_max=0.0)
+                     elem_mismatch=0, rst_abs_max=0.0, rst_abs_max_ties=0.0)
         r["ok"] = (r["found_mismatch"] == 0 and r["t_rel_bad_nontie"] == 0 and
-                   r["u_minus_iso_max"] <= VAL_ABS and r["elem_mismatch"] == 0)
+                   r["u_minus_iso_max"] <= VAL_ABS and r["elem_mismatch"] == 0 and
+                   r["rst_abs_max"] <= RST_ABS)
         ok = ok and r["ok"]
         iso_rep.append(r)
     rep["iso"] = iso_rep

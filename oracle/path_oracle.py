@@ -45,10 +45,13 @@ def sky(d):
     return np.array([(1 - t) + t * 0.5, (1 - t) + t * 0.7, 1.0])
 
 
-def scatter(orc, rays, hits, mats, seed, depth, last, tp, acc, alive):
+def scatter(orc, rays, hits, mats, seed, depth, last, tp, acc, alive, ties=None):
     """One bounce of every live path; rays (6 x n) are the rays that were
     traced, hits the oracle's trace of them.  Updates tp, acc, alive in place
-    and returns the next rays (6 x n, float64)."""
+    and returns the next rays (6 x n, float64).  ties (bool n, optional) is
+    set where a dielectric's reflect/refract decision lies within 1e-5 of its
+    threshold (Fresnel draw or total internal reflection), which FP32 may
+    decide the other way."""
     n = rays.shape[1]
     nxt = np.zeros((6, n))
     nxt[:3] = 1e30
@@ -94,7 +97,10 @@ def scatter(orc, rays, hits, mats, seed, depth, last, tp, acc, alive):
             sinv = np.sqrt(max(0.0, 1 - cosv * cosv))
             r0 = ((1 - eta) / (1 + eta)) ** 2
             sch = r0 + (1 - r0) * (1 - cosv) ** 5
-            if eta * sinv > 1 or sch > path_rand(seed, p, depth + 1, 0):
+            xi = path_rand(seed, p, depth + 1, 0)
+            if ties is not None and (abs(eta * sinv - 1) < 1e-5 or abs(sch - xi) < 1e-5):
+                ties[p] = True
+            if eta * sinv > 1 or sch > xi:
                 sd = d + 2 * cosv * nrm
             else:
                 k = 1 - eta * eta * (1 - cosv * cosv)

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_first_hit(const float* __restrict__ ray
   if (work && active) {
     add_work(work, 0, 1);
     add_work(work, 1, hit ? 1 : 0);
-    flush_work(work, w);
+    flush_work(work, w, true);
   }
 }
 
@@ -1169,11 +1169,26 @@ FEMRT_API int femrt_frame_get_hits(femrt_frame* f, int* he, int* hf, double* ht,
   const size_t N = f->n;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (he) HIPCHK(c, hipMemcpy(he, f->d_hit_elem, N * 4, hipMemcpyDeviceToHost));
+  std::vector<int> own;
+  int* e = he;
+  if (!e && (hf || ht || hr || hu)) {
+    own.resize(N);
+    e = own.data();
+  }
+  if (e) HIPCHK(c, hipMemcpy(e, f->d_hit_elem, N * 4, hipMemcpyDeviceToHost));
   if (hf) HIPCHK(c, hipMemcpy(hf, f->d_hit_face, N * 4, hipMemcpyDeviceToHost));
   if (ht) HIPCHK(c, hipMemcpy(ht, f->d_hit_t, N * 8, hipMemcpyDeviceToHost));
   if (hr) HIPCHK(c, hipMemcpy(hr, f->d_hit_rst, 3 * N * 8, hipMemcpyDeviceToHost));
   if (hu) HIPCHK(c, hipMemcpy(hu, f->d_hit_u, N * 8, hipMemcpyDeviceToHost));
+  // rays without a hit report face -1 and NaN (the device buffers keep
+  // whatever an earlier trace left there)
+  for (size_t r = 0; e && r < N; ++r) {
+    if (e[r] >= 0) continue;
+    if (hf) hf[r] = -1;
+    if (ht) ht[r] = NAN;
+    if (hu) hu[r] = NAN;
+    if (hr) hr[r] = hr[N + r] = hr[2 * N + r] = NAN;
+  }
   return FEMRT_OK;
 }
 
@@ -1185,10 +1200,24 @@ FEMRT_API int femrt_frame_get_iso(femrt_frame* f, int n_iso, int* ie, double* it
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (!K) return FEMRT_OK;
-  if (ie) HIPCHK(c, hipMemcpy(ie, f->d_iso_elem, K * N * 4, hipMemcpyDeviceToHost));
+  std::vector<int> own;
+  int* e = ie;
+  if (!e && (it || ir || iu)) {
+    own.resize(K * N);
+    e = own.data();
+  }
+  if (e) HIPCHK(c, hipMemcpy(e, f->d_iso_elem, K * N * 4, hipMemcpyDeviceToHost));
   if (it) HIPCHK(c, hipMemcpy(it, f->d_iso_t, K * N * 8, hipMemcpyDeviceToHost));
   if (ir) HIPCHK(c, hipMemcpy(ir, f->d_iso_rst, 3 * K * N * 8, hipMemcpyDeviceToHost));
   if (iu) HIPCHK(c, hipMemcpy(iu, f->d_iso_u, K * N * 8, hipMemcpyDeviceToHost));
+  // (ray, isovalue) pairs without a crossing report NaN
+  for (size_t k = 0; e && k < K; ++k)
+    for (size_t r = 0; r < N; ++r) {
+      if (e[k * N + r] >= 0) continue;
+      if (it) it[k * N + r] = NAN;
+      if (iu) iu[k * N + r] = NAN;
+      if (ir) ir[(3 * k) * N + r] = ir[(3 * k + 1) * N + r] = ir[(3 * k + 2) * N + r] = NAN;
+    }
   return FEMRT_OK;
 }
 

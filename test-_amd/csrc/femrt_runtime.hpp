@@ -187,15 +187,17 @@ __device__ __forceinline__ void add_work(long long* work, int idx, long long v)
   if (v) atomicAdd((unsigned long long*)&work[idx], (unsigned long long)v);
 }
 
-__device__ __forceinline__ void flush_work(long long* work, const Work& w)
+// first_hit: the first-hit pass keeps its face-Newton iterations and slab
+// tests in counters of their own ([30], [31]), so the walk's are its own.
+__device__ __forceinline__ void flush_work(long long* work, const Work& w, bool first_hit = false)
 {
   if (!work) return;
-  add_work(work, 2, w.face_it);
+  add_work(work, first_hit ? 30 : 2, w.face_it);
   add_work(work, 3, w.inv_it);
   add_work(work, 4, w.field);
   add_work(work, 5, w.geo64);
   add_work(work, 6, w.field64);
-  add_work(work, 7, w.slab);
+  add_work(work, first_hit ? 31 : 7, w.slab);
   add_work(work, 8, w.mlp);
   add_work(work, 9, w.marched);
   add_work(work, 10, w.exit_pred);

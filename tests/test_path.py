@@ -90,14 +90,23 @@ def _two_blocks(femrt):
     return hm
 
 
+MATERIAL_SETS = {
+    # material 0 on faces 1..5, material 1 on local face 0 (raytracer.cu:248)
+    "lambert_metal": [(0, (0.7, 0.6, 0.5), (0, 0, 0), 0.0, 0.0),
+                      (1, (0.8, 0.8, 0.9), (0, 0, 0), 0.2, 0.0)],
+    "dielectric_lambert": [(2, (1.0, 1.0, 1.0), (0, 0, 0), 0.0, 1.5),
+                           (0, (0.6, 0.7, 0.5), (0, 0, 0), 0.0, 0.0)],
+}
+
+
 @pytest.mark.gpu
-def test_paths_match_oracle_depth_by_depth(femrt, oracle, gpu_ctx):
+@pytest.mark.parametrize("matset", sorted(MATERIAL_SETS))
+def test_paths_match_oracle_depth_by_depth(femrt, oracle, gpu_ctx, matset):
     import oracle.parity as parity
     hm = _two_blocks(femrt)
     mesh = femrt.Mesh(gpu_ctx, hm)
     orc = oracle.Oracle.from_mesh(hm)
-    specs = [(femrt.LAMBERT, (0.7, 0.6, 0.5), (0, 0, 0), 0.0, 0.0),
-             (femrt.METAL, (0.8, 0.8, 0.9), (0, 0, 0), 0.2, 0.0)]
+    specs = MATERIAL_SETS[matset]
     mats = femrt.materials(specs)
     W, H, spp, depth_max, seed = 40, 30, 2, 3, 1234
     cam = femrt.camera_basis((3.2, 2.0, 2.6), (1.25, 0.5, 0.5), (0, 1, 0), 35.0, W / H)
@@ -124,7 +133,10 @@ def test_paths_match_oracle_depth_by_depth(femrt, oracle, gpu_ctx):
             assert np.max(np.abs(gh["t"][both] - oh["t"][both]) / oh["t"][both]) < 1e-6
         if depth > 0:
             rehits += int(both.sum())
-        nxt = po.scatter(orc, rays, oh, specs, seed, depth, depth == depth_max - 1, tp, acc, alive)
+        dties = np.zeros(n, bool)
+        nxt = po.scatter(orc, rays, oh, specs, seed, depth, depth == depth_max - 1, tp, acc, alive,
+                         dties)
+        tie_any |= dties
         fr.scatter(mesh, mats, seed, depth, depth == depth_max - 1)
         g_next = fr.get_rays()
         ok = alive & ~tie_any
@@ -145,3 +157,27 @@ def test_paths_match_oracle_depth_by_depth(femrt, oracle, gpu_ctx):
     np.testing.assert_array_equal(img, img2)
     fr.close()
     mesh.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("W,H,cam", [
+    (64, 48, ((3.0, 2.0, 2.5), (0.5, 0.5, 0.5), (0, 1, 0), 20.0)),
+    (31, 77, ((-1.5, 0.4, -2.0), (0.2, 0.6, 0.4), (0, 1, 0), 35.0)),
+    (128, 16, ((0.5, 4.0, 0.3), (0.5, 0.5, 0.5), (1, 0, 0), 60.0)),
+])
+def test_gpu_camera_rays_match_oracle(femrt, oracle, gpu_ctx, W, H, cam):
+    """Pixel-centre pinhole rays generated on the device (femrt_frame_camera_rays)
+    against the oracle's FP64 camera (raytracer.h:31-48), for several aspect
+    ratios, field-of-view angles and up vectors; ray p = y W + x, y = 0 the
+    bottom row."""
+    cam10 = np.array([*cam[0], *cam[1], *cam[2], cam[3]], np.float64)
+    fr = femrt.Frame(gpu_ctx, W * H)
+    fr.camera_rays(cam10.astype(np.float32), W, H)
+    g = fr.get_rays().astype(np.float64)
+    ref = oracle.camera_rays(cam10.astype(np.float32).astype(np.float64), W, H)
+    np.testing.assert_allclose(g[:3], ref[:3], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(g[3:], ref[3:], rtol=0, atol=2e-6)
+    # corner pixels point where the closed form says: the bottom-left pixel
+    # centre lies half a pixel inside the lower-left corner of the image plane
+    assert np.allclose(np.linalg.norm(g[3:], axis=0), 1.0, atol=1e-6)
+    fr.close()
