@@ -200,6 +200,11 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
  * tests ([7] counts the slab tests of the other passes). */
 #define FEMRT_NCOUNTERS 32
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
+/* Ray ids of the walks of the last counted trace that ended stalled (counter
+ * [29]): copies up to min(cap, FEMRT_MAX_STALLED_IDS) of them, in no set
+ * order, and returns how many walks stalled (>= 0). */
+#define FEMRT_MAX_STALLED_IDS 64
+FEMRT_API int femrt_frame_stalled_rays(femrt_frame* f, int cap, int* ids);
 
 /* ------------------------------------------------ shading and bounces */
 /* Materials of scatter (raytracer.cu:61-97): type 0 Lambertian (cosine-

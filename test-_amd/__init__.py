@@ -139,6 +139,7 @@ def lib():
         "femrt_frame_get_iso": (I, [P, I, P, P, P, P]),
         "femrt_frame_get_rgba": (I, [P, P]),
         "femrt_frame_counters": (I, [P, I, P]),
+        "femrt_frame_stalled_rays": (I, [P, I, P]),
         "femrt_gen_structured_hex": (I, [P, P, P, P, P]),
         "femrt_gen_field_waves": (I, [I, I, P, I, I, C.c_ulonglong, P]),
         "femrt_host_boxes": (I, [I, I, P, P]),
@@ -194,7 +195,7 @@ def exported_symbols():
             "femrt_invert_points", "femrt_trace_params_default", "femrt_frame_create",
             "femrt_frame_destroy", "femrt_frame_set_rays", "femrt_frame_camera_rays",
             "femrt_frame_get_rays", "femrt_trace", "femrt_frame_get_hits", "femrt_frame_get_iso",
-            "femrt_frame_get_rgba", "femrt_frame_counters", "femrt_gen_structured_hex",
+            "femrt_frame_get_rgba", "femrt_frame_counters", "femrt_frame_stalled_rays", "femrt_gen_structured_hex",
             "femrt_gen_field_waves",
             "femrt_load_xml_fld", "femrt_host_boxes", "femrt_host_build_bvh", "femrt_host_field_range",
             "femrt_host_field_lipschitz", "femrt_host_geom_jinv_bound",
@@ -634,6 +635,15 @@ class Frame:
         en = 1 if enable is None else int(enable)
         _check(lib().femrt_frame_counters(self.h, en, _ptr(out)), self.ctx.h, "counters")
         return dict(zip(COUNTER_NAMES, out.tolist()))
+
+    def stalled_rays(self):
+        """(count, ids) of the walks of the last counted trace that ended
+        stalled; at most 64 ids are kept."""
+        ids = np.zeros(64, np.int32)
+        n = lib().femrt_frame_stalled_rays(self.h, ids.size, _ptr(ids))
+        if n < 0:
+            _check(n, self.ctx.h, "stalled_rays")
+        return n, np.sort(ids[:min(n, ids.size)])
 
     def close(self):
         if self.h:

@@ -40,6 +40,7 @@
 using namespace femrt;
 
 #include "femrt_mlp.hpp"
+#include "femrt_point.hpp"
 #include "femrt_runtime.hpp"
 
 
@@ -413,6 +414,11 @@ __global__ __launch_bounds__(256, 2) void k_march(
         if (work) {
           const int len = max(1, S.step & 0xFFFF);
           add_work(work, 19 + min(7, 31 - __clz(len)), 1);
+          if ((S.step >> 16) > FEMRT_MAX_MARCHED_EXITS) {
+            // a stalled walk: keep the ray id for inspection
+            const unsigned long long k = atomicAdd((unsigned long long*)&work[FEMRT_NCOUNTERS], 1ull);
+            if (k < FEMRT_MAX_STALLED_IDS) work[FEMRT_NCOUNTERS + 1 + k] = sink.r;
+          }
         }
         // a bracket written by a pass that was redone is not a result
         for (int k = 0; k < P.n_iso; ++k)
@@ -512,26 +518,7 @@ __global__ void k_shade(int n, int n_iso, const float* __restrict__ rays,
   rgba[r] = px;
 }
 
-// Point kernels: field value + world gradient; inversion.
-template <typename T, int NG, int NF>
-__global__ void k_eval_field(int n, const int* __restrict__ elem, const float* __restrict__ rst,
-                             const GeoMode* __restrict__ geom, const float* __restrict__ field,
-                             T* __restrict__ out)
-{
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int e = elem[p];
-  const T r = rst[p], s = rst[n + p], t = rst[2 * n + p];
-  T u, gref[3], X[3], J[9], gw[3];
-  field_eval<T, NF>(field + (size_t)e * (NF * NF * NF), r, s, t, &u, gref);
-  geom_eval<T, NG>(geom + (size_t)e * (NG * NG * NG), r, s, t, X, J);
-  if (!grad_world<T>(J, gref, gw)) gw[0] = gw[1] = gw[2] = T(NAN);
-  out[p] = u;
-  out[n + p] = gw[0];
-  out[2 * n + p] = gw[1];
-  out[3 * n + p] = gw[2];
-}
-
+// Point kernels: field value + world gradient (femrt_point.hpp); inversion.
 template <int NG>
 __global__ void k_invert(int n, const int* __restrict__ elem, const float* __restrict__ x,
                          const GeoMode* __restrict__ geom, double* __restrict__ rst_out,
@@ -919,13 +906,13 @@ FEMRT_API int femrt_frame_create(femrt_ctx* c, int n, femrt_frame** out)
             hipMalloc(&f->d_qbin, N) == hipSuccess &&
             hipMalloc(&f->d_counts, FEMRT_NCOUNTS * 4) == hipSuccess &&
             hipMalloc(&f->d_rgba, N * 4) == hipSuccess &&
-            hipMalloc(&f->d_work, FEMRT_NCOUNTERS * 8) == hipSuccess;
+            hipMalloc(&f->d_work, FEMRT_WORK_SLOTS * 8) == hipSuccess;
   if (!ok) {
     femrt_frame_destroy(f);
     return fail(c, FEMRT_ENOMEM, "frame_create: hipMalloc failed");
   }
   if (hipMemset(f->d_counts, 0, FEMRT_NCOUNTS * 4) != hipSuccess ||
-      hipMemset(f->d_work, 0, FEMRT_NCOUNTERS * 8) != hipSuccess ||
+      hipMemset(f->d_work, 0, FEMRT_WORK_SLOTS * 8) != hipSuccess ||
       hipMemset(f->d_hit_elem, 0xff, N * 4) != hipSuccess ||
       hipMemset(f->d_iso_elem, 0xff, K * N * 4) != hipSuccess) {
     femrt_frame_destroy(f);
@@ -1020,6 +1007,20 @@ FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out)
   return FEMRT_OK;
 }
 
+FEMRT_API int femrt_frame_stalled_rays(femrt_frame* f, int cap, int* ids)
+{
+  if (!f || cap < 0 || (cap > 0 && !ids)) return FEMRT_EINVAL;
+  femrt_ctx* c = f->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  long long buf[1 + FEMRT_MAX_STALLED_IDS];
+  HIPCHK(c, hipMemcpyAsync(buf, f->d_work + FEMRT_NCOUNTERS, sizeof(buf), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int total = (int)buf[0];
+  const int kept = std::min(total, FEMRT_MAX_STALLED_IDS);
+  for (int i = 0; i < std::min(kept, cap); ++i) ids[i] = (int)buf[1 + i];
+  return total;
+}
+
 FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_params* p)
 {
   if (!m || !f || !p) return FEMRT_EINVAL;
@@ -1042,7 +1043,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   const int n_iso = p->iso_enable ? p->n_iso : 0;
   long long* work = f->counters ? f->d_work : nullptr;
   if (work) {
-    HIPCHK(c, hipMemsetAsync(f->d_work, 0, FEMRT_NCOUNTERS * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(f->d_work, 0, FEMRT_WORK_SLOTS * 8, c->stream));
     HIPCHK(c, hipMemsetAsync(f->d_work + 16, 0xff, 2 * 8, c->stream));  // minima start at the top
   }
   const int blocks = (n + 255) / 256;

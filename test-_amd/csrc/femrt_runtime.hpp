@@ -248,6 +248,9 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter)
 #define FEMRT_QSEG 8
 #define FEMRT_QSEG_CTR 16
 #define FEMRT_NCOUNTS 32
+// Work buffer of a frame: FEMRT_NCOUNTERS counters, then the number of walks
+// that ended stalled and the first FEMRT_MAX_STALLED_IDS of their ray ids.
+#define FEMRT_WORK_SLOTS (FEMRT_NCOUNTERS + 1 + FEMRT_MAX_STALLED_IDS)
 
 // Element walk in element-binned rounds (femrt_walk.hip).
 int walk_run(femrt_ctx* c, femrt_mesh* m, femrt_frame* f, const TraceMesh& TM, const IsoParams& ip,

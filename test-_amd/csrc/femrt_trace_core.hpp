@@ -258,6 +258,23 @@ FEMRT_HD bool face_root_grid(const GeoMode* __restrict__ gc, const float* o, con
 // and a single flat seed's failure mode (converging to a crossing of the
 // face's polynomial extension outside the element while a crossing inside
 // exists) is avoided.
+// Whether the ray meets face f at (u, v) within about 17 degrees of the
+// face's tangent plane: |n.d| < 0.3 |n| |d|, n = X_u x X_v.
+template <int NG>
+FEMRT_HD bool grazing_crossing(const GeoMode* __restrict__ gc, const float* d, int f, float u, float v)
+{
+  float r, s, t, X[3], J[9];
+  face_to_rst(f, u, v, r, s, t);
+  geom_eval<float, NG>(gc, r, s, t, X, J);
+  const int cu = face_u(f) == 1 ? 1 : 0, cv = face_v(f);
+  const float a[3] = {J[cu], J[3 + cu], J[6 + cu]}, b[3] = {J[cv], J[3 + cv], J[6 + cv]};
+  const float n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const float nd = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
+  const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  const float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  return nd * nd < 0.09f * nn * dd;
+}
+
 template <int NG, typename W>
 FEMRT_HD bool face_root_patches(const GeoMode* __restrict__ gc, const float* __restrict__ bx,
                                 const float* o, const float* d, const float* id, int f, float tmin,
@@ -278,6 +295,23 @@ FEMRT_HD bool face_root_patches(const GeoMode* __restrict__ gc, const float* __r
     // face within this sub-patch): seed again at the far end of the box
     if (rc == 0 && tt < tmin)
       rc = face_newton<NG>(gc, o, d, f, uc, vc, s1, tmin, 1e30f, tt, uu, vv, w.face_it);
+    // A ray close to the tangent plane can cross one sub-patch twice, and
+    // Newton from the centre may converge to the farther crossing: then the
+    // four quarter centres of the sub-patch seed Newton too, from the near
+    // end of the box, and a nearer accepted root replaces it.
+    if (rc == 1 && tt < best * 1.0001f && grazing_crossing<NG>(gc, d, f, uu, vv)) {
+      ++w.face_it;
+      for (int k = 0; k < 4; ++k) {
+        float t2, u2, v2;
+        const float uq = uc + ((k & 1) ? 1.f : -1.f) / 6.f, vq = vc + ((k & 2) ? 1.f : -1.f) / 6.f;
+        if (face_newton<NG>(gc, o, d, f, uq, vq, s0, tmin, 1e30f, t2, u2, v2, w.face_it) == 1 &&
+            t2 < tt * (1.f - 1e-6f)) {
+          tt = t2;
+          uu = u2;
+          vv = v2;
+        }
+      }
+    }
     if (rc == 1 && tt < best) {
       best = tt;
       tau_o = tt;
@@ -624,15 +658,21 @@ FEMRT_HD bool find_exit_marching(const GeoMode* __restrict__ gc, const float* __
 // face excluded) at s_f; a face Newton on each crossed face, nearest crossing
 // first (at most 3), seeded at the predicted point, gives the exit.  Returns
 // false when none converges to an accepted root.
+// m_in, m_out (may be null): drst/dt = J^-1 d at the entry and at the exit
+// found (the Jacobian of the last iterate), for the scan's interpolated
+// guesses.
 template <int NG, typename W>
 FEMRT_HD bool find_exit_predicted(const GeoMode* __restrict__ gc, const float* o, const float* d,
                                   int f_in, const float* rst_in, float t_in, float t_lo, int& f_out,
-                                  float& t_out, float& u_out, float& v_out, W& w)
+                                  float& t_out, float& u_out, float& v_out, W& w,
+                                  float* m_in = nullptr, float* m_out = nullptr)
 {
   float X[3], J[9], dl[3];
   geom_eval<float, NG>(gc, rst_in[0], rst_in[1], rst_in[2], X, J);
   ++w.face_it;
   if (!solve3<float>(J, d, dl)) return false;
+  if (m_in)
+    for (int c = 0; c < 3; ++c) m_in[c] = dl[c];
   // crossings of the three planes the line heads for
   float sc[3];
   int fc[3];
@@ -678,6 +718,7 @@ FEMRT_HD bool find_exit_predicted(const GeoMode* __restrict__ gc, const float* o
       t_out = F.tau;
       u_out = F.u;
       v_out = F.v;
+      if (m_out && !solve3<float>(J, d, m_out)) m_out[0] = m_out[1] = m_out[2] = 0.f;
       return true;
     }
     if (st != FN_CONTINUE) have = next_face(F);
@@ -746,6 +787,8 @@ struct MarchState {
   unsigned found;         // isovalues whose bracket has been recorded
   bool ok;
   bool redo;              // samples showed a wrong exit: search again, entry face included
+  bool have_m;            // m_in, m_out hold drst/dt at the entry and exit (predicted exits)
+  float m_in[3], m_out[3];
 };
 
 FEMRT_HD void march_begin(MarchState& S, const HitF32& h)
@@ -761,6 +804,7 @@ FEMRT_HD void march_begin(MarchState& S, const HitF32& h)
   S.found = 0;
   S.ok = false;
   S.redo = false;
+  S.have_m = false;
 }
 
 // Exit of the current segment.  First pass: predicted exit (engineered and
@@ -780,12 +824,14 @@ FEMRT_HD void march_exit(const TraceMesh& M, const IsoParams& P, const float* o,
   float rst_in[3];
   face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
   bool ok = false;
+  S.have_m = false;
   if (!S.redo) {
     ++w.marched;
     if (P.guess != FEMRT_GUESS_PLAIN) {
       ok = find_exit_predicted<NG>(gc, o, d, S.f_in, rst_in, S.t_in, t_lo, S.f_out, S.t_out, S.u_out,
-                                   S.v_out, w);
+                                   S.v_out, w, S.m_in, S.m_out);
       w.exit_pred += ok;
+      S.have_m = ok;
     }
     if (!ok) {
       ok = find_exit_search<NG>(gc, bx, o, d, id, S.f_in, t_lo, P.guess, false, S.f_out, S.t_out,
@@ -1061,6 +1107,264 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
   return false;
 }
 
+// --------------------------------------------- cooperative segment scan
+//
+// The same scan with the samples of a segment evaluated side by side instead
+// of one after another: the kernel hands the interior samples of a few
+// segments to the lanes of a wave, 16 lanes per segment, so a wave stays busy
+// while its rays are at different stages (and when only a few of them are
+// left).  Each sample's point inversion starts from the cubic Hermite
+// interpolant of rst(t) between the segment's ends, which matches rst and
+// drst/dt = J^-1 d at both ends and is O(dt^4) from the preimage (the
+// sequential scan extrapolates from the two previous samples instead).  The
+// samples are the scan's samples, the first sign change per isovalue the
+// bracket, so the brackets are the sequential scan's (ties aside: FP32
+// decisions within rounding of zero).
+
+#define FEMRT_COOP_MAX_SAMPLES 16
+
+// What the lanes sampling a segment need to know about it.
+struct SegInfo {
+  int e;
+  unsigned open;      // isovalues still searched
+  float lip;          // field Lipschitz bound of the element (0: none)
+  float t_in, t_out;
+  float r_in[3], r_out[3];  // rst at the ends
+  float m_in[3], m_out[3];  // drst/dt at the ends
+  float o[3], d[3];
+};
+
+// drst/dt = J^-1 d at both ends of the segment, for exits whose search left
+// no Jacobian behind.
+template <int NG, typename W>
+FEMRT_HD void seg_tangents(const GeoMode* __restrict__ gc, const float* d, const float* r_in,
+                           const float* r_out, float* m_in, float* m_out, W& w)
+{
+  float X[3], J[9];
+  geom_eval<float, NG>(gc, r_in[0], r_in[1], r_in[2], X, J);
+  if (!solve3<float>(J, d, m_in)) m_in[0] = m_in[1] = m_in[2] = 0.f;
+  geom_eval<float, NG>(gc, r_out[0], r_out[1], r_out[2], X, J);
+  if (!solve3<float>(J, d, m_out)) m_out[0] = m_out[1] = m_out[2] = 0.f;
+  w.inv_it += 2;
+}
+
+// Segment info of the walk state (after march_exit found the exit).
+template <int NG, typename W>
+FEMRT_HD void seg_info(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
+                       const MarchState& S, SegInfo& g, W& w)
+{
+  g.e = S.e;
+  g.open = ((1u << P.n_iso) - 1u) & ~S.found;
+  g.lip = M.lip ? M.lip[S.e] : 0.f;
+  g.t_in = S.t_in;
+  g.t_out = S.t_out;
+  face_to_rst(S.f_in, S.u_in, S.v_in, g.r_in[0], g.r_in[1], g.r_in[2]);
+  face_to_rst(S.f_out, S.u_out, S.v_out, g.r_out[0], g.r_out[1], g.r_out[2]);
+  for (int c = 0; c < 3; ++c) {
+    g.o[c] = o[c];
+    g.d[c] = d[c];
+  }
+  if (P.guess == FEMRT_GUESS_PLAIN) {
+    // no interpolated guesses (the Hermite curve of the bracket's start
+    // point then has zero end slopes)
+    for (int c = 0; c < 3; ++c) g.m_in[c] = g.m_out[c] = 0.f;
+    return;
+  }
+  if (S.have_m) {
+    for (int c = 0; c < 3; ++c) {
+      g.m_in[c] = S.m_in[c];
+      g.m_out[c] = S.m_out[c];
+    }
+  } else {
+    seg_tangents<NG>(M.geom + (size_t)S.e * (NG * NG * NG), d, g.r_in, g.r_out, g.m_in, g.m_out, w);
+  }
+}
+
+// Whether the scan of the segment can find a crossing: some isovalue still
+// open lies within the element's bounds of u (else the scan is skipped).
+FEMRT_HD bool seg_may_cross(const TraceMesh& M, const IsoParams& P, const MarchState& S)
+{
+  if (!M.frange) return true;
+  const float lo = M.frange[2 * (size_t)S.e], hi = M.frange[2 * (size_t)S.e + 1];
+  for (int k = 0; k < FEMRT_MAX_ISO; ++k)
+    if (k < P.n_iso && !((S.found >> k) & 1u) && P.iso[k] >= lo && P.iso[k] <= hi) return true;
+  return false;
+}
+
+// Ray parameter of sample j of [t_in, t_out] (the sequential scan's
+// march_sample_t, bit for bit).
+FEMRT_HD float seg_sample_t(const SegInfo& g, int j, int samples)
+{
+  return j == samples ? g.t_out : g.t_in + (g.t_out - g.t_in) * ((float)j / (float)samples);
+}
+
+// Cubic Hermite interpolant of rst at fraction s of the segment, clamped to
+// the cube.
+FEMRT_HD void seg_hermite(const SegInfo& g, float s, float* rst)
+{
+  const float h = g.t_out - g.t_in;
+  const float s2 = s * s, s3 = s2 * s;
+  const float h00 = 2.f * s3 - 3.f * s2 + 1.f, h10 = s3 - 2.f * s2 + s;
+  const float h01 = 3.f * s2 - 2.f * s3, h11 = s3 - s2;
+  for (int c = 0; c < 3; ++c) {
+    const float v = h00 * g.r_in[c] + h10 * h * g.m_in[c] + h01 * g.r_out[c] + h11 * h * g.m_out[c];
+    rst[c] = fminf(fmaxf(v, -1.f), 1.f);
+  }
+}
+
+// Point inversion of one scan sample, one Newton iteration per call
+// (sample_job_newton), then the decision once u is known (sample_job_after):
+// the sequential scan's per-sample rules -- convergence |delta|_inf < 1e-5
+// inside the cube (|rst|_inf <= 1.001), early sign acceptance after a step
+// below 1e-2 when every open isovalue clears 2 lip |step|, restarts from 0
+// (after a network guess) and the octant centres, 50 iterations per run.
+struct SampleJob {
+  float rst[3], x[3];
+  int stage, its;
+  bool cert_tried, zero_retry;
+};
+
+struct SampleTrip {
+  bool fail, conv, try_cert, inside;
+  float st;
+};
+
+enum { SJ_CONTINUE = 0, SJ_DONE = 1, SJ_FAIL = 2 };
+
+FEMRT_HD void sample_job_begin(SampleJob& J, const SegInfo& g, int j, int samples, int guess)
+{
+  const float t = seg_sample_t(g, j, samples);
+  for (int c = 0; c < 3; ++c) J.x[c] = fmaf(t, g.d[c], g.o[c]);
+  if (guess == FEMRT_GUESS_PLAIN)
+    J.rst[0] = J.rst[1] = J.rst[2] = 0.f;
+  else
+    seg_hermite(g, (float)j / (float)samples, J.rst);
+  J.stage = 0;
+  J.its = 0;
+  J.cert_tried = false;
+  J.zero_retry = false;
+}
+
+template <int NG, typename C>
+FEMRT_HD void sample_job_newton(SampleJob& J, const GeoMode* __restrict__ gc, float lip, SampleTrip& T,
+                                C& iters)
+{
+  float X[3], Jm[9], dl[3];
+  geom_eval<float, NG>(gc, J.rst[0], J.rst[1], J.rst[2], X, Jm);
+  ++iters;
+  ++J.its;
+  const float bv[3] = {X[0] - J.x[0], X[1] - J.x[1], X[2] - J.x[2]};
+  T.fail = !solve3<float>(Jm, bv, dl);
+  T.conv = T.try_cert = false;
+  T.st = 0.f;
+  if (!T.fail) {
+    J.rst[0] -= dl[0];
+    J.rst[1] -= dl[1];
+    J.rst[2] -= dl[2];
+    if (!(fabsf(J.rst[0]) < 1e4f && fabsf(J.rst[1]) < 1e4f && fabsf(J.rst[2]) < 1e4f)) {
+      T.fail = true;
+    } else if (fabsf(dl[0]) < 1e-5f && fabsf(dl[1]) < 1e-5f && fabsf(dl[2]) < 1e-5f) {
+      T.conv = true;
+    } else {
+      T.st = fmaxf(fabsf(dl[0]), fmaxf(fabsf(dl[1]), fabsf(dl[2])));
+      T.try_cert = T.st < 1e-2f && !J.cert_tried && lip > 0.f;
+      J.cert_tried = J.cert_tried || T.try_cert;
+    }
+  }
+  T.inside = fmaxf(fabsf(J.rst[0]), fmaxf(fabsf(J.rst[1]), fabsf(J.rst[2]))) <= 1.001f;
+}
+
+FEMRT_HD bool sample_trip_needs_u(const SampleTrip& T) { return !T.fail && ((T.conv && T.inside) || T.try_cert); }
+
+// u: the field at J.rst when sample_trip_needs_u.  SJ_DONE: u is the sample's
+// value; SJ_FAIL: no preimage inside the element; SJ_CONTINUE: next trip.
+FEMRT_HD int sample_job_after(SampleJob& J, const SampleTrip& T, float u, const IsoParams& P, unsigned open,
+                              float lip)
+{
+  bool done = T.conv, fail = T.fail;
+  if (T.try_cert) {
+    const float margin = 2.f * lip * T.st;
+    done = true;
+    for (int k = 0; k < FEMRT_MAX_ISO; ++k)
+      if (k < P.n_iso && ((open >> k) & 1u) && !(fabsf(u - P.iso[k]) > margin)) done = false;
+  }
+  if (!fail && !done && J.its >= 50) fail = true;
+  if (done && !T.inside) {
+    done = false;
+    fail = true;
+  }
+  if (done) return SJ_DONE;
+  if (!fail) return SJ_CONTINUE;
+  ++J.stage;
+  if (J.stage == 1 && !J.zero_retry) J.stage = 2;
+  if (J.stage > 9) return SJ_FAIL;
+  if (J.stage == 1) {
+    J.rst[0] = J.rst[1] = J.rst[2] = 0.f;
+  } else {
+    const int sd = J.stage - 2;
+    J.rst[0] = (sd & 1) ? 0.5f : -0.5f;
+    J.rst[1] = (sd & 2) ? 0.5f : -0.5f;
+    J.rst[2] = (sd & 4) ? 0.5f : -0.5f;
+  }
+  J.its = 0;
+  J.cert_tried = false;
+  return SJ_CONTINUE;
+}
+
+// The scan of one segment from its sample values u(j) = U[j * ustride],
+// j = 0 .. samples (NaN: no preimage inside the element).  Returns true when
+// an interior sample has no preimage on the first pass (the exit is wrong:
+// the caller sets S.redo and finds the exit again); else records the first
+// sign change per open isovalue as the sequential scan does and marks it in
+// S.found.
+template <typename Sink, typename W>
+FEMRT_HD bool seg_scan(const IsoParams& P, const SegInfo& g, MarchState& S, const float* U, int ustride,
+                       Sink& sink, W& w)
+{
+  const int Ms = P.samples;
+  if (!S.redo)
+    for (int j = 1; j < Ms; ++j)
+      if (!(U[j * ustride] == U[j * ustride])) return true;
+  float gprev[FEMRT_MAX_ISO];
+  float tprev = 0.f;
+  bool have_prev = false;
+  unsigned found_here = 0;
+  for (int j = 0; j <= Ms; ++j) {
+    const float u = U[j * ustride];
+    if (!(u == u)) {
+      have_prev = false;
+      continue;
+    }
+    const float t = seg_sample_t(g, j, Ms);
+    for (int k = 0; k < FEMRT_MAX_ISO; ++k) {
+      if (k >= P.n_iso || ((S.found | found_here) >> k) & 1u) continue;
+      const float gk = u - P.iso[k];
+      if (have_prev && gprev[k] * gk <= 0.f) {
+        IsoBracket br;
+        br.elem = S.e;
+        br.ta = tprev;
+        br.tb = t;
+        br.ga = gprev[k];
+        const float den = gprev[k] - gk;
+        const float wt = den != 0.f ? fminf(fmaxf(gprev[k] / den, 0.f), 1.f) : 0.5f;
+        br.tm = tprev + wt * (t - tprev);
+        // rst at the secant point from the segment's interpolant (the FP64
+        // endgame converges from there)
+        const float h = g.t_out - g.t_in;
+        seg_hermite(g, h > 0.f ? fminf(fmaxf((br.tm - g.t_in) / h, 0.f), 1.f) : 0.f, br.rst);
+        sink.put(k, br);
+        ++w.brackets;
+        found_here |= 1u << k;
+      }
+      gprev[k] = gk;
+    }
+    tprev = t;
+    have_prev = true;
+  }
+  S.found |= found_here;
+  return false;
+}
+
 // Next segment through the face adjacency.  Returns MARCH_DONE when the ray
 // is done (every isovalue found or the step cap), MARCH_NEXT when S now holds
 // the neighbour's segment, MARCH_REENTER when the ray left the mesh (boundary
@@ -1140,6 +1444,69 @@ struct BracketArray {
   IsoBracket* out;
   FEMRT_HD void put(int k, const IsoBracket& b) { out[k] = b; }
 };
+
+// Sample values of one segment for the cooperative scan, one sample after
+// another (the kernel spreads them over lanes): U[j], j = 0 .. samples, NaN
+// where a sample has no preimage inside the element.
+template <int NG, int NF, typename W>
+FEMRT_HD void seg_sample_values(const TraceMesh& M, const IsoParams& P, const SegInfo& g, float* U, W& w)
+{
+  const GeoMode* gc = M.geom + (size_t)g.e * (NG * NG * NG);
+  const float* fc = M.field + (size_t)g.e * (NF * NF * NF);
+  const int Ms = P.samples;
+  U[0] = field_value<float, NF>(fc, g.r_in[0], g.r_in[1], g.r_in[2]);
+  U[Ms] = field_value<float, NF>(fc, g.r_out[0], g.r_out[1], g.r_out[2]);
+  w.field += 2;
+  for (int j = 1; j < Ms; ++j) {
+    SampleJob J;
+    sample_job_begin(J, g, j, Ms, P.guess);
+    for (;;) {
+      SampleTrip T;
+      sample_job_newton<NG>(J, gc, g.lip, T, w.inv_it);
+      float u = 0.f;
+      if (sample_trip_needs_u(T)) {
+        u = field_value<float, NF>(fc, J.rst[0], J.rst[1], J.rst[2]);
+        ++w.field;
+      }
+      const int rc = sample_job_after(J, T, u, P, g.open, g.lip);
+      if (rc == SJ_CONTINUE) continue;
+      U[j] = rc == SJ_DONE ? u : NAN;
+      break;
+    }
+  }
+}
+
+// march_ray with the cooperative scan's per-segment rules, one lane.
+template <int NG, int NF, typename W>
+FEMRT_HD unsigned march_ray_coop(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
+                                 HitF32 h, IsoBracket* out, W& w)
+{
+  if (P.n_iso <= 0 || P.max_march <= 0 || P.samples > FEMRT_COOP_MAX_SAMPLES) return 0;
+  float id[3];
+  for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
+  MarchState S;
+  march_begin(S, h);
+  BracketArray sink{out};
+  float U[FEMRT_COOP_MAX_SAMPLES + 1];
+  for (;;) {
+    march_exit<NG>(M, P, o, d, id, S, w);
+    if (S.ok) {
+      if (seg_may_cross(M, P, S)) {
+        SegInfo g;
+        seg_info<NG>(M, P, o, d, S, g, w);
+        seg_sample_values<NG, NF>(M, P, g, U, w);
+        if (seg_scan(P, g, S, U, 1, sink, w)) {
+          S.redo = true;
+          continue;
+        }
+      } else {
+        ++w.skipped;
+      }
+    }
+    if (!march_advance<NG>(M, P, o, d, S, w)) break;
+  }
+  return S.found;
+}
 
 // The whole walk of one ray, one segment after another (host harnesses and
 // single-lane use).  Returns the mask of isovalues found.
