@@ -288,7 +288,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variants", default="learned,learned_every,warm,plain")
+    ap.add_argument("--variants", default="warm,learned,plain,warm_coop,learned_every")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -301,6 +301,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--inflight", type=int, default=3,
                     help="frames in flight, each on its own HIP stream (1 = one frame at a time)")
+    ap.add_argument("--single-steps", type=int, default=5,
+                    help="also time this many frames of the headline variant one at a time")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -333,8 +335,14 @@ def main():
         c2 = femrt.Context(0)
         lanes.append((c2, femrt.Mesh(c2, hm), femrt.Frame(c2, n_rays)))
 
+    # warm / plain: the default walk (each segment scanned in its ray's lane;
+    # warm starts extrapolated along the ray, plain from rst = 0 with the
+    # reference's face seeds); learned: the network's guess for each
+    # segment's first sample, warm starts after it; learned_every: a network
+    # guess for every sample (the network on its own); warm_coop: the
+    # samples of a wave's segments spread over its lanes, Hermite starts
     names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED,
-             "learned_every": femrt.GUESS_LEARNED}
+             "learned_every": femrt.GUESS_LEARNED, "warm_coop": femrt.GUESS_WARM}
     variants = [v for v in args.variants.split(",") if v]
     results = {}
     mlp_info = None
@@ -366,7 +374,7 @@ def main():
     if not args.no_autotune:
         # launch shapes picked by the autotuner on the headline variant's trace,
         # then used by every variant (untimed)
-        tv = "learned" if "learned" in variants else variants[0]
+        tv = "warm" if "warm" in variants else variants[0]
         tflags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
         gen_rays(fr)
         launch_cfg = list(mesh.autotune(fr, femrt.trace_params(ISOS, guess=names[tv], flags=tflags), reps=3))
@@ -377,6 +385,8 @@ def main():
         flags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
         if v == "learned_every":
             flags |= femrt.TRACE_LEARNED_EVERY_SAMPLE
+        if v.endswith("_coop"):
+            flags |= femrt.TRACE_COOP_SCAN
         wall, kt, cnt = run_variant(femrt, lanes, names[v], args.steps, args.warmup, flags)
         f32, f64 = kernel_flops(cnt, hm.ng, hm.nf)
         results[v] = dict(wall=wall, kernels=kt, counters=cnt, f32=f32, f64=f64,
@@ -387,8 +397,26 @@ def main():
                           "launches_per_step": {k: c / args.steps for k, (a, c) in kt.items()},
                           "counters": cnt}), file=sys.stderr, flush=True)
 
-    head = "learned" if "learned" in results else ("warm" if "warm" in results else variants[0])
+    head = "warm" if "warm" in results else variants[0]
     R = results[head]
+    single = None
+    if args.inflight > 1 and args.single_steps > 0:
+        # the headline variant one frame at a time (one stream): the walk's
+        # launches then share the GPU with nothing, so its time is its own
+        hflags = 1 | (femrt.TRACE_BINNED_WALK if args.binned_walk else 0)
+        w1, kt1, cnt1 = run_variant(femrt, lanes[:1], names[head], args.single_steps, 1, hflags)
+        fm1 = flops_model(hm.ng, hm.nf)
+        d1 = max(kt1, key=lambda k: kt1[k][0])
+        ms1 = kt1[d1][0] / args.single_steps
+        fl1 = kernel_flops_of(d1, cnt1, fm1) or kernel_flops(cnt1, hm.ng, hm.nf)[0]
+        single = {"frames_in_flight": 1, "steps": args.single_steps,
+                  "ms_per_step": w1 / args.single_steps * 1e3,
+                  "mrays_s": args.single_steps * n_rays / w1 / 1e6,
+                  "kernel": d1, "kernel_ms_per_step": ms1,
+                  "kernels_ms_per_step": {k: a / args.single_steps for k, (a, c) in kt1.items()},
+                  "achieved": fl1 / (ms1 * 1e-3) / 1e12,
+                  "frac": fl1 / (ms1 * 1e-3) / 1e12 / PEAK_FP32_TFLOPS}
+        print(json.dumps({"single_frame": single}), file=sys.stderr, flush=True)
     # dominant kernel roofline: the kernel with the largest time share
     kt = R["kernels"]
     dom = max(kt, key=lambda k: kt[k][0])
@@ -461,6 +489,21 @@ def main():
         line["config"]["mlp"] = mlp_info
     if "plain" in results and head != "plain":
         line["config"]["speedup_vs_plain_newton"] = R["mrays"] / results["plain"]["mrays"]
+    if "learned" in results and head != "learned":
+        L_ = results["learned"]
+        lg = {"mrays_s": L_["mrays"], "ms_per_step": L_["wall"] / args.steps * 1e3,
+              "speedup_vs_warm": L_["mrays"] / R["mrays"],
+
+              "network_inferences_per_frame": L_["counters"].get("mlp_infer", 0)}
+        if "plain" in results:
+            lg["speedup_vs_plain_newton"] = L_["mrays"] / results["plain"]["mrays"]
+        line["config"]["learned_guess"] = lg
+        line["config"]["note"] = (
+            "value is the default walk with engineered guesses; the learned guess gains nothing over them "
+            "(see learned_guess): the speedup over plain Newton comes from non-learned engineering -- "
+            "predicted exits, sub-patch face seeds, extrapolated sample starts, certified sample skipping")
+    if single:
+        line["config"]["single_frame"] = single
     print(json.dumps(line), flush=True)
     for c, m, f in reversed(lanes):
         f.close()

@@ -120,11 +120,16 @@ FEMRT_API int femrt_invert_points(femrt_mesh* mesh, int n, const int* elem, cons
 enum {
   FEMRT_GUESS_PLAIN = 0,   /* reference initial guesses: face seeds {-0.8,0,0.8}^2,
                               point inversion from rst = 0 */
-  FEMRT_GUESS_WARM = 1,    /* engineered: warm starts along the ray + bilinear seeds */
+  FEMRT_GUESS_WARM = 1,    /* engineered: the predicted exit, sub-patch face seeds,
+                              and scan samples started from the cubic Hermite
+                              interpolant of rst(t) between the segment's ends
+                              (FEMRT_TRACE_COOP_SCAN), else extrapolated from the
+                              two previous samples */
   FEMRT_GUESS_LEARNED = 2  /* MLP initial guesses (femrt_mlp_*) for the point
                               inversions with no earlier sample to start from
                               (all of them with FEMRT_TRACE_LEARNED_EVERY_SAMPLE),
-                              warm starts elsewhere, plain fallback */
+                              warm starts elsewhere, plain fallback (never with
+                              FEMRT_TRACE_COOP_SCAN) */
 };
 
 #define FEMRT_MAX_ISO 8
@@ -155,6 +160,15 @@ typedef struct {
 #define FEMRT_TRACE_NO_SAMPLE_SKIP 8   /* evaluate every scan sample, even where the
                                           element's bounds certify its signs from
                                           an earlier sample (same results)          */
+#define FEMRT_TRACE_COOP_SCAN 16       /* with plain or warm guesses and at most 16
+                                          samples per segment: spread the samples of
+                                          a wave's segments over its lanes, each
+                                          started from the cubic Hermite
+                                          interpolant of rst(t) between the
+                                          segment's ends; by default each segment
+                                          is scanned in its ray's lane, sample after
+                                          sample (warm starts extrapolated from the
+                                          two previous samples, certified skipping) */
 
 FEMRT_API void femrt_trace_params_default(femrt_trace_params* p);
 
@@ -195,14 +209,18 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
  * by segments walked, log2 bins 1, 2-3, 4-7, ..., 128+; [27] segments whose
  * scan the element's field bounds ruled out; [28] scan samples whose signs
  * the field and geometry bounds certified without evaluating them; [29] walks
- * ended as stalled (more than 16 exits found by the marching fallback);
- * [30] face-Newton iterations FP32 of the first-hit pass, [31] its slab
- * tests ([7] counts the slab tests of the other passes). */
-#define FEMRT_NCOUNTERS 32
+ * ended as stalled (8 stalls skipped already; a stall is a run of more than
+ * 16 exits in a row found by the marching fallback, as for a ray inside a
+ * face two elements share); [30] face-Newton iterations FP32 of the
+ * first-hit pass, [31] its slab tests ([7] counts the slab tests of the
+ * other passes); [32] stalls skipped past (the entry moved 1e-3 of the
+ * element along the ray). */
+#define FEMRT_NCOUNTERS 33
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
-/* Ray ids of the walks of the last counted trace that ended stalled (counter
- * [29]): copies up to min(cap, FEMRT_MAX_STALLED_IDS) of them, in no set
- * order, and returns how many walks stalled (>= 0). */
+/* Ray ids of the walks of the last counted trace that stalled (skipped past
+ * a stall or ended stalled, counters [32] and [29]): copies up to
+ * min(cap, FEMRT_MAX_STALLED_IDS) of them, in no set order, and returns how
+ * many walks stalled (>= 0). */
 #define FEMRT_MAX_STALLED_IDS 64
 FEMRT_API int femrt_frame_stalled_rays(femrt_frame* f, int cap, int* ids);
 

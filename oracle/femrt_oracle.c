@@ -510,6 +510,21 @@ static int eval_along(const orc_mesh_t* m, int e, const double* o, const double*
   return 1;
 }
 
+/* Tie margin of a scan sample at ray parameter t with g = u - iso: the sign
+ * of g is not determined by an FP32 scan when |g| is below 1e-5 (evaluation
+ * error), or below 4e-6 t |dg/dt| (the sample's position in FP32, from FP32
+ * face roots, moves it by a few 1e-7 t).  Returned on one scale: the smaller
+ * of |g| and 1e-5 |g| / (4e-6 t |dg/dt|), so a margin below 1e-5 is a tie. */
+static double sample_tie_margin(const orc_mesh_t* m, int e, const double* rst, const double* d, double t,
+                                double g)
+{
+  double u, gr[3], gw[3];
+  orc_field(m, e, rst, &u, gr, gw);
+  const double gp = fabs(gw[0] * d[0] + gw[1] * d[1] + gw[2] * d[2]);
+  const double pos = 4e-6 * fabs(t) * gp;
+  return pos > 0.0 ? fmin(fabs(g), 1e-5 * fabs(g) / pos) : fabs(g);
+}
+
 ORC_EXPORT int orc_iso_segment(const orc_mesh_t* m, int e, const double* o, const double* d,
                                double ta, double tb, int M, double iso, double* out_t,
                                double* out_rst, double* out_u, double* out_minabs)
@@ -522,7 +537,7 @@ ORC_EXPORT int orc_iso_segment(const orc_mesh_t* m, int e, const double* o, cons
     double rst[3], u;
     if (!eval_along(m, e, o, d, t, rst, &u)) continue;
     const double g = u - iso;
-    if (fabs(g) < minabs) minabs = fabs(g);
+    minabs = fmin(minabs, sample_tie_margin(m, e, rst, d, t, g));
     if (have_prev && prev_g * g <= 0.0) {
       double lt = prev_t, lg = prev_g, rt = t;
       for (int it = 0; it < 200 && fabs(rt - lt) > 1e-14 * (fabs(rt) + 1.0); ++it) {
@@ -569,11 +584,13 @@ ORC_EXPORT int orc_iso_segment(const orc_mesh_t* m, int e, const double* o, cons
 /*   root rejected just outside its face, at or before the hit (any root for */
 /*   a miss)                                                                 */
 /* Per ray and isovalue (ORC_ISO_STRIDE):                                     */
-/*   [0] elem (-1 none) [1] t [2..4] rst [5] u [6] min |g| over samples of   */
-/*   the element where the root was found (tie classification)               */
+/*   [0] elem (-1 none) [1] t [2..4] rst [5] u [6] smallest tie margin       */
+/*   (sample_tie_margin) over the samples of every segment scanned before    */
+/*   the root, up to its bracket (all segments when none is found)           */
 /* ------------------------------------------------------------------------ */
 #define ORC_GEOM_STRIDE 10
 #define ORC_ISO_STRIDE 7
+#define ORC_MAX_ISO_REC 8  /* isovalues whose tie margins track every scanned segment */
 
 ORC_EXPORT int orc_geom_stride(void) { return ORC_GEOM_STRIDE; }
 ORC_EXPORT int orc_iso_stride(void) { return ORC_ISO_STRIDE; }
@@ -640,6 +657,17 @@ ORC_EXPORT void orc_trace(const orc_mesh_t* m, const float* rays, int nrays, con
     }
     int cap = 256, nc = 0;
     cand_t* cand = (cand_t*)malloc(sizeof(cand_t) * cap);
+    /* every scanned segment's start and smallest |u - iso| over its samples
+     * (up to its bracket), per isovalue: the tie margin of a first crossing
+     * is the smallest over all segments before it, since "no crossing in
+     * this segment" is a sign decision too */
+    enum { SEGREC = 256 };
+    double seg_ta[ORC_MAX_ISO_REC][SEGREC], seg_mn[ORC_MAX_ISO_REC][SEGREC], seg_over[ORC_MAX_ISO_REC];
+    int nseg[ORC_MAX_ISO_REC];
+    for (int k = 0; k < ORC_MAX_ISO_REC; ++k) {
+      nseg[k] = 0;
+      seg_over[k] = 1e30;
+    }
     orc_rej_t rej;
     rej.n = 0;
     for (int e = 0; e < m->n_elem; ++e) {
@@ -710,22 +738,54 @@ ORC_EXPORT void orc_trace(const orc_mesh_t* m, const float* rays, int nrays, con
           roots[b2] = roots[b2 - 1];
           roots[b2 - 1] = tmp;
         }
-      for (int iv = 0; iv + 1 < nroots; iv += 2) {
-        const double ta = roots[iv], tb = roots[iv + 1];
+      /* which gaps between consecutive roots lie inside the element: the
+       * midpoint's preimage in the closed cube.  Pairing the sorted roots as
+       * (r0, r1), (r2, r3), ... instead goes wrong for rays that run along
+       * an edge or inside a face, where a root is found twice or missed;
+       * consecutive inside gaps make one segment */
+      int inside[64];
+      for (int iv = 0; iv + 1 < nroots; ++iv) {
+        const double tm = 0.5 * (roots[iv] + roots[iv + 1]);
+        const double xm[3] = {o[0] + tm * d[0], o[1] + tm * d[1], o[2] + tm * d[2]};
+        double rm[3];
+        const int itm = orc_invert(m, e, xm, rm);
+        inside[iv] = itm >= 0 && fmax(fabs(rm[0]), fmax(fabs(rm[1]), fabs(rm[2]))) <= 1.0 + 1e-9;
+      }
+      for (int iv = 0; iv + 1 < nroots; ++iv) {
+        if (!inside[iv]) continue;
+        int iw = iv;
+        while (iw + 2 < nroots && inside[iw + 1]) ++iw;
+        const double ta = roots[iv], tb = roots[iw + 1];
+        iv = iw;
         for (int k = 0; k < niso; ++k) {
           double* I = out_iso + ((size_t)r * niso + k) * ORC_ISO_STRIDE;
           if (I[0] >= 0 && I[1] <= ta) continue; /* an earlier root already */
-          double t, rst[3], u, mn;
-          if (orc_iso_segment(m, e, o, d, ta, tb, M, isos[k], &t, rst, &u, &mn)) {
-            if (t < I[1]) {
-              I[0] = e; I[1] = t; I[2] = rst[0]; I[3] = rst[1]; I[4] = rst[2]; I[5] = u;
-              I[6] = mn;
+          double t, rst[3], u, mn = 1e30;
+          const int okk = orc_iso_segment(m, e, o, d, ta, tb, M, isos[k], &t, rst, &u, &mn);
+          if (k < ORC_MAX_ISO_REC) {
+            if (nseg[k] < SEGREC) {
+              seg_ta[k][nseg[k]] = ta;
+              seg_mn[k][nseg[k]] = mn;
+              ++nseg[k];
+            } else {
+              seg_over[k] = fmin(seg_over[k], mn); /* (counted as before any root) */
             }
+          }
+          if (okk && t < I[1]) {
+            I[0] = e; I[1] = t; I[2] = rst[0]; I[3] = rst[1]; I[4] = rst[2]; I[5] = u;
+            I[6] = mn;
           }
         }
       }
     }
     free(cand);
+    for (int k = 0; k < niso && k < ORC_MAX_ISO_REC; ++k) {
+      double* I = out_iso + ((size_t)r * niso + k) * ORC_ISO_STRIDE;
+      double mn = fmin(I[6], seg_over[k]);
+      for (int q = 0; q < nseg[k]; ++q)
+        if (seg_ta[k][q] < I[1]) mn = fmin(mn, seg_mn[k][q]);
+      I[6] = mn;
+    }
     /* near miss: a rejected root at or before the hit (any, for a miss) */
     for (int q = 0; q < rej.n; ++q)
       if (G[0] < 0 || rej.t[q] <= G[2] * (1.0 + 1e-6) + 1e-12) G[9] = fmin(G[9], rej.mg[q]);

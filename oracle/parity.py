@@ -19,8 +19,12 @@ VAL_ABS = 1e-6
 # coordinate is within TIE_MARGIN of the acceptance bound
 TIE_T = 1e-6
 TIE_MARGIN = 1e-5
-# an isovalue is tie-classified in a ray when some sample of g = u - iso in the
-# element that holds the oracle's root is within this of zero
+# an isovalue is tie-classified in a ray when some scan sample of g = u - iso
+# before the oracle's first crossing (in any element the ray crossed; all of
+# them when there is none) is within this of zero, or within 4e-6 t of its
+# zero along the ray (|g| < 4e-6 t |dg/dt|: the FP32 scan places its samples
+# to a few 1e-7 t): the sign decisions there are not determined.  The oracle
+# reports both tests on one scale (minabs < TIE_ISO).
 TIE_ISO = 1e-5
 
 

@@ -20,15 +20,16 @@ OK = 0
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "ESTATE"}
 GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
 TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE, TRACE_NO_SAMPLE_SKIP = 1, 2, 4, 8   # trace flag bits
+TRACE_COOP_SCAN = 16
 MAX_ISO = 8
-NCOUNTERS = 32
+NCOUNTERS = 33
 COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",
                  "geo64_it", "field64_evals", "slab_tests", "mlp_infer", "elems_marched",
                  "exit_pred", "exit_search", "exit_march", "reentry", "redo", "brackets",
                  "walk_t_start", "walk_t_drained", "walk_t_end",
                  "len_1", "len_2_3", "len_4_7", "len_8_15", "len_16_31", "len_32_63", "len_64_127",
                  "len_128_up", "scan_skipped", "samples_skipped", "walks_stalled",
-                 "first_hit_face_it", "first_hit_slab_tests"]
+                 "first_hit_face_it", "first_hit_slab_tests", "walk_skips"]
 
 
 class FemrtError(RuntimeError):
@@ -637,8 +638,8 @@ class Frame:
         return dict(zip(COUNTER_NAMES, out.tolist()))
 
     def stalled_rays(self):
-        """(count, ids) of the walks of the last counted trace that ended
-        stalled; at most 64 ids are kept."""
+        """(count, ids) of the walks of the last counted trace that stalled
+        (skipped past a stall or ended stalled); at most 64 ids are kept."""
         ids = np.zeros(64, np.int32)
         n = lib().femrt_frame_stalled_rays(self.h, ids.size, _ptr(ids))
         if n < 0:

@@ -39,6 +39,7 @@
 
 using namespace femrt;
 
+#include "femrt_march.hpp"
 #include "femrt_mlp.hpp"
 #include "femrt_point.hpp"
 #include "femrt_runtime.hpp"
@@ -144,13 +145,28 @@ __device__ __forceinline__ int chord_bin(const TraceMesh& M, const float* o, con
   return x >= FEMRT_QBINS - 1 ? FEMRT_QBINS - 1 : (int)x;
 }
 
+// FP64 results of a polished hit.
+template <int NF>
+__device__ __forceinline__ void store_hit64(const TraceMesh& M, int n, int r, int e, int f, double u, double v,
+                                            double tau, double* __restrict__ hit_t, double* __restrict__ hit_rst,
+                                            double* __restrict__ hit_u)
+{
+  double rst[3];
+  face_to_rst_t<double>(f, u, v, rst);
+  hit_t[r] = tau;
+  hit_rst[r] = rst[0];
+  hit_rst[(size_t)n + r] = rst[1];
+  hit_rst[2 * (size_t)n + r] = rst[2];
+  hit_u[r] = field_value<double, NF>(M.field + (size_t)e * (NF * NF * NF), rst[0], rst[1], rst[2]);
+}
+
 template <int NG, int NF>
 __global__ __launch_bounds__(256) void k_polish_hit(
-    const float* __restrict__ rays, int n, TraceMesh M, const int* __restrict__ queue,
-    const int* __restrict__ counts, const int* __restrict__ hit_elem,
+    const float* __restrict__ rays, int n, TraceMesh M, float tmin, const int* __restrict__ queue,
+    int* __restrict__ counts, const int* __restrict__ hit_elem,
     const int* __restrict__ hit_face, const float4* __restrict__ hit_f32,
     double* __restrict__ hit_t, double* __restrict__ hit_rst, double* __restrict__ hit_u,
-    unsigned char* __restrict__ qbin, long long* __restrict__ work)
+    unsigned char* __restrict__ qbin, int* __restrict__ retrace, long long* __restrict__ work)
 {
   const int count = counts[0];
   Work w;
@@ -163,15 +179,54 @@ __global__ __launch_bounds__(256) void k_polish_hit(
     const int e = hit_elem[r], f = hit_face[r];
     const float4 h = hit_f32[r];
     double u = h.x, v = h.y, tau = h.z;
-    polish_hit<NG>(M.geom + (size_t)e * (NG * NG * NG), o, d, f, u, v, tau, w);
-    double rst[3];
-    face_to_rst_t<double>(f, u, v, rst);
-    hit_t[r] = tau;
-    hit_rst[r] = rst[0];
-    hit_rst[(size_t)n + r] = rst[1];
-    hit_rst[2 * (size_t)n + r] = rst[2];
-    hit_u[r] = field_value<double, NF>(M.field + (size_t)e * (NF * NF * NF), rst[0], rst[1], rst[2]);
-    ++w.field64;
+    if (polish_hit<NG>(M.geom + (size_t)e * (NG * NG * NG), o, d, f, u, v, tau, (double)tmin, w)) {
+      store_hit64<NF>(M, n, r, e, f, u, v, tau, hit_t, hit_rst, hit_u);
+      ++w.field64;
+    } else {
+      // no FP64 root on the face: a near miss, traced again by k_retrace
+      retrace[atomicAdd(&counts[FEMRT_RETRACE_CTR], 1)] = r;
+    }
+  }
+  flush_work(work, w);
+}
+
+// Rays whose FP32 hit had no FP64 root on its face (a grazing near miss that
+// the FP32 face Newton accepted within its residual tolerance): traced again
+// with that (element, face) excluded, and the new hit polished; up to four
+// exclusions, then the ray counts as a miss.  The walk starts from the new
+// FP32 hit; a ray left without a hit is skipped by the walk.
+template <int NG, int NF>
+__global__ __launch_bounds__(64) void k_retrace(const float* __restrict__ rays, int n, TraceMesh M, int guess,
+                                                float tmin, float tmax, const int* __restrict__ retrace,
+                                                const int* __restrict__ counts, int* __restrict__ hit_elem,
+                                                int* __restrict__ hit_face, float4* __restrict__ hit_f32,
+                                                double* __restrict__ hit_t, double* __restrict__ hit_rst,
+                                                double* __restrict__ hit_u, long long* __restrict__ work)
+{
+  const int count = counts[FEMRT_RETRACE_CTR];
+  Work w;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+    const int r = retrace[q];
+    float of[3], df[3];
+    load_ray(rays, n, r, of, df);
+    const double o[3] = {of[0], of[1], of[2]}, d[3] = {df[0], df[1], df[2]};
+    HitExcl ex;
+    ex.add(hit_elem[r], hit_face[r]);
+    int e = -1;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      const HitF32 h = first_hit_ray<NG>(M, of, df, guess, tmin, tmax, w, -1, &ex);
+      if (h.elem < 0) break;
+      double u = h.u, v = h.v, tau = h.t;
+      if (polish_hit<NG>(M.geom + (size_t)h.elem * (NG * NG * NG), o, d, h.face, u, v, tau, (double)tmin, w)) {
+        e = h.elem;
+        hit_face[r] = h.face;
+        hit_f32[r] = make_float4(h.u, h.v, h.t, 0.f);
+        store_hit64<NF>(M, n, r, h.elem, h.face, u, v, tau, hit_t, hit_rst, hit_u);
+        break;
+      }
+      ex.add(h.elem, h.face);
+    }
+    hit_elem[r] = e;
   }
   flush_work(work, w);
 }
@@ -236,202 +291,8 @@ __global__ __launch_bounds__(FEMRT_ORDER_THREADS) void k_walk_order(
   }
 }
 
-// Bracket output of the walk: FP32 bracket and estimate per (isovalue, ray).
-struct GlobalSink {
-  int* elem;
-  float4* f32;
-  float4* rst32;
-  size_t n;
-  int r;
-  __device__ void put(int k, const IsoBracket& b)
-  {
-    const size_t slot = (size_t)k * n + r;
-    elem[slot] = b.elem;
-    f32[slot] = make_float4(b.ta, b.tb, b.tm, b.ga);
-    rst32[slot] = make_float4(b.rst[0], b.rst[1], b.rst[2], 0.f);
-  }
-};
-
-// Network guesses of the interior samples of the lane's current segment.
-struct LdsGuess {
-  const _Float16* g;  // (samples - 1) x 3
-  int lead;           // samples 1..lead carry a guess
-  __device__ bool get(int j, float* rst) const
-  {
-    if (j > lead) return false;
-    rst[0] = (float)g[(j - 1) * 3 + 0];
-    rst[1] = (float)g[(j - 1) * 3 + 1];
-    rst[2] = (float)g[(j - 1) * 3 + 2];
-    return true;
-  }
-};
-
-// Orders this wave's LDS writes before its following LDS reads (lanes read
-// what other lanes of the same wave wrote).
-__device__ __forceinline__ void wave_lds_sync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-// Persistent element walk.  Each round every lane of a wave advances its ray
-// by one element segment (exit, samples, next element); a lane whose ray
-// ended takes the next hit ray from the queue (one atomic per wave), so the
-// lanes stay busy until the queue drains.  With LEARNED, the point-inversion
-// guesses of the first P.mlp_lead interior samples of the round's segments
-// come from the network, evaluated on the matrix cores for the whole wave
-// between the exit and sample steps (later samples extrapolate).
-// COUNT selects the work counters (validation runs) or none (timed runs).
-template <int NG, int NF, bool LEARNED, bool COUNT>
-__global__ __launch_bounds__(256, 2) void k_march(
-    const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ queue,
-    int* __restrict__ counts, const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
-    const float4* __restrict__ hit_f32, int* __restrict__ iso_elem, float4* __restrict__ iso_f32,
-    float4* __restrict__ iso_rst32, long long* __restrict__ work,
-    const void* __restrict__ mlp_pack, _Float16* __restrict__ mlp_g)
-{
-  const int count = counts[0];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  typename std::conditional<COUNT, Work, NoWork>::type w;
-  GlobalSink sink{iso_elem, iso_f32, iso_rst32, (size_t)n, 0};
-  bf16x8* ops = nullptr;
-  float* bias = nullptr;
-  bf16x8* xs = nullptr;
-  _Float16* G = nullptr;  // this lane's guesses, gs halfs per lane
-  int gs = 3;
-  if constexpr (LEARNED) {
-    __shared__ bf16x8 s_ops[PK_INFER_BF16 / 8];
-    __shared__ float s_bias[PK_NF32];
-    __shared__ bf16x8 s_xs[4][64 * 4];
-    __shared__ _Float16 s_g[4][64 * 3];
-    const bf16x8* gp = (const bf16x8*)mlp_pack;
-    for (int t = threadIdx.x; t < PK_INFER_BF16 / 8; t += blockDim.x) s_ops[t] = gp[t];
-    const float* gf = (const float*)((const __bf16*)mlp_pack + PK_NBF16);
-    for (int t = threadIdx.x; t < PK_NF32; t += blockDim.x) s_bias[t] = gf[t];
-    __syncthreads();
-    ops = s_ops;
-    bias = s_bias;
-    xs = s_xs[wave];
-    if (P.mlp_lead > 1) {
-      // every interior sample guessed: per-lane rows of the global buffer
-      // (the LDS holds one guess per lane, which keeps three blocks per CU)
-      gs = (FEMRT_MLP_MAX_SAMPLES - 1) * 3;
-      G = mlp_g + ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * gs;
-    } else {
-      G = &s_g[wave][lane * 3];
-    }
-  }
-  bool active = false;
-  float o[3] = {0.f, 0.f, 0.f}, d[3] = {1.f, 1.f, 1.f}, id[3] = {1.f, 1.f, 1.f};
-  MarchState S;
-  march_begin(S, HitF32());
-  // The hit queue is split into FEMRT_QSEG contiguous segments (tile order,
-  // so each covers one band of the image) with a counter each.  A block
-  // starts on the segment of its XCD (workgroups are dispatched round-robin
-  // over the XCDs) so that each XCD's L2 holds the elements of one band; a
-  // wave that finds its segment drained moves on to the next one.
-  int seg = blockIdx.x % FEMRT_QSEG, drained = 0;  // wave-uniform
-  // timeline with the work counters: start (min), queue dry (min), end (max)
-  const bool timeline = work != nullptr && lane == 0;
-  if (timeline) atomicMin((unsigned long long*)&work[16], (unsigned long long)wall_clock64());
-  bool saw_dry = false;
-  for (;;) {
-    bool need = !active;
-    while (drained < FEMRT_QSEG && __any(need)) {
-      const int lo = (int)((long long)count * seg / FEMRT_QSEG);
-      const int hi = (int)((long long)count * (seg + 1) / FEMRT_QSEG);
-      const int q = lo + wave_append(need, &counts[FEMRT_QSEG_CTR + seg]);
-      const bool got = need && q < hi;
-      if (got) {
-        const int r = queue[q];
-        sink.r = r;
-        load_ray(rays, n, r, o, d);
-        for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
-        HitF32 h;
-        h.elem = hit_elem[r];
-        h.face = hit_face[r];
-        const float4 hf = hit_f32[r];
-        h.u = hf.x;
-        h.v = hf.y;
-        h.t = hf.z;
-        march_begin(S, h);
-        active = true;
-      }
-      if (__any(need && !got)) {
-        seg = seg + 1 == FEMRT_QSEG ? 0 : seg + 1;
-        ++drained;
-      }
-      need = need && !got;
-    }
-    if (timeline && drained == FEMRT_QSEG && !saw_dry) {
-      saw_dry = true;
-      atomicMin((unsigned long long*)&work[17], (unsigned long long)wall_clock64());
-    }
-    if (!__any(active)) break;
-    if (active) march_exit<NG>(M, P, o, d, id, S, w);
-    if constexpr (LEARNED) {
-      // every lane of the wave is here: the network runs on the whole wave
-      const bool seg = active && S.ok;
-      const float* bx = M.boxes + (size_t)(seg ? S.e : 0) * FEMRT_BOX_STRIDE;
-      const float* vt = M.verts + (size_t)(seg ? S.e : 0) * 24;
-      const int jmax = min(P.mlp_lead, P.samples - 1);
-      for (int j = 1; j <= jmax; ++j) {
-        float f[MLP_IN];
-        if (seg) {
-          const float t = march_sample_t(S, j, P.samples);
-          const float x[3] = {fmaf(t, d[0], o[0]), fmaf(t, d[1], o[1]), fmaf(t, d[2], o[2])};
-          mlp_features(bx, vt, x, f);
-          ++w.mlp;
-        } else {
-          for (int k = 0; k < MLP_IN; ++k) f[k] = 0.f;
-        }
-        mlp_stage_inputs(xs, lane, f);
-        wave_lds_sync();
-        float y0[3], y1[3];
-        wave_mlp_forward(xs, ops, bias, bias + 64, y0, y1);
-        if (lane < 32) {
-          _Float16* g0 = &G[(j - 1) * 3];
-          _Float16* g1 = g0 + 32 * gs;
-          for (int c = 0; c < 3; ++c) {
-            g0[c] = (_Float16)y0[c];
-            g1[c] = (_Float16)y1[c];
-          }
-        }
-        wave_lds_sync();
-      }
-    }
-    if (active) {
-      bool redo;
-      if constexpr (LEARNED)
-        redo = S.ok && march_samples<NG, NF>(M, P, o, d, S, LdsGuess{G, P.mlp_lead}, sink, w);
-      else
-        redo = S.ok && march_samples<NG, NF>(M, P, o, d, S, NoGuess(), sink, w);
-      if (redo) {
-        S.redo = true;
-      } else if (!march_advance<NG>(M, P, o, d, S, w)) {
-        active = false;
-        if (work) {
-          const int len = max(1, S.step & 0xFFFF);
-          add_work(work, 19 + min(7, 31 - __clz(len)), 1);
-          if ((S.step >> 16) > FEMRT_MAX_MARCHED_EXITS) {
-            // a stalled walk: keep the ray id for inspection
-            const unsigned long long k = atomicAdd((unsigned long long*)&work[FEMRT_NCOUNTERS], 1ull);
-            if (k < FEMRT_MAX_STALLED_IDS) work[FEMRT_NCOUNTERS + 1 + k] = sink.r;
-          }
-        }
-        // a bracket written by a pass that was redone is not a result
-        for (int k = 0; k < P.n_iso; ++k)
-          if (!((S.found >> k) & 1u)) iso_elem[(size_t)k * n + sink.r] = -1;
-      }
-    }
-  }
-  if (timeline) atomicMax((unsigned long long*)&work[18], (unsigned long long)wall_clock64());
-  if constexpr (COUNT) flush_work(work, w);
-}
-
 template <int NG, int NF>
-__global__ __launch_bounds__(256, 3) void k_polish_iso(
+__global__ __launch_bounds__(256, 2) void k_polish_iso(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ hit_queue,
     const int* __restrict__ counts, const float4* __restrict__ iso_f32, const float4* __restrict__ iso_rst32,
     int* __restrict__ iso_elem, double* __restrict__ iso_t, double* __restrict__ iso_rst,
@@ -902,6 +763,7 @@ FEMRT_API int femrt_frame_create(femrt_ctx* c, int n, femrt_frame** out)
             hipMalloc(&f->d_iso_f32, K * N * 16) == hipSuccess &&
             hipMalloc(&f->d_iso_rst32, K * N * 16) == hipSuccess &&
             hipMalloc(&f->d_queue_hit, N * 4) == hipSuccess &&
+            hipMalloc(&f->d_retrace, N * 4) == hipSuccess &&
             hipMalloc(&f->d_queue_walk, N * 4) == hipSuccess &&
             hipMalloc(&f->d_qbin, N) == hipSuccess &&
             hipMalloc(&f->d_counts, FEMRT_NCOUNTS * 4) == hipSuccess &&
@@ -929,7 +791,7 @@ FEMRT_API int femrt_frame_destroy(femrt_frame* f)
   (void)hipStreamSynchronize(f->ctx->stream);
   void* ptrs[] = {f->d_rays, f->d_hit_elem, f->d_hit_face, f->d_hit_f32, f->d_hit_t, f->d_hit_rst,
                   f->d_hit_u, f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, f->d_iso_f32,
-                  f->d_iso_rst32, f->d_queue_hit, f->d_queue_walk, f->d_qbin, f->d_counts, f->d_rgba, f->d_work,
+                  f->d_iso_rst32, f->d_queue_hit, f->d_retrace, f->d_queue_walk, f->d_qbin, f->d_counts, f->d_rgba, f->d_work,
                   f->d_path_tp, f->d_path_acc, f->d_mats, f->d_mlp_g, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
                   f->d_walk_sorted, f->d_walk_reent, f->d_walk_hist};
   for (void* p : ptrs) (void)hipFree(p);
@@ -1093,8 +955,14 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
     FEMRT_DISPATCH_NF(m->NF, {
       rc = timed_launch(c, "k_polish_hit", [&] {
         k_polish_hit<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
-            f->d_rays, n, TM, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
-            f->d_hit_f32, f->d_hit_t, f->d_hit_rst, f->d_hit_u, f->d_qbin, work);
+            f->d_rays, n, TM, p->t_min, f->d_queue_hit, f->d_counts, f->d_hit_elem, f->d_hit_face,
+            f->d_hit_f32, f->d_hit_t, f->d_hit_rst, f->d_hit_u, f->d_qbin, f->d_retrace, work);
+      });
+      if (rc) break;
+      rc = timed_launch(c, "k_retrace", [&] {
+        k_retrace<NG_, NF_><<<64, 64, 0, c->stream>>>(f->d_rays, n, TM, p->guess, p->t_min, p->t_max,
+                                                      f->d_retrace, f->d_counts, f->d_hit_elem, f->d_hit_face,
+                                                      f->d_hit_f32, f->d_hit_t, f->d_hit_rst, f->d_hit_u, work);
       });
       if (rc) break;
       if (n_iso > 0 && !(p->flags & FEMRT_TRACE_BINNED_WALK)) {
@@ -1128,20 +996,35 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
               f->mlp_g_cap = need;
             }
           }
+          // the cooperative scan on request, for up to 16 samples per
+          // segment and guesses other than the network's (which start
+          // samples that have no earlier neighbour in the sequential scan)
+          const bool coop = p->samples <= FEMRT_COOP_MAX_SAMPLES && !learned &&
+                            (p->flags & FEMRT_TRACE_COOP_SCAN);
           rc = timed_launch(c, "k_march", [&] {
-            const void* pack = learned ? m->guess_mlp->d_pack : nullptr;
-            _Float16* g = learned ? f->d_mlp_g : nullptr;
-            auto go = [&](auto L, auto CNT) {
-              k_march<NG_, NF_, decltype(L)::value, decltype(CNT)::value><<<mblocks, 256, 0, c->stream>>>(
-                  f->d_rays, n, TM, ip, f->d_queue_walk, f->d_counts, f->d_hit_elem, f->d_hit_face,
-                  f->d_hit_f32, f->d_iso_elem, f->d_iso_f32, f->d_iso_rst32, work, pack, g);
-            };
-            using T1 = std::true_type;
-            using F0 = std::false_type;
-            if (learned && work) go(T1{}, T1{});
-            else if (learned) go(T1{}, F0{});
-            else if (work) go(F0{}, T1{});
-            else go(F0{}, F0{});
+            MarchArgs a;
+            a.blocks = mblocks;
+            a.stream = c->stream;
+            a.nf = m->NF;
+            a.learned = learned;
+            a.count = work != nullptr;
+            a.coop = coop;
+            a.rays = f->d_rays;
+            a.n = n;
+            a.M = TM;
+            a.P = ip;
+            a.queue = f->d_queue_walk;
+            a.counts = f->d_counts;
+            a.hit_elem = f->d_hit_elem;
+            a.hit_face = f->d_hit_face;
+            a.hit_f32 = f->d_hit_f32;
+            a.iso_elem = f->d_iso_elem;
+            a.iso_f32 = f->d_iso_f32;
+            a.iso_rst32 = f->d_iso_rst32;
+            a.work = work;
+            a.mlp_pack = learned ? m->guess_mlp->d_pack : nullptr;
+            a.mlp_g = learned ? f->d_mlp_g : nullptr;
+            march_launch<NG_>(a);
           });
         }
         if (rc) break;

@@ -1,0 +1,6 @@
+// femrt_march_ng4.hip -- the element walk kernels for 4 geometry modes per
+// direction (all field mode counts, learned/unlearned, counted/timed,
+// cooperative/sequential scan).
+#include "femrt_march.hpp"
+
+FEMRT_MARCH_LAUNCH_DEF(4)

@@ -95,6 +95,7 @@ struct femrt_frame {
   float4* d_iso_f32 = nullptr;  // MAX_ISO x n: (t_a, t_b, t*, packed signs)
   float4* d_iso_rst32 = nullptr;  // MAX_ISO x n: rst* (FP32)
   int* d_queue_hit = nullptr;     // n
+  int* d_retrace = nullptr;       // n: hit rays traced again (no FP64 root on the FP32 hit's face)
   int* d_queue_walk = nullptr;    // n: the hit queue in walk order (longest chords first per segment)
   unsigned char* d_qbin = nullptr;  // n: chord bin per hit-queue position
   int* d_counts = nullptr;        // FEMRT_NCOUNTS: [0] hits, [2..3] walk
@@ -209,6 +210,7 @@ __device__ __forceinline__ void flush_work(long long* work, const Work& w, bool 
   add_work(work, 27, w.skipped);
   add_work(work, 28, w.samples_skipped);
   add_work(work, 29, w.stalled);
+  add_work(work, 32, w.skips);
 }
 
 // Append `pred` lanes to a queue: one atomic per wave, ballot + prefix count
@@ -243,8 +245,10 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter)
   }
 
 #define FEMRT_MLP_MAX_SAMPLES 16
-// Frame queue counters: [0] hits, [2..3] binned-walk lists,
+// Frame queue counters: [0] hits, [2..3] binned-walk lists, [4] rays to
+// trace again (FEMRT_RETRACE_CTR), [5] binned-walk start list,
 // [FEMRT_QSEG_CTR + s] progress in segment s of the hit queue (persistent walk).
+#define FEMRT_RETRACE_CTR 4
 #define FEMRT_QSEG 8
 #define FEMRT_QSEG_CTR 16
 #define FEMRT_NCOUNTS 32

@@ -53,7 +53,8 @@ struct Work {
   unsigned exit_pred = 0, exit_search = 0, exit_march = 0, reentry = 0, redo = 0, brackets = 0;
   unsigned skipped = 0;  // segments whose samples the field bounds rule out
   unsigned samples_skipped = 0;  // scan samples whose signs the bounds certified
-  unsigned stalled = 0;  // walks ended after FEMRT_MAX_MARCHED_EXITS marched exits
+  unsigned stalled = 0;  // walks ended stalled (FEMRT_MAX_WALK_SKIPS stalls skipped already)
+  unsigned skips = 0;    // stalls skipped past (runs of FEMRT_MAX_MARCHED_EXITS marched exits)
 };
 
 // The same counters compiled away: the kernels of timed runs take NoWork,
@@ -65,7 +66,7 @@ struct NoCount {
 };
 struct NoWork {
   NoCount face_it, inv_it, field, geo64, field64, slab, mlp, marched;
-  NoCount exit_pred, exit_search, exit_march, reentry, redo, brackets, skipped, samples_skipped, stalled;
+  NoCount exit_pred, exit_search, exit_march, reentry, redo, brackets, skipped, samples_skipped, stalled, skips;
 };
 
 // FP32 result of the first-hit stage.
@@ -83,8 +84,11 @@ struct IsoBracket {
 };
 
 #define FEMRT_STACK 64
-// exits a walk may find by the marching fallback before it counts as stalled
+// exits in a row a walk may find by the marching fallback (with no exit found
+// by prediction or face search in between) before it counts as stalled, and
+// the stalls it may skip past before it ends
 #define FEMRT_MAX_MARCHED_EXITS 16
+#define FEMRT_MAX_WALK_SKIPS 8
 #define FEMRT_FACE_ACC 1e-4f
 // Boxes per element: [0] element, [1..6] faces (Nektar order), [7 + 9 f +
 // 3 b + a] sub-patch (a, b) of face f on the 3 x 3 split of its (u, v) square.
@@ -356,13 +360,33 @@ FEMRT_HD void load_ray(const float* __restrict__ rays, int n, int r, float* o, f
 
 // ------------------------------------------------------------- first hit
 
+// (element, face) pairs a traversal skips: false hits already ruled out.
+struct HitExcl {
+  int n = 0;
+  int e[4], f[4];
+  FEMRT_HD bool has(int el, int fa) const
+  {
+    for (int i = 0; i < 4; ++i)
+      if (i < n && e[i] == el && f[i] == fa) return true;
+    return false;
+  }
+  FEMRT_HD void add(int el, int fa)
+  {
+    if (n < 4) {
+      e[n] = el;
+      f[n] = fa;
+      ++n;
+    }
+  }
+};
+
 // BVH traversal over element boxes, near child first, short stack; per leaf
 // element: element box, then per face its box and the face root.  The
 // nearest accepted root over all elements (but `exclude`) and faces is the
 // hit.
 template <int NG, typename W>
 FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d, int guess,
-                              float tmin, float tmax, W& w, int exclude = -1)
+                              float tmin, float tmax, W& w, int exclude = -1, const HitExcl* excl = nullptr)
 {
   HitF32 h;
   float id[3];
@@ -392,6 +416,7 @@ FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d
         if (!slab(o, id, bx, bx + 3, tmin, best_t, e0, e1)) continue;
         const GeoMode* gc = M.geom + (size_t)e * (NG * NG * NG);
         for (int f = 0; f < 6; ++f) {
+          if (excl && excl->has(e, f)) continue;
           const float* fb = bx + 6 * (1 + f);
           float f0, f1;
           ++w.slab;
@@ -443,13 +468,21 @@ FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d
 }
 
 // FP64 face Newton polish of a hit (unconstrained, quadratic convergence from
-// the FP32 root).
+// the FP32 root; linear near a double root, hence up to 40 iterations).
+// Converged when the step drops below 1e-14, or once it is below 1e-10 and
+// stops halving (the FP64 rounding floor).  Returns whether it converged to
+// a root on the face (max(|u|,|v|) <= 1 + 1e-4) at tau >= tmin: an FP32 root
+// of a ray that passes the face within the FP32 residual tolerance without
+// meeting it (a grazing near miss) has no FP64 root, and the caller traces
+// such a ray again without this face.
 template <int NG, typename W>
-FEMRT_HD void polish_hit(const GeoMode* __restrict__ gc, const double* o, const double* d, int f,
-                         double& u, double& v, double& tau, W& w)
+FEMRT_HD bool polish_hit(const GeoMode* __restrict__ gc, const double* o, const double* d, int f,
+                         double& u, double& v, double& tau, double tmin, W& w)
 {
   const bool us = (f == 2 || f == 4), vs = (f == 0 || f == 5);
-  for (int it = 0; it < 4; ++it) {
+  double prev = 1e300;
+  bool conv = false;
+  for (int it = 0; it < 40; ++it) {
     double rst[3], X[3], J[9];
     face_to_rst_t<double>(f, u, v, rst);
     geom_eval<double, NG>(gc, rst[0], rst[1], rst[2], X, J);
@@ -465,8 +498,15 @@ FEMRT_HD void polish_hit(const GeoMode* __restrict__ gc, const double* o, const 
     u += dl[0];
     v += dl[1];
     tau += dl[2];
-    if (fabs(dl[0]) + fabs(dl[1]) < 1e-15 && fabs(dl[2]) < 1e-15 * (1.0 + fabs(tau))) break;
+    if (!(fabs(u) < 10.0 && fabs(v) < 10.0)) break;
+    const double st = fmax(fabs(dl[0]) + fabs(dl[1]), fabs(dl[2]) / (1.0 + fabs(tau)));
+    if (st < 1e-14 || (st < 1e-10 && st > 0.5 * prev)) {
+      conv = true;
+      break;
+    }
+    prev = st;
   }
+  return conv && fmax(fabs(u), fabs(v)) <= 1.0 + FEMRT_FACE_ACC && tau >= tmin;
 }
 
 // ------------------------------------------------------ point inversion
@@ -681,9 +721,16 @@ FEMRT_HD bool find_exit_predicted(const GeoMode* __restrict__ gc, const float* o
     fc[c] = -1;
     if (dl[c] == 0.f) continue;
     const float target = dl[c] > 0.f ? 1.f : -1.f;
-    const float sv = (target - rst_in[c]) / dl[c];
+    float sv = (target - rst_in[c]) / dl[c];
     const int f = c == 0 ? (target > 0 ? 2 : 4) : (c == 1 ? (target > 0 ? 3 : 1) : (target > 0 ? 5 : 0));
-    if (f == f_in || !(sv > 0.f)) continue;
+    if (f == f_in) continue;
+    if (!(sv > 0.f)) {
+      // the entry point lies on this face's plane already (a ray entering
+      // at an edge of the entry face, moving outward): the element is only
+      // clipped, and the exit through this face follows at once
+      if (!(fabsf(rst_in[c]) >= 0.999f)) continue;
+      sv = 0.f;
+    }
     sc[c] = sv;
     fc[c] = f;
   }
@@ -783,13 +830,19 @@ struct MarchState {
   float t_in, u_in, v_in;
   int f_out;              // exit of the current segment (valid when ok)
   float t_out, u_out, v_out;
-  int step;               // segments walked (low 16 bits), exits found by marching (bits 16+)
+  int step;               // segments walked (bits 0-15), exits found by marching in a row
+                          // (bits 16-23), stalls skipped (bits 24-30)
   unsigned found;         // isovalues whose bracket has been recorded
   bool ok;
   bool redo;              // samples showed a wrong exit: search again, entry face included
   bool have_m;            // m_in, m_out hold drst/dt at the entry and exit (predicted exits)
   float m_in[3], m_out[3];
 };
+
+// Whether the walk's last FEMRT_MAX_MARCHED_EXITS + 1 exits all came from the
+// marching fallback; march_skips(S): stalls skipped so far.
+FEMRT_HD bool march_stalled(const MarchState& S) { return ((S.step >> 16) & 0xFF) > FEMRT_MAX_MARCHED_EXITS; }
+FEMRT_HD int march_skips(const MarchState& S) { return (S.step >> 24) & 0x7F; }
 
 FEMRT_HD void march_begin(MarchState& S, const HitF32& h)
 {
@@ -819,12 +872,31 @@ FEMRT_HD void march_exit(const TraceMesh& M, const IsoParams& P, const float* o,
   const GeoMode* gc = M.geom + (size_t)S.e * (NG * NG * NG);
   const float* bx = M.boxes + (size_t)S.e * FEMRT_BOX_STRIDE;
   const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
-  // exits closer than eps to the entry are not resolved in FP32
-  const float t_lo = S.t_in + 1e-4f * ext;
+  // exits closer than eps (t_lo below) to the entry are not resolved in FP32
   float rst_in[3];
   face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
   bool ok = false;
   S.have_m = false;
+  if (march_stalled(S)) {
+    // The last exits all came from marching: the ray runs inside a face or
+    // along an edge that elements share, within FP32 resolution of it, and
+    // the elements on both sides hand it back and forth with hardly any
+    // advance.  The entry moves 1e-3 of the element along the ray (its (u, v)
+    // on the entry face from the point's preimage), and the walk goes on from
+    // there in this element.
+    S.step = (S.step & 0xFFFF) | ((march_skips(S) + 1) << 24);
+    ++w.skips;
+    S.t_in = fmaxf(S.t_in, S.t_in + 1e-3f * ext);
+    const float x[3] = {fmaf(S.t_in, d[0], o[0]), fmaf(S.t_in, d[1], o[1]), fmaf(S.t_in, d[2], o[2])};
+    float r[3] = {rst_in[0], rst_in[1], rst_in[2]}, J[9];
+    if (invert_in_cube_f32<NG>(gc, x, r, J, w.inv_it)) {
+      rst_to_face(S.f_in, r, S.u_in, S.v_in);
+      S.u_in = fminf(fmaxf(S.u_in, -1.f), 1.f);
+      S.v_in = fminf(fmaxf(S.v_in, -1.f), 1.f);
+    }
+    face_to_rst(S.f_in, S.u_in, S.v_in, rst_in[0], rst_in[1], rst_in[2]);
+  }
+  const float t_lo = S.t_in + 1e-4f * ext;
   if (!S.redo) {
     ++w.marched;
     if (P.guess != FEMRT_GUESS_PLAIN) {
@@ -846,14 +918,13 @@ FEMRT_HD void march_exit(const TraceMesh& M, const IsoParams& P, const float* o,
                               S.u_out, S.v_out, w);
     w.exit_search += ok;
   }
-  if (!ok) {
+  if (ok) {
+    S.step &= ~(0xFF << 16);  // the run of marched exits ends
+  } else {
     ok = find_exit_marching<NG>(gc, bx, o, d, id, rst_in, S.t_in, t_lo, S.f_out, S.t_out, S.u_out,
                                 S.v_out, w);
     w.exit_march += ok;
-    if (ok) {
-      S.step += 1 << 16;
-      if ((S.step >> 16) == FEMRT_MAX_MARCHED_EXITS + 1) ++w.stalled;
-    }
+    if (ok && ((S.step >> 16) & 0xFF) < 0xFF) S.step += 1 << 16;
   }
   S.ok = ok;
 }
@@ -1376,13 +1447,16 @@ FEMRT_HD int march_next(const TraceMesh& M, const IsoParams& P, MarchState& S)
 {
   const unsigned all = (1u << P.n_iso) - 1u;
   S.redo = false;
-  // a walk that keeps needing the marching fallback for its exits has
-  // stalled between elements (each such exit advances the ray by as little
-  // as 1e-4 of the element and costs thousands of inversion iterations): it
-  // ends there
+  // a walk whose exits keep coming from the marching fallback, with no exit
+  // found by prediction or face search in between, has stalled between
+  // elements (each such exit can advance the ray by as little as 1e-4 of the
+  // element and costs thousands of inversion iterations)
   const int seg = (S.step & 0xFFFF) + 1;
   S.step = (S.step & ~0xFFFF) | seg;
-  if (S.found == all || seg >= P.max_march || (S.step >> 16) > FEMRT_MAX_MARCHED_EXITS) return MARCH_DONE;
+  if (S.found == all || seg >= P.max_march) return MARCH_DONE;
+  // a stalled walk skips ahead in its next element (march_exit); with its
+  // skips used up it ends
+  if (march_stalled(S) && march_skips(S) >= FEMRT_MAX_WALK_SKIPS) return MARCH_DONE;
   if (S.ok && M.adj) {
     const int ne = M.adj[(size_t)S.e * 12 + S.f_out * 2];
     const int code = M.adj[(size_t)S.e * 12 + S.f_out * 2 + 1];
@@ -1408,17 +1482,23 @@ FEMRT_HD int march_next(const TraceMesh& M, const IsoParams& P, MarchState& S)
   return MARCH_REENTER;
 }
 
-// Re-entry after march_next returned MARCH_REENTER: the nearest element the
-// ray meets beyond the last resolved point, other than the current one, by
-// the same traversal as the first hit.  Returns false when there is none.
+// Re-entry after march_next returned MARCH_REENTER: the nearest element face
+// the ray meets beyond the last resolved point, by the same traversal as the
+// first hit.  The current element is not excluded -- a ray that leaves
+// through a curved boundary face can re-enter the same element through it
+// further on -- so the search starts 1e-4 of the element beyond the point
+// (as the exit search does), where the exit's own root no longer counts.
+// Returns false when there is none.
 template <int NG, typename W>
 FEMRT_HD bool march_reenter(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
                             MarchState& S, W& w)
 {
   const float t_from = S.ok ? S.t_out : S.t_in;
+  const float* bx = M.boxes + (size_t)S.e * FEMRT_BOX_STRIDE;
+  const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
   ++w.reentry;
-  const HitF32 h2 =
-      first_hit_ray<NG>(M, o, d, P.guess, t_from + 1e-6f * (1.f + fabsf(t_from)), 1e30f, w, S.e);
+  const HitF32 h2 = first_hit_ray<NG>(M, o, d, P.guess,
+                                      t_from + fmaxf(1e-4f * ext, 1e-6f * (1.f + fabsf(t_from))), 1e30f, w);
   if (h2.elem < 0) return false;
   S.e = h2.elem;
   S.f_in = h2.face;
@@ -1435,7 +1515,10 @@ FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float*
                             MarchState& S, W& w)
 {
   const int k = march_next(M, P, S);
-  if (k == MARCH_DONE) return false;
+  if (k == MARCH_DONE) {
+    if (march_stalled(S) && march_skips(S) >= FEMRT_MAX_WALK_SKIPS) ++w.stalled;
+    return false;
+  }
   if (k == MARCH_NEXT) return true;
   return march_reenter<NG>(M, P, o, d, S, w);
 }
@@ -1557,17 +1640,35 @@ FEMRT_HD bool polish_iso(const TraceMesh& M, const double* o, const double* d, d
   double lo = br.ta, hi = br.tb, t = br.tm;
   const bool lo_neg = br.ga < 0.f;
   double rst[3] = {br.rst[0], br.rst[1], br.rst[2]}, J[9];
-  double u = 0.0;
+  double u = 0.0, g_last = 0.0, gp_last = 0.0;
   bool ok = false;
+  // FP64 preimage of x in the element: warm from the last iterate, else
+  // from the bracket's FP32 estimate, else from the octant centres
+  auto locate = [&](const double* x) -> bool {
+    if (invert_f64<NG>(gc, x, rst, J, w.geo64)) return true;
+    for (int s = -1; s < 8; ++s) {
+      for (int c = 0; c < 3; ++c)
+        rst[c] = s < 0 ? (double)br.rst[c] : (((s >> c) & 1) ? 0.5 : -0.5);
+      if (invert_f64<NG>(gc, x, rst, J, w.geo64) &&
+          fmax(fabs(rst[0]), fmax(fabs(rst[1]), fabs(rst[2]))) <= 1.001)
+        return true;
+    }
+    return false;
+  };
   for (int it = 0; it < 60; ++it) {
     const double x[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
-    if (!invert_f64<NG>(gc, x, rst, J, w.geo64)) break;
+    if (!locate(x)) {
+      ok = false;  // no root is reported from an iterate that is not converged
+      break;
+    }
     double gref[3], gw[3];
     field_eval<double, NF>(fc, rst[0], rst[1], rst[2], &u, gref);
     ++w.field64;
     grad_world<double>(J, gref, gw);
     const double g = u - iso;
     const double gp = gw[0] * d[0] + gw[1] * d[1] + gw[2] * d[2];
+    g_last = g;
+    gp_last = gp;
     ok = true;
     t_out = t;
     for (int c = 0; c < 3; ++c) rst_out[c] = rst[c];
@@ -1580,15 +1681,20 @@ FEMRT_HD bool polish_iso(const TraceMesh& M, const double* o, const double* d, d
     t = tn;
     if (step < 1e-13 * (1.0 + fabs(t)) || (hi - lo) < 1e-13 * (1.0 + fabs(t))) {
       const double x2[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
-      if (invert_f64<NG>(gc, x2, rst, J, w.geo64)) {
+      if (locate(x2)) {
         u_out = field_value<double, NF>(fc, rst[0], rst[1], rst[2]);
         ++w.field64;
         t_out = t;
         for (int c = 0; c < 3; ++c) rst_out[c] = rst[c];
+        g_last = u_out - iso;
       }
       break;
     }
   }
+  // a root only where g vanishes to the distance scale: a bracket that the
+  // FP32 scan saw change sign at a tangency, but whose FP64 values keep one
+  // sign, narrows onto one of its ends with g away from zero -- no crossing
+  if (ok && !(g_last == 0.0 || fabs(g_last) <= 1e-9 * (1.0 + fabs(t_out)) * fabs(gp_last))) ok = false;
   return ok;
 }
 

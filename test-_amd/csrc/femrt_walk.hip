@@ -98,7 +98,7 @@ __device__ __forceinline__ void finish_ray(bool fin, int r, unsigned found, int 
 
 }  // namespace
 
-__global__ void k_walk_init(const int* __restrict__ counts, const int* __restrict__ queue_hit,
+__global__ void k_walk_init(int* __restrict__ counts, const int* __restrict__ queue_hit,
                             const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
                             const float4* __restrict__ hit_f32, int4* __restrict__ si,
                             float4* __restrict__ sf, int* __restrict__ list)
@@ -106,10 +106,11 @@ __global__ void k_walk_init(const int* __restrict__ counts, const int* __restric
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= counts[0]) return;
   const int r = queue_hit[q];
+  if (hit_elem[r] < 0) return;  // a false hit that the re-trace could not replace
   const float4 h = hit_f32[r];
   si[r] = make_int4(hit_elem[r], hit_face[r], 0, 0);
   sf[r] = make_float4(h.z, h.x, h.y, 0.f);
-  list[q] = r;
+  list[atomicAdd(&counts[5], 1)] = r;
 }
 
 __global__ void k_walk_zero(int* __restrict__ a, int n)
@@ -321,7 +322,7 @@ int walk_run(femrt_ctx* c, femrt_mesh* m, femrt_frame* f, const TraceMesh& TM, c
   });
   if (rc) return rc;
   int n_act = 0;
-  HIPCHK(c, hipMemcpyAsync(&n_act, f->d_counts, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&n_act, f->d_counts + 5, 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int* cur = f->d_walk_a;
   int* nxt = f->d_walk_b;

@@ -13,7 +13,12 @@ oracle would take hours over the whole frame), and the whole frame is
 checked by a size-independent property: the learned, warm-start and plain
 Newton variants start their FP32 iterations from different points but must
 converge to the same FP64 results on every ray, ties aside; so must the
-scan that evaluates every sample (no certified skipping).
+scan that evaluates every sample (no certified skipping).  Every ray where
+two variants disagree is then traced by the oracle too, and both variants
+must meet the bar there (ties aside); so must every ray whose walk stalled,
+and the 64^3 P=8 frame's rays whose walks once ended stalled.  (On that
+frame the FP64 oracle needs seconds per ray, so its disagreements are
+checked on an even subsample of 240.)
 """
 from __future__ import annotations
 
@@ -33,7 +38,7 @@ MAX_GEOM_DIFF_FRAC = 1e-4
 MAX_ISO_DIFF_FRAC = 2e-3
 
 
-def _sample_check(oracle, hm, fr, isos, seed):
+def _sample_check(oracle, hm, fr, isos, seed, extra=()):
     import oracle.parity as parity
     gh, giso, rays = fr.hits(), fr.iso(len(isos)), fr.get_rays()
     rng = np.random.default_rng(seed)
@@ -41,12 +46,60 @@ def _sample_check(oracle, hm, fr, isos, seed):
     miss = np.flatnonzero(gh["elem"] < 0)
     sel = np.concatenate([rng.choice(hit, min(N_SAMPLE_HIT, hit.size), replace=False),
                           rng.choice(miss, min(N_SAMPLE_MISS, miss.size), replace=False)])
+    sel = np.union1d(sel, np.asarray(extra, np.int64))
 
     def sub(d):
         return {k: (v[..., sel] if isinstance(v, np.ndarray) else v) for k, v in d.items()}
     og, oiso = oracle.Oracle.from_mesh(hm).trace(np.ascontiguousarray(rays[:, sel]), isos)
     rep = parity.compare(sub(gh), sub(giso), og, oiso, isos)
     return rep, gh, giso
+
+
+MAX_ORACLE_DISAGREE = 2000
+
+
+def _disagreement_vs_oracle(oracle, hm, rays, isos, traces, ids, cap=MAX_ORACLE_DISAGREE):
+    """Every ray where two GPU traces disagree (at most `cap` of them, an even
+    subsample beyond that) against the oracle: each trace must meet the bar
+    there, ties aside (a tie is where the FP32 decision is not determined:
+    two faces within 1e-6 t, a face coordinate at the acceptance bound, or a
+    scan sample of u - iso within the oracle's tie margin of zero)."""
+    import oracle.parity as parity
+    ids = np.asarray(ids, np.int64)
+    total = int(ids.size)
+    if ids.size > cap:
+        ids = ids[np.linspace(0, ids.size - 1, cap).astype(np.int64)]
+    if ids.size == 0:
+        return {"disagreeing_rays": 0}
+    orc = oracle.Oracle.from_mesh(hm)
+    parts = []
+    for c0 in range(0, ids.size, 200):   # in chunks, reporting progress
+        parts.append(orc.trace(np.ascontiguousarray(rays[:, ids[c0:c0 + 200]]), isos))
+        print(json.dumps({"oracle_rays_done": min(c0 + 200, int(ids.size)), "of": int(ids.size)}), flush=True)
+    og = {k: np.concatenate([p[0][k] for p in parts], axis=-1) for k in parts[0][0]}
+    oi = {k: np.concatenate([p[1][k] for p in parts], axis=-1) for k in parts[0][1]}
+    out = {"disagreeing_rays": total, "checked": int(ids.size)}
+    for name, (gh, giso) in traces.items():
+        sub = lambda d: {k: (v[..., ids] if isinstance(v, np.ndarray) else v) for k, v in d.items()}  # noqa: E731
+        rep = parity.compare(sub(gh), sub(giso), og, oi, isos)
+        out[name] = {"ok": rep["ok"], "ties": rep["ties"], "hit_miss_mismatch": rep["hit_miss_mismatch"],
+                     "id_mismatch": rep["id_mismatch"],
+                     "iso": [(r["found_mismatch"], r["elem_mismatch"], r["t_rel_bad_nontie"], r["ties"])
+                             for r in rep["iso"]]}
+    return out
+
+
+def _disagreeing_rays(a_hits, a_iso, b_hits, b_iso):
+    ha, hb = a_hits["elem"] >= 0, b_hits["elem"] >= 0
+    both = ha & hb
+    trel = np.abs(a_hits["t"] - b_hits["t"]) / np.maximum(np.abs(b_hits["t"]), 1e-300)
+    bad = (ha != hb) | (both & ((a_hits["elem"] != b_hits["elem"]) | (a_hits["face"] != b_hits["face"]) |
+                                (trel > SAME_T_REL)))
+    for k in range(a_iso["elem"].shape[0]):
+        fa, fb = a_iso["elem"][k] >= 0, b_iso["elem"][k] >= 0
+        tr = np.abs(a_iso["t"][k] - b_iso["t"][k]) / np.maximum(np.abs(b_iso["t"][k]), 1e-300)
+        bad |= (fa != fb) | (fa & fb & ((a_iso["elem"][k] != b_iso["elem"][k]) | (tr > SAME_T_REL)))
+    return np.flatnonzero(bad)
 
 
 def _frame_agreement(a_hits, a_iso, b_hits, b_iso):
@@ -83,7 +136,8 @@ def _trained_mlp(femrt, ctx, pert, amp, seed):
     return mlp
 
 
-def _run(femrt, oracle, ctx, hm, rays_fn, n_rays, isos, pert, amp, seed, variants):
+def _run(femrt, oracle, ctx, hm, rays_fn, n_rays, isos, pert, amp, seed, variants, extra=(),
+         oracle_cap=MAX_ORACLE_DISAGREE):
     mesh = femrt.Mesh(ctx, hm)
     mlp = _trained_mlp(femrt, ctx, pert, amp, seed)
     mesh.set_guess_mlp(mlp)
@@ -92,20 +146,31 @@ def _run(femrt, oracle, ctx, hm, rays_fn, n_rays, isos, pert, amp, seed, variant
     fr.counters(True)
     fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_LEARNED))
     cnt = fr.counters(False)
-    rep, gh, giso = _sample_check(oracle, hm, fr, isos, seed)
+    n_st, st_ids = fr.stalled_rays()
+    rep, gh, giso = _sample_check(oracle, hm, fr, isos, seed, np.union1d(np.asarray(extra, np.int64), st_ids))
     print(json.dumps({"learned_vs_oracle": rep, "counters": {k: cnt[k] for k in ("hit_rays", "mlp_infer",
-                                                                                     "brackets")}}),
+                                                                                     "brackets", "walks_stalled",
+                                                                                     "walk_skips")}}),
           flush=True)
     assert rep["ok"], rep
     assert rep["oracle_hits"] >= N_SAMPLE_HIT - 5
-    assert cnt["mlp_infer"] > 0
+    assert cnt["walks_stalled"] == 0
+    rays = fr.get_rays()
     for name in variants:
         guess, flags = name if isinstance(name, tuple) else (name, 1)
         fr.trace(mesh, femrt.trace_params(isos, guess=guess, flags=flags))
-        agree = _frame_agreement(gh, giso, fr.hits(), fr.iso(len(isos)))
-        print(json.dumps({"learned_vs_variant": [guess, flags], **agree}), flush=True)
+        vh, vi = fr.hits(), fr.iso(len(isos))
+        agree = _frame_agreement(gh, giso, vh, vi)
+        dis = _disagreement_vs_oracle(oracle, hm, rays, isos, {"learned": (gh, giso), "variant": (vh, vi)},
+                                      _disagreeing_rays(gh, giso, vh, vi), oracle_cap)
+        print(json.dumps({"learned_vs_variant": [guess, flags], **agree, "disagreements_vs_oracle": dis}),
+              flush=True)
         assert agree["geom_diff_frac"] <= MAX_GEOM_DIFF_FRAC, agree
         assert all(r["diff_frac"] <= MAX_ISO_DIFF_FRAC for r in agree["iso"]), agree
+        # the plain variant's face roots are the reference's nine seeds, which
+        # miss the nearer crossing of a face crossed twice: reported, not gated
+        gated = ("learned",) if guess == femrt.GUESS_PLAIN else ("learned", "variant")
+        assert all(dis[k]["ok"] for k in gated if k in dis), dis
     for o in (fr, mesh, mlp):
         o.close()
     return rep, cnt
@@ -117,8 +182,8 @@ def test_bench_frame_32cube_p6_1080p(femrt, oracle, gpu_ctx):
     W, H = 1920, 1080
     isos = [-0.1, 0.0, 0.1]
     rep, cnt = _run(femrt, oracle, gpu_ctx, hm, lambda fr: fr.camera_rays(cam, W, H), W * H, isos,
-                    2, 0.08, 101, [femrt.GUESS_WARM, femrt.GUESS_PLAIN,
-                                   (femrt.GUESS_LEARNED, 1 | femrt.TRACE_NO_SAMPLE_SKIP)])
+                    2, 0.08, 101, [femrt.GUESS_WARM, (femrt.GUESS_WARM, 1 | femrt.TRACE_COOP_SCAN),
+                                   femrt.GUESS_PLAIN, (femrt.GUESS_LEARNED, 1 | femrt.TRACE_NO_SAMPLE_SKIP)])
     assert all(r["oracle_found"] > 100 for r in rep["iso"])
 
 
@@ -142,18 +207,29 @@ def test_multifrequency_64cube_p8_four_isos_16spp(femrt, oracle, gpu_ctx):
     cam12 = femrt.camera_basis(cam[:3], cam[3:6], cam[6:9], float(cam[9]), W / H)
     isos = [-0.6, -0.2, 0.2, 0.6]
     rep, cnt = _run(femrt, oracle, gpu_ctx, hm, lambda fr: fr.camera_rays_jitter(cam12, W, H, spp, 507),
-                    W * H * spp, isos, 3, 0.15, 303, [femrt.GUESS_WARM])
+                    W * H * spp, isos, 3, 0.15, 303, [femrt.GUESS_WARM],
+                    extra=_stalled_fixture("p8_1920x1080x16"), oracle_cap=240)
     assert all(r["oracle_found"] > 60 for r in rep["iso"])
 
 
-def test_stalled_walks_end_without_changing_results(femrt, gpu_ctx):
-    """Regression: on this 64^3 P=8 frame one ray's walk stalled between
-    elements (every exit from the marching fallback, advancing by 1e-4 of an
-    element) and walked to the 4096-segment cap in about a minute.  Such walks
-    now end after 16 marched exits: the frame is counted with stalled walks,
-    takes seconds, and gives the same crossings as with the walk capped at
-    256 segments."""
+def _stalled_fixture(name):
+    import pathlib
+    d = json.loads((pathlib.Path(__file__).parent / "golden" / "stalled_walk_rays.json").read_text())
+    return np.array(d[name]["ids"], np.int64)
+
+
+def test_formerly_stalled_walks_finish_and_match_oracle(femrt, oracle, gpu_ctx):
+    """Regression: on this 64^3 P=8 frame eight walks used to end as
+    stalled: the rays run through element edges or inside faces two
+    elements share, where FP32 face roots cannot resolve the exit and the
+    marching fallback finds it (a walk that ran into 16 marched exits in
+    total ended there; one ray, inside a shared face, was handed back and
+    forth between two elements).  Now only an unbroken run of 16 marched
+    exits counts, a stall is skipped past in the element, and no walk ends
+    stalled; the rays' crossings match the oracle (whose segments between
+    face roots are classified by their midpoints, see test_oracle_brute)."""
     import time
+    import oracle.parity as parity
     base = femrt.gen_structured_hex(64, 4, 2, pert=3, amp=0.15, seed=20240611)
     hm = femrt.gen_field_waves(base, 8, n_waves=8, seed=20240611 + 4)
     mesh = femrt.Mesh(gpu_ctx, hm)
@@ -162,23 +238,25 @@ def test_stalled_walks_end_without_changing_results(femrt, gpu_ctx):
     cam12 = femrt.camera_basis(cam[:3], cam[3:6], cam[6:9], float(cam[9]), W / H)
     fr = femrt.Frame(gpu_ctx, W * H * spp)
     isos = [-0.6, -0.2, 0.2, 0.6]
-    out = {}
-    for mm in (4096, 256):
-        fr.camera_rays_jitter(cam12, W, H, spp, 7)
-        fr.counters(True)
-        gpu_ctx.sync()
-        t0 = time.perf_counter()
-        fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_WARM, max_march=mm))
-        gpu_ctx.sync()
-        dt = time.perf_counter() - t0
-        cnt = fr.counters(False)
-        out[mm] = fr.iso(len(isos))
-        print(json.dumps({"max_march": mm, "s": dt, "walks_stalled": cnt["walks_stalled"]}), flush=True)
-        if mm == 4096:
-            assert cnt["walks_stalled"] >= 1
-            assert dt < 15.0
-    for k in range(len(isos)):
-        np.testing.assert_array_equal(out[4096]["elem"][k], out[256]["elem"][k])
-        np.testing.assert_array_equal(out[4096]["t"][k], out[256]["t"][k])
+    fr.camera_rays_jitter(cam12, W, H, spp, 7)
+    fr.counters(True)
+    gpu_ctx.sync()
+    t0 = time.perf_counter()
+    fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_WARM))
+    gpu_ctx.sync()
+    dt = time.perf_counter() - t0
+    cnt = fr.counters(False)
+    n_st, ids = fr.stalled_rays()
+    print(json.dumps({"s": dt, "walks_stalled": cnt["walks_stalled"], "walk_skips": cnt["walk_skips"],
+                      "skipping_rays": int(n_st), "exit_march": cnt["exit_march"]}), flush=True)
+    assert cnt["walks_stalled"] == 0
+    assert dt < 15.0
+    sel = np.union1d(_stalled_fixture("regression_960x540x16"), ids)
+    gh, giso, rays = fr.hits(), fr.iso(len(isos)), fr.get_rays()
+    og, oi = oracle.Oracle.from_mesh(hm).trace(np.ascontiguousarray(rays[:, sel]), isos)
+    sub = lambda d: {k: (v[..., sel] if isinstance(v, np.ndarray) else v) for k, v in d.items()}  # noqa: E731
+    rep = parity.compare(sub(gh), sub(giso), og, oi, isos)
+    print(json.dumps({"formerly_stalled_vs_oracle": rep}), flush=True)
+    assert rep["ok"], rep
     fr.close()
     mesh.close()

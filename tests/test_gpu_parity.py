@@ -78,19 +78,21 @@ def _trace_and_compare(femrt, oracle, ctx, hm, W, H, isos, guess, flags=1):
     return rep
 
 
-@pytest.mark.parametrize("guess", [0, 1])
-def test_trace_config1_geometry_and_iso(femrt, oracle, gpu_ctx, guess):
+@pytest.mark.parametrize("guess,flags", [(0, 1), (1, 1), (0, 17), (1, 17)],
+                         ids=["plain", "warm", "plain_coop", "warm_coop"])
+def test_trace_config1_geometry_and_iso(femrt, oracle, gpu_ctx, guess, flags):
     hm = _mesh_c1(femrt)
     isos = [0.0]
-    rep = _trace_and_compare(femrt, oracle, gpu_ctx, hm, 64, 48, isos, guess)
+    rep = _trace_and_compare(femrt, oracle, gpu_ctx, hm, 64, 48, isos, guess, flags)
     print(json.dumps(rep, indent=1))
     assert rep["oracle_hits"] > 200
     assert rep["ok"], rep
 
 
-@pytest.mark.parametrize("flags", [1, 3], ids=["persistent", "binned"])
+@pytest.mark.parametrize("flags", [1, 3, 17], ids=["persistent", "binned", "persistent_coop"])
 def test_trace_three_isovalues_curved_q4(femrt, oracle, gpu_ctx, flags):
-    """Both walk schedules: one persistent kernel, element-binned rounds."""
+    """The walk schedules: one persistent kernel (sequential or cooperative
+    scan), element-binned rounds."""
     hm = femrt.gen_structured_hex(4, 4, 6, pert=2, amp=0.08, seed=99)
     isos = [-0.1, 0.0, 0.1]
     rep = _trace_and_compare(femrt, oracle, gpu_ctx, hm, 48, 32, isos, 1, flags)

@@ -164,11 +164,21 @@ def test_training_reduces_loss_and_guess_error(femrt, gpu_ctx):
 
 
 @pytest.mark.gpu
-def test_learned_guess_keeps_parity_and_saves_newton_iterations(femrt, oracle, gpu_ctx):
-    """The element walk with network guesses gives the oracle's results, and
-    the trained network needs fewer point-inversion iterations than the same
-    walk with the network replaced by the plain guess rst = 0 (all-zero
-    parameters).  Trained on one seeded mesh, evaluated on another."""
+def test_learned_guess_keeps_parity_and_what_it_gains(femrt, oracle, gpu_ctx):
+    """Every walk form with network guesses gives the oracle's results.  What
+    the trained network gains:
+
+    * over the plain guess rst = 0 (the network with all-zero parameters):
+      fewer point-inversion iterations when it guesses every sample, and when
+      it guesses each segment's first sample of the sequential scan;
+    * over the engineered warm starts of the same walk: nothing measurable.
+      In the sequential scan the network's first-sample guess (median error
+      about 0.05 in reference units) is no better than the extrapolated warm
+      start, and the cooperative scan (warm guesses only) starts every sample
+      from the cubic Hermite interpolant of rst(t) (error O(dt^4)).  The test
+      asserts parity and that the network costs no iterations over warm
+      starts in the sequential scan, and prints the ratios.
+    Trained on one seeded mesh, evaluated on another."""
     import oracle.parity as parity
     train_hm = femrt.gen_structured_hex(6, 4, 6, pert=2, amp=0.08, seed=1)
     train_mesh = femrt.Mesh(gpu_ctx, train_hm)
@@ -187,23 +197,31 @@ def test_learned_guess_keeps_parity_and_saves_newton_iterations(femrt, oracle, g
     fr.camera_rays(cam, W, H)
     isos = [-0.1, 0.0, 0.1]
     og, oiso = oracle.Oracle.from_mesh(hm).trace(fr.get_rays(), isos)
+    forms = {"every": 1 | femrt.TRACE_LEARNED_EVERY_SAMPLE, "seq": 1, "default": 1 | femrt.TRACE_COOP_SCAN}
     work = {}
-    # every interior sample guessed by the network, then the default (only the
-    # first sample of each segment, the rest extrapolated)
-    for every in (True, False):
-        flags = 1 | (femrt.TRACE_LEARNED_EVERY_SAMPLE if every else 0)
-        for name, net in (("trained", mlp), ("zero", zero)):
+    for form, flags in forms.items():
+        # the cooperative scan (samples spread over the lanes, Hermite starts)
+        # takes warm guesses only
+        runs = [("warm", femrt.GUESS_WARM, mlp)]
+        if form != "default":
+            runs += [("trained", femrt.GUESS_LEARNED, mlp), ("zero", femrt.GUESS_LEARNED, zero)]
+        for name, guess, net in runs:
             mesh.set_guess_mlp(net)
             fr.counters(True)
-            fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_LEARNED, flags=flags))
+            fr.trace(mesh, femrt.trace_params(isos, guess=guess, flags=flags))
             rep = parity.compare(fr.hits(), fr.iso(len(isos)), og, oiso, isos)
-            work[name, every] = fr.counters()
-            print(name, every, json.dumps(rep), work[name, every])
-            assert rep["ok"], (name, every, rep)
-            assert work[name, every]["mlp_infer"] > 0
-    assert work["trained", True]["invert_it"] < 0.8 * work["zero", True]["invert_it"]
-    assert work["trained", False]["invert_it"] < work["zero", False]["invert_it"]
-    assert work["trained", False]["mlp_infer"] < 0.25 * work["trained", True]["mlp_infer"]
+            work[name, form] = fr.counters()
+            print(form, name, json.dumps(rep), {k: work[name, form][k] for k in ("invert_it", "mlp_infer")})
+            assert rep["ok"], (form, name, rep)
+    it = {k: v["invert_it"] for k, v in work.items()}
+    print(json.dumps({"learned_over_warm_iterations": {"seq": it["trained", "seq"] / it["warm", "seq"],
+                                                       "seq_vs_default_walk": it["trained", "seq"] / it["warm", "default"]},
+                      "learned_over_zero_net_iterations": {f: it["trained", f] / it["zero", f] for f in ("every", "seq")}}))
+    assert work["trained", "every"]["mlp_infer"] > 0 and work["trained", "seq"]["mlp_infer"] > 0
+    assert it["trained", "every"] < 0.8 * it["zero", "every"]
+    assert it["trained", "seq"] < it["zero", "seq"]
+    assert work["trained", "seq"]["mlp_infer"] < 0.25 * work["trained", "every"]["mlp_infer"]
+    assert it["trained", "seq"] <= 1.05 * it["warm", "seq"]
     with pytest.raises(femrt.FemrtError):
         mesh.set_guess_mlp(None)
         fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_LEARNED))
