@@ -107,6 +107,18 @@ FEMRT_API int femrt_mesh_get_bvh(const femrt_mesh* mesh, float* nodes8, int* pri
  * float or double). Host buffers; synchronous. */
 FEMRT_API int femrt_eval_field(femrt_mesh* mesh, int n, const int* elem, const float* rst,
                                int precision, void* out);
+/* femrt_eval_field without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Inputs and
+ * outputs are staged in a fixed device array of the library (3 MiB per GPU,
+ * held from the first call on that GPU until the process ends), so n is at
+ * most FEMRT_NOALLOC_MAX_POINTS (larger n: FEMRT_EINVAL).  Same arguments and
+ * results as femrt_eval_field.  Calls on one GPU share the array: calls that
+ * overlap on one GPU (contexts of the same device used from several host
+ * threads) race. */
+#define FEMRT_NOALLOC_MAX_POINTS 65536
+FEMRT_API int femrt_eval_field_noalloc(femrt_mesh* mesh, int n, const int* elem, const float* rst,
+                                       int precision, void* out);
 /* Point inversion x -> rst: the preimage inside the element.  Newton from
  * rst = 0 as WorldToReferenceNewton, restarted from the 8 octant centres when
  * it lands outside the element (Newton from 0 finds spurious preimages of the

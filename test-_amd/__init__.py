@@ -23,6 +23,7 @@ TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE, TRACE_NO_SAMPLE_SKIP 
 TRACE_COOP_SCAN = 16
 MAX_ISO = 8
 NCOUNTERS = 33
+NOALLOC_MAX_POINTS = 65536
 COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",
                  "geo64_it", "field64_evals", "slab_tests", "mlp_infer", "elems_marched",
                  "exit_pred", "exit_search", "exit_march", "reentry", "redo", "brackets",
@@ -128,6 +129,7 @@ def lib():
         "femrt_mesh_build_bvh": (I, [P, I, C.POINTER(I)]),
         "femrt_mesh_get_bvh": (I, [P, P, P]),
         "femrt_eval_field": (I, [P, I, P, P, I, P]),
+        "femrt_eval_field_noalloc": (I, [P, I, P, P, I, P]),
         "femrt_invert_points": (I, [P, I, P, P, P, P]),
         "femrt_trace_params_default": (None, [P]),
         "femrt_frame_create": (I, [P, I, C.POINTER(P)]),
@@ -192,7 +194,7 @@ def exported_symbols():
     return ["femrt_version", "femrt_ctx_create", "femrt_ctx_destroy", "femrt_last_error",
             "femrt_sync", "femrt_timing_enable", "femrt_timing_reset", "femrt_timing_read",
             "femrt_mesh_upload_hex", "femrt_mesh_destroy", "femrt_mesh_boxes",
-            "femrt_mesh_build_bvh", "femrt_mesh_get_bvh", "femrt_eval_field",
+            "femrt_mesh_build_bvh", "femrt_mesh_get_bvh", "femrt_eval_field", "femrt_eval_field_noalloc",
             "femrt_invert_points", "femrt_trace_params_default", "femrt_frame_create",
             "femrt_frame_destroy", "femrt_frame_set_rays", "femrt_frame_camera_rays",
             "femrt_frame_get_rays", "femrt_trace", "femrt_frame_get_hits", "femrt_frame_get_iso",
@@ -435,6 +437,17 @@ class Mesh:
         out = np.zeros((4, n), np.float64 if precision else np.float32)
         _check(lib().femrt_eval_field(self.h, n, _ptr(elem), _ptr(rst), precision, _ptr(out)),
                self.ctx.h, "eval_field")
+        return out
+
+    def eval_field_noalloc(self, elem, rst, precision=1):
+        """eval_field staged in the library's fixed device array (no device
+        allocation; at most NOALLOC_MAX_POINTS points)."""
+        elem = np.ascontiguousarray(elem, np.int32)
+        rst = np.ascontiguousarray(rst, np.float32).reshape(3, -1)
+        n = elem.size
+        out = np.zeros((4, n), np.float64 if precision else np.float32)
+        _check(lib().femrt_eval_field_noalloc(self.h, n, _ptr(elem), _ptr(rst), precision, _ptr(out)),
+               self.ctx.h, "eval_field_noalloc")
         return out
 
     def invert(self, elem, x):
