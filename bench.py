@@ -151,10 +151,25 @@ def cpu_baseline(femrt, orc_mod, hm, rays, budget_s=15.0):
         done += sel.size
         chunk = min(4 * chunk, 1024)
     mrays = done / t_used / 1e6
+    # the FP64 oracle (the checker) on the first rays of the same sample,
+    # reported beside the port, bounded to about a third of the budget
+    orc = orc_mod.Oracle.from_mesh(hm)
+    o_done, o_used, chunk = 0, 0.0, 16
+    while o_done < idx.size and o_used < budget_s / 3:
+        sel = idx[o_done:o_done + chunk]
+        t0 = time.perf_counter()
+        orc.trace(np.ascontiguousarray(rays[:, sel]), ISOS)
+        o_used += time.perf_counter() - t0
+        o_done += sel.size
+        chunk = min(2 * chunk, 512)
     return {"value": mrays, "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": (f"{done} of the {idx.size} rays at every 16th pixel in x and y "
                        f"({done / n * 100:.3f}% of the frame), {len(ISOS)} isovalue passes per ray, "
-                       f"{t_used:.1f} s on {cores} threads, {platform.processor() or platform.machine()}")}
+                       f"{t_used:.1f} s on {cores} threads, {platform.processor() or platform.machine()}"),
+            "oracle_fp64": {"value": o_done / o_used / 1e6, "unit": "Mrays/s", "rays": o_done,
+                            "seconds": o_used, "threads": int(os.environ.get("OMP_NUM_THREADS", cores)),
+                            "note": "the FP64 oracle (checker: brute force over element boxes, converged "
+                                    "face roots and iso scans) on the first rays of the same sample"}}
 
 
 def parity_sample(femrt, orc_mod, parity, hm, fr, n_iso, n_check=192):
@@ -178,6 +193,7 @@ def run_variant(femrt, lanes, guess, steps, warmup, flags=1):
     prm = femrt.trace_params(ISOS, guess=guess, flags=flags)
     for _ in range(warmup):
         for ctx, mesh, fr in lanes:
+            gen_rays(fr)
             fr.trace(mesh, prm)
     for ctx, _, _ in lanes:
         ctx.sync()
@@ -288,7 +304,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variants", default="warm,learned,plain,warm_coop,learned_every")
+    ap.add_argument("--variants", default="learned,warm,plain",
+                    help="comma list of learned, warm, plain, learned_every, warm_coop")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -397,7 +414,9 @@ def main():
                           "launches_per_step": {k: c / args.steps for k, (a, c) in kt.items()},
                           "counters": cnt}), file=sys.stderr, flush=True)
 
-    head = "warm" if "warm" in results else variants[0]
+    # the headline is the learned-guess variant (the metric's subject); the
+    # engineered warm starts and plain Newton are reported beside it
+    head = next((v for v in ("learned", "warm") if v in results), variants[0])
     R = results[head]
     single = None
     if args.inflight > 1 and args.single_steps > 0:
@@ -489,19 +508,15 @@ def main():
         line["config"]["mlp"] = mlp_info
     if "plain" in results and head != "plain":
         line["config"]["speedup_vs_plain_newton"] = R["mrays"] / results["plain"]["mrays"]
-    if "learned" in results and head != "learned":
-        L_ = results["learned"]
-        lg = {"mrays_s": L_["mrays"], "ms_per_step": L_["wall"] / args.steps * 1e3,
-              "speedup_vs_warm": L_["mrays"] / R["mrays"],
-
-              "network_inferences_per_frame": L_["counters"].get("mlp_infer", 0)}
-        if "plain" in results:
-            lg["speedup_vs_plain_newton"] = L_["mrays"] / results["plain"]["mrays"]
-        line["config"]["learned_guess"] = lg
+    if head == "learned" and "warm" in results:
+        Wm = results["warm"]
+        line["config"]["speedup_vs_warm"] = R["mrays"] / Wm["mrays"]
+        line["config"]["network_inferences_per_frame"] = R["counters"].get("mlp_infer", 0)
         line["config"]["note"] = (
-            "value is the default walk with engineered guesses; the learned guess gains nothing over them "
-            "(see learned_guess): the speedup over plain Newton comes from non-learned engineering -- "
-            "predicted exits, sub-patch face seeds, extrapolated sample starts, certified sample skipping")
+            "value is the learned-guess variant; the network gains nothing over the engineered warm "
+            "starts (speedup_vs_warm ~1): the speedup over plain Newton comes from non-learned "
+            "engineering shared by both -- predicted exits, sub-patch face seeds, extrapolated sample "
+            "starts, certified sample skipping")
     if single:
         line["config"]["single_frame"] = single
     print(json.dumps(line), flush=True)

@@ -45,13 +45,15 @@ def sky(d):
     return np.array([(1 - t) + t * 0.5, (1 - t) + t * 0.7, 1.0])
 
 
-def scatter(orc, rays, hits, mats, seed, depth, last, tp, acc, alive, ties=None):
+def scatter(orc, rays, hits, mats, seed, depth, last, tp, acc, alive, ties=None, ids=None):
     """One bounce of every live path; rays (6 x n) are the rays that were
     traced, hits the oracle's trace of them.  Updates tp, acc, alive in place
     and returns the next rays (6 x n, float64).  ties (bool n, optional) is
     set where a dielectric's reflect/refract decision lies within 1e-5 of its
     threshold (Fresnel draw or total internal reflection), which FP32 may
-    decide the other way."""
+    decide the other way.  ids (int n, optional) are the frame's path
+    indices of the columns when they are a subset of a frame (the random
+    draws are keyed by path index); by default column p is path p."""
     n = rays.shape[1]
     nxt = np.zeros((6, n))
     nxt[:3] = 1e30
@@ -59,6 +61,7 @@ def scatter(orc, rays, hits, mats, seed, depth, last, tp, acc, alive, ties=None)
     for p in range(n):
         if not alive[p]:
             continue
+        pid = int(ids[p]) if ids is not None else p
         o = rays[:3, p].astype(np.float64)
         d = rays[3:, p].astype(np.float64)
         e = int(hits["elem"][p])
@@ -86,8 +89,8 @@ def scatter(orc, rays, hits, mats, seed, depth, last, tp, acc, alive, ties=None)
         scattered = True
         if ty == 1:
             rv = d - 2 * np.dot(d, nrm) * nrm
-            us = unit_vec(path_rand(seed, p, depth + 1, 0), path_rand(seed, p, depth + 1, 1))
-            rad = np.cbrt(path_rand(seed, p, depth + 1, 2))
+            us = unit_vec(path_rand(seed, pid, depth + 1, 0), path_rand(seed, pid, depth + 1, 1))
+            rad = np.cbrt(path_rand(seed, pid, depth + 1, 2))
             sd = rv + rough * rad * us
             scattered = np.dot(sd, nrm) > 0
         elif ty == 2:
@@ -97,7 +100,7 @@ def scatter(orc, rays, hits, mats, seed, depth, last, tp, acc, alive, ties=None)
             sinv = np.sqrt(max(0.0, 1 - cosv * cosv))
             r0 = ((1 - eta) / (1 + eta)) ** 2
             sch = r0 + (1 - r0) * (1 - cosv) ** 5
-            xi = path_rand(seed, p, depth + 1, 0)
+            xi = path_rand(seed, pid, depth + 1, 0)
             if ties is not None and (abs(eta * sinv - 1) < 1e-5 or abs(sch - xi) < 1e-5):
                 ties[p] = True
             if eta * sinv > 1 or sch > xi:
@@ -106,7 +109,7 @@ def scatter(orc, rays, hits, mats, seed, depth, last, tp, acc, alive, ties=None)
                 k = 1 - eta * eta * (1 - cosv * cosv)
                 sd = eta * (d + cosv * nrm) - np.sqrt(max(k, 0.0)) * nrm
         else:
-            us = unit_vec(path_rand(seed, p, depth + 1, 0), path_rand(seed, p, depth + 1, 1))
+            us = unit_vec(path_rand(seed, pid, depth + 1, 0), path_rand(seed, pid, depth + 1, 1))
             sd = nrm + us
             if np.dot(sd, sd) < 1e-12:
                 sd = nrm
