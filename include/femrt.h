@@ -91,8 +91,18 @@ FEMRT_API int femrt_mesh_destroy(femrt_mesh* mesh);
  * of sub-patch (a, b) of face f on the 3 x 3 split of its (u, v) square; each
  * lo.xyz, hi.xyz. */
 FEMRT_API int femrt_mesh_boxes(const femrt_mesh* mesh, float* out);
-/* builder 0: binned SAH on the host, 16 bins, leaves of <= 4 elements
- * (BVH.cu:35-177).  Returns node count via *n_nodes (may be NULL). */
+/* builder FEMRT_BVH_HOST_SAH (0): binned SAH on the host, 16 bins, leaves of
+ * <= 4 elements (BVH.cu:35-177, replaces BVH::build called by
+ * Scene::build_bvh, raytracer.h:338-344).
+ * builder FEMRT_BVH_GPU_LBVH (1): linear BVH built on the device on the
+ * context's stream (Morton codes of the element-box centroids, radix sort,
+ * binary radix tree, boxes bottom-up, subtrees of <= 4 elements as leaves);
+ * faster to build, slower to trace than SAH.  The reference has no GPU
+ * builder.  Both give the traversal the same answers (the nearest hit does
+ * not depend on the tree, except which of two equally near hits is kept).  Returns the node count via *n_nodes (may be NULL;
+ * LBVH node arrays keep 2 n - 1 nodes, some unreferenced). */
+#define FEMRT_BVH_HOST_SAH 0
+#define FEMRT_BVH_GPU_LBVH 1
 FEMRT_API int femrt_mesh_build_bvh(femrt_mesh* mesh, int builder, int* n_nodes);
 /* Host copy of the BVH: nodes x 8 floats (lo.xyz, a as int bits, hi.xyz, b
  * as int bits).  Internal node: a, b = child node indices.  Leaf: b < 0,

@@ -21,6 +21,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "
 GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
 TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE, TRACE_NO_SAMPLE_SKIP = 1, 2, 4, 8   # trace flag bits
 TRACE_COOP_SCAN = 16
+BVH_HOST_SAH, BVH_GPU_LBVH = 0, 1   # femrt_mesh_build_bvh builders
 MAX_ISO = 8
 NCOUNTERS = 33
 NOALLOC_MAX_POINTS = 65536
@@ -414,7 +415,7 @@ class Mesh:
         if build_bvh:
             self.build_bvh()
 
-    def build_bvh(self, builder=0):
+    def build_bvh(self, builder=BVH_HOST_SAH):
         nn = C.c_int(0)
         _check(lib().femrt_mesh_build_bvh(self.h, builder, C.byref(nn)), self.ctx.h, "build_bvh")
         self.n_nodes = nn.value

@@ -170,7 +170,8 @@ __global__ __launch_bounds__(256) void k_polish_hit(
 {
   const int count = counts[0];
   Work w;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+  for (XcdStride it(count); it.q < it.end; it.q += it.step) {
+    const int q = it.q;
     const int r = queue[q];
     float of[3], df[3];
     load_ray(rays, n, r, of, df);
@@ -302,7 +303,8 @@ __global__ __launch_bounds__(256, 2) void k_polish_iso(
   // neighbouring lanes polish roots in the same elements
   const int count = counts[0] * P.n_iso;
   Work w;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+  for (XcdStride it(count); it.q < it.end; it.q += it.step) {
+    const int q = it.q;
     const int r = hit_queue[q / P.n_iso], k = q % P.n_iso;
     const size_t slot = (size_t)k * n + r;
     if (iso_elem[slot] < 0) continue;
@@ -621,7 +623,17 @@ FEMRT_API int femrt_mesh_build_bvh(femrt_mesh* m, int builder, int* n_nodes)
 {
   if (!m) return FEMRT_EINVAL;
   femrt_ctx* c = m->ctx;
-  if (builder != 0) return fail(c, FEMRT_EUNSUP, "build_bvh: only builder 0 (host binned SAH)");
+  if (builder != FEMRT_BVH_HOST_SAH && builder != FEMRT_BVH_GPU_LBVH)
+    return fail(c, FEMRT_EINVAL, "build_bvh: builder must be 0 (host binned SAH) or 1 (GPU LBVH)");
+  if (builder == FEMRT_BVH_GPU_LBVH) {
+    const int rc = lbvh_build(c, m);
+    if (rc != FEMRT_OK) return rc;
+    m->bvh_depth = bvh_depth(m->bvh.nodes, 0, 0);
+    if (m->bvh_depth >= FEMRT_STACK)
+      return fail(c, FEMRT_EUNSUP, "build_bvh: tree deeper than the traversal stack");
+    if (n_nodes) *n_nodes = m->n_nodes;
+    return FEMRT_OK;
+  }
   build_bvh_sah(m->n_elem, m->h_boxes.data(), m->bvh);
   m->n_nodes = (int)(m->bvh.nodes.size() / 8);
   m->bvh_depth = bvh_depth(m->bvh.nodes, 0, 0);

@@ -228,6 +228,40 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter)
   return base + __popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Frame queue counters: [0] hits, [2..3] binned-walk lists, [4] rays to
+// trace again (FEMRT_RETRACE_CTR), [5] binned-walk start list,
+// [FEMRT_QSEG_CTR + s] progress in segment s of the hit queue (persistent walk).
+#define FEMRT_RETRACE_CTR 4
+#define FEMRT_QSEG 8
+#define FEMRT_QSEG_CTR 16
+#define FEMRT_NCOUNTS 32
+
+// Grid-stride loop over [0, count) split by XCD.  The workgroups of a
+// launch are dispatched round-robin over the FEMRT_QSEG (8) XCDs, so block b
+// runs on XCD b % 8; the blocks of XCD x stride over the x-th of 8
+// contiguous slices of the range.  Over a queue in tile order each slice is
+// one band of the image, so each XCD's L2 caches the elements of its own
+// band instead of every XCD fetching every element.  A grid of fewer than 8
+// blocks strides over the whole range.
+struct XcdStride {
+  int q, end, step;
+  __device__ XcdStride(int count)
+  {
+    const int b = blockIdx.x, nb = gridDim.x, t = threadIdx.x, bs = blockDim.x;
+    if (nb < FEMRT_QSEG) {
+      q = b * bs + t;
+      end = count;
+      step = nb * bs;
+      return;
+    }
+    const int x = b % FEMRT_QSEG, rank = b / FEMRT_QSEG;
+    const int nx = (nb - x + FEMRT_QSEG - 1) / FEMRT_QSEG;  // blocks on XCD x
+    q = (int)((long long)count * x / FEMRT_QSEG) + rank * bs + t;
+    end = (int)((long long)count * (x + 1) / FEMRT_QSEG);
+    step = nx * bs;
+  }
+};
+
 #define FEMRT_DISPATCH_NG(NGV, BODY)      \
   switch (NGV) {                          \
     case 4: { constexpr int NG_ = 4; BODY; } break; \
@@ -245,16 +279,13 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter)
   }
 
 #define FEMRT_MLP_MAX_SAMPLES 16
-// Frame queue counters: [0] hits, [2..3] binned-walk lists, [4] rays to
-// trace again (FEMRT_RETRACE_CTR), [5] binned-walk start list,
-// [FEMRT_QSEG_CTR + s] progress in segment s of the hit queue (persistent walk).
-#define FEMRT_RETRACE_CTR 4
-#define FEMRT_QSEG 8
-#define FEMRT_QSEG_CTR 16
-#define FEMRT_NCOUNTS 32
 // Work buffer of a frame: FEMRT_NCOUNTERS counters, then the number of walks
 // that ended stalled and the first FEMRT_MAX_STALLED_IDS of their ray ids.
 #define FEMRT_WORK_SLOTS (FEMRT_NCOUNTERS + 1 + FEMRT_MAX_STALLED_IDS)
+
+// BVH builder 1: linear BVH on the device (femrt_lbvh.hip).  Sets
+// m->d_nodes, m->d_prims, m->n_nodes and the host copy m->bvh.
+int lbvh_build(femrt_ctx* c, femrt_mesh* m);
 
 // Element walk in element-binned rounds (femrt_walk.hip).
 int walk_run(femrt_ctx* c, femrt_mesh* m, femrt_frame* f, const TraceMesh& TM, const IsoParams& ip,
