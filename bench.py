@@ -358,8 +358,12 @@ def main():
     # segment's first sample, warm starts after it; learned_every: a network
     # guess for every sample (the network on its own); warm_coop: the
     # samples of a wave's segments spread over its lanes, Hermite starts
+    # bisect / learned_bracket: learned guesses with the FP64 iso endgame by
+    # plain bisection of the scan's bracket, or by bisection from the bracket
+    # network's predicted sub-bracket (config 5's comparison)
     names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED,
-             "learned_every": femrt.GUESS_LEARNED, "warm_coop": femrt.GUESS_WARM}
+             "learned_every": femrt.GUESS_LEARNED, "warm_coop": femrt.GUESS_WARM,
+             "bisect": femrt.GUESS_LEARNED, "learned_bracket": femrt.GUESS_LEARNED}
     variants = [v for v in args.variants.split(",") if v]
     results = {}
     mlp_info = None
@@ -387,6 +391,44 @@ def main():
                     "guess_err_median_on_bench_mesh": float(np.median(err)),
                     "guess_err_p99_on_bench_mesh": float(np.percentile(err, 99))}
         print(json.dumps({"mlp": mlp_info}), file=sys.stderr, flush=True)
+    bracket_info = None
+    if "learned_bracket" in variants:
+        # the bracket network is trained on the tracer's own brackets and FP64
+        # roots of four other seeded meshes of the family (16^3, the same
+        # frame size; the last one's pairs held out), untimed
+        t0 = time.perf_counter()
+        parts = []
+        bfr = femrt.Frame(ctx, n_rays)
+        gen_rays(bfr)
+        for j in range(4):
+            bhm = femrt.gen_structured_hex(16, Q, P if not WAVES else 2, pert=PERT, amp=AMP, seed=SEED + 11 + 2 * j)
+            if WAVES:
+                bhm = femrt.gen_field_waves(bhm, P, n_waves=WAVES, seed=SEED + 12 + 2 * j)
+            bmesh = femrt.Mesh(ctx, bhm)
+            bfr.trace(bmesh, femrt.trace_params(ISOS, guess=femrt.GUESS_WARM, flags=1 | femrt.TRACE_SEG_INFO))
+            parts.append(bfr.bracket_pairs(bmesh, ISOS, 1 << 21))
+            bmesh.close()
+        bf = np.concatenate([p_[0] for p_ in parts])
+        bt = np.concatenate([p_[1] for p_ in parts])
+        bmlp = femrt.Mlp(ctx, seed=SEED + 13, kind=femrt.MLP_BRACKET)
+        hold = len(parts[-1][0])
+        blosses = bmlp.train_pairs(bf[:-hold], bt[:-hold], batch=4096, epochs=4, lr=3e-3, lr_final=1e-4,
+                                   seed=SEED + 14)
+        q = bmlp.forward(bf[-hold:])
+        lo, hi = np.minimum(q[:, 0], q[:, 1]), np.maximum(q[:, 0], q[:, 1])
+        bt = bt[-hold:]
+        mesh.set_bracket_mlp(bmlp)
+        for lane_ctx, lane_mesh, _ in lanes[1:]:
+            b2 = femrt.Mlp(lane_ctx, seed=SEED + 13, kind=femrt.MLP_BRACKET)
+            b2.set_params(bmlp.params())
+            lane_mesh.set_bracket_mlp(b2)
+        bfr.close()
+        bracket_info = {"train_meshes": "three 16^3 meshes of the same family (seeds +11, +13, +15); "
+                                         "a fourth (+17) held out", "train_pairs": int(bf.shape[0] - len(bt)),
+                        "epoch_loss": [float(x) for x in blosses], "train_s": time.perf_counter() - t0,
+                        "holdout_cover_5_95": float(np.mean((bt[:, 0] >= lo) & (bt[:, 0] <= hi))),
+                        "holdout_median_width": float(np.median(hi - lo))}
+        print(json.dumps({"bracket_mlp": bracket_info}), file=sys.stderr, flush=True)
     launch_cfg = list(mesh.launch_cfg())
     if not args.no_autotune:
         # launch shapes picked by the autotuner on the headline variant's trace,
@@ -404,6 +446,10 @@ def main():
             flags |= femrt.TRACE_LEARNED_EVERY_SAMPLE
         if v.endswith("_coop"):
             flags |= femrt.TRACE_COOP_SCAN
+        if v == "bisect":
+            flags |= femrt.TRACE_ISO_BISECT
+        if v == "learned_bracket":
+            flags |= femrt.TRACE_ISO_LEARNED_BRACKET
         wall, kt, cnt = run_variant(femrt, lanes, names[v], args.steps, args.warmup, flags)
         f32, f64 = kernel_flops(cnt, hm.ng, hm.nf)
         results[v] = dict(wall=wall, kernels=kt, counters=cnt, f32=f32, f64=f64,
@@ -519,6 +565,24 @@ def main():
             "starts, certified sample skipping")
     if single:
         line["config"]["single_frame"] = single
+    if "bisect" in results and "learned_bracket" in results:
+        # config 5's comparison: the FP64 endgame by plain bisection of the
+        # scan's bracket against bisection from the bracket network's
+        # prediction (both to the same tolerance), and the default Newton
+        # endgame of the learned variant
+        def endgame(v):
+            r = results[v]
+            nb = max(1, r["counters"]["brackets"])
+            return {"mrays_s": r["mrays"], "k_polish_iso_ms": r["kernels"].get("k_polish_iso", (0.0, 1))[0] / args.steps,
+                    "k_bracket_infer_ms": r["kernels"].get("k_bracket_infer", (0.0, 1))[0] / args.steps,
+                    "field64_per_bracket": r["counters"]["field64_evals"] / nb,
+                    "geo64_it_per_bracket": r["counters"]["geo64_it"] / nb}
+        eg = {v: endgame(v) for v in ("bisect", "learned_bracket", "learned") if v in results}
+        eg["note"] = ("field64/geo64 counts include the hit polish's (one field evaluation per hit); "
+                      "the endgames differ only in the iso polish")
+        line["config"]["iso_endgames"] = eg
+        if bracket_info:
+            line["config"]["bracket_mlp"] = bracket_info
     print(json.dumps(line), flush=True)
     for c, m, f in reversed(lanes):
         f.close()

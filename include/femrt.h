@@ -191,6 +191,24 @@ typedef struct {
                                           is scanned in its ray's lane, sample after
                                           sample (warm starts extrapolated from the
                                           two previous samples, certified skipping) */
+#define FEMRT_TRACE_ISO_BISECT 32      /* FP64 iso endgame by plain bisection of the
+                                          scan's bracket (default: Newton from the
+                                          bracket's secant point, bisection
+                                          safeguard); stops within 1e-10 (1 + |t|)
+                                          with |u - iso| <= 1e-8 */
+#define FEMRT_TRACE_ISO_LEARNED_BRACKET 64 /* FP64 bisection endgame started from the
+                                          bracket network's predicted [t_lo, t_hi]
+                                          (femrt_mesh_set_bracket_mlp) where its ends
+                                          narrow the scan's bracket; the crossing
+                                          is still the scan's first.  Records the
+                                          segments like FEMRT_TRACE_SEG_INFO.  Not
+                                          with FEMRT_TRACE_BINNED_WALK.  No
+                                          counterpart in the reference (its endgame:
+                                          fem/RayIsosurface.cu:121-171) */
+#define FEMRT_TRACE_SEG_INFO 128       /* record each bracket's ends (rst and the
+                                          FP32 g at ta and tb), the bracket
+                                          network's inputs, for
+                                          femrt_frame_bracket_pairs */
 
 FEMRT_API void femrt_trace_params_default(femrt_trace_params* p);
 
@@ -330,6 +348,17 @@ FEMRT_API int femrt_autotune(femrt_mesh* mesh, femrt_frame* f, const femrt_trace
  * Flat FP32 parameters (6147): W1 [28][64] (row 27 = first-layer bias),
  * W2 [64][64] (in, out), b2 [64], W3 [64][3], b3 [3]. */
 #define FEMRT_MLP_GUESS 0
+/* FEMRT_MLP_BRACKET: the isosurface-bracket network (no counterpart in the
+ * reference, whose bracket is the first sign change of 16 uniform samples,
+ * fem/RayIsosurface.cu:67-188).  32 inputs -> 64 tanh (bias b1) -> 64 tanh
+ * -> 2: the 5% and 95% quantiles of where in the scan's bracket [ta, tb] the
+ * crossing lies (s in [0, 1]), trained by the pinball loss.  Inputs: rst
+ * where the ray enters and leaves the bracket (at ta and tb), the secant
+ * weight g(ta) / (g(ta) - g(tb)) of the FP32 samples there, and the 25 modal
+ * coefficients of u - iso with (i, j, k) in {0, 1, 2}^3 other than
+ * (2, 2, 2) and (2, 2, 1), scaled to unit maximum.  Flat FP32 parameters
+ * (6402): W1 [32][64], b1 [64], W2 [64][64], b2 [64], W3 [64][2], b3 [2]. */
+#define FEMRT_MLP_BRACKET 1
 typedef struct femrt_mlp femrt_mlp;
 typedef struct {
   int n_samples;      /* training pairs generated from the mesh             */
@@ -347,12 +376,16 @@ FEMRT_API int femrt_mlp_destroy(femrt_mlp* m);
 FEMRT_API int femrt_mlp_get_params(femrt_mlp* m, float* params);
 /* Replaces the parameters and resets the optimiser state. */
 FEMRT_API int femrt_mlp_set_params(femrt_mlp* m, const float* params);
-/* Inference with the element walk's own device routine: feats n x 27,
- * out n x 3 (host arrays, synchronous). */
+/* Inference with the tracer's own device routine: feats n x IN, out n x OUT
+ * (host arrays, synchronous; IN, OUT = 27, 3 for the guess network and
+ * 32, 2 for the bracket network). */
 FEMRT_API int femrt_mlp_forward(femrt_mlp* m, int n, const float* feats, float* out);
-/* Loss sum_i sum_c (y_ic - target_ic)^2 / n and its gradient with respect to
- * the flat parameters, from the training kernel's forward and backward pass
- * (no update).  feats n x 27, targets n x 3, grad 6147 floats. */
+/* Loss and its gradient with respect to the flat parameters, from the
+ * training kernel's forward and backward pass (no update).  Guess network:
+ * sum_i sum_c (y_ic - target_ic)^2 / n; bracket network: sum_i sum_c
+ * rho_tau_c(target_ic - y_ic) / n, rho_tau(r) = max(tau r, (tau - 1) r),
+ * tau = (0.05, 0.95).  feats n x IN, targets n x OUT, grad
+ * femrt_mlp_num_params(kind) floats. */
 FEMRT_API int femrt_mlp_loss_grad(femrt_mlp* m, int n, const float* feats, const float* targets,
                                   float* loss, float* grad);
 /* Training pairs from the tracer's own point inversions: a seeded element and
@@ -366,9 +399,25 @@ FEMRT_API void femrt_mlp_train_params_default(femrt_mlp_train_params* p);
  * each epoch.  Synchronous. */
 FEMRT_API int femrt_mlp_train(femrt_mlp* m, femrt_mesh* mesh, const femrt_mlp_train_params* p,
                               float* epoch_loss);
+/* Adam over caller-given training pairs (host arrays feats n x IN, targets
+ * n x OUT), p->epochs epochs in batches of p->batch (p->n_samples unused);
+ * either kind of network.  Synchronous. */
+FEMRT_API int femrt_mlp_train_pairs(femrt_mlp* m, int n, const float* feats, const float* targets,
+                                    const femrt_mlp_train_params* p, float* epoch_loss);
 /* Network used by FEMRT_GUESS_LEARNED traces of this mesh (NULL detaches);
  * it must stay alive while such traces run. */
 FEMRT_API int femrt_mesh_set_guess_mlp(femrt_mesh* mesh, const femrt_mlp* m);
+/* Bracket network used by FEMRT_TRACE_ISO_LEARNED_BRACKET traces of this mesh
+ * (NULL detaches); it must stay alive while such traces run. */
+FEMRT_API int femrt_mesh_set_bracket_mlp(femrt_mesh* mesh, const femrt_mlp* m);
+/* Training pairs of the bracket network from the last trace of frame f,
+ * which must have recorded segments (FEMRT_TRACE_SEG_INFO or
+ * FEMRT_TRACE_ISO_LEARNED_BRACKET) with isovalues iso[0 .. n_iso): for each
+ * crossing found, the network inputs (feats, 32 floats) and twice the
+ * position s of the FP64 root in the scan's bracket (targets, 2 floats).  At
+ * most max_pairs pairs (host arrays), their count in *n_pairs. */
+FEMRT_API int femrt_frame_bracket_pairs(femrt_mesh* mesh, femrt_frame* f, int n_iso, const double* iso,
+                                        int max_pairs, float* feats, float* targets, int* n_pairs);
 
 /* ------------------------------------------------ host-only utilities */
 /* The same box and BVH builders the mesh upload uses, on host arrays (no

@@ -539,6 +539,22 @@ ORC_EXPORT int orc_iso_segment(const orc_mesh_t* m, int e, const double* o, cons
     const double g = u - iso;
     minabs = fmin(minabs, sample_tie_margin(m, e, rst, d, t, g));
     if (have_prev && prev_g * g <= 0.0) {
+      /* more than one crossing inside this sample interval (seen by 32
+         sub-samples): which of them an endgame converges to is not fixed by
+         the scan -- the reference's Newton / bisection in the bracket
+         (fem/RayIsosurface.cu:121-171) takes any -- so the result is a tie */
+      {
+        double pg = prev_g;
+        int changes = 0;
+        for (int s = 1; s <= 32; ++s) {
+          double sr[3], su;
+          if (!eval_along(m, e, o, d, prev_t + (t - prev_t) * ((double)s / 32.0), sr, &su)) continue;
+          const double sg = su - iso;
+          if ((pg < 0.0) != (sg < 0.0)) ++changes;
+          pg = sg;
+        }
+        if (changes > 1) minabs = 0.0;
+      }
       double lt = prev_t, lg = prev_g, rt = t;
       for (int it = 0; it < 200 && fabs(rt - lt) > 1e-14 * (fabs(rt) + 1.0); ++it) {
         const double mt = 0.5 * (lt + rt);

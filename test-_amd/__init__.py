@@ -21,6 +21,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "
 GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
 TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE, TRACE_NO_SAMPLE_SKIP = 1, 2, 4, 8   # trace flag bits
 TRACE_COOP_SCAN = 16
+TRACE_ISO_BISECT, TRACE_ISO_LEARNED_BRACKET, TRACE_SEG_INFO = 32, 64, 128
 BVH_HOST_SAH, BVH_GPU_LBVH = 0, 1   # femrt_mesh_build_bvh builders
 MAX_ISO = 8
 NCOUNTERS = 33
@@ -50,6 +51,8 @@ class MlpTrainParams(C.Structure):
 
 
 MLP_GUESS = 0
+MLP_BRACKET = 1
+MLP_SHAPE = {MLP_GUESS: (27, 3), MLP_BRACKET: (32, 2)}   # (inputs, outputs)
 MLP_NPARAM = 6147
 MLP_IN = 27
 
@@ -171,6 +174,9 @@ def lib():
         "femrt_mlp_train_params_default": (None, [P]),
         "femrt_mlp_train": (I, [P, P, P, P]),
         "femrt_mesh_set_guess_mlp": (I, [P, P]),
+        "femrt_mesh_set_bracket_mlp": (I, [P, P]),
+        "femrt_mlp_train_pairs": (I, [P, I, P, P, P, P]),
+        "femrt_frame_bracket_pairs": (I, [P, P, I, P, I, P, P, P]),
         "femrt_frame_set_materials": (I, [P, P, I]),
         "femrt_batch_create": (I, [P, C.POINTER(P)]),
         "femrt_batch_destroy": (I, [P]),
@@ -210,6 +216,7 @@ def exported_symbols():
             "femrt_mlp_get_params", "femrt_mlp_set_params", "femrt_mlp_forward",
             "femrt_mlp_loss_grad", "femrt_mlp_make_samples", "femrt_mlp_train_params_default",
             "femrt_mlp_train", "femrt_mesh_set_guess_mlp", "femrt_frame_set_materials",
+            "femrt_mesh_set_bracket_mlp", "femrt_mlp_train_pairs", "femrt_frame_bracket_pairs",
             "femrt_batch_create", "femrt_batch_destroy", "femrt_batch_add_trace",
             "femrt_batch_add_paths", "femrt_batch_capture", "femrt_batch_launch",
             "femrt_mesh_get_launch_cfg", "femrt_mesh_set_launch_cfg", "femrt_autotune"]
@@ -467,6 +474,11 @@ class Mesh:
                self.ctx.h, "set_guess_mlp")
         self._mlp = mlp  # keep the network alive while the mesh uses it
 
+    def set_bracket_mlp(self, mlp):
+        self._bracket_mlp = mlp
+        _check(lib().femrt_mesh_set_bracket_mlp(self.h, None if mlp is None else mlp.h),
+               self.ctx.h, "set_bracket_mlp")
+
     def launch_cfg(self, cfg=None):
         """Launch shapes (first-hit block, march and polish workgroups per CU);
         sets them when cfg is given, returns the current ones."""
@@ -491,11 +503,15 @@ class Mesh:
 
 
 class Mlp:
-    """Learned Newton-guess network (27 -> 64 tanh -> 64 tanh -> 3, BF16 MFMA)."""
+    """The library's networks on BF16 MFMA: the Newton-guess network
+    (MLP_GUESS, 27 -> 64 tanh -> 64 tanh -> 3) or the isosurface-bracket
+    network (MLP_BRACKET, 32 -> 64 tanh -> 64 tanh -> 2 quantiles)."""
 
     def __init__(self, ctx: Context, seed=0, kind=MLP_GUESS):
         self.ctx = ctx
         self.h = C.c_void_p()
+        self.kind = kind
+        self.n_in, self.n_out = MLP_SHAPE[kind]
         _check(lib().femrt_mlp_create(ctx.h, kind, seed, C.byref(self.h)), ctx.h, "mlp_create")
         self.n_param = lib().femrt_mlp_num_params(kind)
 
@@ -510,15 +526,15 @@ class Mlp:
         _check(lib().femrt_mlp_set_params(self.h, _ptr(p)), self.ctx.h, "mlp_set_params")
 
     def forward(self, feats):
-        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, MLP_IN)
-        out = np.zeros((feats.shape[0], 3), np.float32)
+        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, self.n_in)
+        out = np.zeros((feats.shape[0], self.n_out), np.float32)
         _check(lib().femrt_mlp_forward(self.h, feats.shape[0], _ptr(feats), _ptr(out)),
                self.ctx.h, "mlp_forward")
         return out
 
     def loss_grad(self, feats, targets):
-        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, MLP_IN)
-        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, self.n_in)
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, self.n_out)
         loss = C.c_float(0)
         grad = np.zeros(self.n_param, np.float32)
         _check(lib().femrt_mlp_loss_grad(self.h, feats.shape[0], _ptr(feats), _ptr(targets),
@@ -536,6 +552,18 @@ class Mlp:
         losses = np.zeros(p.epochs, np.float32)
         _check(lib().femrt_mlp_train(self.h, mesh.h, C.byref(p), _ptr(losses)), self.ctx.h,
                "mlp_train")
+        return losses
+
+    def train_pairs(self, feats, targets, batch=4096, epochs=4, lr=3e-3, lr_final=1e-4, seed=1):
+        """Adam over given pairs (feats n x n_in, targets n x n_out)."""
+        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, self.n_in)
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, self.n_out)
+        p = MlpTrainParams()
+        lib().femrt_mlp_train_params_default(C.byref(p))
+        p.batch, p.epochs, p.lr, p.lr_final, p.seed = batch, epochs, lr, lr_final, seed
+        losses = np.zeros(epochs, np.float32)
+        _check(lib().femrt_mlp_train_pairs(self.h, feats.shape[0], _ptr(feats), _ptr(targets), C.byref(p),
+                                           _ptr(losses)), self.ctx.h, "mlp_train_pairs")
         return losses
 
     def close(self):
@@ -659,6 +687,17 @@ class Frame:
         if n < 0:
             _check(n, self.ctx.h, "stalled_rays")
         return n, np.sort(ids[:min(n, ids.size)])
+
+    def bracket_pairs(self, mesh, isos, max_pairs):
+        """Training pairs of the bracket network from the last trace (which
+        recorded segments): (features n x 32, targets n x 2)."""
+        iso = np.ascontiguousarray(isos, np.float64)
+        feats = np.zeros((max_pairs, 32), np.float32)
+        tgt = np.zeros((max_pairs, 2), np.float32)
+        got = C.c_int(0)
+        _check(lib().femrt_frame_bracket_pairs(mesh.h, self.h, iso.size, _ptr(iso), max_pairs, _ptr(feats),
+                                               _ptr(tgt), C.byref(got)), self.ctx.h, "bracket_pairs")
+        return feats[:got.value], tgt[:got.value]
 
     def close(self):
         if self.h:

@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256, 2) void k_polish_iso(
     const float* __restrict__ rays, int n, TraceMesh M, IsoParams P, const int* __restrict__ hit_queue,
     const int* __restrict__ counts, const float4* __restrict__ iso_f32, const float4* __restrict__ iso_rst32,
     int* __restrict__ iso_elem, double* __restrict__ iso_t, double* __restrict__ iso_rst,
-    double* __restrict__ iso_u, long long* __restrict__ work)
+    double* __restrict__ iso_u, long long* __restrict__ work, const float2* __restrict__ iso_pred)
 {
   // (hit ray, isovalue) pairs in hit-queue order, which is tile order, so
   // neighbouring lanes polish roots in the same elements
@@ -323,7 +323,15 @@ __global__ __launch_bounds__(256, 2) void k_polish_iso(
     br.rst[1] = r4.y;
     br.rst[2] = r4.z;
     double t, rst[3], u;
-    if (polish_iso<NG, NF>(M, o, d, P.iso64[k], br, t, rst, u, w)) {
+    bool ok;
+    if (P.endgame == ISO_NEWTON) {
+      ok = polish_iso<NG, NF>(M, o, d, P.iso64[k], br, t, rst, u, w);
+    } else {
+      const bool pred = P.endgame == ISO_LEARNED_BISECT;
+      const float2 pr = pred ? iso_pred[slot] : make_float2(0.f, 0.f);
+      ok = polish_iso_bisect<NG, NF>(M, o, d, P.iso64[k], br, pred, pr.x, pr.y, t, rst, u, w);
+    }
+    if (ok) {
       iso_t[slot] = t;
       iso_rst[(size_t)k * 3 * n + r] = rst[0];
       iso_rst[(size_t)k * 3 * n + n + r] = rst[1];
@@ -805,7 +813,7 @@ FEMRT_API int femrt_frame_destroy(femrt_frame* f)
                   f->d_hit_u, f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, f->d_iso_f32,
                   f->d_iso_rst32, f->d_queue_hit, f->d_retrace, f->d_queue_walk, f->d_qbin, f->d_counts, f->d_rgba, f->d_work,
                   f->d_path_tp, f->d_path_acc, f->d_mats, f->d_mlp_g, f->d_walk_si, f->d_walk_sf, f->d_walk_a, f->d_walk_b,
-                  f->d_walk_sorted, f->d_walk_reent, f->d_walk_hist};
+                  f->d_walk_sorted, f->d_walk_reent, f->d_walk_hist, f->d_iso_seg, f->d_iso_pred};
   for (void* p : ptrs) (void)hipFree(p);
   delete f;
   return FEMRT_OK;
@@ -912,6 +920,25 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
     return fail(c, FEMRT_EUNSUP, "trace: learned guesses support at most 16 samples per segment");
   if (c->capturing && (f->counters || (p->flags & FEMRT_TRACE_BINNED_WALK)))
     return fail(c, FEMRT_EUNSUP, "trace: work counters and the binned walk cannot be captured in a batch");
+  const bool learned_br = (p->flags & FEMRT_TRACE_ISO_LEARNED_BRACKET) != 0;
+  const bool want_seg = learned_br || (p->flags & FEMRT_TRACE_SEG_INFO);
+  if (learned_br && !m->bracket_mlp)
+    return fail(c, FEMRT_ESTATE, "trace: the learned bracket needs a network (femrt_mesh_set_bracket_mlp)");
+  if (want_seg && (p->flags & FEMRT_TRACE_BINNED_WALK))
+    return fail(c, FEMRT_EUNSUP, "trace: the binned walk records no bracket segments");
+  if (want_seg && !f->d_iso_seg) {
+    if (c->capturing)
+      return fail(c, FEMRT_ESTATE, "trace: run a trace that records segments once before a batch records");
+    const size_t slots = (size_t)FEMRT_MAX_ISO * f->n;
+    if (hipMalloc(&f->d_iso_seg, slots * 2 * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&f->d_iso_pred, slots * sizeof(float2)) != hipSuccess) {
+      (void)hipFree(f->d_iso_seg);
+      f->d_iso_seg = nullptr;
+      f->d_iso_pred = nullptr;
+      return fail(c, FEMRT_ENOMEM, "trace: hipMalloc of the bracket segments failed");
+    }
+  }
+  f->seg_valid = false;
   if (!c->capturing) HIPCHK(c, hipSetDevice(c->device));
   const int n = f->n;
   const int n_iso = p->iso_enable ? p->n_iso : 0;
@@ -945,6 +972,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   ip.mlp_lead = (p->flags & FEMRT_TRACE_LEARNED_EVERY_SAMPLE) ? p->samples - 1 : 1;
   ip.tmin = p->t_min;
   ip.skip = (p->flags & FEMRT_TRACE_NO_SAMPLE_SKIP) ? 0 : 1;
+  ip.endgame = learned_br ? ISO_LEARNED_BISECT : ((p->flags & FEMRT_TRACE_ISO_BISECT) ? ISO_BISECT : ISO_NEWTON);
   TraceMesh TM;
   TM.geom = m->d_geom;
   TM.field = m->d_field;
@@ -1036,20 +1064,26 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
             a.work = work;
             a.mlp_pack = learned ? m->guess_mlp->d_pack : nullptr;
             a.mlp_g = learned ? f->d_mlp_g : nullptr;
+            a.iso_seg = want_seg ? f->d_iso_seg : nullptr;
             march_launch<NG_>(a);
           });
         }
         if (rc) break;
+        if (learned_br) {
+          rc = bracket_infer_launch(c, m, f, TM, ip, pblocks);
+          if (rc) break;
+        }
         rc = timed_launch(c, "k_polish_iso", [&] {
           k_polish_iso<NG_, NF_><<<pblocks, 256, 0, c->stream>>>(
               f->d_rays, n, TM, ip, f->d_queue_hit, f->d_counts, f->d_iso_f32, f->d_iso_rst32,
-              f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, work);
+              f->d_iso_elem, f->d_iso_t, f->d_iso_rst, f->d_iso_u, work, f->d_iso_pred);
         });
         if (rc) break;
       }
     });
   });
   if (rc) return rc;
+  f->seg_valid = want_seg && n_iso > 0;
   if (p->flags & 1)
     rc = timed_launch(c, "k_shade", [&] {
       k_shade<<<blocks, 256, 0, c->stream>>>(n, n_iso, f->d_rays, f->d_hit_elem, f->d_hit_u,

@@ -15,19 +15,26 @@
 #include "femrt_runtime.hpp"
 #include "femrt_trace_core.hpp"
 
-// Bracket output of the walk: FP32 bracket and estimate per (isovalue, ray).
+// Bracket output of the walk: FP32 bracket and estimate per (isovalue, ray),
+// and, when seg is set, the bracket's ends for the bracket network (two
+// float4 per slot: rst at ta, rst at tb, g at ta, g at tb).
 struct GlobalSink {
   int* elem;
   float4* f32;
   float4* rst32;
   size_t n;
   int r;
+  float4* seg;
   __device__ void put(int k, const IsoBracket& b)
   {
     const size_t slot = (size_t)k * n + r;
     elem[slot] = b.elem;
     f32[slot] = make_float4(b.ta, b.tb, b.tm, b.ga);
     rst32[slot] = make_float4(b.rst[0], b.rst[1], b.rst[2], 0.f);
+    if (seg) {
+      seg[2 * slot] = make_float4(b.seg[0], b.seg[1], b.seg[2], b.seg[3]);
+      seg[2 * slot + 1] = make_float4(b.seg[4], b.seg[5], b.seg[6], b.seg[7]);
+    }
   }
 };
 
@@ -192,12 +199,12 @@ __global__ __launch_bounds__(256, 2) void k_march(
     int* __restrict__ counts, const int* __restrict__ hit_elem, const int* __restrict__ hit_face,
     const float4* __restrict__ hit_f32, int* __restrict__ iso_elem, float4* __restrict__ iso_f32,
     float4* __restrict__ iso_rst32, long long* __restrict__ work,
-    const void* __restrict__ mlp_pack, _Float16* __restrict__ mlp_g)
+    const void* __restrict__ mlp_pack, _Float16* __restrict__ mlp_g, float4* __restrict__ iso_seg)
 {
   const int count = counts[0];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   typename std::conditional<COUNT, Work, NoWork>::type w;
-  GlobalSink sink{iso_elem, iso_f32, iso_rst32, (size_t)n, 0};
+  GlobalSink sink{iso_elem, iso_f32, iso_rst32, (size_t)n, 0, iso_seg};
   bf16x8* ops = nullptr;
   float* bias = nullptr;
   bf16x8* xs = nullptr;
@@ -368,6 +375,7 @@ struct MarchArgs {
   long long* work = nullptr;
   const void* mlp_pack = nullptr;
   _Float16* mlp_g = nullptr;
+  float4* iso_seg = nullptr;
 };
 
 template <int NG, int NF>
@@ -376,7 +384,7 @@ void march_launch_nf(const MarchArgs& a)
   auto go = [&](auto L, auto CNT, auto CO) {
     k_march<NG, NF, decltype(L)::value, decltype(CNT)::value, decltype(CO)::value><<<a.blocks, 256, 0, a.stream>>>(
         a.rays, a.n, a.M, a.P, a.queue, a.counts, a.hit_elem, a.hit_face, a.hit_f32, a.iso_elem, a.iso_f32,
-        a.iso_rst32, a.work, a.mlp_pack, a.mlp_g);
+        a.iso_rst32, a.work, a.mlp_pack, a.mlp_g, a.iso_seg);
   };
   using T1 = std::true_type;
   using F0 = std::false_type;

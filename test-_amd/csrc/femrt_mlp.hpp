@@ -1,8 +1,14 @@
-// femrt_mlp.hpp -- the learned Newton-guess network on gfx950 matrix cores.
+// femrt_mlp.hpp -- the library's two networks on gfx950 matrix cores.
 //
-// Network: 27 inputs (hit point and the 8 corner vertices, each normalised to
-// the element's bounding box as 2 (x - lo) / (hi - lo) - 1) plus a constant 1
-// for the first-layer bias -> 64 tanh -> 64 tanh -> 3 reference coordinates.
+// Guess network (FEMRT_MLP_GUESS): 27 inputs (hit point and the 8 corner
+// vertices, each normalised to the element's bounding box as
+// 2 (x - lo) / (hi - lo) - 1) plus a constant 1 for the first-layer bias
+// -> 64 tanh -> 64 tanh -> 3 reference coordinates (squared-error loss).
+// Bracket network (FEMRT_MLP_BRACKET): 32 inputs (the reference coordinates
+// where the ray enters and leaves the element segment, and the 26 low-order
+// modal coefficients of u - iso on the element, scaled to unit maximum)
+// -> 64 tanh (explicit bias b1) -> 64 tanh -> 2: the 5% and 95% quantiles of
+// the crossing's position s in [0, 1] along the segment (pinball loss).
 // Operands are BF16, products accumulate in FP32 on v_mfma_f32_32x32x16_bf16.
 //
 // Inference works on one wave of 64 queries at a time in the transposed form
@@ -24,13 +30,35 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int MLP_IN = 27;
 constexpr int MLP_HID = 64;
 constexpr int MLP_OUT = 3;
-// flat FP32 parameter vector
+// flat FP32 parameter vector of the guess network
 constexpr int MLP_OFF_W1 = 0;                        // [28][64] (input 27 = bias)
 constexpr int MLP_OFF_W2 = 28 * 64;                  // [64][64] (in, out)
 constexpr int MLP_OFF_B2 = MLP_OFF_W2 + 64 * 64;     // [64]
 constexpr int MLP_OFF_W3 = MLP_OFF_B2 + 64;          // [64][3] (in, out)
 constexpr int MLP_OFF_B3 = MLP_OFF_W3 + 64 * 3;      // [3]
 constexpr int MLP_NPARAM = MLP_OFF_B3 + 3;           // 6147
+
+// Network kinds (the values of FEMRT_MLP_GUESS and FEMRT_MLP_BRACKET) and
+// their shapes and flat parameter layouts.
+constexpr int NET_GUESS = 0, NET_BRACKET = 1;
+template <int KIND>
+struct Net;
+template <>
+struct Net<NET_GUESS> {
+  static constexpr int IN = MLP_IN, OUT = 3, W1_ROWS = 28;
+  static constexpr bool BIAS_INPUT = true, PINBALL = false;
+  static constexpr int OFF_W1 = MLP_OFF_W1, OFF_B1 = -1, OFF_W2 = MLP_OFF_W2, OFF_B2 = MLP_OFF_B2,
+                       OFF_W3 = MLP_OFF_W3, OFF_B3 = MLP_OFF_B3, NPARAM = MLP_NPARAM;
+};
+template <>
+struct Net<NET_BRACKET> {
+  static constexpr int IN = 32, OUT = 2, W1_ROWS = 32;
+  static constexpr bool BIAS_INPUT = false, PINBALL = true;
+  static constexpr int OFF_W1 = 0, OFF_B1 = 32 * 64, OFF_W2 = OFF_B1 + 64, OFF_B2 = OFF_W2 + 64 * 64,
+                       OFF_W3 = OFF_B2 + 64, OFF_B3 = OFF_W3 + 64 * 2, NPARAM = OFF_B3 + 2;  // 6402
+};
+// quantiles of the bracket network's two outputs
+constexpr float BRACKET_TAU0 = 0.05f, BRACKET_TAU1 = 0.95f;
 
 // Packed operand buffer: bf16 section, then FP32 biases.
 constexpr int PK_W1OP = 0;       // [2 a][2 q][64 lane] x 8  inference A operands, layer 1
@@ -41,8 +69,10 @@ constexpr int PK_W2B = 10240;    // [64][64]
 constexpr int PK_W3B = 14336;    // [64][32], columns 3..31 zero
 constexpr int PK_NBF16 = 16384;
 constexpr int PK_INFER_BF16 = 8192;  // inference operands only
-constexpr int PK_NF32 = 68;          // b2[64], b3[3], pad
-constexpr int PK_BYTES = PK_NBF16 * 2 + PK_NF32 * 4;
+constexpr int PK_NF32 = 68;          // b2[64], b3[<= 3], pad (what the guess network reads)
+constexpr int PK_B1 = 68;            // b1[64] of the bracket network
+constexpr int PK_NF32_ALL = PK_B1 + 64;
+constexpr int PK_BYTES = PK_NBF16 * 2 + PK_NF32_ALL * 4;
 
 // Hidden unit fed by k slot (q, h, x) of a layer-2/3 step: lane half h,
 // element x of k step q (16 units per step) -- the accumulator order of
@@ -65,8 +95,10 @@ __device__ __forceinline__ f32x16 mfma_bf16(bf16x8 a, bf16x8 b, f32x16 c)
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-// Writes this lane's 32 BF16 network inputs (27 features, 1, zeros) to row
-// `lane` of the wave's LDS input block xs (64 rows x 4 bf16x8).
+// Writes this lane's 32 BF16 network inputs to row `lane` of the wave's LDS
+// input block xs (64 rows x 4 bf16x8): the guess network's 27 features, 1,
+// zeros; the bracket network's 32 features.
+template <int KIND = NET_GUESS>
 __device__ __forceinline__ void mlp_stage_inputs(bf16x8* xs, int lane, const float* feat)
 {
   bf16x8 c0, c1, c2, c3;
@@ -74,7 +106,10 @@ __device__ __forceinline__ void mlp_stage_inputs(bf16x8* xs, int lane, const flo
     c0[x] = (__bf16)feat[x];
     c1[x] = (__bf16)feat[8 + x];
     c2[x] = (__bf16)feat[16 + x];
-    c3[x] = (__bf16)(x < 3 ? feat[24 + x] : (x == 3 ? 1.f : 0.f));
+    if constexpr (KIND == NET_GUESS)
+      c3[x] = (__bf16)(x < 3 ? feat[24 + x] : (x == 3 ? 1.f : 0.f));
+    else
+      c3[x] = (__bf16)feat[24 + x];
   }
   xs[lane * 4 + 0] = c0;
   xs[lane * 4 + 1] = c1;
@@ -86,10 +121,13 @@ __device__ __forceinline__ void mlp_stage_inputs(bf16x8* xs, int lane, const flo
 // of the wave must call it together: the matrix instructions take operands
 // from all 64 lanes.  Lanes 0..31 receive the outputs of query `lane` in y0
 // and of query `lane + 32` in y1 (lanes 32..63 receive unused rows).
+// b1 (the bracket network's first-layer bias) is read when KIND has one.
+template <int KIND = NET_GUESS>
 __device__ __forceinline__ void wave_mlp_forward(const bf16x8* xs, const bf16x8* ops,
                                                  const float* b2, const float* b3, float* y0,
-                                                 float* y1)
+                                                 float* y1, const float* b1 = nullptr)
 {
+  constexpr int OUT = Net<KIND>::OUT;
   const int lane = __lane_id();
   const int i = lane & 31, h = lane >> 5;
   const bf16x8* w1 = ops + PK_W1OP / 8;
@@ -102,6 +140,8 @@ __device__ __forceinline__ void wave_mlp_forward(const bf16x8* xs, const bf16x8*
       const bf16x8 x1 = xs[(32 * c + i) * 4 + 2 + h];
       for (int a = 0; a < 2; ++a) {
         f32x16 acc = {};
+        if constexpr (!Net<KIND>::BIAS_INPUT)
+          for (int v = 0; v < 16; ++v) acc[v] = b1[32 * a + 8 * (v >> 2) + 4 * h + (v & 3)];
         acc = mfma_bf16(w1[(a * 2 + 0) * 64 + lane], x0, acc);
         acc = mfma_bf16(w1[(a * 2 + 1) * 64 + lane], x1, acc);
         for (int x = 0; x < 8; ++x) {
@@ -123,13 +163,11 @@ __device__ __forceinline__ void wave_mlp_forward(const bf16x8* xs, const bf16x8*
     f32x16 acc;
     for (int v = 0; v < 16; ++v) {
       const int u = 8 * (v >> 2) + 4 * h + (v & 3);
-      acc[v] = u < 3 ? b3[u] : 0.f;
+      acc[v] = u < OUT ? b3[u] : 0.f;
     }
     for (int q = 0; q < 4; ++q) acc = mfma_bf16(w3[q * 64 + lane], gb[q >> 1][q & 1], acc);
     float* y = c ? y1 : y0;
-    y[0] = acc[0];
-    y[1] = acc[1];
-    y[2] = acc[2];
+    for (int o = 0; o < OUT; ++o) y[o] = acc[o];
   }
 }
 

@@ -56,6 +56,7 @@ struct femrt_mesh {
   femrt_ctx* ctx = nullptr;
   LaunchCfg cfg;
   const femrt_mlp* guess_mlp = nullptr;  // network used by FEMRT_GUESS_LEARNED
+  const femrt_mlp* bracket_mlp = nullptr;  // network used by FEMRT_TRACE_ISO_LEARNED_BRACKET
   int n_elem = 0, ng = 0, nf = 0;      // true mode counts
   int NG = 0, NF = 0;                  // kernel mode counts (padded)
   GeoMode* d_geom = nullptr;           // n_elem x NG^3 (x, y, z)
@@ -108,6 +109,12 @@ struct femrt_frame {
   void* d_mats = nullptr;        // scatter materials (64 slots), uploaded when they change
   _Float16* d_mlp_g = nullptr;   // every-sample network guesses, per walk lane
   size_t mlp_g_cap = 0;          // halfs
+  // per (isovalue, ray) bracket: its ends as the bracket network reads them
+  // (rst at ta and tb, FP32 g at ta and tb: 2 float4) and the network's
+  // prediction (t_lo, t_hi); allocated by the first trace that records them
+  float4* d_iso_seg = nullptr;   // MAX_ISO x n x 2
+  float2* d_iso_pred = nullptr;  // MAX_ISO x n
+  bool seg_valid = false;        // the last trace recorded segments
   std::vector<unsigned char> h_mats;
   // element-walk rounds (allocated by the first trace with isovalues)
   int4* d_walk_si = nullptr;     // per ray: element, entry face, step, found | redo << 8
@@ -282,6 +289,11 @@ struct XcdStride {
 // Work buffer of a frame: FEMRT_NCOUNTERS counters, then the number of walks
 // that ended stalled and the first FEMRT_MAX_STALLED_IDS of their ray ids.
 #define FEMRT_WORK_SLOTS (FEMRT_NCOUNTERS + 1 + FEMRT_MAX_STALLED_IDS)
+
+// Bracket network inference before the FP64 endgame (femrt_bracket.hip):
+// d_iso_pred of every (hit ray, isovalue) bracket of the frame.
+int bracket_infer_launch(femrt_ctx* c, femrt_mesh* m, femrt_frame* f, const TraceMesh& TM,
+                         const IsoParams& ip, int blocks);
 
 // BVH builder 1: linear BVH on the device (femrt_lbvh.hip).  Sets
 // m->d_nodes, m->d_prims, m->n_nodes and the host copy m->bvh.

@@ -40,7 +40,11 @@ struct IsoParams {
   float tmin;
   int mlp_lead;   // leading interior samples of a segment guessed by the network
   int skip;       // certified sample skipping on (see march_samples)
+  int endgame;    // FP64 iso endgame: ISO_NEWTON, ISO_BISECT or ISO_LEARNED_BISECT
 };
+
+// FP64 iso endgames (IsoParams::endgame)
+enum { ISO_NEWTON = 0, ISO_BISECT = 1, ISO_LEARNED_BISECT = 2 };
 
 // Work counters accumulated per lane.
 // (32-bit per lane: a lane's counts stay far below 2^32; the kernels add them
@@ -76,11 +80,13 @@ struct HitF32 {
 };
 
 // FP32 result of the iso stage for one isovalue: sample bracket [ta, tb],
-// FP32 estimate tm, g at ta, rst estimate.
+// FP32 estimate tm, g at ta, rst estimate; and the bracket's ends as the
+// bracket network reads them: seg = (rst at ta, rst at tb, g at ta, g at tb).
 struct IsoBracket {
   int elem;
   float ta, tb, tm, ga;
   float rst[3];
+  float seg[8];
 };
 
 #define FEMRT_STACK 64
@@ -1009,7 +1015,13 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
         const float den = gprev[k] - g;
         const float wt = den != 0.f ? fminf(fmaxf(gprev[k] / den, 0.f), 1.f) : 0.5f;
         br.tm = tprev + wt * (t - tprev);
-        for (int c = 0; c < 3; ++c) br.rst[c] = rprev[c] + wt * (r[c] - rprev[c]);
+        for (int c = 0; c < 3; ++c) {
+          br.rst[c] = rprev[c] + wt * (r[c] - rprev[c]);
+          br.seg[c] = rprev[c];
+          br.seg[3 + c] = r[c];
+        }
+        br.seg[6] = gprev[k];
+        br.seg[7] = g;
         sink.put(k, br);
         ++w.brackets;
         found_here |= 1u << k;
@@ -1423,6 +1435,11 @@ FEMRT_HD bool seg_scan(const IsoParams& P, const SegInfo& g, MarchState& S, cons
         // endgame converges from there)
         const float h = g.t_out - g.t_in;
         seg_hermite(g, h > 0.f ? fminf(fmaxf((br.tm - g.t_in) / h, 0.f), 1.f) : 0.f, br.rst);
+        // the bracket's ends from the same interpolant (network inputs only)
+        seg_hermite(g, h > 0.f ? fminf(fmaxf((tprev - g.t_in) / h, 0.f), 1.f) : 0.f, br.seg);
+        seg_hermite(g, h > 0.f ? fminf(fmaxf((t - g.t_in) / h, 0.f), 1.f) : 0.f, br.seg + 3);
+        br.seg[6] = gprev[k];
+        br.seg[7] = gk;
         sink.put(k, br);
         ++w.brackets;
         found_here |= 1u << k;
@@ -1696,6 +1713,87 @@ FEMRT_HD bool polish_iso(const TraceMesh& M, const double* o, const double* d, d
   // sign, narrows onto one of its ends with g away from zero -- no crossing
   if (ok && !(g_last == 0.0 || fabs(g_last) <= 1e-9 * (1.0 + fabs(t_out)) * fabs(gp_last))) ok = false;
   return ok;
+}
+
+// Tolerances of the bisection endgames: stop when the bracket is narrower
+// than ISO_BISECT_TREL (1 + |t|) and |g| at its midpoint is at most
+// ISO_BISECT_GTOL, or the bracket is narrower than 1e-13 (1 + |t|).  The
+// midpoint is then within 1e-10 relative of the crossing (its reference
+// coordinates within |J^-1 d| 1e-10 (1 + |t|), below 1e-7 for elements down
+// to 1/64 of the unit cube) and u within 1e-8 of the isovalue: inside the
+// parity bar (t to 1e-6 relative, rst to 1e-7, u to 1e-6).
+#define ISO_BISECT_TREL 1e-10
+#define ISO_BISECT_GTOL 1e-8
+
+// FP64 bisection endgame of one bracket (fem/RayIsosurface.cu:121-171 is
+// the reference's Newton-in-bracket / bisection).  Each step evaluates g =
+// u - iso at the bracket's midpoint (FP64 point inversion warm from the
+// previous step's preimage, then the field).  With `pred` (the bracket
+// network's [lo, hi] in ray parameter, learned bisection), the predicted ends
+// that lie inside the scan's bracket are evaluated first and each narrows
+// the bracket when g there keeps the sign of the end it replaces (only when
+// the prediction is narrower than a quarter of the bracket); the crossing
+// stays the one the scan bracketed.  The pair count of FP64 field
+// evaluations per bracket is what the bracket network saves.
+template <int NG, int NF, typename W>
+FEMRT_HD bool polish_iso_bisect(const TraceMesh& M, const double* o, const double* d, double iso,
+                                const IsoBracket& br, bool use_pred, float pred_lo, float pred_hi,
+                                double& t_out, double* rst_out, double& u_out, W& w)
+{
+  const GeoMode* gc = M.geom + (size_t)br.elem * (NG * NG * NG);
+  const float* fc = M.field + (size_t)br.elem * (NF * NF * NF);
+  double lo = br.ta, hi = br.tb;
+  const bool lo_neg = br.ga < 0.f;
+  double rst[3] = {br.rst[0], br.rst[1], br.rst[2]}, J[9];
+  auto locate = [&](const double* x) -> bool {
+    if (invert_f64<NG>(gc, x, rst, J, w.geo64) && fmax(fabs(rst[0]), fmax(fabs(rst[1]), fabs(rst[2]))) <= 1.001)
+      return true;
+    for (int s = -1; s < 8; ++s) {
+      for (int c = 0; c < 3; ++c)
+        rst[c] = s < 0 ? (double)br.rst[c] : (((s >> c) & 1) ? 0.5 : -0.5);
+      if (invert_f64<NG>(gc, x, rst, J, w.geo64) &&
+          fmax(fabs(rst[0]), fmax(fabs(rst[1]), fabs(rst[2]))) <= 1.001)
+        return true;
+    }
+    return false;
+  };
+  // g at t; false where t has no preimage in the element
+  auto g_at = [&](double t, double& g) -> bool {
+    const double x[3] = {o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
+    if (!locate(x)) return false;
+    u_out = field_value<double, NF>(fc, rst[0], rst[1], rst[2]);
+    ++w.field64;
+    g = u_out - iso;
+    t_out = t;
+    for (int c = 0; c < 3; ++c) rst_out[c] = rst[c];
+    return true;
+  };
+  double g;
+  const double pl = fmin(pred_lo, pred_hi), ph = fmax(pred_lo, pred_hi);
+  // the two evaluations at the predicted ends pay off only when they narrow
+  // the bracket by more than the two bisection steps they cost
+  if (use_pred && ph - pl < 0.25 * (hi - lo)) {
+    if (pl > lo && pl < hi && g_at(pl, g)) {
+      if (g == 0.0) return true;
+      if ((g < 0.0) == lo_neg) lo = pl;
+    }
+    if (ph > lo && ph < hi && g_at(ph, g)) {
+      if (g == 0.0) return true;
+      if ((g < 0.0) != lo_neg) hi = ph;
+    }
+  }
+  for (int it = 0; it < 96; ++it) {
+    const double t = 0.5 * (lo + hi);
+    if (!g_at(t, g)) return false;
+    const double scale = 1.0 + fabs(t);
+    if (g == 0.0 || ((hi - lo) <= ISO_BISECT_TREL * scale && fabs(g) <= ISO_BISECT_GTOL)) return true;
+    // narrowed to the distance scale with g still away from zero: the FP32
+    // scan saw a sign change that the FP64 values do not have (a tangency),
+    // so there is no crossing, as in the Newton endgame
+    if ((hi - lo) <= 1e-13 * scale) return fabs(g) <= ISO_BISECT_GTOL;
+    if ((g < 0.0) == lo_neg) lo = t; else hi = t;
+  }
+  return false;
 }
 
 }  // namespace femrt

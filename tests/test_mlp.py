@@ -72,6 +72,49 @@ def test_oracle_emulation_is_close_to_exact():
     assert np.dot(g_e, g_x) / (np.linalg.norm(g_e) * np.linalg.norm(g_x)) > 0.999
 
 
+def test_bracket_oracle_gradient_matches_finite_differences():
+    """The bracket network (explicit first-layer bias, two outputs, pinball
+    loss at quantiles 0.05 and 0.95): FD of the exact FP64 loss, away from
+    the pinball kinks (continuous random targets put no residual at 0)."""
+    spec = mo.BRACKET
+    p = mo.glorot_params(11, spec=spec)
+    rng = np.random.default_rng(12)
+    feats = rng.uniform(-1.2, 1.2, (48, spec.n_in))
+    t = rng.uniform(-0.5, 1.5, 48)
+    tgt = np.stack([t, t], axis=1)
+    loss, g = mo.loss_grad(p, feats, tgt, emulate=False, spec=spec)
+    o = spec.offsets
+    idx = np.concatenate([rng.choice(np.arange(o["W1"], o["b1"]), 10, replace=False),
+                          np.arange(o["b1"], o["b1"] + 6),
+                          rng.choice(np.arange(o["W2"], o["b2"]), 10, replace=False),
+                          np.arange(o["b2"], o["b2"] + 4),
+                          rng.choice(np.arange(o["W3"], o["b3"]), 8, replace=False),
+                          np.arange(o["b3"], o["end"])])
+    h = 1e-7
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (mo.loss_grad(pp, feats, tgt, False, spec)[0] - mo.loss_grad(pm, feats, tgt, False, spec)[0]) / (2 * h)
+        assert g[k] == pytest.approx(fd, rel=1e-4, abs=1e-9), k
+
+
+def test_pinball_loss_closed_form():
+    """Pinball loss of a constant prediction: minimised at the quantile of
+    the targets (brute force over candidate constants)."""
+    spec = mo.BRACKET
+    t = np.random.default_rng(3).normal(0, 1, 4001)
+    cands = np.linspace(-3, 3, 1201)
+    for j, tau in enumerate(spec.quantiles):
+        losses = []
+        for c in cands:
+            y = np.zeros((t.size, 2))
+            y[:, j] = c
+            losses.append(mo.loss_and_output_grad(y, np.stack([t, t], 1), t.size, spec)[0])
+        best = cands[int(np.argmin(losses))]
+        assert abs(best - np.quantile(t, tau)) < 0.02
+
+
 # -------------------------------------------------------------------- GPU
 
 def _blocks(g):

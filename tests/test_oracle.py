@@ -195,6 +195,45 @@ def test_iso_on_linear_field_is_plane(oracle):
         assert u == pytest.approx(c, abs=1e-12)
 
 
+def _field_of_t(oracle, f, nf):
+    """Modal coefficients (nf^3) of u(r, s, t) = f(t), f a polynomial of
+    degree < nf: the t-modes solve f at nf points; constant in r and s through
+    phi_0 + phi_1 = 1."""
+    ts = np.linspace(-0.9, 0.9, nf)
+    A = np.array([[oracle.lib().orc_modA(k, x) for k in range(nf)] for x in ts])
+    ck = np.linalg.solve(A, f(ts))
+    c = np.zeros((nf, nf, nf))
+    for k in range(nf):
+        c[k, :2, :2] = ck[k]
+    return c.ravel()
+
+
+@pytest.mark.parametrize("roots,tie", [((0.06,), False), ((0.03, 0.06, 0.09), True)])
+def test_iso_several_crossings_in_one_sample_interval_are_ties(oracle, roots, tie):
+    """Straight unit cube, u = 1e3 prod (t - t_i) in the reference t (z =
+    (t + 1) / 2): with all roots inside the 16-sample scan's interval
+    z in [0.5, 0.5625] the scan sees one sign change, but which crossing an
+    endgame converges to is not determined, so the oracle marks it a tie
+    (margin 0); a single root is not a tie.  The root found is one of them."""
+    geom = hex_curved_face_geom(5)
+    geom[2, (0 * 5 + 2) * 5 + 2] = 0.0          # flat bottom face
+    nf = 4
+
+    def f(x):
+        return 1e3 * np.prod([x - r for r in roots], axis=0)
+    field = _field_of_t(oracle, f, nf)
+    orc = oracle.Oracle(geom, field.astype(np.float32), 5, nf)
+    o = np.array([0.3, 0.6, -1.0])
+    d = np.array([0.0, 0.0, 1.0])
+    ok, t, rst, u = orc.iso_segment(0, o, d, 1.0, 2.0, 0.0)
+    assert ok
+    z = t - 1.0
+    assert min(abs((2 * z - 1) - r) for r in roots) < 1e-4   # float32 coefficients move the roots
+    rays = np.array([[*o, *d]], np.float32).T
+    _, iso = orc.trace(rays, [0.0])
+    assert (iso["minabs"][0][0] == 0.0) == tie
+
+
 def test_camera_rays_closed_form(oracle):
     cam = np.array([0.5 + 2.5, 0.5 - 2.5, 0.5 - 2.0, 0.5, 0.5, 0.5, 0, 1, 0, 20.0])
     W, H = 64, 36
