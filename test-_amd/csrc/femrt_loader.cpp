@@ -19,11 +19,16 @@
 // entry, xml_fld_common.cpp:63-100); the FLD's ID lists place coefficients
 // (the reference assigns them in expansion order from the first ELEMENTS
 // block only, xml_fld_field_parser.cpp:36-51).
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <zlib.h>
 
 #include <algorithm>
 #include <array>
 #include <cctype>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,7 +36,9 @@
 #include <fstream>
 #include <map>
 #include <sstream>
+#include <memory>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -45,112 +52,158 @@ struct P3 {
   double x = 0, y = 0, z = 0;
 };
 
-bool read_text(const char* path, std::string& out)
-{
-  std::ifstream f(path, std::ios::binary);
-  if (!f) return false;
-  std::ostringstream ss;
-  ss << f.rdbuf();
-  out = ss.str();
-  return true;
-}
+// A read-only view of a range of characters.
+struct Text {
+  const char* b = nullptr;
+  const char* e = nullptr;
+  bool empty() const { return b >= e; }
+  std::string str() const { return std::string(b, e); }
+};
 
-std::string strip_comments(const std::string& s)
-{
-  std::string out;
-  out.reserve(s.size());
-  size_t p = 0;
-  while (p < s.size()) {
-    const size_t a = s.find("<!--", p);
-    if (a == std::string::npos) {
-      out.append(s, p, std::string::npos);
-      break;
+// A file mapped read-only: the scanner reads it in place, page by page as it
+// goes, with no copy of the text.
+struct MappedFile {
+  int fd = -1;
+  void* p = nullptr;
+  size_t n = 0;
+  long long mtime_ns = 0;
+  bool open(const char* path)
+  {
+    fd = ::open(path, O_RDONLY);
+    if (fd < 0) return false;
+    struct stat st;
+    if (fstat(fd, &st) != 0) return false;
+    n = (size_t)st.st_size;
+    mtime_ns = (long long)st.st_mtim.tv_sec * 1000000000LL + st.st_mtim.tv_nsec;
+    if (n == 0) return true;
+    p = mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (p == MAP_FAILED) {
+      p = nullptr;
+      return false;
     }
-    out.append(s, p, a - p);
-    const size_t b = s.find("-->", a + 4);
-    if (b == std::string::npos) break;
-    p = b + 3;
+    madvise(p, n, MADV_SEQUENTIAL);
+    return true;
   }
-  return out;
+  Text text() const { return Text{(const char*)p, (const char*)p + (p ? n : 0)}; }
+  ~MappedFile()
+  {
+    if (p) munmap(p, n);
+    if (fd >= 0) ::close(fd);
+  }
+};
+
+inline char up(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 32) : c; }
+inline bool is_space(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; }
+
+// Case-insensitive match of the upper-case literal U at p (p < e).
+inline bool imatch(const char* p, const char* e, const char* U)
+{
+  for (; *U; ++U, ++p)
+    if (p >= e || up(*p) != *U) return false;
+  return true;
 }
 
 std::string upper(std::string s)
 {
-  for (auto& c : s) c = (char)std::toupper((unsigned char)c);
+  for (auto& c : s) c = up(c);
   return s;
 }
 
-// Value of attribute key in an opening tag (case-insensitive key).
-bool attr(const std::string& tag, const char* key, std::string& out)
+// The next opening tag <T ...> (T upper case, matched whole and
+// case-insensitive) at or after p; comments <!-- ... --> are skipped.
+const char* find_open(const char* p, const char* e, const char* T)
 {
-  const std::string U = upper(tag), K = upper(key);
-  size_t p = 0;
-  while ((p = U.find(K, p)) != std::string::npos) {
-    const bool left_ok = p > 0 && (std::isspace((unsigned char)U[p - 1]) || U[p - 1] == '<');
-    size_t q = p + K.size();
-    while (q < U.size() && std::isspace((unsigned char)U[q])) ++q;
-    if (left_ok && q < U.size() && U[q] == '=') {
-      ++q;
-      while (q < U.size() && std::isspace((unsigned char)U[q])) ++q;
-      if (q < U.size() && (U[q] == '"' || U[q] == '\'')) {
-        const char quote = U[q];
-        const size_t e = tag.find(quote, q + 1);
-        if (e == std::string::npos) return false;
-        out = tag.substr(q + 1, e - q - 1);
-        return true;
-      }
+  const size_t tl = std::strlen(T);
+  while (p < e) {
+    p = (const char*)std::memchr(p, '<', (size_t)(e - p));
+    if (!p) return nullptr;
+    if (imatch(p, e, "<!--")) {
+      const char* q = p + 4;
+      while (q + 2 < e && !(q[0] == '-' && q[1] == '-' && q[2] == '>')) ++q;
+      p = q + 3;
+      continue;
     }
-    p += K.size();
+    if (imatch(p + 1, e, T)) {
+      const char* q = p + 1 + tl;
+      if (q < e && (is_space(*q) || *q == '>' || *q == '/')) return p;
+    }
+    ++p;
+  }
+  return nullptr;
+}
+
+// The closing tag </T> at or after p (case-insensitive).
+const char* find_close(const char* p, const char* e, const char* T)
+{
+  while (p < e) {
+    p = (const char*)std::memchr(p, '<', (size_t)(e - p));
+    if (!p) return nullptr;
+    if (p + 1 < e && p[1] == '/' && imatch(p + 2, e, T)) {
+      const char* q = p + 2 + std::strlen(T);
+      while (q < e && is_space(*q)) ++q;
+      if (q < e && *q == '>') return p;
+    }
+    ++p;
+  }
+  return nullptr;
+}
+
+// Value of attribute key in an opening tag (case-insensitive key).
+bool attr(Text tag, const char* key, std::string& out)
+{
+  const std::string K = upper(key);
+  for (const char* p = tag.b; p < tag.e; ++p) {
+    if (!imatch(p, tag.e, K.c_str())) continue;
+    const bool left_ok = p > tag.b && (is_space(p[-1]) || p[-1] == '<');
+    const char* q = p + K.size();
+    while (q < tag.e && is_space(*q)) ++q;
+    if (!left_ok || q >= tag.e || *q != '=') continue;
+    ++q;
+    while (q < tag.e && is_space(*q)) ++q;
+    if (q >= tag.e || (*q != '"' && *q != '\'')) continue;
+    const char quote = *q;
+    const char* v = q + 1;
+    const char* w = (const char*)std::memchr(v, quote, (size_t)(tag.e - v));
+    if (!w) return false;
+    out.assign(v, w);
+    return true;
   }
   return false;
 }
 
 // Body of the section <name ...> ... </name>, empty when absent.
-std::string section(const std::string& s, const char* name, std::string* header = nullptr)
+Text section(Text s, const char* name)
 {
-  const std::string U = upper(s);
-  const std::string open = std::string("<") + name, close = std::string("</") + name + ">";
-  size_t p = 0;
-  while ((p = U.find(open, p)) != std::string::npos) {
-    const char c = U[p + open.size()];
-    if (std::isspace((unsigned char)c) || c == '>') break;
-    p += open.size();
-  }
-  if (p == std::string::npos) return {};
-  const size_t gt = s.find('>', p);
-  const size_t e = U.find(close, gt);
-  if (gt == std::string::npos || e == std::string::npos) return {};
-  if (header) *header = s.substr(p, gt - p + 1);
-  return s.substr(gt + 1, e - gt - 1);
+  const std::string N = upper(name);
+  const char* p = find_open(s.b, s.e, N.c_str());
+  if (!p) return {};
+  const char* gt = (const char*)std::memchr(p, '>', (size_t)(s.e - p));
+  if (!gt) return {};
+  const char* c = find_close(gt + 1, s.e, N.c_str());
+  if (!c) return {};
+  return Text{gt + 1, c};
 }
 
 // Calls f(header, body) for every element <tag ...>body</tag> or <tag .../>
-// (tag matched exactly, case-insensitive).
+// (tag matched whole, case-insensitive), in order.
 template <typename F>
-void each(const std::string& s, const char* tag, F f)
+void each(Text s, const char* tag, F f)
 {
-  const std::string U = upper(s);
   const std::string T = upper(tag);
-  const std::string open = "<" + T, close = "</" + T + ">";
-  size_t p = 0;
-  while ((p = U.find(open, p)) != std::string::npos) {
-    const char c = p + open.size() < U.size() ? U[p + open.size()] : '\0';
-    if (!(std::isspace((unsigned char)c) || c == '>' || c == '/')) {
-      p += open.size();
-      continue;
-    }
-    const size_t gt = s.find('>', p);
-    if (gt == std::string::npos) return;
-    const std::string header = s.substr(p, gt - p + 1);
-    if (gt > 0 && s[gt - 1] == '/') {
-      f(header, std::string());
+  const char* p = s.b;
+  while ((p = find_open(p, s.e, T.c_str())) != nullptr) {
+    const char* gt = (const char*)std::memchr(p, '>', (size_t)(s.e - p));
+    if (!gt) return;
+    const Text header{p, gt + 1};
+    if (gt > p && gt[-1] == '/') {
+      f(header, Text{});
       p = gt + 1;
       continue;
     }
-    const size_t e = U.find(close, gt);
-    if (e == std::string::npos) return;
-    f(header, s.substr(gt + 1, e - gt - 1));
-    p = e + close.size();
+    const char* c = find_close(gt + 1, s.e, T.c_str());
+    if (!c) return;
+    f(header, Text{gt + 1, c});
+    p = c + 2 + T.size();
   }
 }
 
@@ -188,7 +241,7 @@ bool parse_modes(const std::string& s, int m[3])
   return true;
 }
 
-std::vector<uint8_t> b64_decode(const std::string& in)
+std::vector<uint8_t> b64_decode(Text in)
 {
   static int lut[256];
   static bool init = false;
@@ -199,10 +252,11 @@ std::vector<uint8_t> b64_decode(const std::string& in)
     init = true;
   }
   std::vector<uint8_t> out;
-  out.reserve(in.size() * 3 / 4);
+  out.reserve((size_t)(in.e - in.b) * 3 / 4);
   unsigned acc = 0;
   int bits = 0;
-  for (unsigned char c : in) {
+  for (const char* p = in.b; p < in.e; ++p) {
+    const unsigned char c = (unsigned char)*p;
     if (c == '=') break;
     const int v = lut[c];
     if (v < 0) continue;  // whitespace
@@ -256,20 +310,28 @@ std::string b64_encode(const uint8_t* d, size_t n)
   return out;
 }
 
-// Whitespace-separated numbers from the start of s; *used (may be NULL)
-// receives the offset where parsing stopped.
-std::vector<double> parse_numbers(const std::string& s, size_t* used = nullptr)
+// Whitespace-separated numbers from the start of s (appended to v); *stop
+// (may be NULL) receives where parsing stopped.
+void parse_numbers(Text s, std::vector<double>& v, const char** stop = nullptr)
+{
+  const char* p = s.b;
+  for (;;) {
+    while (p < s.e && is_space(*p)) ++p;
+    if (p >= s.e) break;
+    const char* q = (*p == '+') ? p + 1 : p;
+    double x;
+    const auto r = std::from_chars(q, s.e, x);
+    if (r.ec != std::errc()) break;
+    v.push_back(x);
+    p = r.ptr;
+  }
+  if (stop) *stop = p;
+}
+
+std::vector<double> numbers(Text s)
 {
   std::vector<double> v;
-  const char* p = s.c_str();
-  char* e = nullptr;
-  for (;;) {
-    const double x = std::strtod(p, &e);
-    if (e == p) break;
-    v.push_back(x);
-    p = e;
-  }
-  if (used) *used = (size_t)(p - s.c_str());
+  parse_numbers(s, v);
   return v;
 }
 
@@ -309,76 +371,110 @@ double dist2(const P3& a, const P3& b)
 
 }  // namespace
 
-int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_elem_out, int* ng_out,
-                 int* nf_out, float* vertices, float* geom, float* field, int* adjacency,
-                 std::string* err)
+namespace {
+
+struct Hex {
+  int id;
+  std::array<int, 8> v;
+  std::array<int, 6> f;
+  bool has_faces;
+};
+
+// Everything the reader takes from the two files, before the per-element
+// output: kept between the size query and the fill call of the two-call
+// protocol, so each file is scanned once.
+struct Parsed {
+  std::string xml_path, fld_path;
+  int flags = 0;
+  long long xml_size = -1, xml_mtime = 0, fld_size = -1, fld_mtime = 0;
+  std::unordered_map<int, P3> verts;
+  std::unordered_map<int, std::array<int, 2>> edges;
+  std::vector<Hex> hexes;
+  std::unordered_map<int, Curved> cedges, cfaces;
+  std::map<std::pair<int, int>, int> edge_id;  // vertex pair -> edge id
+  std::vector<double> coeffs;
+  std::vector<int> coeff_ids;
+  int ng = 0, nf = 0;
+};
+
+// The parse of the last size query on this thread.
+thread_local std::unique_ptr<Parsed> t_parsed;
+
+int parse_files(const char* xml_path, const char* fld_path, int flags, Parsed& P, std::string* err)
 {
   auto bad = [&](int code, const std::string& m) {
     if (err) *err = "load_xml_fld: " + m;
     return code;
   };
-  if (!xml_path || !n_elem_out || !ng_out || !nf_out) return bad(FEMRT_EINVAL, "null argument");
-  std::string xml_raw, fld_raw;
-  if (!read_text(xml_path, xml_raw)) return bad(FEMRT_EIO, std::string("cannot read ") + xml_path);
-  if (fld_path && *fld_path && !read_text(fld_path, fld_raw))
-    return bad(FEMRT_EIO, std::string("cannot read ") + fld_path);
-  const std::string xml = strip_comments(xml_raw);
+  MappedFile xf, ff;
+  if (!xf.open(xml_path)) return bad(FEMRT_EIO, std::string("cannot read ") + xml_path);
+  if (fld_path && *fld_path && !ff.open(fld_path)) return bad(FEMRT_EIO, std::string("cannot read ") + fld_path);
+  P.xml_path = xml_path;
+  P.fld_path = fld_path ? fld_path : "";
+  P.flags = flags;
+  P.xml_size = (long long)xf.n;
+  P.xml_mtime = xf.mtime_ns;
+  P.fld_size = ff.fd >= 0 ? (long long)ff.n : -1;
+  P.fld_mtime = ff.mtime_ns;
+  const Text xml = xf.text();
 
   // ---- vertices, edges, faces
-  std::unordered_map<int, P3> verts;
-  each(section(xml, "VERTEX"), "V", [&](const std::string& h, const std::string& b) {
-    std::string id;
+  std::string id;
+  std::vector<double> v;
+  each(section(xml, "VERTEX"), "V", [&](Text h, Text body) {
     if (!attr(h, "ID", id)) return;
-    const std::vector<double> v = parse_numbers(b);
-    if (v.size() >= 3) verts[std::atoi(id.c_str())] = P3{v[0], v[1], v[2]};
+    v.clear();
+    parse_numbers(body, v);
+    if (v.size() >= 3) P.verts[std::atoi(id.c_str())] = P3{v[0], v[1], v[2]};
   });
-  if (verts.empty()) return bad(FEMRT_EIO, "no vertices");
-  if ((flags & 1) && verts.count(0)) verts[0] = P3{};
-  std::unordered_map<int, std::array<int, 2>> edges;
-  each(section(xml, "EDGE"), "E", [&](const std::string& h, const std::string& b) {
-    std::string id;
+  if (P.verts.empty()) return bad(FEMRT_EIO, "no vertices");
+  if ((flags & 1) && P.verts.count(0)) P.verts[0] = P3{};
+  each(section(xml, "EDGE"), "E", [&](Text h, Text body) {
     if (!attr(h, "ID", id)) return;
-    const std::vector<double> v = parse_numbers(b);
-    if (v.size() >= 2) edges[std::atoi(id.c_str())] = {(int)v[0], (int)v[1]};
+    v.clear();
+    parse_numbers(body, v);
+    if (v.size() >= 2) P.edges[std::atoi(id.c_str())] = {(int)v[0], (int)v[1]};
   });
-  std::unordered_map<int, std::vector<int>> face_verts;  // ring of vertices
-  each(section(xml, "FACE"), "Q", [&](const std::string& h, const std::string& b) {
-    std::string id;
+  std::unordered_map<int, std::array<int, 4>> face_verts;  // ring of vertices
+  each(section(xml, "FACE"), "Q", [&](Text h, Text body) {
     if (!attr(h, "ID", id)) return;
-    const std::vector<double> ev = parse_numbers(b);
-    std::vector<int> ring;
-    for (double x : ev) {
-      const auto it = edges.find((int)x);
-      if (it == edges.end()) return;
-      for (int v : it->second)
-        if (std::find(ring.begin(), ring.end(), v) == ring.end()) ring.push_back(v);
+    v.clear();
+    parse_numbers(body, v);
+    std::array<int, 4> ring;
+    int nr = 0;
+    for (double x : v) {
+      const auto it = P.edges.find((int)x);
+      if (it == P.edges.end()) return;
+      for (int w : it->second) {
+        bool seen = false;
+        for (int k = 0; k < nr; ++k) seen = seen || ring[k] == w;
+        if (!seen) {
+          if (nr == 4) return;
+          ring[nr++] = w;
+        }
+      }
     }
-    if (ring.size() == 4) face_verts[std::atoi(id.c_str())] = ring;
+    if (nr == 4) face_verts[std::atoi(id.c_str())] = ring;
   });
 
   // ---- hexes
-  struct Hex {
-    int id;
-    std::array<int, 8> v;
-    std::array<int, 6> f;
-    bool has_faces;
-  };
-  std::vector<Hex> hexes;
-  const std::string elem_sec = section(xml, "ELEMENT");
+  const Text elem_sec = section(xml, "ELEMENT");
   for (const char* tag : {"H", "HEX"})
-    each(elem_sec, tag, [&](const std::string& h, const std::string& b) {
-      std::string id;
+    each(elem_sec, tag, [&](Text h, Text body) {
       Hex x{};
-      x.id = attr(h, "ID", id) ? std::atoi(id.c_str()) : (int)hexes.size();
-      const std::vector<double> v = parse_numbers(b);
+      x.id = attr(h, "ID", id) ? std::atoi(id.c_str()) : (int)P.hexes.size();
+      v.clear();
+      parse_numbers(body, v);
       if (v.size() == 6) {
         x.has_faces = true;
         for (int k = 0; k < 6; ++k) x.f[k] = (int)v[k];
         for (int c = 0; c < 8; ++c) {
           x.v[c] = -1;
-          const auto& A = face_verts[x.f[kCornerFaces[c][0]]];
-          const auto& B = face_verts[x.f[kCornerFaces[c][1]]];
-          const auto& C = face_verts[x.f[kCornerFaces[c][2]]];
+          const auto ia = face_verts.find(x.f[kCornerFaces[c][0]]);
+          const auto ib = face_verts.find(x.f[kCornerFaces[c][1]]);
+          const auto ic = face_verts.find(x.f[kCornerFaces[c][2]]);
+          if (ia == face_verts.end() || ib == face_verts.end() || ic == face_verts.end()) continue;
+          const auto &A = ia->second, &B = ib->second, &C = ic->second;
           for (int a : A)
             if (std::count(B.begin(), B.end(), a) && std::count(C.begin(), C.end(), a)) x.v[c] = a;
         }
@@ -389,46 +485,44 @@ int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_e
       } else {
         return;
       }
-      hexes.push_back(x);
+      P.hexes.push_back(x);
     });
-  if (hexes.empty()) return bad(FEMRT_EIO, "no hexahedral elements");
-  for (const Hex& x : hexes)
+  if (P.hexes.empty()) return bad(FEMRT_EIO, "no hexahedral elements");
+  for (const Hex& x : P.hexes)
     for (int c = 0; c < 8; ++c)
-      if (x.v[c] < 0 || !verts.count(x.v[c]))
+      if (x.v[c] < 0 || !P.verts.count(x.v[c]))
         return bad(FEMRT_EIO, "element " + std::to_string(x.id) + ": unresolved corner vertex");
-  std::sort(hexes.begin(), hexes.end(), [](const Hex& a, const Hex& b) { return a.id < b.id; });
+  std::sort(P.hexes.begin(), P.hexes.end(), [](const Hex& a, const Hex& b) { return a.id < b.id; });
 
   // ---- curved edges and faces
-  std::unordered_map<int, Curved> cedges, cfaces;
-  const std::string cur = section(xml, "CURVED");
+  const Text cur = section(xml, "CURVED");
   auto read_curved = [&](const char* tag, const char* key, std::unordered_map<int, Curved>& dst) {
-    each(cur, tag, [&](const std::string& h, const std::string& b) {
-      std::string id, np, ty;
+    std::string np, ty;
+    each(cur, tag, [&](Text h, Text body) {
       if (!attr(h, key, id) || !attr(h, "NUMPOINTS", np)) return;
       Curved c;
       c.npts = std::atoi(np.c_str());
       c.gll = attr(h, "TYPE", ty) && upper(ty).find("LOBATTO") != std::string::npos;
-      const std::vector<double> v = parse_numbers(b);
+      v.clear();
+      parse_numbers(body, v);
+      c.pts.reserve(v.size() / 3);
       for (size_t k = 0; k + 2 < v.size(); k += 3) c.pts.push_back(P3{v[k], v[k + 1], v[k + 2]});
-      if (!c.pts.empty()) dst[std::atoi(id.c_str())] = c;
+      if (!c.pts.empty()) dst[std::atoi(id.c_str())] = std::move(c);
     });
   };
-  read_curved("E", "EDGEID", cedges);
-  read_curved("F", "FACEID", cfaces);
+  read_curved("E", "EDGEID", P.cedges);
+  read_curved("F", "FACEID", P.cfaces);
 
   // ---- field modes: FLD NUMMODESPERDIR, else the XML expansion
   int fm[3] = {0, 0, 0};
-  std::string exp_hdr;
-  each(section(xml, "EXPANSIONS"), "E", [&](const std::string& h, const std::string&) {
+  each(section(xml, "EXPANSIONS"), "E", [&](Text h, Text) {
     std::string m;
     if (!fm[0] && (attr(h, "NUMMODESPERDIR", m) || attr(h, "NUMMODES", m))) parse_modes(m, fm);
   });
-  std::vector<double> coeffs;
-  std::vector<int> coeff_ids;
   int fld_modes[3] = {0, 0, 0};
-  if (!fld_raw.empty()) {
+  if (ff.fd >= 0) {
     bool ok = true;
-    each(fld_raw, "ELEMENTS", [&](const std::string& h, const std::string& b) {
+    each(ff.text(), "ELEMENTS", [&](Text h, Text body) {
       if (!ok) return;
       std::string m, ids, comp, bits, fields;
       int bm[3] = {0, 0, 0};
@@ -440,30 +534,30 @@ int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_e
       // COMPRESSED attribute)
       bool ascii = !(attr(h, "COMPRESSED", comp) && upper(comp).find("B64Z") != std::string::npos);
       if (ascii) {
-        size_t used = 0;
-        vals = parse_numbers(b, &used);
-        if (b.find_first_not_of(" \t\r\n", used) != std::string::npos) {
+        const char* stop = nullptr;
+        parse_numbers(body, vals, &stop);
+        while (stop < body.e && is_space(*stop)) ++stop;
+        if (stop < body.e) {
           ascii = false;
           vals.clear();
         }
       }
       if (!ascii) {
         std::vector<uint8_t> raw;
-        if (!inflate_all(b64_decode(b), raw)) {
+        if (!inflate_all(b64_decode(body), raw)) {
           ok = false;
           return;
         }
         const bool b32 = attr(h, "BITSIZE", bits) && std::atoi(bits.c_str()) == 32;
         const size_t w = b32 ? 4 : 8;
-        for (size_t k = 0; k + w <= raw.size(); k += w) {
+        vals.resize(raw.size() / w);
+        for (size_t k = 0; k < vals.size(); ++k) {
           if (b32) {
             float f;
-            std::memcpy(&f, &raw[k], 4);
-            vals.push_back(f);
+            std::memcpy(&f, &raw[k * 4], 4);
+            vals[k] = f;
           } else {
-            double d;
-            std::memcpy(&d, &raw[k], 8);
-            vals.push_back(d);
+            std::memcpy(&vals[k], &raw[k * 8], 8);
           }
         }
       }
@@ -472,11 +566,11 @@ int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_e
       const size_t per = (size_t)(bm[0] ? bm[0] * bm[1] * bm[2] : 0);
       if (!idl.empty() && per && vals.size() >= per * idl.size() * nfield) {
         for (size_t k = 0; k < idl.size(); ++k) {
-          coeff_ids.push_back(idl[k]);
-          coeffs.insert(coeffs.end(), vals.begin() + k * per, vals.begin() + (k + 1) * per);
+          P.coeff_ids.push_back(idl[k]);
+          P.coeffs.insert(P.coeffs.end(), vals.begin() + k * per, vals.begin() + (k + 1) * per);
         }
       } else {
-        coeffs.insert(coeffs.end(), vals.begin(), vals.end());
+        P.coeffs.insert(P.coeffs.end(), vals.begin(), vals.end());
       }
     });
     if (!ok) return bad(FEMRT_EIO, "FLD payload is not valid base64 + zlib");
@@ -484,200 +578,268 @@ int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_e
   }
   if (!fm[0]) return bad(FEMRT_EIO, "no field mode counts (EXPANSIONS or FLD)");
   if (fm[0] != fm[1] || fm[1] != fm[2]) return bad(FEMRT_EUNSUP, "anisotropic field orders");
-  const int nf = fm[0];
+  P.nf = fm[0];
 
   // ---- geometry modes
-  std::map<std::pair<int, int>, int> edge_id;  // vertex pair -> edge id
-  for (const auto& kv : edges)
-    edge_id[{std::min(kv.second[0], kv.second[1]), std::max(kv.second[0], kv.second[1])}] = kv.first;
+  for (const auto& kv : P.edges)
+    P.edge_id[{std::min(kv.second[0], kv.second[1]), std::max(kv.second[0], kv.second[1])}] = kv.first;
   int ng = 2;
-  for (const Hex& x : hexes) {
+  for (const Hex& x : P.hexes) {
     if (x.has_faces)
       for (int f : x.f) {
-        const auto it = cfaces.find(f);
-        if (it != cfaces.end()) ng = std::max(ng, (int)std::lround(std::sqrt((double)it->second.npts)));
+        const auto it = P.cfaces.find(f);
+        if (it != P.cfaces.end()) ng = std::max(ng, (int)std::lround(std::sqrt((double)it->second.npts)));
       }
     for (int e = 0; e < 12; ++e) {
       const int a = x.v[kEdges[e][0]], b = x.v[kEdges[e][1]];
-      const auto ei = edge_id.find({std::min(a, b), std::max(a, b)});
-      if (ei == edge_id.end()) continue;
-      const auto ce = cedges.find(ei->second);
-      if (ce != cedges.end()) ng = std::max(ng, (int)ce->second.pts.size());
+      const auto ei = P.edge_id.find({std::min(a, b), std::max(a, b)});
+      if (ei == P.edge_id.end()) continue;
+      const auto ce = P.cedges.find(ei->second);
+      if (ce != P.cedges.end()) ng = std::max(ng, (int)ce->second.pts.size());
     }
   }
-  if (flags & 2) ng = std::max(ng, nf);
-  if (ng > 16 || nf > 16) return bad(FEMRT_EUNSUP, "more than 16 modes per direction");
+  if (flags & 2) ng = std::max(ng, P.nf);
+  if (ng > 16 || P.nf > 16) return bad(FEMRT_EUNSUP, "more than 16 modes per direction");
+  P.ng = ng;
+  return FEMRT_OK;
+}
 
-  const int n = (int)hexes.size();
+// Geometry map of element ie on the GLL grid, fitted to modes (the
+// reference's BuildHexGeometry, xml_fld_geometry_builder.cpp:68-381).
+void build_geometry(const Parsed& P, int ie, const std::vector<double>& gll, const std::vector<double>& inv,
+                    std::vector<double>& nodal, std::vector<double>& coef, float* geom)
+{
+  const int ng = P.ng;
+  const size_t g3 = (size_t)ng * ng * ng;
+  const Hex& x = P.hexes[ie];
+  P3 C[8];
+  for (int c = 0; c < 8; ++c) C[c] = P.verts.at(x.v[c]);
+  auto at = [&](int i, int j, int k) { return ((size_t)k * ng + j) * ng + i; };
+  for (int k = 0; k < ng; ++k)
+    for (int j = 0; j < ng; ++j)
+      for (int i = 0; i < ng; ++i) {
+        const double r = gll[i], s = gll[j], t = gll[k];
+        double p[3] = {0, 0, 0};
+        for (int c = 0; c < 8; ++c) {
+          const double w = 0.125 * (1 + kCornerRst[c][0] * r) * (1 + kCornerRst[c][1] * s) *
+                           (1 + kCornerRst[c][2] * t);
+          p[0] += w * C[c].x;
+          p[1] += w * C[c].y;
+          p[2] += w * C[c].z;
+        }
+        for (int d = 0; d < 3; ++d) nodal[d * g3 + at(i, j, k)] = p[d];
+      }
+  // curved edges: interpolate the point set to the GLL nodes, in the
+  // direction of increasing local parameter
+  for (int e = 0; e < 12; ++e) {
+    const int lo = x.v[kEdges[e][0]], hi = x.v[kEdges[e][1]];
+    const auto ei = P.edge_id.find({std::min(lo, hi), std::max(lo, hi)});
+    if (ei == P.edge_id.end()) continue;
+    const auto ce = P.cedges.find(ei->second);
+    if (ce == P.cedges.end() || (int)ce->second.pts.size() < 2) continue;
+    const bool rev = P.edges.at(ei->second)[0] != lo;
+    const int ns = (int)ce->second.pts.size();
+    const std::vector<double> L = lagrange_matrix(nodes_of(ns, ce->second.gll), gll);
+    const int ax = kEdges[e][2];
+    const int* cl = kCornerRst[kEdges[e][0]];
+    for (int m = 0; m < ng; ++m) {
+      P3 q;
+      for (int a = 0; a < ns; ++a) {
+        const P3& s = ce->second.pts[rev ? ns - 1 - a : a];
+        const double w = L[(size_t)m * ns + a];
+        q.x += w * s.x;
+        q.y += w * s.y;
+        q.z += w * s.z;
+      }
+      int idx[3];
+      for (int d = 0; d < 3; ++d) idx[d] = cl[d] < 0 ? 0 : ng - 1;
+      idx[ax] = m;
+      const size_t id = at(idx[0], idx[1], idx[2]);
+      nodal[id] = q.x;
+      nodal[g3 + id] = q.y;
+      nodal[2 * g3 + id] = q.z;
+    }
+  }
+  // curved faces: orient the point grid to the local face, interpolate to
+  // the GLL grid, blend the difference along the normal coordinate
+  if (x.has_faces)
+    for (int lf = 0; lf < 6; ++lf) {
+      const auto cf = P.cfaces.find(x.f[lf]);
+      if (cf == P.cfaces.end()) continue;
+      const int ns = (int)std::lround(std::sqrt((double)cf->second.pts.size()));
+      if (ns * ns != (int)cf->second.pts.size() || ns < 2) continue;
+      const auto& G = cf->second.pts;
+      P3 want[4];
+      for (int c = 0; c < 4; ++c) want[c] = C[kFaceCorners[lf][c]];
+      // the one of the 8 grid symmetries whose corners match the face's
+      int best = -1;
+      double best_err = 1e300;
+      for (int sym = 0; sym < 8; ++sym) {
+        auto src = [&](int i, int j) {
+          int a = (sym & 1) ? j : i, b = (sym & 1) ? i : j;
+          if (sym & 2) a = ns - 1 - a;
+          if (sym & 4) b = ns - 1 - b;
+          return G[(size_t)b * ns + a];
+        };
+        const double e = dist2(src(0, 0), want[0]) + dist2(src(ns - 1, 0), want[1]) +
+                         dist2(src(ns - 1, ns - 1), want[2]) + dist2(src(0, ns - 1), want[3]);
+        if (e < best_err) {
+          best_err = e;
+          best = sym;
+        }
+      }
+      const std::vector<double> L = lagrange_matrix(nodes_of(ns, cf->second.gll), gll);
+      const int fix = (lf == 0 || lf == 5) ? 2 : ((lf == 1 || lf == 3) ? 1 : 0);
+      const bool plus = lf == 2 || lf == 3 || lf == 5;
+      const int ua = (lf == 2 || lf == 4) ? 1 : 0, va = (lf == 0 || lf == 5) ? 1 : 2;
+      for (int b2 = 0; b2 < ng; ++b2)
+        for (int a2 = 0; a2 < ng; ++a2) {
+          P3 q;
+          for (int b = 0; b < ns; ++b)
+            for (int a = 0; a < ns; ++a) {
+              int sa = (best & 1) ? b : a, sb = (best & 1) ? a : b;
+              if (best & 2) sa = ns - 1 - sa;
+              if (best & 4) sb = ns - 1 - sb;
+              const P3& s = G[(size_t)sb * ns + sa];
+              const double w = L[(size_t)a2 * ns + a] * L[(size_t)b2 * ns + b];
+              q.x += w * s.x;
+              q.y += w * s.y;
+              q.z += w * s.z;
+            }
+          int idx[3];
+          idx[ua] = a2;
+          idx[va] = b2;
+          idx[fix] = plus ? ng - 1 : 0;
+          const size_t fid = at(idx[0], idx[1], idx[2]);
+          const double diff[3] = {q.x - nodal[fid], q.y - nodal[g3 + fid], q.z - nodal[2 * g3 + fid]};
+          for (int m = 0; m < ng; ++m) {
+            idx[fix] = m;
+            const double w = plus ? 0.5 * (1 + gll[m]) : 0.5 * (1 - gll[m]);
+            const size_t id = at(idx[0], idx[1], idx[2]);
+            for (int d = 0; d < 3; ++d) nodal[d * g3 + id] += diff[d] * w;
+          }
+        }
+    }
+  for (int d = 0; d < 3; ++d) {
+    tensor_fit(ng, inv, &nodal[d * g3], coef.data());
+    for (size_t m = 0; m < g3; ++m) geom[((size_t)ie * 3 + d) * g3 + m] = (float)coef[m];
+  }
+}
+
+// Runs f(lo, hi) over [0, n) split across the host's cores.
+template <typename F>
+void parallel_ranges(int n, F f)
+{
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int T = std::max(1, std::min({hw, 16, n / 256 + 1}));
+  if (T == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t) th.emplace_back(f, (int)((long long)n * t / T), (int)((long long)n * (t + 1) / T));
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+// Two-call protocol: with every output pointer NULL only the sizes are
+// returned (the parse is kept for the next call on this thread with the same
+// files, flags and file times); otherwise the outputs are filled.
+int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_elem_out, int* ng_out,
+                 int* nf_out, float* vertices, float* geom, float* field, int* adjacency,
+                 std::string* err)
+{
+  if (!xml_path || !n_elem_out || !ng_out || !nf_out) {
+    if (err) *err = "load_xml_fld: null argument";
+    return FEMRT_EINVAL;
+  }
+  const bool sizes_only = !vertices && !geom && !field && !adjacency;
+  std::unique_ptr<Parsed> P = std::move(t_parsed);
+  bool reuse = false;
+  if (P) {
+    struct stat sx, sf;
+    const std::string fp = fld_path ? fld_path : "";
+    reuse = P->xml_path == xml_path && P->fld_path == fp && P->flags == flags && stat(xml_path, &sx) == 0 &&
+            (long long)sx.st_size == P->xml_size &&
+            (long long)sx.st_mtim.tv_sec * 1000000000LL + sx.st_mtim.tv_nsec == P->xml_mtime &&
+            (fp.empty() ? P->fld_size < 0
+                        : (stat(fp.c_str(), &sf) == 0 && (long long)sf.st_size == P->fld_size &&
+                           (long long)sf.st_mtim.tv_sec * 1000000000LL + sf.st_mtim.tv_nsec == P->fld_mtime));
+  }
+  if (!reuse) {
+    P.reset(new Parsed);
+    const int rc = parse_files(xml_path, fld_path, flags, *P, err);
+    if (rc != FEMRT_OK) return rc;
+  }
+  const int n = (int)P->hexes.size(), ng = P->ng, nf = P->nf;
   *n_elem_out = n;
   *ng_out = ng;
   *nf_out = nf;
-  if (!vertices && !geom && !field && !adjacency) return FEMRT_OK;
+  if (sizes_only) {
+    t_parsed = std::move(P);
+    return FEMRT_OK;
+  }
 
-  // ---- per-element output
+  // ---- per-element output, on all host cores
   std::unordered_map<int, size_t> coeff_pos;
-  for (size_t k = 0; k < coeff_ids.size(); ++k) coeff_pos[coeff_ids[k]] = k;
+  for (size_t k = 0; k < P->coeff_ids.size(); ++k) coeff_pos[P->coeff_ids[k]] = k;
   const size_t f3 = (size_t)nf * nf * nf, g3 = (size_t)ng * ng * ng;
   const std::vector<double> gll = gll_nodes(ng);
   std::vector<double> inv;
   modal_fit_1d_matrix(ng, gll, inv);
-  std::vector<double> nodal(3 * g3), coef(g3);
-  for (int ie = 0; ie < n; ++ie) {
-    const Hex& x = hexes[ie];
-    P3 C[8];
-    for (int c = 0; c < 8; ++c) C[c] = verts[x.v[c]];
-    if (vertices) {
-      for (int c = 0; c < 8; ++c) {
-        vertices[(size_t)ie * 24 + 3 * c + 0] = (float)C[c].x;
-        vertices[(size_t)ie * 24 + 3 * c + 1] = (float)C[c].y;
-        vertices[(size_t)ie * 24 + 3 * c + 2] = (float)C[c].z;
-      }
-    }
-    if (field) {
-      size_t pos = (size_t)-1;
-      if (!coeff_ids.empty()) {
-        const auto it = coeff_pos.find(x.id);
-        if (it != coeff_pos.end()) pos = it->second * f3;
-      } else if ((size_t)(ie + 1) * f3 <= coeffs.size()) {
-        pos = (size_t)ie * f3;  // expansion order
-      }
-      for (size_t m = 0; m < f3; ++m)
-        field[(size_t)ie * f3 + m] = pos == (size_t)-1 ? 0.f : (float)coeffs[pos + m];
-    }
-    if (geom) {
-      auto at = [&](int i, int j, int k) { return ((size_t)k * ng + j) * ng + i; };
-      for (int k = 0; k < ng; ++k)
-        for (int j = 0; j < ng; ++j)
-          for (int i = 0; i < ng; ++i) {
-            const double r = gll[i], s = gll[j], t = gll[k];
-            double p[3] = {0, 0, 0};
-            for (int c = 0; c < 8; ++c) {
-              const double w = 0.125 * (1 + kCornerRst[c][0] * r) * (1 + kCornerRst[c][1] * s) *
-                               (1 + kCornerRst[c][2] * t);
-              p[0] += w * C[c].x;
-              p[1] += w * C[c].y;
-              p[2] += w * C[c].z;
-            }
-            for (int d = 0; d < 3; ++d) nodal[d * g3 + at(i, j, k)] = p[d];
-          }
-      // curved edges: interpolate the point set to the GLL nodes, in the
-      // direction of increasing local parameter
-      for (int e = 0; e < 12; ++e) {
-        const int lo = x.v[kEdges[e][0]], hi = x.v[kEdges[e][1]];
-        const auto ei = edge_id.find({std::min(lo, hi), std::max(lo, hi)});
-        if (ei == edge_id.end()) continue;
-        const auto ce = cedges.find(ei->second);
-        if (ce == cedges.end() || (int)ce->second.pts.size() < 2) continue;
-        const bool rev = edges[ei->second][0] != lo;
-        const int ns = (int)ce->second.pts.size();
-        const std::vector<double> L = lagrange_matrix(nodes_of(ns, ce->second.gll), gll);
-        const int ax = kEdges[e][2];
-        const int* cl = kCornerRst[kEdges[e][0]];
-        for (int m = 0; m < ng; ++m) {
-          P3 q;
-          for (int a = 0; a < ns; ++a) {
-            const P3& s = ce->second.pts[rev ? ns - 1 - a : a];
-            const double w = L[(size_t)m * ns + a];
-            q.x += w * s.x;
-            q.y += w * s.y;
-            q.z += w * s.z;
-          }
-          int idx[3];
-          for (int d = 0; d < 3; ++d) idx[d] = cl[d] < 0 ? 0 : ng - 1;
-          idx[ax] = m;
-          const size_t id = at(idx[0], idx[1], idx[2]);
-          nodal[id] = q.x;
-          nodal[g3 + id] = q.y;
-          nodal[2 * g3 + id] = q.z;
+  const Parsed& Q = *P;
+  parallel_ranges(n, [&](int lo, int hi) {
+    std::vector<double> nodal(3 * g3), coef(g3);
+    for (int ie = lo; ie < hi; ++ie) {
+      const Hex& x = Q.hexes[ie];
+      if (vertices)
+        for (int c = 0; c < 8; ++c) {
+          const P3& C = Q.verts.at(x.v[c]);
+          vertices[(size_t)ie * 24 + 3 * c + 0] = (float)C.x;
+          vertices[(size_t)ie * 24 + 3 * c + 1] = (float)C.y;
+          vertices[(size_t)ie * 24 + 3 * c + 2] = (float)C.z;
         }
-      }
-      // curved faces: orient the point grid to the local face, interpolate to
-      // the GLL grid, blend the difference along the normal coordinate
-      if (x.has_faces)
-        for (int lf = 0; lf < 6; ++lf) {
-          const auto cf = cfaces.find(x.f[lf]);
-          if (cf == cfaces.end()) continue;
-          const int ns = (int)std::lround(std::sqrt((double)cf->second.pts.size()));
-          if (ns * ns != (int)cf->second.pts.size() || ns < 2) continue;
-          const auto& G = cf->second.pts;
-          P3 want[4];
-          for (int c = 0; c < 4; ++c) want[c] = C[kFaceCorners[lf][c]];
-          // the one of the 8 grid symmetries whose corners match the face's
-          int best = -1;
-          double best_err = 1e300;
-          for (int sym = 0; sym < 8; ++sym) {
-            auto src = [&](int i, int j) {
-              int a = (sym & 1) ? j : i, b = (sym & 1) ? i : j;
-              if (sym & 2) a = ns - 1 - a;
-              if (sym & 4) b = ns - 1 - b;
-              return G[(size_t)b * ns + a];
-            };
-            const double e = dist2(src(0, 0), want[0]) + dist2(src(ns - 1, 0), want[1]) +
-                             dist2(src(ns - 1, ns - 1), want[2]) + dist2(src(0, ns - 1), want[3]);
-            if (e < best_err) {
-              best_err = e;
-              best = sym;
-            }
-          }
-          const std::vector<double> L = lagrange_matrix(nodes_of(ns, cf->second.gll), gll);
-          const int fix = (lf == 0 || lf == 5) ? 2 : ((lf == 1 || lf == 3) ? 1 : 0);
-          const bool plus = lf == 2 || lf == 3 || lf == 5;
-          const int ua = (lf == 2 || lf == 4) ? 1 : 0, va = (lf == 0 || lf == 5) ? 1 : 2;
-          for (int b2 = 0; b2 < ng; ++b2)
-            for (int a2 = 0; a2 < ng; ++a2) {
-              P3 q;
-              for (int b = 0; b < ns; ++b)
-                for (int a = 0; a < ns; ++a) {
-                  int sa = (best & 1) ? b : a, sb = (best & 1) ? a : b;
-                  if (best & 2) sa = ns - 1 - sa;
-                  if (best & 4) sb = ns - 1 - sb;
-                  const P3& s = G[(size_t)sb * ns + sa];
-                  const double w = L[(size_t)a2 * ns + a] * L[(size_t)b2 * ns + b];
-                  q.x += w * s.x;
-                  q.y += w * s.y;
-                  q.z += w * s.z;
-                }
-              int idx[3];
-              idx[ua] = a2;
-              idx[va] = b2;
-              idx[fix] = plus ? ng - 1 : 0;
-              const size_t fid = at(idx[0], idx[1], idx[2]);
-              const double diff[3] = {q.x - nodal[fid], q.y - nodal[g3 + fid], q.z - nodal[2 * g3 + fid]};
-              for (int m = 0; m < ng; ++m) {
-                idx[fix] = m;
-                const double w = plus ? 0.5 * (1 + gll[m]) : 0.5 * (1 - gll[m]);
-                const size_t id = at(idx[0], idx[1], idx[2]);
-                for (int d = 0; d < 3; ++d) nodal[d * g3 + id] += diff[d] * w;
-              }
-            }
+      if (field) {
+        size_t pos = (size_t)-1;
+        if (!Q.coeff_ids.empty()) {
+          const auto it = coeff_pos.find(x.id);
+          if (it != coeff_pos.end()) pos = it->second * f3;
+        } else if ((size_t)(ie + 1) * f3 <= Q.coeffs.size()) {
+          pos = (size_t)ie * f3;  // expansion order
         }
-      for (int d = 0; d < 3; ++d) {
-        tensor_fit(ng, inv, &nodal[d * g3], coef.data());
-        for (size_t m = 0; m < g3; ++m) geom[((size_t)ie * 3 + d) * g3 + m] = (float)coef[m];
+        for (size_t m = 0; m < f3; ++m)
+          field[(size_t)ie * f3 + m] = pos == (size_t)-1 ? 0.f : (float)Q.coeffs[pos + m];
       }
+      if (geom) build_geometry(Q, ie, gll, inv, nodal, coef, geom);
     }
-  }
+  });
 
-  // ---- adjacency: elements sharing a face (by face id, or by its 4 vertices)
+  // ---- adjacency: elements sharing a face (by its 4 vertices)
   if (adjacency) {
-    std::map<std::array<int, 4>, std::vector<std::pair<int, int>>> owners;
+    std::vector<std::pair<std::array<int, 4>, int>> keys((size_t)n * 6);  // face vertices, ie * 6 + lf
     for (int ie = 0; ie < n; ++ie)
       for (int lf = 0; lf < 6; ++lf) {
         std::array<int, 4> key;
-        for (int c = 0; c < 4; ++c) key[c] = hexes[ie].v[kFaceCorners[lf][c]];
+        for (int c = 0; c < 4; ++c) key[c] = Q.hexes[ie].v[kFaceCorners[lf][c]];
         std::sort(key.begin(), key.end());
-        owners[key].push_back({ie, lf});
+        keys[(size_t)ie * 6 + lf] = {key, ie * 6 + lf};
         adjacency[((size_t)ie * 6 + lf) * 2] = -1;
         adjacency[((size_t)ie * 6 + lf) * 2 + 1] = -1;
       }
-    for (const auto& kv : owners)
-      if (kv.second.size() == 2) {
-        const auto a = kv.second[0], b = kv.second[1];
-        adjacency[((size_t)a.first * 6 + a.second) * 2] = b.first;
-        adjacency[((size_t)a.first * 6 + a.second) * 2 + 1] = b.second;
-        adjacency[((size_t)b.first * 6 + b.second) * 2] = a.first;
-        adjacency[((size_t)b.first * 6 + b.second) * 2 + 1] = a.second;
+    std::sort(keys.begin(), keys.end());
+    for (size_t k = 0; k < keys.size();) {
+      size_t j = k + 1;
+      while (j < keys.size() && keys[j].first == keys[k].first) ++j;
+      if (j - k == 2) {
+        const int a = keys[k].second, b = keys[k + 1].second;
+        adjacency[(size_t)a * 2] = b / 6;
+        adjacency[(size_t)a * 2 + 1] = b % 6;
+        adjacency[(size_t)b * 2] = a / 6;
+        adjacency[(size_t)b * 2 + 1] = a % 6;
       }
+      k = j;
+    }
   }
   return FEMRT_OK;
 }
@@ -760,32 +922,54 @@ int save_xml_fld(const char* xml_path, const char* fld_path, int n, int ng, int 
                  hf[e][3], hf[e][4], hf[e][5]);
   std::fprintf(fx, "    </ELEMENT>\n    <CURVED>\n");
   // face points from the owning element's modal map, on the face's own
-  // (u, v) grid: point (a, b) at u = -1 + 2a/(ng-1), v = -1 + 2b/(ng-1)
+  // (u, v) grid: point (a, b) at u = -1 + 2a/(ng-1), v = -1 + 2b/(ng-1);
+  // the faces are evaluated on all host cores into text blocks written in
+  // face order
   const size_t g3 = (size_t)ng * ng * ng;
-  for (size_t k = 0; k < fowner.size(); ++k) {
-    const int e = fowner[k].first, lf = fowner[k].second;
-    std::fprintf(fx, "      <F ID=\"%zu\" FACEID=\"%zu\" NUMPOINTS=\"%d\" TYPE=\"PolyEvenlySpaced\">", k, k, ng * ng);
-    const int fix = (lf == 0 || lf == 5) ? 2 : ((lf == 1 || lf == 3) ? 1 : 0);
-    const double fv = (lf == 2 || lf == 3 || lf == 5) ? 1.0 : -1.0;
-    const int ua = (lf == 2 || lf == 4) ? 1 : 0, va = (lf == 0 || lf == 5) ? 1 : 2;
-    for (int b = 0; b < ng; ++b)
-      for (int a = 0; a < ng; ++a) {
-        double rst[3];
-        rst[fix] = fv;
-        rst[ua] = -1.0 + 2.0 * a / (ng - 1);
-        rst[va] = -1.0 + 2.0 * b / (ng - 1);
-        double x[3] = {0, 0, 0};
-        for (int kk = 0; kk < ng; ++kk)
-          for (int jj = 0; jj < ng; ++jj)
-            for (int ii = 0; ii < ng; ++ii) {
-              const double w = modA(ii, rst[0]) * modA(jj, rst[1]) * modA(kk, rst[2]);
-              for (int d = 0; d < 3; ++d)
-                x[d] += w * geom[((size_t)e * 3 + d) * g3 + ((size_t)kk * ng + jj) * ng + ii];
-            }
-        std::fprintf(fx, " %.9g %.9g %.9g", x[0], x[1], x[2]);
-      }
-    std::fprintf(fx, " </F>\n");
+  std::vector<double> phi((size_t)(ng + 2) * ng);  // phi[x][i]: x = grid point a, ng: -1, ng + 1: +1
+  for (int a = 0; a < ng + 2; ++a) {
+    const double xa = a < ng ? -1.0 + 2.0 * a / (ng - 1) : (a == ng ? -1.0 : 1.0);
+    for (int i = 0; i < ng; ++i) phi[(size_t)a * ng + i] = modA(i, xa);
   }
+  const int nfaces = (int)fowner.size();
+  std::vector<std::string> ftext(nfaces);
+  parallel_ranges(nfaces, [&](int lo, int hi) {
+    char buf[96];
+    for (int k = lo; k < hi; ++k) {
+      const int e = fowner[k].first, lf = fowner[k].second;
+      std::string& out = ftext[k];
+      std::snprintf(buf, sizeof buf, "      <F ID=\"%d\" FACEID=\"%d\" NUMPOINTS=\"%d\" TYPE=\"PolyEvenlySpaced\">",
+                    k, k, ng * ng);
+      out += buf;
+      const int fix = (lf == 0 || lf == 5) ? 2 : ((lf == 1 || lf == 3) ? 1 : 0);
+      const int fv = (lf == 2 || lf == 3 || lf == 5) ? ng + 1 : ng;
+      const int ua = (lf == 2 || lf == 4) ? 1 : 0, va = (lf == 0 || lf == 5) ? 1 : 2;
+      for (int b = 0; b < ng; ++b)
+        for (int a = 0; a < ng; ++a) {
+          int row[3];
+          row[fix] = fv;
+          row[ua] = a;
+          row[va] = b;
+          const double* pr = &phi[(size_t)row[0] * ng];
+          const double* ps = &phi[(size_t)row[1] * ng];
+          const double* pt = &phi[(size_t)row[2] * ng];
+          double x[3] = {0, 0, 0};
+          for (int kk = 0; kk < ng; ++kk)
+            for (int jj = 0; jj < ng; ++jj) {
+              const double wjk = ps[jj] * pt[kk];
+              for (int ii = 0; ii < ng; ++ii) {
+                const double w = pr[ii] * wjk;
+                const size_t m = ((size_t)kk * ng + jj) * ng + ii;
+                for (int d = 0; d < 3; ++d) x[d] += w * geom[((size_t)e * 3 + d) * g3 + m];
+              }
+            }
+          std::snprintf(buf, sizeof buf, " %.9g %.9g %.9g", x[0], x[1], x[2]);
+          out += buf;
+        }
+      out += " </F>\n";
+    }
+  });
+  for (const std::string& t : ftext) std::fwrite(t.data(), 1, t.size(), fx);
   std::fprintf(fx, "    </CURVED>\n    <COMPOSITE>\n      <C ID=\"0\"> H[0-%d] </C>\n    </COMPOSITE>\n", n - 1);
   std::fprintf(fx, "    <DOMAIN> C[0] </DOMAIN>\n  </GEOMETRY>\n  <EXPANSIONS>\n");
   std::fprintf(fx, "    <E COMPOSITE=\"C[0]\" NUMMODES=\"%d\" FIELDS=\"u\" TYPE=\"MODIFIED\" />\n", nf);
@@ -798,7 +982,7 @@ int save_xml_fld(const char* xml_path, const char* fld_path, int n, int ng, int 
   for (size_t k = 0; k < d.size(); ++k) d[k] = field[k];
   uLongf zlen = compressBound((uLong)(d.size() * 8));
   std::vector<uint8_t> z(zlen);
-  if (compress2(z.data(), &zlen, (const Bytef*)d.data(), (uLong)(d.size() * 8), 6) != Z_OK)
+  if (compress2(z.data(), &zlen, (const Bytef*)d.data(), (uLong)(d.size() * 8), Z_BEST_SPEED) != Z_OK)
     return bad(FEMRT_EIO, "zlib compression failed");
   FILE* ff = std::fopen(fld_path, "w");
   if (!ff) return bad(FEMRT_EIO, std::string("cannot write ") + fld_path);

@@ -151,3 +151,76 @@ def test_errors_are_reported(femrt, tmp_path):
                       [(0, 1, 2, 3, 4, 5, 6, 7)], 2)
     with pytest.raises(femrt.FemrtError, match="base64"):
         femrt.load_xml_fld(xml, fld)
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+def test_curved_edge_is_resampled_onto_the_edge(femrt, oracle, tmp_path, reverse):
+    """One unit-cube hex whose edge from vertex 0 (0,0,0) to vertex 1 (1,0,0)
+    is curved: 4 evenly spaced points on z = -0.2 s (1 - s) (the edge listed
+    in either direction).  The loaded map along that edge (s = t = -1) is the
+    parabola (degree 2, the cubic geometry order holds it exactly), every
+    other GLL node keeps its trilinear position, and the other edges stay
+    straight."""
+    verts = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    edges = [(0, 1), (1, 2), (3, 2), (0, 3), (0, 4), (1, 5), (2, 6), (3, 7), (4, 5), (5, 6), (7, 6), (4, 7)]
+    if reverse:
+        edges[0] = (1, 0)
+    s = np.linspace(0, 1, 4)
+    pts = np.stack([s, 0 * s, -0.2 * s * (1 - s)], 1)
+    if reverse:
+        pts = pts[::-1]
+    vx = "\n".join(f'<V ID="{i}"> {x} {y} {z} </V>' for i, (x, y, z) in enumerate(verts))
+    ex = "\n".join(f'<E ID="{i}"> {a} {b} </E>' for i, (a, b) in enumerate(edges))
+    cx = " ".join(f"{v:.17g}" for v in pts.ravel())
+    xml = tmp_path / "ce.xml"
+    xml.write_text(f"""<?xml version="1.0"?>
+<NEKTAR><GEOMETRY DIM="3" SPACE="3">
+<VERTEX>{vx}</VERTEX>
+<EDGE>{ex}</EDGE>
+<ELEMENT><H ID="0"> 0 1 2 3 4 5 6 7 </H></ELEMENT>
+<CURVED><E ID="0" EDGEID="0" NUMPOINTS="4" TYPE="PolyEvenlySpaced"> {cx} </E></CURVED>
+<COMPOSITE><C ID="0"> H[0] </C></COMPOSITE>
+</GEOMETRY>
+<EXPANSIONS><E COMPOSITE="C[0]" NUMMODES="2" FIELDS="u" TYPE="MODIFIED"/></EXPANSIONS>
+</NEKTAR>""")
+    hm = femrt.load_xml_fld(xml, None)
+    assert hm.ng == 4
+    orc = oracle.Oracle.from_mesh(hm)
+    for r in np.linspace(-1, 1, 21):
+        x, _ = orc.map(0, [r, -1, -1])
+        sp = (r + 1) / 2
+        np.testing.assert_allclose(x, [sp, 0, -0.2 * sp * (1 - sp)], atol=2e-6)
+    # the opposite edges (s = +1 or t = +1) stay straight
+    for r in np.linspace(-1, 1, 11):
+        for (sv, tv) in ((1, -1), (-1, 1), (1, 1)):
+            x, _ = orc.map(0, [r, sv, tv])
+            np.testing.assert_allclose(x, [(r + 1) / 2, (sv + 1) / 2, (tv + 1) / 2], atol=2e-6)
+    # GLL nodes off the curved edge keep their trilinear positions
+    g = np.array([-1.0, -1 / np.sqrt(5), 1 / np.sqrt(5), 1.0])
+    for r, s_, t in itertools.product(g, g, g):
+        if s_ == -1 and t == -1:
+            continue
+        x, _ = orc.map(0, [r, s_, t])
+        np.testing.assert_allclose(x, [(r + 1) / 2, (s_ + 1) / 2, (t + 1) / 2], atol=2e-6)
+
+
+def test_round_trip_at_32_cubed_is_exact_and_timed(femrt, tmp_path):
+    """The 32^3 benchmark mesh (32768 hexes, Q=4, P=6: 86 MB of XML and 73 MB
+    of FLD) written and read back: identical field coefficients, geometry to
+    the modal fit's rounding, and both within a time bound (the reader maps
+    the files and scans them once; the per-element fits run on all cores)."""
+    import time
+    hm = femrt.gen_structured_hex(32, 4, 6, pert=2, amp=0.08, seed=20240611)
+    xml, fld = tmp_path / "m.xml", tmp_path / "m.fld"
+    t0 = time.perf_counter()
+    femrt.save_xml_fld(hm, xml, fld)
+    t1 = time.perf_counter()
+    h2 = femrt.load_xml_fld(xml, fld)
+    t2 = time.perf_counter()
+    print({"save_s": t1 - t0, "load_s": t2 - t1, "xml_mb": xml.stat().st_size / 1e6,
+           "fld_mb": fld.stat().st_size / 1e6})
+    assert (h2.n_elem, h2.ng, h2.nf) == (hm.n_elem, hm.ng, hm.nf)
+    np.testing.assert_array_equal(h2.field, hm.field.reshape(h2.field.shape))
+    assert np.abs(h2.geom - hm.geom.reshape(h2.geom.shape)).max() < 2e-6
+    np.testing.assert_array_equal(h2.adjacency[..., 0] >= 0, hm.adjacency.reshape(h2.adjacency.shape)[..., 0] >= 0)
+    assert t2 - t1 < 30 and t1 - t0 < 60
