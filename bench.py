@@ -520,6 +520,28 @@ def main():
         prm = femrt.trace_params(ISOS, guess=names[head])
         fr.trace(mesh, prm)
         check = parity_sample(femrt, orc_mod, parity, hm, fr, len(ISOS))
+    grad = None
+    if ISOS:
+        # the world gradient at every crossing of the last trace (the iso
+        # normal), by the element-binned matrix-core engine and by the per-lane
+        # vector engine, three calls each, kernel times by HIP events
+        gi = fr.iso(len(ISOS))
+        crossings = int((gi["elem"] >= 0).sum())
+        ctx.sync()
+        ctx.timing(True)
+        ctx.timing_reset()
+        for _ in range(3):
+            gm = fr.iso_gradients(mesh, len(ISOS), femrt.GRAD_MFMA)
+            gv = fr.iso_gradients(mesh, len(ISOS), femrt.GRAD_VALU)
+        gk = {k: v[0] / 3 for k, v in ctx.timing_read().items()}
+        ctx.timing(False)
+        f_ = ~np.isnan(gv[:, 0, :])
+        a_ = np.transpose(gm, (1, 0, 2))[:, f_]
+        b_ = np.transpose(gv, (1, 0, 2))[:, f_]
+        grad = {"crossings": crossings, "ms_per_call": gk,
+                "mfma_vs_valu_rel_max": float(np.max(np.linalg.norm(a_ - b_, axis=0) /
+                                                     np.maximum(np.linalg.norm(b_, axis=0), 1e-30)))
+                if crossings else 0.0}
     cpu = None
     if not args.no_cpu:
         orc_mod = importlib.import_module("oracle.oracle")
@@ -565,6 +587,8 @@ def main():
             "starts, certified sample skipping")
     if single:
         line["config"]["single_frame"] = single
+    if grad:
+        line["config"]["iso_gradients"] = grad
     if "bisect" in results and "learned_bracket" in results:
         # config 5's comparison: the FP64 endgame by plain bisection of the
         # scan's bracket against bisection from the bracket network's

@@ -410,6 +410,19 @@ FEMRT_API int femrt_mesh_set_guess_mlp(femrt_mesh* mesh, const femrt_mlp* m);
 /* Bracket network used by FEMRT_TRACE_ISO_LEARNED_BRACKET traces of this mesh
  * (NULL detaches); it must stay alive while such traces run. */
 FEMRT_API int femrt_mesh_set_bracket_mlp(femrt_mesh* mesh, const femrt_mlp* m);
+/* World-space gradient of the field at every isosurface crossing of the
+ * last trace of frame f, from the crossing's FP64 reference coordinates, in
+ * FP32: the reference shades an iso hit with the normal +-grad u / |grad u|
+ * (raytracer.cu:207-216; the gradient: fem/HexElement.cu:274-330).
+ * engine FEMRT_GRAD_MFMA: the crossings binned by element (sorted by element
+ * id), each element's field coefficients staged in LDS, the contraction over
+ * the r modes on the FP32 matrix cores for 16 crossings at a time, the rest
+ * on the vector ALUs; FEMRT_GRAD_VALU: one crossing per lane on the vector
+ * ALUs.  out: n_iso x 3 x n floats (host array; NaN where there is no
+ * crossing).  Synchronous. */
+#define FEMRT_GRAD_MFMA 0
+#define FEMRT_GRAD_VALU 1
+FEMRT_API int femrt_frame_iso_gradients(femrt_mesh* mesh, femrt_frame* f, int n_iso, int engine, float* out);
 /* Training pairs of the bracket network from the last trace of frame f,
  * which must have recorded segments (FEMRT_TRACE_SEG_INFO or
  * FEMRT_TRACE_ISO_LEARNED_BRACKET) with isovalues iso[0 .. n_iso): for each

@@ -22,6 +22,7 @@ GUESS_PLAIN, GUESS_WARM, GUESS_LEARNED = 0, 1, 2
 TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE, TRACE_NO_SAMPLE_SKIP = 1, 2, 4, 8   # trace flag bits
 TRACE_COOP_SCAN = 16
 TRACE_ISO_BISECT, TRACE_ISO_LEARNED_BRACKET, TRACE_SEG_INFO = 32, 64, 128
+GRAD_MFMA, GRAD_VALU = 0, 1   # femrt_frame_iso_gradients engines
 BVH_HOST_SAH, BVH_GPU_LBVH = 0, 1   # femrt_mesh_build_bvh builders
 MAX_ISO = 8
 NCOUNTERS = 33
@@ -177,6 +178,7 @@ def lib():
         "femrt_mesh_set_bracket_mlp": (I, [P, P]),
         "femrt_mlp_train_pairs": (I, [P, I, P, P, P, P]),
         "femrt_frame_bracket_pairs": (I, [P, P, I, P, I, P, P, P]),
+        "femrt_frame_iso_gradients": (I, [P, P, I, I, P]),
         "femrt_frame_set_materials": (I, [P, P, I]),
         "femrt_batch_create": (I, [P, C.POINTER(P)]),
         "femrt_batch_destroy": (I, [P]),
@@ -217,6 +219,7 @@ def exported_symbols():
             "femrt_mlp_loss_grad", "femrt_mlp_make_samples", "femrt_mlp_train_params_default",
             "femrt_mlp_train", "femrt_mesh_set_guess_mlp", "femrt_frame_set_materials",
             "femrt_mesh_set_bracket_mlp", "femrt_mlp_train_pairs", "femrt_frame_bracket_pairs",
+            "femrt_frame_iso_gradients",
             "femrt_batch_create", "femrt_batch_destroy", "femrt_batch_add_trace",
             "femrt_batch_add_paths", "femrt_batch_capture", "femrt_batch_launch",
             "femrt_mesh_get_launch_cfg", "femrt_mesh_set_launch_cfg", "femrt_autotune"]
@@ -687,6 +690,14 @@ class Frame:
         if n < 0:
             _check(n, self.ctx.h, "stalled_rays")
         return n, np.sort(ids[:min(n, ids.size)])
+
+    def iso_gradients(self, mesh, n_iso, engine=GRAD_MFMA):
+        """World gradient of the field at each isosurface crossing of the last
+        trace: (n_iso, 3, n) float32, NaN where there is none."""
+        out = np.zeros((n_iso, 3, self.n), np.float32)
+        _check(lib().femrt_frame_iso_gradients(mesh.h, self.h, n_iso, engine, _ptr(out)), self.ctx.h,
+               "iso_gradients")
+        return out
 
     def bracket_pairs(self, mesh, isos, max_pairs):
         """Training pairs of the bracket network from the last trace (which
