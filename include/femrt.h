@@ -83,6 +83,17 @@ FEMRT_API int femrt_timing_read(femrt_ctx* ctx, int cap, char* names48, double* 
 FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* ctx, int n_elem, int ng, int nf,
                                     const float* vertices, const float* geom, const float* field,
                                     const int* adjacency, femrt_mesh** out);
+/* Curved quadrilateral surface elements (the reference's QuadElementData,
+ * fem/QuadElement.cuh:9-18, Scene::add_quad, raytracer.h:213-257, traversal
+ * branch raytracer.cu:297-320).  vertices: n x 4 x 3 floats, vertex q = i + 2 j
+ * at (r, s) = (2 i - 1, 2 j - 1); geom: n x 3 x ng^2 Modified_A coefficients
+ * (x, y, z), mode (i, j) at j ng + i, or NULL for the bilinear map of the
+ * vertices; field: n x nf^2, or NULL (zero field).  The mesh traces like a
+ * hex mesh (same BVH, face Newton with sub-patch seeds, FP64 polish) with
+ * each quad as face 0 of a map constant in the third coordinate: hit face 0,
+ * rst = (r, s, -1), u = the quad's field.  No isosurfaces (iso_enable = 0). */
+FEMRT_API int femrt_mesh_upload_quads(femrt_ctx* ctx, int n_quad, int ng, int nf, const float* vertices,
+                                      const float* geom, const float* field, femrt_mesh** out);
 FEMRT_API int femrt_mesh_destroy(femrt_mesh* mesh);
 /* Conservative boxes from the exact Bernstein hull of the modal geometry,
  * split 3 ways per reference direction (replaces the sampled +15% box of

@@ -33,6 +33,17 @@ struct HexElementData {
   float3 aabbMin;
   float3 aabbMax;
 };
+
+// Quad surface element record (FEM-AABB/fem/QuadElement.cuh:9-18): corner
+// vertices (corner i at r = +-1 by bit 0, s = +-1 by bit 1), modal field and
+// geometry coefficients (mode (i, j) at j * modes.x + i).
+struct QuadElementData {
+  float3 vertices[4];
+  const float* fieldCoefficients{nullptr};
+  uint2 fieldModes{0, 0};
+  const float* geomCoefficients[3]{nullptr, nullptr, nullptr};
+  uint2 geomModes{0, 0};
+};
 }  // namespace zidingyi
 
 // Pinhole camera (raytracer.h:22-49); the lens is ignored (the reference
@@ -77,6 +88,12 @@ struct FEMRT_CXX_API Scene {
   std::vector<float> hex_geom_coefficients;
   std::vector<int> hex_geom_offsets;
   std::vector<std::vector<FieldSlice>> hex_field_slices;
+  std::vector<zidingyi::QuadElementData> quads;
+  std::vector<float> quad_coefficients;
+  std::vector<int> quad_coeff_offsets;
+  std::vector<float> quad_geom_coefficients;
+  std::vector<int> quad_geom_offsets;
+  std::vector<std::vector<FieldSlice>> quad_field_slices;
 
   // materials (raytracer.h Material subset as femrt_material); empty means
   // the reference's defaults, grey and blue Lambertian (main.cu:39-41)
@@ -104,9 +121,18 @@ struct FEMRT_CXX_API Scene {
                const float* geomZ = nullptr, uint3 geomModes = {0, 0, 0}, size_t geom_count = 0,
                const std::vector<FieldSlice>* slices = nullptr);
 
-  // Uploads the hexes (uniform mode counts) to the GPU and builds the BVH.
-  // On failure prints to stderr and leaves the scene unbuilt (render then
-  // returns false).
+  // Curved quad surface element (raytracer.h:213-257): pooled like add_hex.
+  // Without geometry coefficients the quad is bilinear in its corners.  A
+  // scene holds hexes or quads, not both; a quad scene renders first hits
+  // only (use_iso is ignored).
+  void add_quad(const zidingyi::QuadElementData& q, const float* fieldCoeffs = nullptr,
+                uint2 fieldModes = {0, 0}, size_t field_count = 0, const float* geomX = nullptr,
+                const float* geomY = nullptr, const float* geomZ = nullptr, uint2 geomModes = {0, 0},
+                size_t geom_count = 0, const std::vector<FieldSlice>* slices = nullptr);
+
+  // Uploads the hexes (uniform mode counts), or the quads of a quad scene,
+  // to the GPU and builds the BVH.  On failure prints to stderr and leaves
+  // the scene unbuilt (render then returns false).
   void build_bvh();
 
   // device objects (owned)

@@ -857,3 +857,161 @@ ORC_EXPORT void orc_camera_rays(const double* lookfrom, const double* lookat, co
       }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Curved quadrilateral surface elements (fem/QuadElement.cu).  A quad is   */
+/* the 2D Modified_A map x(r,s) = sum_{j,i} G[c][j n + i] phi_i(r) phi_j(s)   */
+/* (EvaluateModal2D, QuadElement.cu:22-37: i fastest) with the field u(r,s)  */
+/* alike.  Ray intersection (IntersectRayWithQuad, QuadElement.cu:467-527:   */
+/* Newton on (r, s, t) from a seed the reference finds by sampling): here    */
+/* the NEAREST root over a dense 7x7 seed grid per quad, converged in FP64,  */
+/* accepted when max(|r|,|s|) <= 1 + ORC_FACE_ACC and t >= tmin, brute      */
+/* force over every quad.                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int n_quad, ng, nf, pad_;
+  const float* geom;   /* n x 3 x ng^2 */
+  const float* field;  /* n x nf^2 */
+} orc_quads_t;
+
+ORC_EXPORT void orc_quad_map(const orc_quads_t* q, int e, double r, double s, double* x, double* xr,
+                             double* xs)
+{
+  const int n = q->ng;
+  double pr[ORC_MAXN], ps[ORC_MAXN], dpr[ORC_MAXN], dps[ORC_MAXN];
+  for (int i = 0; i < n; ++i) {
+    pr[i] = orc_modA(i, r);
+    ps[i] = orc_modA(i, s);
+    dpr[i] = orc_modA_d(i, r);
+    dps[i] = orc_modA_d(i, s);
+  }
+  for (int c = 0; c < 3; ++c) {
+    const float* g = q->geom + ((size_t)e * 3 + c) * n * n;
+    double v = 0, vr = 0, vs = 0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const double a = g[j * n + i];
+        v += a * pr[i] * ps[j];
+        vr += a * dpr[i] * ps[j];
+        vs += a * pr[i] * dps[j];
+      }
+    x[c] = v;
+    if (xr) xr[c] = vr;
+    if (xs) xs[c] = vs;
+  }
+}
+
+ORC_EXPORT double orc_quad_field(const orc_quads_t* q, int e, double r, double s)
+{
+  const int n = q->nf;
+  const float* f = q->field + (size_t)e * n * n;
+  double u = 0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) u += f[j * n + i] * orc_modA(i, r) * orc_modA(j, s);
+  return u;
+}
+
+/* Newton on (r, s, t) from one seed, damped to steps of 0.5 in the
+ * inf-norm; converged when the residual is below 1e-12 (1 + |x|), then two
+ * undamped steps take the root to the FP64 floor.  1 = converged (any r, s). */
+static int quad_newton(const orc_quads_t* q, int e, const double* o, const double* d, double r, double s,
+                       double t, double* ro, double* so, double* to)
+{
+  int conv = 0;
+  for (int it = 0; it < 200 && conv < 3; ++it) {
+    double x[3], xr[3], xs[3];
+    orc_quad_map(q, e, r, s, x, xr, xs);
+    const double R[3] = {x[0] - o[0] - t * d[0], x[1] - o[1] - t * d[1], x[2] - o[2] - t * d[2]};
+    const double res = fmax(fabs(R[0]), fmax(fabs(R[1]), fabs(R[2])));
+    const double sx = 1 + fmax(fabs(x[0]), fmax(fabs(x[1]), fabs(x[2])));
+    if (conv || res < 1e-12 * sx) ++conv;
+    const double A[9] = {xr[0], xs[0], -d[0], xr[1], xs[1], -d[1], xr[2], xs[2], -d[2]};
+    double inv[9];
+    if (!inv3(A, inv)) return 0;
+    const double dr = -(inv[0] * R[0] + inv[1] * R[1] + inv[2] * R[2]);
+    const double ds = -(inv[3] * R[0] + inv[4] * R[1] + inv[5] * R[2]);
+    const double dt = -(inv[6] * R[0] + inv[7] * R[1] + inv[8] * R[2]);
+    const double mx = fmax(fabs(dr), fmax(fabs(ds), fabs(dt)));
+    const double sc = (!conv && mx > 0.5) ? 0.5 / mx : 1.0;
+    if (conv == 3) break;   /* converged and polished twice */
+    r += sc * dr;
+    s += sc * ds;
+    t += sc * dt;
+    if (fabs(r) > 3 || fabs(s) > 3) return 0;
+  }
+  if (conv < 3) return 0;
+  *ro = r;
+  *so = s;
+  *to = t;
+  return 1;
+}
+
+/* Per ray (ORC_GEOM_STRIDE doubles, as orc_trace): [0] quad (-1 miss)
+ * [1] face (0) [2] t [3..5] (r, s, -1) [6] u [7] t of the nearest root on
+ * another quad [8] margin 1 + ORC_FACE_ACC - max(|r|,|s|) of the hit
+ * [9] smallest |margin| of a converged root rejected just outside a quad, at
+ * or before the hit (any such root for a miss). */
+ORC_EXPORT void orc_quad_trace(const orc_quads_t* q, const float* rays, int nrays, double tmin, double* out)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int ray = 0; ray < nrays; ++ray) {
+    const double o[3] = {rays[ray], rays[(size_t)nrays + ray], rays[2 * (size_t)nrays + ray]};
+    const double d[3] = {rays[3 * (size_t)nrays + ray], rays[4 * (size_t)nrays + ray],
+                         rays[5 * (size_t)nrays + ray]};
+    double* G = out + (size_t)ray * ORC_GEOM_STRIDE;
+    for (int k = 0; k < ORC_GEOM_STRIDE; ++k) G[k] = 0.0;
+    G[0] = -1;
+    G[2] = 1e30;
+    G[7] = 1e30;
+    G[9] = 1e30;
+    double best_r = 0, best_s = 0;
+    double rej_t[64], rej_m[64];
+    int n_rej = 0;
+    for (int e = 0; e < q->n_quad; ++e) {
+      double bt = 1e30, br_ = 0, bs_ = 0;
+      for (int sj = 0; sj < 7; ++sj)
+        for (int si = 0; si < 7; ++si) {
+          const double r0 = -1 + (2.0 * si + 1) / 7.0, s0 = -1 + (2.0 * sj + 1) / 7.0;
+          double x[3];
+          orc_quad_map(q, e, r0, s0, x, NULL, NULL);
+          const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          const double t0 = ((x[0] - o[0]) * d[0] + (x[1] - o[1]) * d[1] + (x[2] - o[2]) * d[2]) / dd;
+          double r, s, t;
+          if (!quad_newton(q, e, o, d, r0, s0, t0, &r, &s, &t) || t < tmin) continue;
+          const double mg = 1 + ORC_FACE_ACC - fmax(fabs(r), fabs(s));
+          if (mg < 0) {
+            if (-mg < 1e-3 && n_rej < 64) {   /* near miss, classified once the hit is known */
+              rej_t[n_rej] = t;
+              rej_m[n_rej++] = -mg;
+            }
+            continue;
+          }
+          if (t < bt) {
+            bt = t;
+            br_ = r;
+            bs_ = s;
+          }
+        }
+      if (bt >= 1e30) continue;
+      if (bt < G[2]) {
+        if (G[0] >= 0) G[7] = fmin(G[7], G[2]);
+        G[0] = e;
+        G[2] = bt;
+        best_r = br_;
+        best_s = bs_;
+      } else {
+        G[7] = fmin(G[7], bt);
+      }
+    }
+    if (G[0] >= 0) {
+      G[1] = 0;
+      G[3] = best_r;
+      G[4] = best_s;
+      G[5] = -1;
+      G[6] = orc_quad_field(q, (int)G[0], best_r, best_s);
+      G[8] = 1 + ORC_FACE_ACC - fmax(fabs(best_r), fabs(best_s));
+    }
+    for (int k = 0; k < n_rej; ++k)
+      if (G[0] < 0 || rej_t[k] <= G[2] * (1.0 + 1e-6) + 1e-12) G[9] = fmin(G[9], rej_m[k]);
+  }
+}

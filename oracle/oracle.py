@@ -20,6 +20,11 @@ class OrcMesh(C.Structure):
                 ("geom", C.c_void_p), ("field", C.c_void_p)]
 
 
+class OrcQuads(C.Structure):
+    _fields_ = [("n_quad", C.c_int), ("ng", C.c_int), ("nf", C.c_int), ("pad_", C.c_int),
+                ("geom", C.c_void_p), ("field", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -43,6 +48,9 @@ def lib():
             "orc_iso_segment": (I, [P, I, P, P, D, D, I, D, P, P, P, P]),
             "orc_trace": (None, [P, P, I, P, I, I, D, P, P]),
             "orc_camera_rays": (None, [P, P, P, D, I, I, P]),
+            "orc_quad_map": (None, [P, I, D, D, P, P, P]),
+            "orc_quad_field": (D, [P, I, D, D]),
+            "orc_quad_trace": (None, [P, P, I, D, P]),
             "orc_geom_stride": (I, []),
             "orc_iso_stride": (I, []),
             "orc_face_acc": (D, []),
@@ -140,6 +148,38 @@ class Oracle:
                    rst=np.transpose(I[:, :, 2:5], (1, 2, 0)).copy(), u=I[:, :, 5].T.copy(),
                    minabs=I[:, :, 6].T.copy())
         return geom, iso
+
+
+class QuadOracle:
+    """FP64 oracle over curved quads (layout of femrt_mesh_upload_quads):
+    nearest root over every quad from a dense seed grid."""
+
+    def __init__(self, geom, field, ng, nf):
+        self.geom = np.ascontiguousarray(geom, np.float32)
+        self.field = np.ascontiguousarray(field, np.float32)
+        self.ng, self.nf = int(ng), int(nf)
+        self.n_quad = self.geom.size // (3 * self.ng ** 2)
+        self.m = OrcQuads(self.n_quad, self.ng, self.nf, 0, self.geom.ctypes.data, self.field.ctypes.data)
+
+    @classmethod
+    def from_mesh(cls, qm):
+        return cls(qm.geom, qm.field, qm.ng, qm.nf)
+
+    def map(self, e, r, s):
+        x, xr, xs = np.zeros(3), np.zeros(3), np.zeros(3)
+        lib().orc_quad_map(C.byref(self.m), int(e), float(r), float(s), _p(x), _p(xr), _p(xs))
+        return x, xr, xs
+
+    def field_at(self, e, r, s):
+        return lib().orc_quad_field(C.byref(self.m), int(e), float(r), float(s))
+
+    def trace(self, rays, tmin=1e-3):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(6, -1)
+        n = rays.shape[1]
+        G = np.zeros((n, lib().orc_geom_stride()))
+        lib().orc_quad_trace(C.byref(self.m), _p(rays), n, tmin, _p(G))
+        return dict(elem=G[:, 0].astype(np.int64), face=G[:, 1].astype(np.int64), t=G[:, 2],
+                    rst=G[:, 3:6].T.copy(), u=G[:, 6], t2=G[:, 7], margin=G[:, 8], nearmiss=G[:, 9])
 
 
 def modA(i, x):

@@ -129,6 +129,7 @@ def lib():
         "femrt_timing_reset": (I, [P]),
         "femrt_timing_read": (I, [P, I, P, P, P]),
         "femrt_mesh_upload_hex": (I, [P, I, I, I, P, P, P, P, C.POINTER(P)]),
+        "femrt_mesh_upload_quads": (I, [P, I, I, I, P, P, P, C.POINTER(P)]),
         "femrt_mesh_destroy": (I, [P]),
         "femrt_mesh_boxes": (I, [P, P]),
         "femrt_mesh_build_bvh": (I, [P, I, C.POINTER(I)]),
@@ -202,7 +203,7 @@ def exported_symbols():
     """Names declared in include/femrt.h that the loaded library must export."""
     return ["femrt_version", "femrt_ctx_create", "femrt_ctx_destroy", "femrt_last_error",
             "femrt_sync", "femrt_timing_enable", "femrt_timing_reset", "femrt_timing_read",
-            "femrt_mesh_upload_hex", "femrt_mesh_destroy", "femrt_mesh_boxes",
+            "femrt_mesh_upload_hex", "femrt_mesh_upload_quads", "femrt_mesh_destroy", "femrt_mesh_boxes",
             "femrt_mesh_build_bvh", "femrt_mesh_get_bvh", "femrt_eval_field", "femrt_eval_field_noalloc",
             "femrt_invert_points", "femrt_trace_params_default", "femrt_frame_create",
             "femrt_frame_destroy", "femrt_frame_set_rays", "femrt_frame_camera_rays",
@@ -253,6 +254,44 @@ class HexMesh:
 
     def geom_blocks(self):
         return self.geom.reshape(self.n_elem, 3, self.ng ** 3)
+
+
+class QuadMesh:
+    """Curved quadrilateral surface elements in the layout of
+    femrt_mesh_upload_quads: vertices n x 4 x 3 (corner q = i + 2 j at
+    (r, s) = (2 i - 1, 2 j - 1)), geometry n x 3 x ng^2 and field n x nf^2
+    Modified_A coefficients, mode (i, j) at j * n + i (fem/QuadElement.cu)."""
+
+    def __init__(self, vertices, geom, field, ng, nf):
+        self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 4, 3)
+        self.n_elem = self.vertices.shape[0]
+        self.ng, self.nf = int(ng), int(nf)
+        self.geom = np.ascontiguousarray(geom, np.float32).reshape(self.n_elem, 3, self.ng ** 2)
+        self.field = np.ascontiguousarray(field, np.float32).reshape(self.n_elem, self.nf ** 2)
+
+
+def gen_bumpy_quads(n, amp=0.2, seed=0, tilt=0.0) -> QuadMesh:
+    """An n x n grid of curved quads over [0, 1]^2 at z = tilt * x, each
+    raised by a seeded bump z += a (1 - r^2)(1 - s^2) (|a| <= amp per quad,
+    the Modified_A mode (2, 2) with coefficient 16 a), at geometry order 2;
+    the field has seeded order-2 coefficients uniform in [-1, 1]."""
+    rng = np.random.default_rng(seed)
+    ng, nf = 3, 3
+    v = np.zeros((n * n, 4, 3), np.float32)
+    g = np.zeros((n * n, 3, ng * ng), np.float32)
+    f = rng.uniform(-1, 1, (n * n, nf * nf)).astype(np.float32)
+    h = 1.0 / n
+    for a in range(n):
+        for b in range(n):
+            e = a * n + b
+            for q in range(4):
+                i, j = q & 1, q >> 1
+                x, y = (b + i) * h, (a + j) * h
+                v[e, q] = (x, y, tilt * x)
+                for c in range(3):
+                    g[e, c, j * ng + i] = v[e, q, c]
+            g[e, 2, 2 * ng + 2] = 16.0 * rng.uniform(-amp, amp) * h
+    return QuadMesh(v, g, f, ng, nf)
 
 
 def gen_structured_hex(n, q, p, pert=0, amp=0.0, seed=0) -> HexMesh:
@@ -424,6 +463,22 @@ class Mesh:
         self.n_nodes = 0
         if build_bvh:
             self.build_bvh()
+
+    @classmethod
+    def from_quads(cls, ctx: Context, qm: QuadMesh, build_bvh=True):
+        """A surface mesh of curved quads (femrt_mesh_upload_quads): first
+        hits only, no isosurfaces."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.host = qm
+        self.h = C.c_void_p()
+        _check(lib().femrt_mesh_upload_quads(ctx.h, qm.n_elem, qm.ng, qm.nf, _ptr(qm.vertices),
+                                             _ptr(qm.geom), _ptr(qm.field), C.byref(self.h)),
+               ctx.h, "mesh_upload_quads")
+        self.n_nodes = 0
+        if build_bvh:
+            self.build_bvh()
+        return self
 
     def build_bvh(self, builder=BVH_HOST_SAH):
         nn = C.c_int(0)

@@ -592,6 +592,48 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* c, int n_elem, int ng, int nf, co
   return FEMRT_OK;
 }
 
+FEMRT_API int femrt_mesh_upload_quads(femrt_ctx* c, int n_quad, int ng, int nf, const float* vertices,
+                                      const float* geom, const float* field, femrt_mesh** out)
+{
+  if (!c || !out || !vertices || n_quad <= 0) return fail(c, FEMRT_EINVAL, "mesh_upload_quads: bad argument");
+  if (!geom) ng = 2;
+  if (!field) nf = 1;
+  if (ng < 2 || ng > 7 || nf < 1 || nf > 9)
+    return fail(c, FEMRT_EUNSUP, "mesh_upload_quads: geometry modes must be 2..7 and field modes 1..9");
+  // Each quad becomes the face t = -1 of a map constant in t: the mode
+  // (i, j) coefficient goes to modes (i, j, 0) and (i, j, 1), since
+  // phi_0(t) + phi_1(t) = 1.  Corners: quad vertex q = i + 2 j sits at
+  // (r, s) = (2 i - 1, 2 j - 1) (QuadElement.cu ShapeWeight); hex corners are
+  // in XML order (r, s) = (-1,-1), (1,-1), (1,1), (-1,1), twice.
+  const size_t G2 = (size_t)ng * ng, G3 = G2 * ng, F2 = (size_t)nf * nf, F3 = F2 * nf;
+  std::vector<float> v8((size_t)n_quad * 24), g3((size_t)n_quad * 3 * G3, 0.f), f3((size_t)n_quad * F3, 0.f);
+  const int qof[4] = {0, 1, 3, 2};
+  for (int e = 0; e < n_quad; ++e) {
+    const float* q = vertices + (size_t)e * 12;
+    for (int c8 = 0; c8 < 8; ++c8)
+      for (int d = 0; d < 3; ++d) v8[(size_t)e * 24 + 3 * c8 + d] = q[3 * qof[c8 & 3] + d];
+    for (int d = 0; d < 3; ++d) {
+      float* dst = &g3[((size_t)e * 3 + d) * G3];
+      for (int j = 0; j < ng; ++j)
+        for (int i = 0; i < ng; ++i) {
+          const float v = geom ? geom[((size_t)e * 3 + d) * G2 + (size_t)j * ng + i] : q[3 * (i + 2 * j) + d];
+          dst[(size_t)j * ng + i] = v;                 // k = 0
+          dst[G2 + (size_t)j * ng + i] = v;            // k = 1
+        }
+    }
+    if (field)
+      for (int j = 0; j < nf; ++j)
+        for (int i = 0; i < nf; ++i) {
+          const float v = field[(size_t)e * F2 + (size_t)j * nf + i];
+          f3[(size_t)e * F3 + (size_t)j * nf + i] = v;
+          if (nf > 1) f3[(size_t)e * F3 + F2 + (size_t)j * nf + i] = v;
+        }
+  }
+  const int rc = femrt_mesh_upload_hex(c, n_quad, ng, nf, v8.data(), g3.data(), f3.data(), nullptr, out);
+  if (rc == FEMRT_OK) (*out)->surface = true;
+  return rc;
+}
+
 FEMRT_API int femrt_mesh_destroy(femrt_mesh* m)
 {
   if (!m) return FEMRT_EINVAL;
@@ -920,6 +962,8 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
     return fail(c, FEMRT_EUNSUP, "trace: learned guesses support at most 16 samples per segment");
   if (c->capturing && (f->counters || (p->flags & FEMRT_TRACE_BINNED_WALK)))
     return fail(c, FEMRT_EUNSUP, "trace: work counters and the binned walk cannot be captured in a batch");
+  if (m->surface && p->iso_enable && p->n_iso > 0)
+    return fail(c, FEMRT_EINVAL, "trace: a surface mesh (curved quads) has no isosurfaces (set iso_enable = 0)");
   const bool learned_br = (p->flags & FEMRT_TRACE_ISO_LEARNED_BRACKET) != 0;
   const bool want_seg = learned_br || (p->flags & FEMRT_TRACE_SEG_INFO);
   if (learned_br && !m->bracket_mlp)
@@ -984,6 +1028,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   TM.lip = m->d_lip;
   TM.frange = m->d_frange;
   TM.jinv = m->d_jinv;
+  TM.surface = m->surface ? 1 : 0;
   rc = FEMRT_EUNSUP;
   FEMRT_DISPATCH_NG(m->NG, {
     rc = timed_launch(c, "k_first_hit", [&] {

@@ -83,6 +83,84 @@ void Scene::add_hex(const zidingyi::HexElementData& h, const float* fieldCoeffs,
   }
 }
 
+void Scene::add_quad(const zidingyi::QuadElementData& q, const float* fieldCoeffs, uint2 fieldModes,
+                     size_t field_count, const float* geomX, const float* geomY, const float* geomZ,
+                     uint2 geomModes, size_t geom_count, const std::vector<FieldSlice>* slices)
+{
+  zidingyi::QuadElementData e = q;
+  if (fieldCoeffs && field_count > 0) {
+    quad_coeff_offsets.push_back((int)quad_coefficients.size());
+    quad_coefficients.insert(quad_coefficients.end(), fieldCoeffs, fieldCoeffs + field_count);
+    if (fieldModes.x > 0) e.fieldModes = fieldModes;
+  } else {
+    quad_coeff_offsets.push_back(-1);
+  }
+  if (geomX && geomY && geomZ && geom_count > 0) {
+    quad_geom_offsets.push_back((int)quad_geom_coefficients.size());
+    quad_geom_coefficients.insert(quad_geom_coefficients.end(), geomX, geomX + geom_count);
+    quad_geom_coefficients.insert(quad_geom_coefficients.end(), geomY, geomY + geom_count);
+    quad_geom_coefficients.insert(quad_geom_coefficients.end(), geomZ, geomZ + geom_count);
+    if (geomModes.x > 0) e.geomModes = geomModes;
+  } else {
+    quad_geom_offsets.push_back(-1);
+  }
+  e.fieldCoefficients = nullptr;
+  e.geomCoefficients[0] = e.geomCoefficients[1] = e.geomCoefficients[2] = nullptr;
+  quads.push_back(e);
+  quad_field_slices.push_back(slices ? *slices : std::vector<FieldSlice>{});
+  if (mesh) {
+    femrt_mesh_destroy(mesh);
+    mesh = nullptr;
+  }
+}
+
+// Quad scene upload: uniform mode counts; a quad without geometry
+// coefficients takes the bilinear map of its corners (order 1, two modes),
+// which needs every quad of the scene to be so.
+static bool build_quads(Scene& S)
+{
+  const int n = (int)S.quads.size();
+  const uint2 fm = S.quads[0].fieldModes, gm = S.quads[0].geomModes;
+  const bool curved = S.quad_geom_offsets[0] >= 0;
+  if (fm.x != fm.y || (curved && gm.x != gm.y)) {
+    std::cerr << "Scene::build_bvh: anisotropic mode counts are not supported" << std::endl;
+    return false;
+  }
+  const int nf = fm.x > 0 ? (int)fm.x : 1, ng = curved ? (int)gm.x : 2;
+  const size_t f2 = (size_t)nf * nf, g2 = (size_t)ng * ng;
+  std::vector<float> verts((size_t)n * 12), geom(curved ? (size_t)n * 3 * g2 : 0), field((size_t)n * f2, 0.f);
+  for (int e = 0; e < n; ++e) {
+    const auto& q = S.quads[e];
+    if ((S.quad_geom_offsets[e] >= 0) != curved || (curved && (q.geomModes.x != gm.x || q.geomModes.y != gm.y)) ||
+        (S.quad_coeff_offsets[e] >= 0 && (q.fieldModes.x != fm.x || q.fieldModes.y != fm.y))) {
+      std::cerr << "Scene::build_bvh: quads of different orders (upload them as separate scenes)" << std::endl;
+      return false;
+    }
+    for (int c = 0; c < 4; ++c) {
+      verts[(size_t)e * 12 + 3 * c] = q.vertices[c].x;
+      verts[(size_t)e * 12 + 3 * c + 1] = q.vertices[c].y;
+      verts[(size_t)e * 12 + 3 * c + 2] = q.vertices[c].z;
+    }
+    if (curved) std::copy_n(&S.quad_geom_coefficients[S.quad_geom_offsets[e]], 3 * g2, &geom[(size_t)e * 3 * g2]);
+    if (S.quad_coeff_offsets[e] >= 0)
+      std::copy_n(&S.quad_coefficients[S.quad_coeff_offsets[e]], f2, &field[(size_t)e * f2]);
+  }
+  if (!S.ctx && femrt_ctx_create(S.device, &S.ctx) != FEMRT_OK) {
+    std::cerr << "Scene::build_bvh: no GPU context" << std::endl;
+    S.ctx = nullptr;
+    return false;
+  }
+  if (femrt_mesh_upload_quads(S.ctx, n, ng, nf, verts.data(), curved ? geom.data() : nullptr, field.data(),
+                              &S.mesh) != FEMRT_OK ||
+      femrt_mesh_build_bvh(S.mesh, 0, nullptr) != FEMRT_OK) {
+    std::cerr << "Scene::build_bvh: " << femrt_last_error(S.ctx) << std::endl;
+    if (S.mesh) femrt_mesh_destroy(S.mesh);
+    S.mesh = nullptr;
+    return false;
+  }
+  return true;
+}
+
 void Scene::build_bvh()
 {
   if (mesh) {
@@ -90,6 +168,14 @@ void Scene::build_bvh()
     mesh = nullptr;
   }
   const int n = (int)hexes.size();
+  if (n > 0 && !quads.empty()) {
+    std::cerr << "Scene::build_bvh: hexahedra and quads in one scene are not supported" << std::endl;
+    return;
+  }
+  if (n == 0 && !quads.empty()) {
+    build_quads(*this);
+    return;
+  }
   if (n == 0) {
     std::cerr << "Scene::build_bvh: no hexahedra" << std::endl;
     return;
@@ -284,8 +370,9 @@ bool render(Scene& scene, const Camera& cam, int width, int height, std::vector<
   }
   femrt_trace_params tp;
   femrt_trace_params_default(&tp);
-  tp.iso_enable = scene.use_iso ? 1 : 0;
-  tp.n_iso = scene.use_iso ? 1 : 0;
+  const bool iso = scene.use_iso && scene.quads.empty();
+  tp.iso_enable = iso ? 1 : 0;
+  tp.n_iso = iso ? 1 : 0;
   tp.iso[0] = scene.iso_value;
   for (double v : scene.extra_iso_values)
     if (tp.n_iso < FEMRT_MAX_ISO) tp.iso[tp.n_iso++] = v;

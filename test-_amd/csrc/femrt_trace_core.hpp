@@ -28,6 +28,7 @@ struct TraceMesh {
   const float* lip;     // n_elem: bound of |u(a) - u(b)| / |a - b|_inf, or null
   const float* frange;  // n_elem x 2: bounds of u on the element (widened), or null
   const float* jinv;    // n_elem: bound of |J^-1|_inf over the element (0: none), or null
+  int surface;          // 1: a surface mesh (curved quads as face 0 of maps constant in t)
 };
 
 struct IsoParams {
@@ -421,7 +422,8 @@ FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d
         ++w.slab;
         if (!slab(o, id, bx, bx + 3, tmin, best_t, e0, e1)) continue;
         const GeoMode* gc = M.geom + (size_t)e * (NG * NG * NG);
-        for (int f = 0; f < 6; ++f) {
+        const int n_faces = M.surface ? 1 : 6;   // a surface element is its face 0
+        for (int f = 0; f < n_faces; ++f) {
           if (excl && excl->has(e, f)) continue;
           const float* fb = bx + 6 * (1 + f);
           float f0, f1;

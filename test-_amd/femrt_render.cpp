@@ -6,6 +6,11 @@
 //   femrt_render mesh.xml field.fld out.ppm [width height [iso ...]]
 //                [--guess plain|warm] [--aov file] [--rays file] [--load-only]
 //                [--spp n] [--depth n] [--seed n]
+//   femrt_render --quads scene.quads out.ppm [width height] [options]
+//
+// --quads reads a curved-quad surface scene (raw, host byte order: int32
+// n, ng, nf; float32 corners n x 4 x 3, geometry n x 3 x ng^2, field
+// n x nf^2) and adds it quad by quad with Scene::add_quad.
 //
 // --spp / --depth > 1 path-trace the mesh surface (the reference's default
 // grey and blue diffuse materials) instead of the primary-ray AOV render.
@@ -31,7 +36,7 @@ int main(int argc, char** argv)
     return 2;
   }
   std::vector<std::string> pos;
-  std::string aov_path, rays_path;
+  std::string aov_path, rays_path, quads_path;
   bool load_only = false;
   int spp = 1, depth = 1;
   RenderParams rp;
@@ -40,6 +45,8 @@ int main(int argc, char** argv)
       aov_path = argv[++i];
     } else if (!std::strcmp(argv[i], "--rays") && i + 1 < argc) {
       rays_path = argv[++i];
+    } else if (!std::strcmp(argv[i], "--quads") && i + 1 < argc) {
+      quads_path = argv[++i];
     } else if (!std::strcmp(argv[i], "--load-only")) {
       load_only = true;
     } else if (!std::strcmp(argv[i], "--spp") && i + 1 < argc) {
@@ -55,14 +62,48 @@ int main(int argc, char** argv)
       pos.push_back(argv[i]);
     }
   }
-  const int width = pos.size() > 3 ? std::atoi(pos[3].c_str()) : 640;
-  const int height = pos.size() > 4 ? std::atoi(pos[4].c_str()) : 360;
+  const size_t b = quads_path.empty() ? 2 : 0;   // positional index of out.ppm
+  if (pos.size() <= b) {
+    std::cerr << "femrt_render: no output file" << std::endl;
+    return 2;
+  }
+  const int width = pos.size() > b + 1 ? std::atoi(pos[b + 1].c_str()) : 640;
+  const int height = pos.size() > b + 2 ? std::atoi(pos[b + 2].c_str()) : 360;
   Scene scene;
   scene.width = width;
   scene.height = height;
   scene.samples_per_pixel = spp;
   scene.max_depth = depth;
-  if (!load_xml_fld(pos[0], pos[1], scene)) return 1;
+  std::vector<float> qdata;
+  if (!quads_path.empty()) {
+    FILE* f = std::fopen(quads_path.c_str(), "rb");
+    int hdr[3] = {0, 0, 0};
+    if (!f || std::fread(hdr, 4, 3, f) != 3 || hdr[0] <= 0 || hdr[1] < 1 || hdr[2] < 1) {
+      std::cerr << "cannot read " << quads_path << std::endl;
+      if (f) std::fclose(f);
+      return 1;
+    }
+    const size_t n = hdr[0], g2 = (size_t)hdr[1] * hdr[1], f2 = (size_t)hdr[2] * hdr[2];
+    qdata.resize(n * (12 + 3 * g2 + f2));
+    const bool ok = std::fread(qdata.data(), 4, qdata.size(), f) == qdata.size();
+    std::fclose(f);
+    if (!ok) {
+      std::cerr << "cannot read " << quads_path << std::endl;
+      return 1;
+    }
+    const float* V = qdata.data();
+    const float* G = V + n * 12;
+    const float* F = G + n * 3 * g2;
+    for (size_t e = 0; e < n; ++e) {
+      zidingyi::QuadElementData q{};
+      for (int c = 0; c < 4; ++c) q.vertices[c] = make_float3(V[e * 12 + 3 * c], V[e * 12 + 3 * c + 1], V[e * 12 + 3 * c + 2]);
+      const float* g = G + e * 3 * g2;
+      scene.add_quad(q, F + e * f2, make_uint2(hdr[2], hdr[2]), f2, g, g + g2, g + 2 * g2,
+                     make_uint2(hdr[1], hdr[1]), g2);
+    }
+  } else if (!load_xml_fld(pos[0], pos[1], scene)) {
+    return 1;
+  }
   if (load_only) {
     double gs = 0.0, fs = 0.0;
     for (float v : scene.hex_geom_coefficients) gs += v;
@@ -72,17 +113,20 @@ int main(int argc, char** argv)
                 scene.hexes.empty() ? 0u : scene.hexes[0].fieldModes.x, gs, fs);
     return 0;
   }
-  scene.use_iso = pos.size() > 5;
+  scene.use_iso = pos.size() > b + 3;
   if (scene.use_iso) {
-    scene.iso_value = (float)std::atof(pos[5].c_str());
-    for (size_t k = 6; k < pos.size(); ++k) scene.extra_iso_values.push_back(std::atof(pos[k].c_str()));
+    scene.iso_value = (float)std::atof(pos[b + 3].c_str());
+    for (size_t k = b + 4; k < pos.size(); ++k) scene.extra_iso_values.push_back(std::atof(pos[k].c_str()));
   }
   float3 lo = make_float3(1e30f, 1e30f, 1e30f), hi = make_float3(-1e30f, -1e30f, -1e30f);
+  auto grow = [&](const float3& v) {
+    lo = make_float3(std::min(lo.x, v.x), std::min(lo.y, v.y), std::min(lo.z, v.z));
+    hi = make_float3(std::max(hi.x, v.x), std::max(hi.y, v.y), std::max(hi.z, v.z));
+  };
   for (const auto& h : scene.hexes)
-    for (const auto& v : h.vertices) {
-      lo = make_float3(std::min(lo.x, v.x), std::min(lo.y, v.y), std::min(lo.z, v.z));
-      hi = make_float3(std::max(hi.x, v.x), std::max(hi.y, v.y), std::max(hi.z, v.z));
-    }
+    for (const auto& v : h.vertices) grow(v);
+  for (const auto& q : scene.quads)
+    for (const auto& v : q.vertices) grow(v);
   const float3 c = make_float3(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z));
   const float md = std::max(hi.x - lo.x, std::max(hi.y - lo.y, hi.z - lo.z));
   const float3 from = make_float3(c.x + 2.5f * md, c.y - 2.5f * md, c.z - 2.0f * md);
@@ -91,7 +135,7 @@ int main(int argc, char** argv)
   std::vector<HitAOV> aov;
   const bool paths = spp > 1 || depth > 1;
   if (!render(scene, cam, width, height, img, aov_path.empty() || paths ? nullptr : &aov, &rp)) return 1;
-  save_image_ppm(img, width, height, pos[2]);
+  save_image_ppm(img, width, height, pos[b]);
   if (!rays_path.empty()) {
     std::vector<float> rays;
     camera_rays(cam, width, height, rays);
@@ -112,7 +156,11 @@ int main(int argc, char** argv)
   }
   int hits = 0;
   for (const auto& a : aov) hits += a.elem >= 0;
-  std::cout << "rendered " << width << "x" << height << " from " << scene.hexes.size() << " hexes";
+  std::cout << "rendered " << width << "x" << height << " from ";
+  if (scene.quads.empty())
+    std::cout << scene.hexes.size() << " hexes";
+  else
+    std::cout << scene.quads.size() << " quads";
   if (!aov_path.empty()) std::cout << ", " << hits << " hit pixels";
   std::cout << std::endl;
   return 0;
