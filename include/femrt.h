@@ -94,6 +94,31 @@ FEMRT_API int femrt_mesh_upload_hex(femrt_ctx* ctx, int n_elem, int ng, int nf,
  * rst = (r, s, -1), u = the quad's field.  No isosurfaces (iso_enable = 0). */
 FEMRT_API int femrt_mesh_upload_quads(femrt_ctx* ctx, int n_quad, int ng, int nf, const float* vertices,
                                       const float* geom, const float* field, femrt_mesh** out);
+/* Prisms (the reference's PrismElementData, fem/PrismElement.cuh:9-18,
+ * Scene::add_prism, raytracer.h:167-212, traversal branch raytracer.cu:255-295,
+ * where the isosurface search of prisms is switched off).  vertices: n x 6 x 3
+ * floats; geom: n x 3 blocks of Modified_A(i, r) Modified_A(j, s)
+ * Modified_B(i, k, t) coefficients, i slowest, then j, then k < mz - i
+ * (EvaluatePrismModal, fem/PrismElement.cu:16-37), with geom_modes = (mx, my,
+ * mz), or NULL for the reference's linear map of the vertices
+ * (fem/PrismElement.cu:194-226); field alike with field_modes, or NULL (zero
+ * field).  (r, s, t) are the reference's collapsed coordinates, so a prism is
+ * a hex map whose t = +1 face is the edge v4 v5: the mesh is traced as that
+ * hex (exact embedding, hex modes max(mx, my, mz), coefficients rounded to
+ * float once).  Hits report the hex face (0: t = -1, 1: s = -1, 2: r = +1,
+ * 3: s = +1, 4: r = -1), rst in the prism's own coordinates and u.  No
+ * isosurfaces (iso_enable = 0). */
+FEMRT_API int femrt_mesh_upload_prisms(femrt_ctx* ctx, int n_prism, const int* geom_modes,
+                                       const int* field_modes, const float* vertices, const float* geom,
+                                       const float* field, femrt_mesh** out);
+/* The embedding femrt_mesh_upload_prisms uploads, in double: ng, nf (hex
+ * modes per direction: max of the mode counts, 2 and 1 for NULL geometry
+ * and field), and, where the output pointers are not NULL, hex_vertices
+ * (n x 8 x 3, XML corner order), hex_geom (n x 3 x ng^3) and hex_field
+ * (n x nf^3) in the hex layout of femrt_mesh_upload_hex. */
+FEMRT_API int femrt_host_prism_to_hex(int n_prism, const int* geom_modes, const int* field_modes,
+                                      const float* vertices, const float* geom, const float* field, int* ng,
+                                      int* nf, float* hex_vertices, double* hex_geom, double* hex_field);
 FEMRT_API int femrt_mesh_destroy(femrt_mesh* mesh);
 /* Conservative boxes from the exact Bernstein hull of the modal geometry,
  * split 3 ways per reference direction (replaces the sampled +15% box of

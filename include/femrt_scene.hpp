@@ -34,6 +34,17 @@ struct HexElementData {
   float3 aabbMax;
 };
 
+// Prism record (FEM-AABB/fem/PrismElement.cuh:9-18): vertices v0..v3 the
+// t = -1 quad, v4 v5 the collapsed t = +1 edge; coefficients in the order
+// i (r) slowest, j (s), k (t) < modes.z - i fastest.
+struct PrismElementData {
+  float3 vertices[6];
+  const float* fieldCoefficients{nullptr};
+  uint3 fieldModes{0, 0, 0};
+  const float* geomCoefficients[3]{nullptr, nullptr, nullptr};
+  uint3 geomModes{0, 0, 0};
+};
+
 // Quad surface element record (FEM-AABB/fem/QuadElement.cuh:9-18): corner
 // vertices (corner i at r = +-1 by bit 0, s = +-1 by bit 1), modal field and
 // geometry coefficients (mode (i, j) at j * modes.x + i).
@@ -88,6 +99,12 @@ struct FEMRT_CXX_API Scene {
   std::vector<float> hex_geom_coefficients;
   std::vector<int> hex_geom_offsets;
   std::vector<std::vector<FieldSlice>> hex_field_slices;
+  std::vector<zidingyi::PrismElementData> prisms;
+  std::vector<float> prism_coefficients;
+  std::vector<int> prism_coeff_offsets;
+  std::vector<float> prism_geom_coefficients;
+  std::vector<int> prism_geom_offsets;
+  std::vector<std::vector<FieldSlice>> prism_field_slices;
   std::vector<zidingyi::QuadElementData> quads;
   std::vector<float> quad_coefficients;
   std::vector<int> quad_coeff_offsets;
@@ -121,17 +138,26 @@ struct FEMRT_CXX_API Scene {
                const float* geomZ = nullptr, uint3 geomModes = {0, 0, 0}, size_t geom_count = 0,
                const std::vector<FieldSlice>* slices = nullptr);
 
+  // Prism (raytracer.h:167-212): pooled like add_hex.  Without geometry
+  // coefficients the prism is the reference's linear map of its vertices.
+  // A prism scene renders first hits only, as the reference's does (its
+  // prism isosurface branch is off, raytracer.cu:266).
+  void add_prism(const zidingyi::PrismElementData& p, const float* fieldCoeffs = nullptr,
+                 uint3 fieldModes = {0, 0, 0}, size_t field_count = 0, const float* geomX = nullptr,
+                 const float* geomY = nullptr, const float* geomZ = nullptr, uint3 geomModes = {0, 0, 0},
+                 size_t geom_count = 0, const std::vector<FieldSlice>* slices = nullptr);
+
   // Curved quad surface element (raytracer.h:213-257): pooled like add_hex.
   // Without geometry coefficients the quad is bilinear in its corners.  A
-  // scene holds hexes or quads, not both; a quad scene renders first hits
-  // only (use_iso is ignored).
+  // scene holds one element type; a quad scene renders first hits only
+  // (use_iso is ignored).
   void add_quad(const zidingyi::QuadElementData& q, const float* fieldCoeffs = nullptr,
                 uint2 fieldModes = {0, 0}, size_t field_count = 0, const float* geomX = nullptr,
                 const float* geomY = nullptr, const float* geomZ = nullptr, uint2 geomModes = {0, 0},
                 size_t geom_count = 0, const std::vector<FieldSlice>* slices = nullptr);
 
   // Uploads the hexes (uniform mode counts), or the quads of a quad scene,
-  // to the GPU and builds the BVH.  On failure prints to stderr and leaves
+  // or the prisms of a prism scene, to the GPU and builds the BVH.  On failure prints to stderr and leaves
   // the scene unbuilt (render then returns false).
   void build_bvh();
 

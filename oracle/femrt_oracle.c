@@ -1015,3 +1015,89 @@ ORC_EXPORT void orc_quad_trace(const orc_quads_t* q, const float* rays, int nray
       if (G[0] < 0 || rej_t[k] <= G[2] * (1.0 + 1e-6) + 1e-12) G[9] = fmin(G[9], rej_m[k]);
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Prisms (fem/PrismElement.cu).  The map and field are sums over           */
+/* Modified_A(i, r) Modified_A(j, s) Modified_B(i, k, t), i slowest, then j, */
+/* then k < mz - i (EvaluatePrismModal, PrismElement.cu:16-37), Modified_B   */
+/* as fem/ModalBasis.cuh:72-82 defines it; without geometry coefficients the */
+/* map is the linear prism of PrismElement.cu:194-226.  Restated directly,  */
+/* not through the hex embedding the product traces.                        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int n, pad_;
+  int gm[3], fm[3];
+  const float* vertices; /* n x 6 x 3 */
+  const float* geom;     /* n x 3 x count(gm), or NULL: linear */
+  const float* field;    /* n x count(fm), or NULL: zero */
+} orc_prisms_t;
+
+/* Jacobi P_n^(a,b) by its explicit sum (Szego 4.3.2):
+ * P_n(x) = sum_s C(n+a, n-s) C(n+b, s) ((x-1)/2)^s ((x+1)/2)^(n-s). */
+static double orc_binom(double top, int k)
+{
+  double r = 1.0;
+  for (int q = 1; q <= k; ++q) r *= (top - k + q) / q;
+  return r;
+}
+
+ORC_EXPORT double orc_jacobi(int n, double a, double b, double x)
+{
+  double s = 0.0;
+  for (int q = 0; q <= n; ++q)
+    s += orc_binom(n + a, n - q) * orc_binom(n + b, q) * pow(0.5 * (x - 1.0), q) * pow(0.5 * (x + 1.0), n - q);
+  return s;
+}
+
+ORC_EXPORT double orc_modB(int i, int k, double x)
+{
+  if (i == 0) return orc_modA(k, x);
+  const double w = pow(0.5 * (1.0 - x), i);
+  if (k == 0) return w;
+  return w * 0.5 * (1.0 + x) * orc_jacobi(k - 1, 2.0 * i - 1.0, 1.0, x);
+}
+
+ORC_EXPORT int orc_prism_count(const int* m)
+{
+  int c = 0;
+  for (int i = 0; i < m[0]; ++i) c += m[1] * (m[2] > i ? m[2] - i : 0);
+  return c;
+}
+
+static double prism_sum(const float* c, const int* m, const double* rst)
+{
+  double v = 0.0;
+  int idx = 0;
+  for (int i = 0; i < m[0]; ++i) {
+    const double a = orc_modA(i, rst[0]);
+    for (int j = 0; j < m[1]; ++j) {
+      const double b = orc_modA(j, rst[1]);
+      for (int k = 0; k + i < m[2]; ++k) v += c[idx++] * a * b * orc_modB(i, k, rst[2]);
+    }
+  }
+  return v;
+}
+
+ORC_EXPORT void orc_prism_map(const orc_prisms_t* P, int e, const double* rst, double* x)
+{
+  if (P->geom) {
+    const int nc = orc_prism_count(P->gm);
+    for (int c = 0; c < 3; ++c) x[c] = prism_sum(P->geom + ((size_t)e * 3 + c) * nc, P->gm, rst);
+    return;
+  }
+  const double r = rst[0], s = rst[1], t = rst[2];
+  const double w[6] = {0.125 * (1 - r) * (1 - s) * (1 - t), 0.125 * (1 + r) * (1 - s) * (1 - t),
+                       0.125 * (1 + r) * (1 + s) * (1 - t), 0.125 * (1 - r) * (1 + s) * (1 - t),
+                       0.25 * (1 - s) * (1 + t), 0.25 * (1 + s) * (1 + t)};
+  const float* v = P->vertices + (size_t)e * 18;
+  for (int c = 0; c < 3; ++c) {
+    x[c] = 0.0;
+    for (int q = 0; q < 6; ++q) x[c] += w[q] * v[3 * q + c];
+  }
+}
+
+ORC_EXPORT double orc_prism_field(const orc_prisms_t* P, int e, const double* rst)
+{
+  if (!P->field) return 0.0;
+  return prism_sum(P->field + (size_t)e * orc_prism_count(P->fm), P->fm, rst);
+}

@@ -25,6 +25,11 @@ class OrcQuads(C.Structure):
                 ("geom", C.c_void_p), ("field", C.c_void_p)]
 
 
+class OrcPrisms(C.Structure):
+    _fields_ = [("n", C.c_int), ("pad_", C.c_int), ("gm", C.c_int * 3), ("fm", C.c_int * 3),
+                ("vertices", C.c_void_p), ("geom", C.c_void_p), ("field", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -51,6 +56,11 @@ def lib():
             "orc_quad_map": (None, [P, I, D, D, P, P, P]),
             "orc_quad_field": (D, [P, I, D, D]),
             "orc_quad_trace": (None, [P, P, I, D, P]),
+            "orc_jacobi": (D, [I, D, D, D]),
+            "orc_modB": (D, [I, I, D]),
+            "orc_prism_count": (I, [P]),
+            "orc_prism_map": (None, [P, I, P, P]),
+            "orc_prism_field": (D, [P, I, P]),
             "orc_geom_stride": (I, []),
             "orc_iso_stride": (I, []),
             "orc_face_acc": (D, []),
@@ -180,6 +190,40 @@ class QuadOracle:
         lib().orc_quad_trace(C.byref(self.m), _p(rays), n, tmin, _p(G))
         return dict(elem=G[:, 0].astype(np.int64), face=G[:, 1].astype(np.int64), t=G[:, 2],
                     rst=G[:, 3:6].T.copy(), u=G[:, 6], t2=G[:, 7], margin=G[:, 8], nearmiss=G[:, 9])
+
+
+class PrismOracle:
+    """FP64 prism map and field in the reference's own basis (no hex
+    embedding).  geom / field None: the linear map / a zero field."""
+
+    def __init__(self, vertices, geom=None, field=None, gm=(0, 0, 0), fm=(0, 0, 0)):
+        self.vertices = np.ascontiguousarray(vertices, np.float32)
+        self.geom = None if geom is None else np.ascontiguousarray(geom, np.float32)
+        self.field = None if field is None else np.ascontiguousarray(field, np.float32)
+        n = self.vertices.size // 18
+        self.m = OrcPrisms(n, 0, (C.c_int * 3)(*gm), (C.c_int * 3)(*fm), self.vertices.ctypes.data,
+                           None if self.geom is None else self.geom.ctypes.data,
+                           None if self.field is None else self.field.ctypes.data)
+
+    def map(self, e, rst):
+        x = np.zeros(3)
+        lib().orc_prism_map(C.byref(self.m), int(e), _p(_d3(rst)), _p(x))
+        return x
+
+    def field_at(self, e, rst):
+        return lib().orc_prism_field(C.byref(self.m), int(e), _p(_d3(rst)))
+
+
+def jacobi(n, a, b, x):
+    return lib().orc_jacobi(int(n), float(a), float(b), float(x))
+
+
+def modB(i, k, x):
+    return lib().orc_modB(int(i), int(k), float(x))
+
+
+def prism_count(m):
+    return lib().orc_prism_count(_p(np.ascontiguousarray(m, np.int32)))
 
 
 def modA(i, x):

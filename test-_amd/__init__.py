@@ -130,6 +130,8 @@ def lib():
         "femrt_timing_read": (I, [P, I, P, P, P]),
         "femrt_mesh_upload_hex": (I, [P, I, I, I, P, P, P, P, C.POINTER(P)]),
         "femrt_mesh_upload_quads": (I, [P, I, I, I, P, P, P, C.POINTER(P)]),
+        "femrt_mesh_upload_prisms": (I, [P, I, P, P, P, P, P, C.POINTER(P)]),
+        "femrt_host_prism_to_hex": (I, [I, P, P, P, P, P, C.POINTER(I), C.POINTER(I), P, P, P]),
         "femrt_mesh_destroy": (I, [P]),
         "femrt_mesh_boxes": (I, [P, P]),
         "femrt_mesh_build_bvh": (I, [P, I, C.POINTER(I)]),
@@ -203,7 +205,8 @@ def exported_symbols():
     """Names declared in include/femrt.h that the loaded library must export."""
     return ["femrt_version", "femrt_ctx_create", "femrt_ctx_destroy", "femrt_last_error",
             "femrt_sync", "femrt_timing_enable", "femrt_timing_reset", "femrt_timing_read",
-            "femrt_mesh_upload_hex", "femrt_mesh_upload_quads", "femrt_mesh_destroy", "femrt_mesh_boxes",
+            "femrt_mesh_upload_hex", "femrt_mesh_upload_quads", "femrt_mesh_upload_prisms",
+            "femrt_host_prism_to_hex", "femrt_mesh_destroy", "femrt_mesh_boxes",
             "femrt_mesh_build_bvh", "femrt_mesh_get_bvh", "femrt_eval_field", "femrt_eval_field_noalloc",
             "femrt_invert_points", "femrt_trace_params_default", "femrt_frame_create",
             "femrt_frame_destroy", "femrt_frame_set_rays", "femrt_frame_camera_rays",
@@ -292,6 +295,64 @@ def gen_bumpy_quads(n, amp=0.2, seed=0, tilt=0.0) -> QuadMesh:
                     g[e, c, j * ng + i] = v[e, q, c]
             g[e, 2, 2 * ng + 2] = 16.0 * rng.uniform(-amp, amp) * h
     return QuadMesh(v, g, f, ng, nf)
+
+
+def prism_count(m):
+    """Coefficients per prism for modes (mx, my, mz): sum over i < mx of
+    my (mz - i) (fem/PrismElement.cu:16-37)."""
+    return sum(m[1] * max(m[2] - i, 0) for i in range(m[0]))
+
+
+class PrismMesh:
+    """Prisms in the layout of femrt_mesh_upload_prisms: vertices n x 6 x 3,
+    geometry n x 3 x count(gm) (or None: linear), field n x count(fm) (or
+    None: zero), the reference's Modified_A / Modified_B coefficient order."""
+
+    def __init__(self, vertices, geom=None, field=None, gm=(0, 0, 0), fm=(0, 0, 0)):
+        self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 6, 3)
+        self.n_elem = self.vertices.shape[0]
+        self.gm, self.fm = tuple(int(v) for v in gm), tuple(int(v) for v in fm)
+        self.geom = None if geom is None else np.ascontiguousarray(geom, np.float32).reshape(
+            self.n_elem, 3, prism_count(self.gm))
+        self.field = None if field is None else np.ascontiguousarray(field, np.float32).reshape(
+            self.n_elem, prism_count(self.fm))
+
+    def to_hex(self):
+        """The hex embedding the library traces, in double:
+        (vertices n x 8 x 3, geom n x 3 x ng^3, field n x nf^3, ng, nf)."""
+        gm = np.array(self.gm, np.int32)
+        fm = np.array(self.fm, np.int32)
+        ng, nf = C.c_int(0), C.c_int(0)
+        args = (self.n_elem, _ptr(gm), _ptr(fm), _ptr(self.vertices), _ptr(self.geom), _ptr(self.field))
+        _check(lib().femrt_host_prism_to_hex(*args, C.byref(ng), C.byref(nf), None, None, None),
+               what="host_prism_to_hex")
+        v = np.zeros((self.n_elem, 8, 3), np.float32)
+        g = np.zeros((self.n_elem, 3, ng.value ** 3))
+        f = np.zeros((self.n_elem, nf.value ** 3))
+        _check(lib().femrt_host_prism_to_hex(*args, C.byref(ng), C.byref(nf), _ptr(v), _ptr(g), _ptr(f)),
+               what="host_prism_to_hex")
+        return v, g, f, ng.value, nf.value
+
+
+def gen_prisms(n, fm=(4, 4, 4), seed=0):
+    """An n^3 grid of unit-cube cells over [0, 1]^3, each split along its
+    x-z diagonal into two linear prisms extruded along y (bottom quad, two
+    triangle ends at s = -1 and s = +1, top edge v4 v5), with seeded field
+    coefficients uniform in [-1, 1] in the reference's prism basis."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    v = []
+    for a in range(n):
+        for b in range(n):
+            for c in range(n):
+                x0, y0, z0 = c * h, b * h, a * h
+                v.append([(x0, y0, z0), (x0 + h, y0, z0), (x0 + h, y0 + h, z0), (x0, y0 + h, z0),
+                          (x0, y0, z0 + h), (x0, y0 + h, z0 + h)])
+                v.append([(x0 + h, y0, z0 + h), (x0, y0, z0 + h), (x0, y0 + h, z0 + h),
+                          (x0 + h, y0 + h, z0 + h), (x0 + h, y0, z0), (x0 + h, y0 + h, z0)])
+    v = np.array(v, np.float32)
+    f = rng.uniform(-1, 1, (len(v), prism_count(fm))).astype(np.float32)
+    return PrismMesh(v, None, f, (0, 0, 0), fm)
 
 
 def gen_structured_hex(n, q, p, pert=0, amp=0.0, seed=0) -> HexMesh:
@@ -475,6 +536,23 @@ class Mesh:
         _check(lib().femrt_mesh_upload_quads(ctx.h, qm.n_elem, qm.ng, qm.nf, _ptr(qm.vertices),
                                              _ptr(qm.geom), _ptr(qm.field), C.byref(self.h)),
                ctx.h, "mesh_upload_quads")
+        self.n_nodes = 0
+        if build_bvh:
+            self.build_bvh()
+        return self
+
+    @classmethod
+    def from_prisms(cls, ctx: Context, pm: PrismMesh, build_bvh=True):
+        """A prism mesh (femrt_mesh_upload_prisms): first hits only."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.host = pm
+        self.h = C.c_void_p()
+        gm = np.array(pm.gm, np.int32)
+        fm = np.array(pm.fm, np.int32)
+        _check(lib().femrt_mesh_upload_prisms(ctx.h, pm.n_elem, _ptr(gm), _ptr(fm), _ptr(pm.vertices),
+                                              _ptr(pm.geom), _ptr(pm.field), C.byref(self.h)),
+               ctx.h, "mesh_upload_prisms")
         self.n_nodes = 0
         if build_bvh:
             self.build_bvh()

@@ -56,6 +56,81 @@ double modA(int p, double x)
   return b * pk;
 }
 
+// Jacobi polynomial P_n^(a,b)(x) by the textbook three-term recurrence
+// (Karniadakis & Sherwin, Appendix A), in double.
+static double jacobi(int n, double a, double b, double x)
+{
+  if (n == 0) return 1.0;
+  double p0 = 1.0, p1 = 0.5 * ((a + b + 2.0) * x + (a - b));
+  for (int k = 2; k <= n; ++k) {
+    const double c = 2.0 * k + a + b;
+    const double p2 = ((c - 1.0) * (c * (c - 2.0) * x + a * a - b * b) * p1 -
+                       2.0 * (k + a - 1.0) * (k + b - 1.0) * c * p0) /
+                      (2.0 * k * (k + a + b) * (c - 2.0));
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+// The prism's collapsed-direction basis as the reference defines it
+// (ModifiedB, fem/ModalBasis.cuh:72-82): Modified_A(k) for i = 0,
+// ((1-x)/2)^i for k = 0, else ((1-x)/2)^i (1+x)/2 P_{k-1}^(2i-1,1)(x).  A
+// polynomial of degree i + k in x.
+double modB(int i, int k, double x)
+{
+  if (i == 0) return modA(k, x);
+  const double w = std::pow(0.5 * (1.0 - x), i);
+  if (k == 0) return w;
+  return w * 0.5 * (1.0 + x) * jacobi(k - 1, 2.0 * i - 1.0, 1.0, x);
+}
+
+// Number of prism coefficients for modes (mx, my, mz): i < mx slowest, then
+// j < my, then k < mz - i fastest (EvaluatePrismModal,
+// fem/PrismElement.cu:16-37).
+int prism_n_coeffs(const int* m)
+{
+  int c = 0;
+  for (int i = 0; i < m[0]; ++i) c += m[1] * std::max(m[2] - i, 0);
+  return c;
+}
+
+// Prism coefficients (per element `comps` blocks in the reference's layout)
+// as hex coefficients with n modes per direction, (k n + j) n + i, in double:
+// sum_{i,j,k} c_ijk A_i(r) A_j(s) B_ik(t) = sum_{i,j,m} H_mji A_i(r) A_j(s) A_m(t)
+// with B_ik(t) = sum_m b_ikm A_m(t), exact since B_ik has degree
+// i + k <= mz - 1 <= n - 1 (b by interpolation at n GLL nodes).  Needs
+// n >= mx, my, mz.
+void prism_to_hex(int n_elem, const int* m, int n, const float* coeffs, int comps, double* out)
+{
+  const std::vector<double> nodes = gll_nodes(n);
+  std::vector<double> inv;
+  modal_fit_1d_matrix(n, nodes, inv);
+  // b[(i * mz + k) * n + q]
+  std::vector<double> b((size_t)m[0] * m[2] * n, 0.0);
+  for (int i = 0; i < m[0]; ++i)
+    for (int k = 0; k + i < m[2]; ++k)
+      for (int q = 0; q < n; ++q) {
+        double s = 0.0;
+        for (int p = 0; p < n; ++p) s += inv[(size_t)q * n + p] * modB(i, k, nodes[p]);
+        b[((size_t)i * m[2] + k) * n + q] = s;
+      }
+  const int nc = prism_n_coeffs(m);
+  const size_t n3 = (size_t)n * n * n;
+  parallel_chunks(n_elem * comps, [&](int e0, int e1) {
+    for (int ec = e0; ec < e1; ++ec) {
+      const float* c = coeffs + (size_t)ec * nc;
+      double* h = out + (size_t)ec * n3;
+      std::fill(h, h + n3, 0.0);
+      int idx = 0;
+      for (int i = 0; i < m[0]; ++i)
+        for (int j = 0; j < m[1]; ++j)
+          for (int k = 0; k + i < m[2]; ++k, ++idx)
+            for (int q = 0; q < n; ++q) h[((size_t)q * n + j) * n + i] += (double)c[idx] * b[((size_t)i * m[2] + k) * n + q];
+    }
+  });
+}
+
 static double legendre(int n, double x, double* pnm1)
 {
   double p0 = 1.0, p1 = x;

@@ -39,6 +39,9 @@ bool modal_fit_1d_matrix(int n, const std::vector<double>& nodes, std::vector<do
 double modA(int p, double x);
 std::vector<double> lagrange_matrix(const std::vector<double>& src, const std::vector<double>& dst);
 void tensor_fit(int n, const std::vector<double>& inv, const double* nodal, double* out);
+double modB(int i, int k, double x);
+int prism_n_coeffs(const int* modes);
+void prism_to_hex(int n_elem, const int* modes, int n, const float* coeffs, int comps, double* out);
 int save_xml_fld(const char* xml_path, const char* fld_path, int n, int ng, int nf,
                  const float* vertices, const float* geom, const float* field, std::string* err);
 int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_elem, int* ng,

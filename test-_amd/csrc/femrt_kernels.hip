@@ -634,6 +634,73 @@ FEMRT_API int femrt_mesh_upload_quads(femrt_ctx* c, int n_quad, int ng, int nf, 
   return rc;
 }
 
+FEMRT_API int femrt_host_prism_to_hex(int n, const int* geom_modes, const int* field_modes, const float* vertices,
+                                      const float* geom, const float* field, int* ng, int* nf,
+                                      float* hex_vertices, double* hex_geom, double* hex_field)
+{
+  if (n <= 0 || !vertices || !ng || !nf) return FEMRT_EINVAL;
+  if ((geom && !geom_modes) || (field && !field_modes)) return FEMRT_EINVAL;
+  const int G = geom ? std::max(geom_modes[0], std::max(geom_modes[1], geom_modes[2])) : 2;
+  const int F = field ? std::max(field_modes[0], std::max(field_modes[1], field_modes[2])) : 1;
+  if (geom && (std::min(geom_modes[0], std::min(geom_modes[1], geom_modes[2])) < 1 || G < 2 || G > 7))
+    return FEMRT_EUNSUP;
+  if (field && (std::min(field_modes[0], std::min(field_modes[1], field_modes[2])) < 1 || F > 9))
+    return FEMRT_EUNSUP;
+  *ng = G;
+  *nf = F;
+  // corners of the hex embedding in XML order: the reference's linear prism
+  // (ReferenceToWorldLinearPrism, fem/PrismElement.cu:194-226) is the
+  // trilinear map of v0 v1 v2 v3 v4 v4 v5 v5 (the t = +1 face collapsed onto
+  // the edge v4 v5)
+  static const int kCorner[8] = {0, 1, 2, 3, 4, 4, 5, 5};
+  if (hex_vertices)
+    for (int e = 0; e < n; ++e)
+      for (int c = 0; c < 8; ++c)
+        for (int d = 0; d < 3; ++d) hex_vertices[((size_t)e * 8 + c) * 3 + d] = vertices[((size_t)e * 6 + kCorner[c]) * 3 + d];
+  if (hex_geom) {
+    if (geom) {
+      femrt::prism_to_hex(n, geom_modes, G, geom, 3, hex_geom);
+    } else {
+      // trilinear coefficients: mode (i, j, k) is the corner (r, s, t) =
+      // (2i-1, 2j-1, 2k-1), XML corner 0 1 3 2 4 5 7 6 of i + 2 j + 4 k
+      static const int kXml[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+      for (int e = 0; e < n; ++e)
+        for (int d = 0; d < 3; ++d)
+          for (int b = 0; b < 8; ++b)
+            hex_geom[((size_t)e * 3 + d) * 8 + b] = vertices[((size_t)e * 6 + kCorner[kXml[b]]) * 3 + d];
+    }
+  }
+  if (hex_field) {
+    if (field)
+      femrt::prism_to_hex(n, field_modes, F, field, 1, hex_field);
+    else
+      std::fill(hex_field, hex_field + n, 0.0);
+  }
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_mesh_upload_prisms(femrt_ctx* c, int n, const int* geom_modes, const int* field_modes,
+                                       const float* vertices, const float* geom, const float* field,
+                                       femrt_mesh** out)
+{
+  if (!c || !out || !vertices || n <= 0) return fail(c, FEMRT_EINVAL, "mesh_upload_prisms: bad argument");
+  *out = nullptr;
+  int ng = 0, nf = 0;
+  int rc = femrt_host_prism_to_hex(n, geom_modes, field_modes, vertices, geom, field, &ng, &nf, nullptr, nullptr,
+                                   nullptr);
+  if (rc == FEMRT_EINVAL) return fail(c, rc, "mesh_upload_prisms: mode counts missing");
+  if (rc != FEMRT_OK)
+    return fail(c, rc, "mesh_upload_prisms: geometry modes must be 1..7 (at least 2 in one direction) and field modes 1..9");
+  const size_t G3 = (size_t)ng * ng * ng, F3 = (size_t)nf * nf * nf;
+  std::vector<float> v8((size_t)n * 24);
+  std::vector<double> g((size_t)n * 3 * G3), f((size_t)n * F3);
+  femrt_host_prism_to_hex(n, geom_modes, field_modes, vertices, geom, field, &ng, &nf, v8.data(), g.data(), f.data());
+  std::vector<float> g32(g.begin(), g.end()), f32(f.begin(), f.end());
+  rc = femrt_mesh_upload_hex(c, n, ng, nf, v8.data(), g32.data(), f32.data(), nullptr, out);
+  if (rc == FEMRT_OK) (*out)->prism = true;
+  return rc;
+}
+
 FEMRT_API int femrt_mesh_destroy(femrt_mesh* m)
 {
   if (!m) return FEMRT_EINVAL;
@@ -964,6 +1031,8 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
     return fail(c, FEMRT_EUNSUP, "trace: work counters and the binned walk cannot be captured in a batch");
   if (m->surface && p->iso_enable && p->n_iso > 0)
     return fail(c, FEMRT_EINVAL, "trace: a surface mesh (curved quads) has no isosurfaces (set iso_enable = 0)");
+  if (m->prism && p->iso_enable && p->n_iso > 0)
+    return fail(c, FEMRT_EINVAL, "trace: a prism mesh is traced for first hits only (set iso_enable = 0)");
   const bool learned_br = (p->flags & FEMRT_TRACE_ISO_LEARNED_BRACKET) != 0;
   const bool want_seg = learned_br || (p->flags & FEMRT_TRACE_SEG_INFO);
   if (learned_br && !m->bracket_mlp)

@@ -58,6 +58,7 @@ struct femrt_mesh {
   const femrt_mlp* guess_mlp = nullptr;  // network used by FEMRT_GUESS_LEARNED
   const femrt_mlp* bracket_mlp = nullptr;  // network used by FEMRT_TRACE_ISO_LEARNED_BRACKET
   bool surface = false;                // curved quads (femrt_mesh_upload_quads)
+  bool prism = false;                  // prisms (femrt_mesh_upload_prisms): first hits only
   int n_elem = 0, ng = 0, nf = 0;      // true mode counts
   int NG = 0, NF = 0;                  // kernel mode counts (padded)
   GeoMode* d_geom = nullptr;           // n_elem x NG^3 (x, y, z)

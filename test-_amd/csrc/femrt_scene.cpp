@@ -114,6 +114,85 @@ void Scene::add_quad(const zidingyi::QuadElementData& q, const float* fieldCoeff
   }
 }
 
+void Scene::add_prism(const zidingyi::PrismElementData& p, const float* fieldCoeffs, uint3 fieldModes,
+                      size_t field_count, const float* geomX, const float* geomY, const float* geomZ,
+                      uint3 geomModes, size_t geom_count, const std::vector<FieldSlice>* slices)
+{
+  zidingyi::PrismElementData e = p;
+  if (fieldCoeffs && field_count > 0) {
+    prism_coeff_offsets.push_back((int)prism_coefficients.size());
+    prism_coefficients.insert(prism_coefficients.end(), fieldCoeffs, fieldCoeffs + field_count);
+    if (fieldModes.x > 0) e.fieldModes = fieldModes;
+  } else {
+    prism_coeff_offsets.push_back(-1);
+  }
+  if (geomX && geomY && geomZ && geom_count > 0) {
+    prism_geom_offsets.push_back((int)prism_geom_coefficients.size());
+    prism_geom_coefficients.insert(prism_geom_coefficients.end(), geomX, geomX + geom_count);
+    prism_geom_coefficients.insert(prism_geom_coefficients.end(), geomY, geomY + geom_count);
+    prism_geom_coefficients.insert(prism_geom_coefficients.end(), geomZ, geomZ + geom_count);
+    if (geomModes.x > 0) e.geomModes = geomModes;
+  } else {
+    prism_geom_offsets.push_back(-1);
+  }
+  e.fieldCoefficients = nullptr;
+  e.geomCoefficients[0] = e.geomCoefficients[1] = e.geomCoefficients[2] = nullptr;
+  prisms.push_back(e);
+  prism_field_slices.push_back(slices ? *slices : std::vector<FieldSlice>{});
+  if (mesh) {
+    femrt_mesh_destroy(mesh);
+    mesh = nullptr;
+  }
+}
+
+// Prism scene upload: uniform mode counts; prisms without geometry
+// coefficients take the linear map of their vertices, which needs every
+// prism of the scene to be so.
+static bool build_prisms(Scene& S)
+{
+  const int n = (int)S.prisms.size();
+  const uint3 fm = S.prisms[0].fieldModes, gm = S.prisms[0].geomModes;
+  const bool curved = S.prism_geom_offsets[0] >= 0;
+  bool has_field = false;
+  for (int e = 0; e < n; ++e) has_field |= S.prism_coeff_offsets[e] >= 0;
+  const int gmi[3] = {(int)gm.x, (int)gm.y, (int)gm.z}, fmi[3] = {(int)fm.x, (int)fm.y, (int)fm.z};
+  int gc = 0, fc = 0;
+  for (int i = 0; i < gmi[0]; ++i) gc += gmi[1] * std::max(gmi[2] - i, 0);
+  for (int i = 0; i < fmi[0]; ++i) fc += fmi[1] * std::max(fmi[2] - i, 0);
+  std::vector<float> verts((size_t)n * 18), geom(curved ? (size_t)n * 3 * gc : 0), field((size_t)n * fc, 0.f);
+  for (int e = 0; e < n; ++e) {
+    const auto& p = S.prisms[e];
+    const bool gsame = p.geomModes.x == gm.x && p.geomModes.y == gm.y && p.geomModes.z == gm.z;
+    const bool fsame = p.fieldModes.x == fm.x && p.fieldModes.y == fm.y && p.fieldModes.z == fm.z;
+    if ((S.prism_geom_offsets[e] >= 0) != curved || (curved && !gsame) || (S.prism_coeff_offsets[e] >= 0 && !fsame)) {
+      std::cerr << "Scene::build_bvh: prisms of different orders (upload them as separate scenes)" << std::endl;
+      return false;
+    }
+    for (int c = 0; c < 6; ++c) {
+      verts[(size_t)e * 18 + 3 * c] = p.vertices[c].x;
+      verts[(size_t)e * 18 + 3 * c + 1] = p.vertices[c].y;
+      verts[(size_t)e * 18 + 3 * c + 2] = p.vertices[c].z;
+    }
+    if (curved) std::copy_n(&S.prism_geom_coefficients[S.prism_geom_offsets[e]], 3 * gc, &geom[(size_t)e * 3 * gc]);
+    if (S.prism_coeff_offsets[e] >= 0) std::copy_n(&S.prism_coefficients[S.prism_coeff_offsets[e]], fc, &field[(size_t)e * fc]);
+  }
+  if (!S.ctx && femrt_ctx_create(S.device, &S.ctx) != FEMRT_OK) {
+    std::cerr << "Scene::build_bvh: no GPU context" << std::endl;
+    S.ctx = nullptr;
+    return false;
+  }
+  if (femrt_mesh_upload_prisms(S.ctx, n, curved ? gmi : nullptr, has_field ? fmi : nullptr, verts.data(),
+                               curved ? geom.data() : nullptr, has_field ? field.data() : nullptr,
+                               &S.mesh) != FEMRT_OK ||
+      femrt_mesh_build_bvh(S.mesh, 0, nullptr) != FEMRT_OK) {
+    std::cerr << "Scene::build_bvh: " << femrt_last_error(S.ctx) << std::endl;
+    if (S.mesh) femrt_mesh_destroy(S.mesh);
+    S.mesh = nullptr;
+    return false;
+  }
+  return true;
+}
+
 // Quad scene upload: uniform mode counts; a quad without geometry
 // coefficients takes the bilinear map of its corners (order 1, two modes),
 // which needs every quad of the scene to be so.
@@ -168,12 +247,16 @@ void Scene::build_bvh()
     mesh = nullptr;
   }
   const int n = (int)hexes.size();
-  if (n > 0 && !quads.empty()) {
-    std::cerr << "Scene::build_bvh: hexahedra and quads in one scene are not supported" << std::endl;
+  if ((n > 0) + !quads.empty() + !prisms.empty() > 1) {
+    std::cerr << "Scene::build_bvh: one element type per scene (hexahedra, prisms or quads)" << std::endl;
     return;
   }
-  if (n == 0 && !quads.empty()) {
+  if (!quads.empty()) {
     build_quads(*this);
+    return;
+  }
+  if (!prisms.empty()) {
+    build_prisms(*this);
     return;
   }
   if (n == 0) {
@@ -370,7 +453,7 @@ bool render(Scene& scene, const Camera& cam, int width, int height, std::vector<
   }
   femrt_trace_params tp;
   femrt_trace_params_default(&tp);
-  const bool iso = scene.use_iso && scene.quads.empty();
+  const bool iso = scene.use_iso && scene.quads.empty() && scene.prisms.empty();
   tp.iso_enable = iso ? 1 : 0;
   tp.n_iso = iso ? 1 : 0;
   tp.iso[0] = scene.iso_value;
