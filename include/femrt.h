@@ -119,6 +119,27 @@ FEMRT_API int femrt_mesh_upload_prisms(femrt_ctx* ctx, int n_prism, const int* g
 FEMRT_API int femrt_host_prism_to_hex(int n_prism, const int* geom_modes, const int* field_modes,
                                       const float* vertices, const float* geom, const float* field, int* ng,
                                       int* nf, float* hex_vertices, double* hex_geom, double* hex_field);
+/* Curved triangles (the reference's TriElementData, fem/TriElement.cuh:9-17,
+ * Scene::add_curved_tri, raytracer.h:259-304).  vertices: n x 3 x 3 floats;
+ * geom: n x 3 blocks of Modified_A(i, r) Modified_B(i, j, s) coefficients,
+ * i slowest, j < my - i (EvaluateTriModal, fem/TriElement.cu:46-63), with
+ * geom_modes = (mx, my), or NULL for the flat triangle v0 v1 v2; field alike
+ * with field_modes, or NULL (zero).  (r, s) are collapsed coordinates, the
+ * collapsed edge at s = +1 (for the flat triangle: barycentric weights of
+ * v1, v2 = (1+r)(1-s)/4, (1+s)/2), so a triangle is a quad map with one
+ * collapsed edge: uploaded as that quad (femrt_mesh_upload_quads; exact
+ * embedding, coefficients rounded to float once).  Hits report face 0,
+ * rst = (r, s, -1) and u.  No isosurfaces. */
+FEMRT_API int femrt_mesh_upload_tris(femrt_ctx* ctx, int n_tri, const int* geom_modes, const int* field_modes,
+                                     const float* vertices, const float* geom, const float* field,
+                                     femrt_mesh** out);
+/* The embedding femrt_mesh_upload_tris uploads, in double: ng, nf (quad
+ * modes: max of the mode counts, 2 and 1 for NULL geometry and field) and,
+ * where not NULL, quad_vertices (n x 4 x 3), quad_geom (n x 3 x ng^2) and
+ * quad_field (n x nf^2) in the layout of femrt_mesh_upload_quads. */
+FEMRT_API int femrt_host_tri_to_quad(int n_tri, const int* geom_modes, const int* field_modes,
+                                     const float* vertices, const float* geom, const float* field, int* ng, int* nf,
+                                     float* quad_vertices, double* quad_geom, double* quad_field);
 FEMRT_API int femrt_mesh_destroy(femrt_mesh* mesh);
 /* Conservative boxes from the exact Bernstein hull of the modal geometry,
  * split 3 ways per reference direction (replaces the sampled +15% box of

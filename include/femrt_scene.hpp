@@ -45,6 +45,16 @@ struct PrismElementData {
   uint3 geomModes{0, 0, 0};
 };
 
+// Curved triangle record (FEM-AABB/fem/TriElement.cuh:9-17): coefficients
+// in the order i (r) slowest, j (s) < modes.y - i fastest.
+struct TriElementData {
+  float3 vertices[3];
+  const float* fieldCoefficients{nullptr};
+  uint2 fieldModes{0, 0};
+  const float* geomCoefficients[3]{nullptr, nullptr, nullptr};
+  uint2 geomModes{0, 0};
+};
+
 // Quad surface element record (FEM-AABB/fem/QuadElement.cuh:9-18): corner
 // vertices (corner i at r = +-1 by bit 0, s = +-1 by bit 1), modal field and
 // geometry coefficients (mode (i, j) at j * modes.x + i).
@@ -105,6 +115,12 @@ struct FEMRT_CXX_API Scene {
   std::vector<float> prism_geom_coefficients;
   std::vector<int> prism_geom_offsets;
   std::vector<std::vector<FieldSlice>> prism_field_slices;
+  std::vector<zidingyi::TriElementData> curved_tris;
+  std::vector<float> tri_coefficients;
+  std::vector<int> tri_coeff_offsets;
+  std::vector<float> tri_geom_coefficients;
+  std::vector<int> tri_geom_offsets;
+  std::vector<std::vector<FieldSlice>> tri_field_slices;
   std::vector<zidingyi::QuadElementData> quads;
   std::vector<float> quad_coefficients;
   std::vector<int> quad_coeff_offsets;
@@ -156,8 +172,17 @@ struct FEMRT_CXX_API Scene {
                 const float* geomY = nullptr, const float* geomZ = nullptr, uint2 geomModes = {0, 0},
                 size_t geom_count = 0, const std::vector<FieldSlice>* slices = nullptr);
 
-  // Uploads the hexes (uniform mode counts), or the quads of a quad scene,
-  // or the prisms of a prism scene, to the GPU and builds the BVH.  On failure prints to stderr and leaves
+  // Curved triangle (raytracer.h:259-304): pooled like add_hex.  Without
+  // geometry coefficients the triangle is flat.  A triangle scene renders
+  // first hits only.
+  void add_curved_tri(const zidingyi::TriElementData& tri, const float* fieldCoeffs = nullptr,
+                      uint2 fieldModes = {0, 0}, size_t field_count = 0, const float* geomX = nullptr,
+                      const float* geomY = nullptr, const float* geomZ = nullptr, uint2 geomModes = {0, 0},
+                      size_t geom_count = 0, const std::vector<FieldSlice>* slices = nullptr);
+
+  // Uploads the hexes (uniform mode counts), or the prisms, quads or
+  // triangles of a scene of that element type, to the GPU and builds the
+  // BVH.  On failure prints to stderr and leaves
   // the scene unbuilt (render then returns false).
   void build_bvh();
 

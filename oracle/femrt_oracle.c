@@ -1101,3 +1101,55 @@ ORC_EXPORT double orc_prism_field(const orc_prisms_t* P, int e, const double* rs
   if (!P->field) return 0.0;
   return prism_sum(P->field + (size_t)e * orc_prism_count(P->fm), P->fm, rst);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Curved triangles (fem/TriElement.cu).  High-order: the sum over          */
+/* Modified_A(i, r) Modified_B(i, j, s), i slowest, j < my - i              */
+/* (EvaluateTriModal, TriElement.cu:46-63).  Without geometry coefficients: */
+/* the flat triangle v0 v1 v2 in the same collapsed coordinates,            */
+/* barycentric weights of v1, v2 = (1+r)(1-s)/4, (1+s)/2.                   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int n;
+  int gm[2], fm[2];
+  int pad_;
+  const float* vertices; /* n x 3 x 3 */
+  const float* geom;     /* n x 3 x count(gm), or NULL */
+  const float* field;    /* n x count(fm), or NULL */
+} orc_tris_t;
+
+ORC_EXPORT int orc_tri_count(const int* m)
+{
+  int c = 0;
+  for (int i = 0; i < m[0]; ++i) c += m[1] > i ? m[1] - i : 0;
+  return c;
+}
+
+static double tri_sum(const float* c, const int* m, double r, double s)
+{
+  double v = 0.0;
+  int idx = 0;
+  for (int i = 0; i < m[0]; ++i) {
+    const double a = orc_modA(i, r);
+    for (int j = 0; j + i < m[1]; ++j) v += c[idx++] * a * orc_modB(i, j, s);
+  }
+  return v;
+}
+
+ORC_EXPORT void orc_tri_map(const orc_tris_t* T, int e, double r, double s, double* x)
+{
+  if (T->geom) {
+    const int nc = orc_tri_count(T->gm);
+    for (int c = 0; c < 3; ++c) x[c] = tri_sum(T->geom + ((size_t)e * 3 + c) * nc, T->gm, r, s);
+    return;
+  }
+  const double b1 = 0.25 * (1 + r) * (1 - s), b2 = 0.5 * (1 + s), b0 = 1 - b1 - b2;
+  const float* v = T->vertices + (size_t)e * 9;
+  for (int c = 0; c < 3; ++c) x[c] = b0 * v[c] + b1 * v[3 + c] + b2 * v[6 + c];
+}
+
+ORC_EXPORT double orc_tri_field(const orc_tris_t* T, int e, double r, double s)
+{
+  if (!T->field) return 0.0;
+  return tri_sum(T->field + (size_t)e * orc_tri_count(T->fm), T->fm, r, s);
+}

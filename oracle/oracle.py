@@ -30,6 +30,11 @@ class OrcPrisms(C.Structure):
                 ("vertices", C.c_void_p), ("geom", C.c_void_p), ("field", C.c_void_p)]
 
 
+class OrcTris(C.Structure):
+    _fields_ = [("n", C.c_int), ("gm", C.c_int * 2), ("fm", C.c_int * 2), ("pad_", C.c_int),
+                ("vertices", C.c_void_p), ("geom", C.c_void_p), ("field", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -61,6 +66,9 @@ def lib():
             "orc_prism_count": (I, [P]),
             "orc_prism_map": (None, [P, I, P, P]),
             "orc_prism_field": (D, [P, I, P]),
+            "orc_tri_count": (I, [P]),
+            "orc_tri_map": (None, [P, I, D, D, P]),
+            "orc_tri_field": (D, [P, I, D, D]),
             "orc_geom_stride": (I, []),
             "orc_iso_stride": (I, []),
             "orc_face_acc": (D, []),
@@ -212,6 +220,32 @@ class PrismOracle:
 
     def field_at(self, e, rst):
         return lib().orc_prism_field(C.byref(self.m), int(e), _p(_d3(rst)))
+
+
+class TriOracle:
+    """FP64 triangle map and field in the reference's own basis; geom None:
+    the flat triangle in collapsed coordinates."""
+
+    def __init__(self, vertices, geom=None, field=None, gm=(0, 0), fm=(0, 0)):
+        self.vertices = np.ascontiguousarray(vertices, np.float32)
+        self.geom = None if geom is None else np.ascontiguousarray(geom, np.float32)
+        self.field = None if field is None else np.ascontiguousarray(field, np.float32)
+        n = self.vertices.size // 9
+        self.m = OrcTris(n, (C.c_int * 2)(*gm), (C.c_int * 2)(*fm), 0, self.vertices.ctypes.data,
+                         None if self.geom is None else self.geom.ctypes.data,
+                         None if self.field is None else self.field.ctypes.data)
+
+    def map(self, e, r, s):
+        x = np.zeros(3)
+        lib().orc_tri_map(C.byref(self.m), int(e), float(r), float(s), _p(x))
+        return x
+
+    def field_at(self, e, r, s):
+        return lib().orc_tri_field(C.byref(self.m), int(e), float(r), float(s))
+
+
+def tri_count(m):
+    return lib().orc_tri_count(_p(np.ascontiguousarray(m, np.int32)))
 
 
 def jacobi(n, a, b, x):

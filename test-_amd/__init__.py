@@ -132,6 +132,8 @@ def lib():
         "femrt_mesh_upload_quads": (I, [P, I, I, I, P, P, P, C.POINTER(P)]),
         "femrt_mesh_upload_prisms": (I, [P, I, P, P, P, P, P, C.POINTER(P)]),
         "femrt_host_prism_to_hex": (I, [I, P, P, P, P, P, C.POINTER(I), C.POINTER(I), P, P, P]),
+        "femrt_mesh_upload_tris": (I, [P, I, P, P, P, P, P, C.POINTER(P)]),
+        "femrt_host_tri_to_quad": (I, [I, P, P, P, P, P, C.POINTER(I), C.POINTER(I), P, P, P]),
         "femrt_mesh_destroy": (I, [P]),
         "femrt_mesh_boxes": (I, [P, P]),
         "femrt_mesh_build_bvh": (I, [P, I, C.POINTER(I)]),
@@ -206,7 +208,8 @@ def exported_symbols():
     return ["femrt_version", "femrt_ctx_create", "femrt_ctx_destroy", "femrt_last_error",
             "femrt_sync", "femrt_timing_enable", "femrt_timing_reset", "femrt_timing_read",
             "femrt_mesh_upload_hex", "femrt_mesh_upload_quads", "femrt_mesh_upload_prisms",
-            "femrt_host_prism_to_hex", "femrt_mesh_destroy", "femrt_mesh_boxes",
+            "femrt_host_prism_to_hex", "femrt_mesh_upload_tris", "femrt_host_tri_to_quad",
+            "femrt_mesh_destroy", "femrt_mesh_boxes",
             "femrt_mesh_build_bvh", "femrt_mesh_get_bvh", "femrt_eval_field", "femrt_eval_field_noalloc",
             "femrt_invert_points", "femrt_trace_params_default", "femrt_frame_create",
             "femrt_frame_destroy", "femrt_frame_set_rays", "femrt_frame_camera_rays",
@@ -332,6 +335,62 @@ class PrismMesh:
         _check(lib().femrt_host_prism_to_hex(*args, C.byref(ng), C.byref(nf), _ptr(v), _ptr(g), _ptr(f)),
                what="host_prism_to_hex")
         return v, g, f, ng.value, nf.value
+
+
+def tri_count(m):
+    """Coefficients per triangle for modes (mx, my): sum over i < mx of
+    my - i (fem/TriElement.cu:46-63)."""
+    return sum(max(m[1] - i, 0) for i in range(m[0]))
+
+
+class TriMesh:
+    """Curved triangles in the layout of femrt_mesh_upload_tris: vertices
+    n x 3 x 3, geometry n x 3 x count(gm) (or None: flat), field
+    n x count(fm) (or None: zero), the reference's coefficient order."""
+
+    def __init__(self, vertices, geom=None, field=None, gm=(0, 0), fm=(0, 0)):
+        self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3, 3)
+        self.n_elem = self.vertices.shape[0]
+        self.gm, self.fm = tuple(int(v) for v in gm), tuple(int(v) for v in fm)
+        self.geom = None if geom is None else np.ascontiguousarray(geom, np.float32).reshape(
+            self.n_elem, 3, tri_count(self.gm))
+        self.field = None if field is None else np.ascontiguousarray(field, np.float32).reshape(
+            self.n_elem, tri_count(self.fm))
+
+    def to_quad(self):
+        """The quad embedding the library traces, in double: (vertices
+        n x 4 x 3, geom n x 3 x ng^2, field n x nf^2, ng, nf)."""
+        gm = np.array(self.gm, np.int32)
+        fm = np.array(self.fm, np.int32)
+        ng, nf = C.c_int(0), C.c_int(0)
+        args = (self.n_elem, _ptr(gm), _ptr(fm), _ptr(self.vertices), _ptr(self.geom), _ptr(self.field))
+        _check(lib().femrt_host_tri_to_quad(*args, C.byref(ng), C.byref(nf), None, None, None),
+               what="host_tri_to_quad")
+        v = np.zeros((self.n_elem, 4, 3), np.float32)
+        g = np.zeros((self.n_elem, 3, ng.value ** 2))
+        f = np.zeros((self.n_elem, nf.value ** 2))
+        _check(lib().femrt_host_tri_to_quad(*args, C.byref(ng), C.byref(nf), _ptr(v), _ptr(g), _ptr(f)),
+               what="host_tri_to_quad")
+        return v, g, f, ng.value, nf.value
+
+
+def gen_tris(n, fm=(4, 4), amp=0.2, seed=0):
+    """A piecewise-linear terrain over [0, 1]^2: an (n+1)^2 grid of seeded
+    heights in [-amp, amp] / n, each cell split into two flat triangles, with
+    seeded field coefficients uniform in [-1, 1] in the reference's
+    triangle basis."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    z = rng.uniform(-amp, amp, (n + 1, n + 1)) * h
+    P = lambda a, b: (b * h, a * h, z[a, b])  # noqa: E731
+    v = []
+    for a in range(n):
+        for b in range(n):
+            v.append([P(a, b), P(a, b + 1), P(a + 1, b)])
+            v.append([P(a + 1, b + 1), P(a + 1, b), P(a, b + 1)])
+    v = np.array(v, np.float32)
+    f = rng.uniform(-1, 1, (len(v), tri_count(fm))).astype(np.float32)
+    return TriMesh(v, None, f, (0, 0), fm)
 
 
 def gen_prisms(n, fm=(4, 4, 4), seed=0):
@@ -553,6 +612,24 @@ class Mesh:
         _check(lib().femrt_mesh_upload_prisms(ctx.h, pm.n_elem, _ptr(gm), _ptr(fm), _ptr(pm.vertices),
                                               _ptr(pm.geom), _ptr(pm.field), C.byref(self.h)),
                ctx.h, "mesh_upload_prisms")
+        self.n_nodes = 0
+        if build_bvh:
+            self.build_bvh()
+        return self
+
+    @classmethod
+    def from_tris(cls, ctx: Context, tm: TriMesh, build_bvh=True):
+        """A curved-triangle surface mesh (femrt_mesh_upload_tris): first
+        hits only."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.host = tm
+        self.h = C.c_void_p()
+        gm = np.array(tm.gm, np.int32)
+        fm = np.array(tm.fm, np.int32)
+        _check(lib().femrt_mesh_upload_tris(ctx.h, tm.n_elem, _ptr(gm), _ptr(fm), _ptr(tm.vertices),
+                                            _ptr(tm.geom), _ptr(tm.field), C.byref(self.h)),
+               ctx.h, "mesh_upload_tris")
         self.n_nodes = 0
         if build_bvh:
             self.build_bvh()

@@ -95,6 +95,48 @@ int prism_n_coeffs(const int* m)
   return c;
 }
 
+// Modified_B(i, k) in the Modified_A basis of n modes: b[(i * mc + k) * n + q]
+// for i < mi, k < mc - i, by interpolation at n GLL nodes (exact: degree
+// i + k <= mc - 1 <= n - 1).
+static std::vector<double> modB_in_modA(int mi, int mc, int n)
+{
+  const std::vector<double> nodes = gll_nodes(n);
+  std::vector<double> inv;
+  modal_fit_1d_matrix(n, nodes, inv);
+  std::vector<double> b((size_t)mi * mc * n, 0.0);
+  for (int i = 0; i < mi; ++i)
+    for (int k = 0; k + i < mc; ++k)
+      for (int q = 0; q < n; ++q) {
+        double s = 0.0;
+        for (int p = 0; p < n; ++p) s += inv[(size_t)q * n + p] * modB(i, k, nodes[p]);
+        b[((size_t)i * mc + k) * n + q] = s;
+      }
+  return b;
+}
+
+// Triangle coefficients (per element `comps` blocks: i < mx slowest, j <
+// my - i, Modified_A(i, r) Modified_B(i, j, s); EvaluateTriModal,
+// fem/TriElement.cu:46-63) as quad coefficients with n modes per direction,
+// j n + i, in double.  Needs n >= mx, my.
+void tri_to_quad(int n_elem, const int* m, int n, const float* coeffs, int comps, double* out)
+{
+  const std::vector<double> b = modB_in_modA(m[0], m[1], n);
+  int nc = 0;
+  for (int i = 0; i < m[0]; ++i) nc += std::max(m[1] - i, 0);
+  const size_t n2 = (size_t)n * n;
+  parallel_chunks(n_elem * comps, [&](int e0, int e1) {
+    for (int ec = e0; ec < e1; ++ec) {
+      const float* c = coeffs + (size_t)ec * nc;
+      double* h = out + (size_t)ec * n2;
+      std::fill(h, h + n2, 0.0);
+      int idx = 0;
+      for (int i = 0; i < m[0]; ++i)
+        for (int j = 0; j + i < m[1]; ++j, ++idx)
+          for (int q = 0; q < n; ++q) h[(size_t)q * n + i] += (double)c[idx] * b[((size_t)i * m[1] + j) * n + q];
+    }
+  });
+}
+
 // Prism coefficients (per element `comps` blocks in the reference's layout)
 // as hex coefficients with n modes per direction, (k n + j) n + i, in double:
 // sum_{i,j,k} c_ijk A_i(r) A_j(s) B_ik(t) = sum_{i,j,m} H_mji A_i(r) A_j(s) A_m(t)
@@ -103,18 +145,7 @@ int prism_n_coeffs(const int* m)
 // n >= mx, my, mz.
 void prism_to_hex(int n_elem, const int* m, int n, const float* coeffs, int comps, double* out)
 {
-  const std::vector<double> nodes = gll_nodes(n);
-  std::vector<double> inv;
-  modal_fit_1d_matrix(n, nodes, inv);
-  // b[(i * mz + k) * n + q]
-  std::vector<double> b((size_t)m[0] * m[2] * n, 0.0);
-  for (int i = 0; i < m[0]; ++i)
-    for (int k = 0; k + i < m[2]; ++k)
-      for (int q = 0; q < n; ++q) {
-        double s = 0.0;
-        for (int p = 0; p < n; ++p) s += inv[(size_t)q * n + p] * modB(i, k, nodes[p]);
-        b[((size_t)i * m[2] + k) * n + q] = s;
-      }
+  const std::vector<double> b = modB_in_modA(m[0], m[2], n);   // b[(i * mz + k) * n + q]
   const int nc = prism_n_coeffs(m);
   const size_t n3 = (size_t)n * n * n;
   parallel_chunks(n_elem * comps, [&](int e0, int e1) {

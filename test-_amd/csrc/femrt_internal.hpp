@@ -42,6 +42,7 @@ void tensor_fit(int n, const std::vector<double>& inv, const double* nodal, doub
 double modB(int i, int k, double x);
 int prism_n_coeffs(const int* modes);
 void prism_to_hex(int n_elem, const int* modes, int n, const float* coeffs, int comps, double* out);
+void tri_to_quad(int n_elem, const int* modes, int n, const float* coeffs, int comps, double* out);
 int save_xml_fld(const char* xml_path, const char* fld_path, int n, int ng, int nf,
                  const float* vertices, const float* geom, const float* field, std::string* err);
 int load_xml_fld(const char* xml_path, const char* fld_path, int flags, int* n_elem, int* ng,

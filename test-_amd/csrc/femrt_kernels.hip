@@ -701,6 +701,61 @@ FEMRT_API int femrt_mesh_upload_prisms(femrt_ctx* c, int n, const int* geom_mode
   return rc;
 }
 
+FEMRT_API int femrt_host_tri_to_quad(int n, const int* geom_modes, const int* field_modes, const float* vertices,
+                                     const float* geom, const float* field, int* ng, int* nf, float* quad_vertices,
+                                     double* quad_geom, double* quad_field)
+{
+  if (n <= 0 || !vertices || !ng || !nf) return FEMRT_EINVAL;
+  if ((geom && !geom_modes) || (field && !field_modes)) return FEMRT_EINVAL;
+  const int G = geom ? std::max(geom_modes[0], geom_modes[1]) : 2;
+  const int F = field ? std::max(field_modes[0], field_modes[1]) : 1;
+  if (geom && (std::min(geom_modes[0], geom_modes[1]) < 1 || G < 2 || G > 7)) return FEMRT_EUNSUP;
+  if (field && (std::min(field_modes[0], field_modes[1]) < 1 || F > 9)) return FEMRT_EUNSUP;
+  *ng = G;
+  *nf = F;
+  // the flat triangle in collapsed coordinates: the bilinear quad of
+  // v0 v1 v2 v2 (corner i + 2 j at (2i-1, 2j-1)), s = +1 the collapsed edge
+  static const int kCorner[4] = {0, 1, 2, 2};
+  if (quad_vertices)
+    for (int e = 0; e < n; ++e)
+      for (int q = 0; q < 4; ++q)
+        for (int d = 0; d < 3; ++d) quad_vertices[((size_t)e * 4 + q) * 3 + d] = vertices[((size_t)e * 3 + kCorner[q]) * 3 + d];
+  if (quad_geom) {
+    if (geom) {
+      femrt::tri_to_quad(n, geom_modes, G, geom, 3, quad_geom);
+    } else {
+      for (int e = 0; e < n; ++e)
+        for (int d = 0; d < 3; ++d)
+          for (int q = 0; q < 4; ++q) quad_geom[((size_t)e * 3 + d) * 4 + q] = vertices[((size_t)e * 3 + kCorner[q]) * 3 + d];
+    }
+  }
+  if (quad_field) {
+    if (field)
+      femrt::tri_to_quad(n, field_modes, F, field, 1, quad_field);
+    else
+      std::fill(quad_field, quad_field + n, 0.0);
+  }
+  return FEMRT_OK;
+}
+
+FEMRT_API int femrt_mesh_upload_tris(femrt_ctx* c, int n, const int* geom_modes, const int* field_modes,
+                                     const float* vertices, const float* geom, const float* field, femrt_mesh** out)
+{
+  if (!c || !out || !vertices || n <= 0) return fail(c, FEMRT_EINVAL, "mesh_upload_tris: bad argument");
+  *out = nullptr;
+  int ng = 0, nf = 0;
+  int rc = femrt_host_tri_to_quad(n, geom_modes, field_modes, vertices, geom, field, &ng, &nf, nullptr, nullptr,
+                                  nullptr);
+  if (rc == FEMRT_EINVAL) return fail(c, rc, "mesh_upload_tris: mode counts missing");
+  if (rc != FEMRT_OK)
+    return fail(c, rc, "mesh_upload_tris: geometry modes must be 1..7 (at least 2 in one direction) and field modes 1..9");
+  std::vector<float> v4((size_t)n * 12);
+  std::vector<double> g((size_t)n * 3 * ng * ng), f((size_t)n * nf * nf);
+  femrt_host_tri_to_quad(n, geom_modes, field_modes, vertices, geom, field, &ng, &nf, v4.data(), g.data(), f.data());
+  std::vector<float> g32(g.begin(), g.end()), f32(f.begin(), f.end());
+  return femrt_mesh_upload_quads(c, n, ng, nf, v4.data(), g32.data(), f32.data(), out);
+}
+
 FEMRT_API int femrt_mesh_destroy(femrt_mesh* m)
 {
   if (!m) return FEMRT_EINVAL;

@@ -193,6 +193,83 @@ static bool build_prisms(Scene& S)
   return true;
 }
 
+void Scene::add_curved_tri(const zidingyi::TriElementData& tri, const float* fieldCoeffs, uint2 fieldModes,
+                           size_t field_count, const float* geomX, const float* geomY, const float* geomZ,
+                           uint2 geomModes, size_t geom_count, const std::vector<FieldSlice>* slices)
+{
+  zidingyi::TriElementData e = tri;
+  if (fieldCoeffs && field_count > 0) {
+    tri_coeff_offsets.push_back((int)tri_coefficients.size());
+    tri_coefficients.insert(tri_coefficients.end(), fieldCoeffs, fieldCoeffs + field_count);
+    if (fieldModes.x > 0) e.fieldModes = fieldModes;
+  } else {
+    tri_coeff_offsets.push_back(-1);
+  }
+  if (geomX && geomY && geomZ && geom_count > 0) {
+    tri_geom_offsets.push_back((int)tri_geom_coefficients.size());
+    tri_geom_coefficients.insert(tri_geom_coefficients.end(), geomX, geomX + geom_count);
+    tri_geom_coefficients.insert(tri_geom_coefficients.end(), geomY, geomY + geom_count);
+    tri_geom_coefficients.insert(tri_geom_coefficients.end(), geomZ, geomZ + geom_count);
+    if (geomModes.x > 0) e.geomModes = geomModes;
+  } else {
+    tri_geom_offsets.push_back(-1);
+  }
+  e.fieldCoefficients = nullptr;
+  e.geomCoefficients[0] = e.geomCoefficients[1] = e.geomCoefficients[2] = nullptr;
+  curved_tris.push_back(e);
+  tri_field_slices.push_back(slices ? *slices : std::vector<FieldSlice>{});
+  if (mesh) {
+    femrt_mesh_destroy(mesh);
+    mesh = nullptr;
+  }
+}
+
+// Triangle scene upload: uniform mode counts; flat triangles (no geometry
+// coefficients) need every triangle of the scene to be flat.
+static bool build_tris(Scene& S)
+{
+  const int n = (int)S.curved_tris.size();
+  const uint2 fm = S.curved_tris[0].fieldModes, gm = S.curved_tris[0].geomModes;
+  const bool curved = S.tri_geom_offsets[0] >= 0;
+  bool has_field = false;
+  for (int e = 0; e < n; ++e) has_field |= S.tri_coeff_offsets[e] >= 0;
+  const int gmi[2] = {(int)gm.x, (int)gm.y}, fmi[2] = {(int)fm.x, (int)fm.y};
+  int gc = 0, fc = 0;
+  for (int i = 0; i < gmi[0]; ++i) gc += std::max(gmi[1] - i, 0);
+  for (int i = 0; i < fmi[0]; ++i) fc += std::max(fmi[1] - i, 0);
+  std::vector<float> verts((size_t)n * 9), geom(curved ? (size_t)n * 3 * gc : 0), field((size_t)n * fc, 0.f);
+  for (int e = 0; e < n; ++e) {
+    const auto& t = S.curved_tris[e];
+    const bool gsame = t.geomModes.x == gm.x && t.geomModes.y == gm.y;
+    const bool fsame = t.fieldModes.x == fm.x && t.fieldModes.y == fm.y;
+    if ((S.tri_geom_offsets[e] >= 0) != curved || (curved && !gsame) || (S.tri_coeff_offsets[e] >= 0 && !fsame)) {
+      std::cerr << "Scene::build_bvh: triangles of different orders (upload them as separate scenes)" << std::endl;
+      return false;
+    }
+    for (int c = 0; c < 3; ++c) {
+      verts[(size_t)e * 9 + 3 * c] = t.vertices[c].x;
+      verts[(size_t)e * 9 + 3 * c + 1] = t.vertices[c].y;
+      verts[(size_t)e * 9 + 3 * c + 2] = t.vertices[c].z;
+    }
+    if (curved) std::copy_n(&S.tri_geom_coefficients[S.tri_geom_offsets[e]], 3 * gc, &geom[(size_t)e * 3 * gc]);
+    if (S.tri_coeff_offsets[e] >= 0) std::copy_n(&S.tri_coefficients[S.tri_coeff_offsets[e]], fc, &field[(size_t)e * fc]);
+  }
+  if (!S.ctx && femrt_ctx_create(S.device, &S.ctx) != FEMRT_OK) {
+    std::cerr << "Scene::build_bvh: no GPU context" << std::endl;
+    S.ctx = nullptr;
+    return false;
+  }
+  if (femrt_mesh_upload_tris(S.ctx, n, curved ? gmi : nullptr, has_field ? fmi : nullptr, verts.data(),
+                             curved ? geom.data() : nullptr, has_field ? field.data() : nullptr, &S.mesh) != FEMRT_OK ||
+      femrt_mesh_build_bvh(S.mesh, 0, nullptr) != FEMRT_OK) {
+    std::cerr << "Scene::build_bvh: " << femrt_last_error(S.ctx) << std::endl;
+    if (S.mesh) femrt_mesh_destroy(S.mesh);
+    S.mesh = nullptr;
+    return false;
+  }
+  return true;
+}
+
 // Quad scene upload: uniform mode counts; a quad without geometry
 // coefficients takes the bilinear map of its corners (order 1, two modes),
 // which needs every quad of the scene to be so.
@@ -247,8 +324,13 @@ void Scene::build_bvh()
     mesh = nullptr;
   }
   const int n = (int)hexes.size();
-  if ((n > 0) + !quads.empty() + !prisms.empty() > 1) {
-    std::cerr << "Scene::build_bvh: one element type per scene (hexahedra, prisms or quads)" << std::endl;
+  if ((n > 0) + !quads.empty() + !prisms.empty() + !curved_tris.empty() > 1) {
+    std::cerr << "Scene::build_bvh: one element type per scene (hexahedra, prisms, quads or triangles)"
+              << std::endl;
+    return;
+  }
+  if (!curved_tris.empty()) {
+    build_tris(*this);
     return;
   }
   if (!quads.empty()) {
@@ -453,7 +535,7 @@ bool render(Scene& scene, const Camera& cam, int width, int height, std::vector<
   }
   femrt_trace_params tp;
   femrt_trace_params_default(&tp);
-  const bool iso = scene.use_iso && scene.quads.empty() && scene.prisms.empty();
+  const bool iso = scene.use_iso && scene.hexes.size() > 0;
   tp.iso_enable = iso ? 1 : 0;
   tp.n_iso = iso ? 1 : 0;
   tp.iso[0] = scene.iso_value;

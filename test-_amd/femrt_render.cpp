@@ -8,6 +8,7 @@
 //                [--spp n] [--depth n] [--seed n]
 //   femrt_render --quads scene.quads out.ppm [width height] [options]
 //   femrt_render --prisms scene.prisms out.ppm [width height] [options]
+//   femrt_render --tris scene.tris out.ppm [width height] [options]
 //
 // --quads reads a curved-quad surface scene (raw, host byte order: int32
 // n, ng, nf; float32 corners n x 4 x 3, geometry n x 3 x ng^2, field
@@ -15,7 +16,10 @@
 // a prism scene (int32 n, gx, gy, gz, fx, fy, fz; float32 vertices
 // n x 6 x 3, geometry n x 3 x count(g) when gx > 0, field n x count(f)
 // when fx > 0, count(m) = sum over i < mx of my (mz - i)) and adds it with
-// Scene::add_prism.
+// Scene::add_prism.  --tris reads a triangle scene (int32 n, gx, gy, fx, fy;
+// float32 vertices n x 3 x 3, geometry n x 3 x count(g) when gx > 0, field
+// n x count(f) when fx > 0, count(m) = sum over i < mx of my - i) and adds
+// it with Scene::add_curved_tri.
 //
 // --spp / --depth > 1 path-trace the mesh surface (the reference's default
 // grey and blue diffuse materials) instead of the primary-ray AOV render.
@@ -42,7 +46,7 @@ int main(int argc, char** argv)
     return 2;
   }
   std::vector<std::string> pos;
-  std::string aov_path, rays_path, quads_path, prisms_path;
+  std::string aov_path, rays_path, quads_path, prisms_path, tris_path;
   bool load_only = false;
   int spp = 1, depth = 1;
   RenderParams rp;
@@ -55,6 +59,8 @@ int main(int argc, char** argv)
       quads_path = argv[++i];
     } else if (!std::strcmp(argv[i], "--prisms") && i + 1 < argc) {
       prisms_path = argv[++i];
+    } else if (!std::strcmp(argv[i], "--tris") && i + 1 < argc) {
+      tris_path = argv[++i];
     } else if (!std::strcmp(argv[i], "--load-only")) {
       load_only = true;
     } else if (!std::strcmp(argv[i], "--spp") && i + 1 < argc) {
@@ -70,7 +76,7 @@ int main(int argc, char** argv)
       pos.push_back(argv[i]);
     }
   }
-  const size_t b = quads_path.empty() && prisms_path.empty() ? 2 : 0;   // positional index of out.ppm
+  const size_t b = quads_path.empty() && prisms_path.empty() && tris_path.empty() ? 2 : 0;   // index of out.ppm
   if (pos.size() <= b) {
     std::cerr << "femrt_render: no output file" << std::endl;
     return 2;
@@ -140,6 +146,37 @@ int main(int argc, char** argv)
       scene.add_prism(p, fc ? F + e * fc : nullptr, make_uint3(hdr[4], hdr[5], hdr[6]), fc, gc ? g : nullptr,
                       gc ? g + gc : nullptr, gc ? g + 2 * gc : nullptr, make_uint3(hdr[1], hdr[2], hdr[3]), gc);
     }
+  } else if (!tris_path.empty()) {
+    FILE* f = std::fopen(tris_path.c_str(), "rb");
+    int hdr[5] = {0, 0, 0, 0, 0};
+    if (!f || std::fread(hdr, 4, 5, f) != 5 || hdr[0] <= 0) {
+      std::cerr << "cannot read " << tris_path << std::endl;
+      if (f) std::fclose(f);
+      return 1;
+    }
+    auto count = [](const int* m) {
+      size_t c = 0;
+      for (int i = 0; i < m[0]; ++i) c += (size_t)std::max(m[1] - i, 0);
+      return c;
+    };
+    const size_t n = hdr[0], gc = count(hdr + 1), fc = count(hdr + 3);
+    qdata.resize(n * (9 + 3 * gc + fc));
+    const bool ok = std::fread(qdata.data(), 4, qdata.size(), f) == qdata.size();
+    std::fclose(f);
+    if (!ok) {
+      std::cerr << "cannot read " << tris_path << std::endl;
+      return 1;
+    }
+    const float* V = qdata.data();
+    const float* G = V + n * 9;
+    const float* F = G + n * 3 * gc;
+    for (size_t e = 0; e < n; ++e) {
+      zidingyi::TriElementData t{};
+      for (int c = 0; c < 3; ++c) t.vertices[c] = make_float3(V[e * 9 + 3 * c], V[e * 9 + 3 * c + 1], V[e * 9 + 3 * c + 2]);
+      const float* g = G + e * 3 * gc;
+      scene.add_curved_tri(t, fc ? F + e * fc : nullptr, make_uint2(hdr[3], hdr[4]), fc, gc ? g : nullptr,
+                           gc ? g + gc : nullptr, gc ? g + 2 * gc : nullptr, make_uint2(hdr[1], hdr[2]), gc);
+    }
   } else if (!load_xml_fld(pos[0], pos[1], scene)) {
     return 1;
   }
@@ -168,6 +205,8 @@ int main(int argc, char** argv)
     for (const auto& v : q.vertices) grow(v);
   for (const auto& p : scene.prisms)
     for (const auto& v : p.vertices) grow(v);
+  for (const auto& t : scene.curved_tris)
+    for (const auto& v : t.vertices) grow(v);
   const float3 c = make_float3(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z));
   const float md = std::max(hi.x - lo.x, std::max(hi.y - lo.y, hi.z - lo.z));
   const float3 from = make_float3(c.x + 2.5f * md, c.y - 2.5f * md, c.z - 2.0f * md);
@@ -202,6 +241,8 @@ int main(int argc, char** argv)
     std::cout << scene.quads.size() << " quads";
   else if (!scene.prisms.empty())
     std::cout << scene.prisms.size() << " prisms";
+  else if (!scene.curved_tris.empty())
+    std::cout << scene.curved_tris.size() << " triangles";
   else
     std::cout << scene.hexes.size() << " hexes";
   if (!aov_path.empty()) std::cout << ", " << hits << " hit pixels";
