@@ -369,7 +369,10 @@ FEMRT_API int femrt_render_path(femrt_mesh* mesh, femrt_frame* f, const float* c
  * launch, raytracer.cu:410-476): a batch records traces and path-traced
  * renders of many meshes of uneven size, each into its own frame, into one
  * HIP graph that femrt_batch_launch replays with one launch on the context
- * stream.  Jobs run in the order they were added.  Not capturable: frames
+ * stream.  Jobs on one frame run in the order they were added; jobs on
+ * different frames may run side by side (the graph has up to four parallel
+ * branches, each frame on one), so a job must not read a frame another job
+ * writes.  A mesh may serve several jobs.  Not capturable: frames
  * with work counters on, the binned walk (FEMRT_TRACE_BINNED_WALK).  Replays
  * rewrite each job's frame from scratch; read results after femrt_sync. */
 typedef struct femrt_batch femrt_batch;

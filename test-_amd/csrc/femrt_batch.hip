@@ -8,9 +8,17 @@
 // must be capture-safe: no work counters, no binned walk (its rounds need the
 // host), materials and path buffers set up beforehand (femrt_batch_add_paths
 // does that).
+//
+// The jobs are recorded on FEMRT_BATCH_BRANCHES streams forked from the
+// context stream inside the capture, so the graph holds that many
+// independent branches and one job's short or latency-bound kernels (the
+// near-miss retrace, the per-depth path kernels) run beside another job's
+// large ones.  Jobs on the same frame stay on one branch, in order; meshes
+// are only read by a trace, so jobs on one mesh may run side by side.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <initializer_list>
 #include <string>
@@ -31,11 +39,14 @@ struct BatchJob {
   unsigned long long seed = 0;
 };
 
+#define FEMRT_BATCH_BRANCHES 4
+
 struct femrt_batch {
   femrt_ctx* ctx = nullptr;
   std::vector<BatchJob> jobs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipStream_t branch[FEMRT_BATCH_BRANCHES - 1] = {};  // forked streams (default priority)
 };
 
 static void batch_drop_graph(femrt_batch* b)
@@ -82,6 +93,8 @@ FEMRT_API int femrt_batch_destroy(femrt_batch* b)
   if (!b) return FEMRT_EINVAL;
   (void)hipStreamSynchronize(b->ctx->stream);
   batch_drop_graph(b);
+  for (hipStream_t& s : b->branch)
+    if (s) (void)hipStreamDestroy(s);
   delete b;
   return FEMRT_OK;
 }
@@ -143,16 +156,58 @@ FEMRT_API int femrt_batch_capture(femrt_batch* b)
   batch_drop_graph(b);
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  // branch of each job: frames in order of first use, each new frame to the
+  // branch with the least estimated work so far (rays x log2 elements)
+  const int nb = std::min<int>(FEMRT_BATCH_BRANCHES, (int)b->jobs.size());
+  std::vector<int> jb(b->jobs.size(), 0);
+  {
+    std::vector<double> load(nb, 0.0);
+    std::vector<std::pair<const femrt_frame*, int>> seen;
+    for (size_t i = 0; i < b->jobs.size(); ++i) {
+      const BatchJob& j = b->jobs[i];
+      int k = -1;
+      for (const auto& fs : seen)
+        if (fs.first == j.frame) k = fs.second;
+      if (k < 0) {
+        k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        seen.push_back({j.frame, k});
+      }
+      jb[i] = k;
+      load[k] += (double)j.frame->n * (1.0 + std::log2(1.0 + j.mesh->n_elem)) * (j.kind == 1 ? j.depth : 1);
+    }
+  }
+  for (int k = 0; k + 1 < nb; ++k)
+    if (!b->branch[k]) HIPCHK(c, hipStreamCreateWithFlags(&b->branch[k], hipStreamNonBlocking));
+  hipEvent_t ev[FEMRT_BATCH_BRANCHES] = {};
+  for (int k = 0; k < nb; ++k) HIPCHK(c, hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+  auto drop_events = [&] {
+    for (hipEvent_t& x : ev)
+      if (x) (void)hipEventDestroy(x);
+  };
+  const hipStream_t main = c->stream;
+  if (hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    drop_events();
+    return fail(c, FEMRT_EHIP, "batch_capture: begin capture failed");
+  }
   c->capturing = true;
-  int rc = FEMRT_OK;
-  for (const BatchJob& j : b->jobs) {
-    rc = enqueue_job(j);
-    if (rc) break;
+  // fork: the branch streams join the capture by waiting on the main stream
+  hipError_t fe = hipEventRecord(ev[0], main);
+  for (int k = 0; k + 1 < nb && fe == hipSuccess; ++k) fe = hipStreamWaitEvent(b->branch[k], ev[0], 0);
+  int rc = fe == hipSuccess ? FEMRT_OK : fail(c, FEMRT_EHIP, "batch_capture: fork failed");
+  for (size_t i = 0; i < b->jobs.size() && rc == FEMRT_OK; ++i) {
+    c->stream = jb[i] == 0 ? main : b->branch[jb[i] - 1];
+    rc = enqueue_job(b->jobs[i]);
+  }
+  c->stream = main;
+  // join: the main stream waits for every branch before the capture ends
+  for (int k = 0; k + 1 < nb; ++k) {
+    if (hipEventRecord(ev[k + 1], b->branch[k]) != hipSuccess || hipStreamWaitEvent(main, ev[k + 1], 0) != hipSuccess)
+      if (rc == FEMRT_OK) rc = fail(c, FEMRT_EHIP, "batch_capture: join failed");
   }
   c->capturing = false;
   hipGraph_t g = nullptr;
-  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  const hipError_t e = hipStreamEndCapture(main, &g);
+  drop_events();
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
     return rc;

@@ -96,6 +96,43 @@ def test_batch_graph_replays_match_eager(femrt, gpu_ctx):
 
 
 @pytest.mark.gpu
+def test_batch_jobs_on_one_frame_keep_their_order(femrt, gpu_ctx):
+    """The graph runs the jobs of different frames side by side on parallel
+    branches; jobs on one frame stay in order.  Six jobs: three frames, two
+    traces each, the second trace of a frame overwriting the first."""
+    ctx = gpu_ctx
+    hms = [femrt.gen_structured_hex(n, 3, 4, pert=1, amp=0.05, seed=10 + n) for n in (2, 3, 4)]
+    ms = [femrt.Mesh(ctx, hm) for hm in hms]
+    W, H = 40, 30
+    frames, eager, jobs = [], [], []
+    for i in range(3):
+        f = femrt.Frame(ctx, W * H)
+        f.camera_rays(_camera(femrt, hms[i]), W, H)
+        frames.append(f)
+    for i in range(3):
+        jobs.append((ms[i], frames[i], femrt.trace_params([0.0], guess=femrt.GUESS_WARM, flags=1)))
+    for i in range(3):
+        jobs.append((ms[(i + 1) % 3], frames[i], femrt.trace_params([0.1, -0.1], guess=femrt.GUESS_WARM, flags=1)))
+    for m, f, p in jobs:          # eager: the last job of each frame wins
+        f.trace(m, p)
+    eager = [_snapshot(f, 2) for f in frames]
+    b = femrt.Batch(ctx)
+    for m, f, p in jobs:
+        b.add_trace(m, f, p)
+    b.capture()
+    for rep in range(2):
+        for i, f in enumerate(frames):   # overwrite with something else first
+            f.trace(ms[i], femrt.trace_params([0.3], guess=femrt.GUESS_PLAIN, flags=1))
+        b.launch()
+        ctx.sync()
+        for i, f in enumerate(frames):
+            _same(_snapshot(f, 2), eager[i])
+    b.close()
+    for o in frames + ms:
+        o.close()
+
+
+@pytest.mark.gpu
 def test_batch_rejects_uncapturable_jobs(femrt, gpu_ctx):
     ctx = gpu_ctx
     hm = femrt.gen_structured_hex(2, 3, 3, pert=1, amp=0.04, seed=4)
