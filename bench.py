@@ -361,9 +361,12 @@ def main():
     # bisect / learned_bracket: learned guesses with the FP64 iso endgame by
     # plain bisection of the scan's bracket, or by bisection from the bracket
     # network's predicted sub-bracket (config 5's comparison)
+    # learned_nested / warm_nested: each lane's first-hit face searches run to
+    # completion (no flattened Newton loop), same hits
     names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED,
              "learned_every": femrt.GUESS_LEARNED, "warm_coop": femrt.GUESS_WARM,
-             "bisect": femrt.GUESS_LEARNED, "learned_bracket": femrt.GUESS_LEARNED}
+             "bisect": femrt.GUESS_LEARNED, "learned_bracket": femrt.GUESS_LEARNED,
+             "learned_nested": femrt.GUESS_LEARNED, "warm_nested": femrt.GUESS_WARM}
     variants = [v for v in args.variants.split(",") if v]
     results = {}
     mlp_info = None
@@ -450,6 +453,8 @@ def main():
             flags |= femrt.TRACE_ISO_BISECT
         if v == "learned_bracket":
             flags |= femrt.TRACE_ISO_LEARNED_BRACKET
+        if v.endswith("_nested"):
+            flags |= femrt.TRACE_NESTED_FIRST_HIT
         wall, kt, cnt = run_variant(femrt, lanes, names[v], args.steps, args.warmup, flags)
         f32, f64 = kernel_flops(cnt, hm.ng, hm.nf)
         results[v] = dict(wall=wall, kernels=kt, counters=cnt, f32=f32, f64=f64,

@@ -262,6 +262,14 @@ typedef struct {
                                           with FEMRT_TRACE_BINNED_WALK.  No
                                           counterpart in the reference (its endgame:
                                           fem/RayIsosurface.cu:121-171) */
+#define FEMRT_TRACE_NESTED_FIRST_HIT 256 /* WARM / LEARNED: run each lane's
+                                          first-hit face searches to completion
+                                          before it traverses on (as the walk's
+                                          re-entries do); by default the
+                                          traversal hands face-Newton jobs to a
+                                          loop that runs one Newton iteration on
+                                          all the wave's lanes at a time.  Same
+                                          hits either way                         */
 #define FEMRT_TRACE_SEG_INFO 128       /* record each bracket's ends (rst and the
                                           FP32 g at ta and tb), the bracket
                                           network's inputs, for

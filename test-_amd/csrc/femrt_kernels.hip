@@ -2,7 +2,7 @@
 // C ABI that drives them.
 //
 // Pipeline of one trace (all on the context stream):
-//   k_first_hit<NG>      one lane per ray: BVH traversal over element boxes,
+//   k_first_hit<NG,FLAT> one lane per ray: BVH traversal over element boxes,
 //                        per-face box cull, face-constrained Newton in FP32;
 //                        hit rays are compacted into a queue with a wave
 //                        ballot + prefix and one atomic per wave.
@@ -93,7 +93,10 @@ struct TileOrder {
   }
 };
 
-template <int NG>
+// FLAT: the face-Newton work of the sub-patch seeds flattened across the
+// wave (first_hit_flat); else each lane's search to completion
+// (first_hit_ray, the reference's seed grid with PLAIN guesses).  Same hits.
+template <int NG, bool FLAT>
 __global__ __launch_bounds__(256) void k_first_hit(const float* __restrict__ rays, int n, TraceMesh M,
                                                    int guess, float tmin, float tmax, TileOrder order,
                                                    int* __restrict__ hit_elem, int* __restrict__ hit_face,
@@ -107,7 +110,10 @@ __global__ __launch_bounds__(256) void k_first_hit(const float* __restrict__ ray
   if (active) {
     float o[3], d[3];
     load_ray(rays, n, r, o, d);
-    h = first_hit_ray<NG>(M, o, d, guess, tmin, tmax, w);
+    if constexpr (FLAT)
+      h = first_hit_flat<NG>(M, o, d, tmin, tmax, w);
+    else
+      h = first_hit_ray<NG>(M, o, d, guess, tmin, tmax, w);
     hit_elem[r] = h.elem;
     hit_face[r] = h.face;
     hit_f32[r] = make_float4(h.u, h.v, h.t, 0.f);
@@ -1154,11 +1160,18 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   TM.jinv = m->d_jinv;
   TM.surface = m->surface ? 1 : 0;
   rc = FEMRT_EUNSUP;
+  const bool flat = p->guess != FEMRT_GUESS_PLAIN && !(p->flags & FEMRT_TRACE_NESTED_FIRST_HIT);
   FEMRT_DISPATCH_NG(m->NG, {
     rc = timed_launch(c, "k_first_hit", [&] {
-      k_first_hit<NG_><<<(order.threads(n) + hblock - 1) / hblock, hblock, 0, c->stream>>>(
-          f->d_rays, n, TM, p->guess, p->t_min, p->t_max, order, f->d_hit_elem, f->d_hit_face,
-          f->d_hit_f32, f->d_queue_hit, f->d_counts, work);
+      const int grid = (order.threads(n) + hblock - 1) / hblock;
+      if (flat)
+        k_first_hit<NG_, true><<<grid, hblock, 0, c->stream>>>(f->d_rays, n, TM, p->guess, p->t_min, p->t_max,
+                                                              order, f->d_hit_elem, f->d_hit_face, f->d_hit_f32,
+                                                              f->d_queue_hit, f->d_counts, work);
+      else
+        k_first_hit<NG_, false><<<grid, hblock, 0, c->stream>>>(f->d_rays, n, TM, p->guess, p->t_min, p->t_max,
+                                                               order, f->d_hit_elem, f->d_hit_face, f->d_hit_f32,
+                                                               f->d_queue_hit, f->d_counts, work);
     });
     if (rc) break;
     FEMRT_DISPATCH_NF(m->NF, {

@@ -475,6 +475,191 @@ FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d
   return h;
 }
 
+// first_hit_ray with sub-patch seeds (WARM, LEARNED), the face-Newton work
+// flattened.  Each trip of the loop first advances the traversal of the lanes
+// that hold no Newton job until each holds one or has finished, then runs one
+// Newton iteration on every lane that holds a job: a wave's geometry
+// evaluations run together, where in first_hit_ray each lane's whole solve
+// diverges from its neighbours' traversal.  Per lane the jobs, their order
+// and the culling by the nearest root so far are first_hit_ray's (a face's
+// sub-patch seeds culled at the running nearest root, which is the face
+// search's own bound there), so the hit is the same.
+template <int NG, typename W>
+FEMRT_HD HitF32 first_hit_flat(const TraceMesh& M, const float* o, const float* d, float tmin, float tmax, W& w)
+{
+  HitF32 h;
+  float id[3];
+  for (int c = 0; c < 3; ++c) id[c] = 1.f / d[c];
+  float best_t = tmax;
+  int stack[FEMRT_STACK];
+  float stack_t[FEMRT_STACK];
+  int sp = 0, node = 0;
+  {
+    const float4 ra = M.nodes[0], rb = M.nodes[1];
+    const float rlo[3] = {ra.x, ra.y, ra.z}, rhi[3] = {rb.x, rb.y, rb.z};
+    float t0, t1;
+    ++w.slab;
+    if (!slab(o, id, rlo, rhi, tmin, best_t, t0, t1)) return h;
+  }
+  const int n_faces = M.surface ? 1 : 6;
+  // traversal: the leaf being scanned (prims [la, la + ln), next li; ln = 0:
+  // none), its element e (-1: none), face f and sub-patch q (-1: the face's
+  // box not tested yet)
+  int la = 0, ln = 0, li = 0, e = -1, f = 0, q = -1;
+  const GeoMode* gc = M.geom;
+  const float* bx = M.boxes;
+  bool done = false;
+  // the Newton job: stage 0 the sub-patch centre seed, 1 its far-end reseed,
+  // 2..5 the four quarter seeds of a grazing crossing
+  bool job = false;
+  int stage = 0, rc = -1;
+  FaceIter F;
+  float s0 = 0.f, s1 = 0.f, uc = 0.f, vc = 0.f, tt = 0.f, uu = 0.f, vv = 0.f;
+  for (;;) {
+    while (!job && !done) {
+      if (e >= 0) {
+        if (f >= n_faces) {
+          e = -1;
+          continue;
+        }
+        if (q < 0) {
+          const float* fb = bx + 6 * (1 + f);
+          float f0, f1;
+          ++w.slab;
+          if (slab(o, id, fb, fb + 3, tmin, best_t, f0, f1))
+            q = 0;
+          else
+            ++f;
+          continue;
+        }
+        if (q >= 9) {
+          ++f;
+          q = -1;
+          continue;
+        }
+        const float* sb = bx + 6 * (7 + 9 * f) + 6 * q;
+        const int qq = q++;
+        float a0, a1;
+        ++w.slab;
+        if (!slab(o, id, sb, sb + 3, tmin, best_t, a0, a1)) continue;
+        uc = -1.f + (2.f * (qq % 3) + 1.f) / 3.f;
+        vc = -1.f + (2.f * (qq / 3) + 1.f) / 3.f;
+        s0 = a0;
+        s1 = a1;
+        face_iter_begin(F, f, uc, vc, 0.5f * (s0 + s1), tmin, 1e30f);
+        stage = 0;
+        job = true;
+        continue;
+      }
+      if (li < ln) {
+        const int ee = M.prims[la + li];
+        ++li;
+        const float* eb = M.boxes + (size_t)ee * FEMRT_BOX_STRIDE;
+        float e0, e1;
+        ++w.slab;
+        if (!slab(o, id, eb, eb + 3, tmin, best_t, e0, e1)) continue;
+        e = ee;
+        bx = eb;
+        gc = M.geom + (size_t)e * (NG * NG * NG);
+        f = 0;
+        q = -1;
+        continue;
+      }
+      bool pop = ln > 0;   // a leaf finished
+      ln = 0;
+      if (!pop) {
+        const float4 na = M.nodes[2 * node], nb = M.nodes[2 * node + 1];
+        const int a = as_int(na.w), b = as_int(nb.w);
+        if (b < 0) {
+          la = a;
+          ln = -b;
+          li = 0;
+          continue;
+        }
+        const float4 ca = M.nodes[2 * a], cb = M.nodes[2 * a + 1];
+        const float4 da = M.nodes[2 * b], db = M.nodes[2 * b + 1];
+        const float llo[3] = {ca.x, ca.y, ca.z}, lhi[3] = {cb.x, cb.y, cb.z};
+        const float qlo[3] = {da.x, da.y, da.z}, qhi[3] = {db.x, db.y, db.z};
+        float l0, l1, q0, q1;
+        w.slab += 2;
+        const bool hl = slab(o, id, llo, lhi, tmin, best_t, l0, l1);
+        const bool hq = slab(o, id, qlo, qhi, tmin, best_t, q0, q1);
+        if (hl && hq) {
+          const bool lfirst = l0 <= q0;
+          if (sp < FEMRT_STACK) {
+            stack[sp] = lfirst ? b : a;
+            stack_t[sp] = lfirst ? q0 : l0;
+            ++sp;
+          }
+          node = lfirst ? a : b;
+          continue;
+        }
+        if (hl || hq) {
+          node = hl ? a : b;
+          continue;
+        }
+      }
+      done = true;
+      while (sp > 0) {
+        --sp;
+        if (stack_t[sp] < best_t) {
+          node = stack[sp];
+          done = false;
+          break;
+        }
+      }
+    }
+    if (!job) break;
+    float r, s, t, X[3], J[9];
+    face_to_rst(F.f, F.u, F.v, r, s, t);
+    geom_eval<float, NG>(gc, r, s, t, X, J);
+    ++w.face_it;
+    const int st = face_newton_step(F, o, d, X, J);
+    if (st == FN_CONTINUE) continue;
+    const int res = st == FN_ACCEPT ? 1 : (st == FN_REJECT ? 0 : -1);
+    if (stage <= 1) {
+      rc = res;
+      tt = F.tau;
+      uu = F.u;
+      vv = F.v;
+      // converged to a crossing behind the range: seed again at the box's far end
+      if (stage == 0 && rc == 0 && tt < tmin) {
+        stage = 1;
+        face_iter_begin(F, f, uc, vc, s1, tmin, 1e30f);
+        continue;
+      }
+      if (rc == 1 && tt < best_t * 1.0001f && grazing_crossing<NG>(gc, d, f, uu, vv)) {
+        ++w.face_it;
+        stage = 2;
+        face_iter_begin(F, f, uc - 1.f / 6.f, vc - 1.f / 6.f, s0, tmin, 1e30f);
+        continue;
+      }
+    } else {
+      if (res == 1 && F.tau < tt * (1.f - 1e-6f)) {
+        tt = F.tau;
+        uu = F.u;
+        vv = F.v;
+      }
+      const int k = stage - 1;   // the next quarter seed
+      if (k < 4) {
+        stage = k + 2;
+        face_iter_begin(F, f, uc + ((k & 1) ? 1.f : -1.f) / 6.f, vc + ((k & 2) ? 1.f : -1.f) / 6.f, s0, tmin, 1e30f);
+        continue;
+      }
+    }
+    if (rc == 1 && tt < best_t) {
+      best_t = tt;
+      h.elem = e;
+      h.face = f;
+      h.u = uu;
+      h.v = vv;
+      h.t = tt;
+    }
+    job = false;
+  }
+  return h;
+}
+
 // FP64 face Newton polish of a hit (unconstrained, quadratic convergence from
 // the FP32 root; linear near a double root, hence up to 40 iterations).
 // Converged when the step drops below 1e-14, or once it is below 1e-10 and

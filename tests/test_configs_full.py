@@ -260,3 +260,37 @@ def test_formerly_stalled_walks_finish_and_match_oracle(femrt, oracle, gpu_ctx):
     assert rep["ok"], rep
     fr.close()
     mesh.close()
+
+
+@pytest.mark.parametrize("n,amp,seed,W,H", [(32, 0.08, 20240611, 1920, 1080), (64, 0.15, 404, 1280, 720)])
+def test_flat_first_hit_equals_nested(femrt, gpu_ctx, n, amp, seed, W, H):
+    """The first hit with its face-Newton work flattened across the wave
+    (the default for warm and learned guesses) gives every ray the same hit,
+    bit for bit, as each lane's search run to completion
+    (FEMRT_TRACE_NESTED_FIRST_HIT): the same jobs in the same order per lane,
+    culled by the same nearest root."""
+    hm = femrt.gen_structured_hex(n, 4, 6, pert=2 if n == 32 else 3, amp=amp, seed=seed)
+    mesh = femrt.Mesh(gpu_ctx, hm)
+    fr = femrt.Frame(gpu_ctx, W * H)
+    fr.camera_rays(femrt.auto_camera([0, 0, 0], [1, 1, 1]), W, H)
+    out = []
+    for flags in (1, 1 | femrt.TRACE_NESTED_FIRST_HIT):
+        fr.counters(True)
+        fr.trace(mesh, femrt.trace_params([], guess=femrt.GUESS_WARM, iso_enable=False, flags=flags))
+        cnt = fr.counters(False)
+        out.append((fr.hits(), cnt))
+    (a, ca), (b, cb) = out
+    print(json.dumps({"hits": int((a["elem"] >= 0).sum()), "face_newton_it": [ca["first_hit_face_it"],
+                                                                             cb["first_hit_face_it"]]}), flush=True)
+    assert int((a["elem"] >= 0).sum()) > 0.2 * W * H
+    np.testing.assert_array_equal(a["elem"], b["elem"])
+    np.testing.assert_array_equal(a["face"], b["face"])
+    h = a["elem"] >= 0
+    np.testing.assert_array_equal(a["t"][h], b["t"][h])
+    np.testing.assert_array_equal(a["rst"][:, h], b["rst"][:, h])
+    np.testing.assert_array_equal(a["u"][h], b["u"][h])
+    # the same work: face-Newton iterations and slab tests per lane
+    assert ca["first_hit_face_it"] == cb["first_hit_face_it"]
+    assert ca["first_hit_slab_tests"] == cb["first_hit_slab_tests"]
+    fr.close()
+    mesh.close()
