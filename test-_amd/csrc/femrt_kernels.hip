@@ -170,7 +170,7 @@ template <int NG, int NF>
 __global__ __launch_bounds__(256) void k_polish_hit(
     const float* __restrict__ rays, int n, TraceMesh M, float tmin, const int* __restrict__ queue,
     int* __restrict__ counts, const int* __restrict__ hit_elem,
-    const int* __restrict__ hit_face, const float4* __restrict__ hit_f32,
+    const int* __restrict__ hit_face, float4* __restrict__ hit_f32,
     double* __restrict__ hit_t, double* __restrict__ hit_rst, double* __restrict__ hit_u,
     unsigned char* __restrict__ qbin, int* __restrict__ retrace, long long* __restrict__ work)
 {
@@ -188,6 +188,10 @@ __global__ __launch_bounds__(256) void k_polish_hit(
     double u = h.x, v = h.y, tau = h.z;
     if (polish_hit<NG>(M.geom + (size_t)e * (NG * NG * NG), o, d, f, u, v, tau, (double)tmin, w)) {
       store_hit64<NF>(M, n, r, e, f, u, v, tau, hit_t, hit_rst, hit_u);
+      // the walk starts at the polished root: where the FP32 root was a near
+      // miss just outside the face and the polish moved on to the face's real
+      // crossing, the FP32 point lies outside the element
+      hit_f32[r] = make_float4((float)u, (float)v, (float)tau, 0.f);
       ++w.field64;
     } else {
       // no FP64 root on the face: a near miss, traced again by k_retrace
@@ -227,7 +231,7 @@ __global__ __launch_bounds__(64) void k_retrace(const float* __restrict__ rays, 
       if (polish_hit<NG>(M.geom + (size_t)h.elem * (NG * NG * NG), o, d, h.face, u, v, tau, (double)tmin, w)) {
         e = h.elem;
         hit_face[r] = h.face;
-        hit_f32[r] = make_float4(h.u, h.v, h.t, 0.f);
+        hit_f32[r] = make_float4((float)u, (float)v, (float)tau, 0.f);   // the polished root, as k_polish_hit
         store_hit64<NF>(M, n, r, h.elem, h.face, u, v, tau, hit_t, hit_rst, hit_u);
         break;
       }

@@ -294,3 +294,37 @@ def test_flat_first_hit_equals_nested(femrt, gpu_ctx, n, amp, seed, W, H):
     assert ca["first_hit_slab_tests"] == cb["first_hit_slab_tests"]
     fr.close()
     mesh.close()
+
+
+def test_walk_starts_at_the_polished_hit(femrt, oracle, gpu_ctx):
+    """Regression: on the 64^3 P=8 frame these two rays' FP32 first hit is a
+    near miss just outside face 0 of the hit element (t 3.99988 and 4.05179);
+    the FP64 polish moves on to the face's real crossing (t 4.000034 and
+    4.052010).  The walk used to start at the FP32 point, outside the
+    element, and missed a crossing 2.5e-3 and 3.3e-4 past the entry; it now
+    starts at the polished root."""
+    import oracle.parity as parity
+    base = femrt.gen_structured_hex(64, 4, 2, pert=3, amp=0.15, seed=505)
+    hm = femrt.gen_field_waves(base, 8, n_waves=8, seed=506)
+    isos = [-0.6, -0.2, 0.2, 0.6]
+    o = [3.0, -2.0, -1.5]
+    dirs = [[-0.6632617115974426, 0.6477730870246887, 0.37479862570762634],
+            [-0.5812870860099792, 0.7246705293655396, 0.37007826566696167]]
+    rays = np.ascontiguousarray(np.array([[o[0]] * 2, [o[1]] * 2, [o[2]] * 2,
+                                          [dd[0] for dd in dirs], [dd[1] for dd in dirs], [dd[2] for dd in dirs]]),
+                                np.float32)
+    mesh = femrt.Mesh(gpu_ctx, hm)
+    fr = femrt.Frame(gpu_ctx, 2)
+    fr.set_rays(rays)
+    og, oi = oracle.Oracle.from_mesh(hm).trace(rays, isos)
+    assert oi["elem"][1][0] == 2390 and oi["elem"][2][1] == 3817
+    for guess, flags in ((femrt.GUESS_WARM, 1), (femrt.GUESS_WARM, 1 | femrt.TRACE_NESTED_FIRST_HIT),
+                         (femrt.GUESS_WARM, 1 | femrt.TRACE_NO_SAMPLE_SKIP),
+                         (femrt.GUESS_WARM, 1 | femrt.TRACE_COOP_SCAN)):
+        fr.trace(mesh, femrt.trace_params(isos, guess=guess, flags=flags))
+        rep = parity.compare(fr.hits(), fr.iso(len(isos)), og, oi, isos)
+        print(json.dumps({"guess": guess, "flags": flags, "ok": rep["ok"],
+                          "iso_found": [r["gpu_found"] for r in rep["iso"]]}), flush=True)
+        assert rep["ok"], rep
+    fr.close()
+    mesh.close()
