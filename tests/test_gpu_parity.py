@@ -164,3 +164,34 @@ def test_scan_skip_by_field_bounds_keeps_parity(femrt, oracle, gpu_ctx):
     assert cnt["scan_skipped"] > 0.3 * cnt["elems_marched"]
     fr.close()
     mesh.close()
+
+
+def test_face_crossed_twice_gpu_takes_the_nearer_crossing(femrt, gpu_ctx):
+    """Nearly horizontal rays below z = 0 cross the curved bottom face of the
+    reference's Hex_CurvedFace fixture twice (into the bowl and out again).
+    The GPU's hit must be the nearer crossing, which dense sampling of the
+    closed-form face finds (tests/test_oracle.py), with warm guesses, both
+    first-hit forms; the reference's nine seeds (plain) are reported."""
+    from conftest import GOLDEN
+    from test_oracle import _bowl_crossings
+    hm = femrt.load_xml_fld(GOLDEN / "Hex_CurvedFace.xml", GOLDEN / "Hex_CurvedFace.fld")
+    mesh = femrt.Mesh(gpu_ctx, hm)
+    cases = [(-0.05, 0.5, 0.0, 0.0), (-0.12, 0.43, 0.07, 0.01), (-0.2, 0.6, -0.05, -0.02), (-0.27, 0.5, 0.0, 0.015)]
+    o = np.array([[-0.5, y0, z0] for z0, y0, _, _ in cases], np.float32)
+    d = np.array([[1.0, dy, dz] for _, _, dy, dz in cases], np.float32)
+    rays = np.ascontiguousarray(np.concatenate([o.T, d.T]), np.float32)
+    near = np.array([_bowl_crossings(o[i].astype(float), d[i].astype(float))[0] for i in range(len(cases))])
+    fr = femrt.Frame(gpu_ctx, len(cases))
+    fr.set_rays(rays)
+    out = {}
+    for name, guess, flags in (("warm", femrt.GUESS_WARM, 1), ("warm_nested", femrt.GUESS_WARM, 1 | femrt.TRACE_NESTED_FIRST_HIT),
+                               ("plain", femrt.GUESS_PLAIN, 1)):
+        fr.trace(mesh, femrt.trace_params([], guess=guess, iso_enable=False, flags=flags))
+        h = fr.hits()
+        out[name] = {"face": h["face"].tolist(), "t_err": np.abs(h["t"] - near).tolist()}
+        if name != "plain":
+            assert np.all(h["elem"] == 0) and np.all(h["face"] == 0), out
+            np.testing.assert_allclose(h["t"], near, rtol=1e-9, err_msg=name)
+    print(json.dumps(out), flush=True)
+    fr.close()
+    mesh.close()

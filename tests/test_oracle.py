@@ -252,3 +252,51 @@ def test_camera_rays_closed_form(oracle):
     bot = d[:, W // 2]
     ang = math.degrees(math.acos(np.clip(top @ bot, -1, 1)))
     assert ang == pytest.approx(20.0 * (H - 1) / H, rel=2e-3)
+
+
+def _bowl_crossings(o, d, amp=HCF_AMP, n=20001, t1=3.0):
+    """Brute force, independent of any Newton: crossings of the ray with the
+    fixture's bottom face z = -amp (1 - r^2)(1 - s^2), x = (r+1)/2,
+    y = (s+1)/2 over the unit square, by dense sign changes of the ray's
+    height above the surface and FP64 bisection."""
+    o, d = np.asarray(o, float), np.asarray(d, float)
+
+    def h(t):
+        p = o + t * d
+        r, s = 2 * p[0] - 1, 2 * p[1] - 1
+        return p[2] + amp * (1 - r * r) * (1 - s * s), abs(r) <= 1 and abs(s) <= 1
+    ts = np.linspace(0.0, t1, n)
+    roots = []
+    prev = h(ts[0])
+    for t in ts[1:]:
+        cur = h(t)
+        if prev[0] * cur[0] <= 0 and (prev[1] or cur[1]):
+            a, b = t - (ts[1] - ts[0]), t
+            for _ in range(200):
+                m = 0.5 * (a + b)
+                if h(a)[0] * h(m)[0] <= 0:
+                    b = m
+                else:
+                    a = m
+            if h(0.5 * (a + b))[1]:
+                roots.append(0.5 * (a + b))
+        prev = cur
+    return roots
+
+
+@pytest.mark.parametrize("z0,y0,dy,dz", [(-0.05, 0.5, 0.0, 0.0), (-0.12, 0.43, 0.07, 0.01),
+                                         (-0.2, 0.6, -0.05, -0.02), (-0.27, 0.5, 0.0, 0.015)])
+def test_face_crossed_twice_nearest_root_matches_brute_force(oracle, z0, y0, dy, dz):
+    """A nearly horizontal ray below z = 0 enters the bowl of the curved
+    bottom face and leaves it again: the face is crossed twice.  The oracle's
+    face root must be the nearer crossing that dense sampling finds."""
+    orc, _ = _hcf(oracle)
+    o = np.array([-0.5, y0, z0], np.float32).astype(float)
+    d = np.array([1.0, dy, dz], np.float32).astype(float)
+    roots = _bowl_crossings(o, d)
+    assert len(roots) == 2, roots
+    ok, t, rst, _ = orc.face_hit(0, 0, o, d)
+    assert ok
+    assert t == pytest.approx(roots[0], abs=1e-10)
+    x, _ = orc.map(0, rst)
+    np.testing.assert_allclose(x, o + t * d, atol=1e-11)
