@@ -8,7 +8,8 @@
 //                  with FP32 accumulation; gradient sums by atomics
 //                  (squared error for the guess network, pinball loss at the
 //                  5% and 95% quantiles for the bracket network)
-//   k_mlp_adam     Adam step on the master parameters, clears the gradients
+//   k_mlp_adam     Adam step on the master parameters from the per-block partial
+//                  gradients, summed in block order (reproducible training)
 //   k_mlp_samples  training pairs from the tracer's own point inversions:
 //                  features of x = X_e(rst) -> Newton's preimage of x
 #include <hip/hip_runtime.h>
@@ -108,8 +109,12 @@ __global__ __launch_bounds__(64) void k_mlp_train(const float* __restrict__ feat
                                                   const float* __restrict__ targets, int n,
                                                   int base, int end, unsigned pa, unsigned pbias,
                                                   float inv_count, const void* __restrict__ pack,
-                                                  float* __restrict__ grad)
+                                                  float* __restrict__ part)
 {
+  // this block's partial gradient (and loss in the last slot): every entry is
+  // written once by the wave, so the sum over blocks (k_mlp_adam,
+  // k_mlp_reduce) runs in block order and training is reproducible
+  float* __restrict__ grad = part + (size_t)blockIdx.x * (Net<KIND>::NPARAM + 1);
   __shared__ __bf16 Xb[64 * 32], H1b[64 * 64], H2b[64 * 64], GZb[64 * 64], GYb[64 * 32];
   __shared__ float H1f[64 * 64], H2f[64 * 64], T[64 * 4], GY[64 * 4];
   using N = Net<KIND>;
@@ -201,12 +206,12 @@ __global__ __launch_bounds__(64) void k_mlp_train(const float* __restrict__ feat
     acc = tile_gemm(H2b + ut * 32, 1, 64, GYb, 32, 1, 64, acc);
     if (i < OUT)
       for (int v = 0; v < 16; ++v)
-        atomicAdd(&grad[N::OFF_W3 + (ut * 32 + tile_row(v, lane)) * OUT + i], acc[v]);
+        grad[N::OFF_W3 + (ut * 32 + tile_row(v, lane)) * OUT + i] = acc[v];
   }
   if (lane < OUT) {
     float s = 0.f;
     for (int r = 0; r < 64; ++r) s += GY[r * 4 + lane];
-    atomicAdd(&grad[N::OFF_B3 + lane], s);
+    grad[N::OFF_B3 + lane] = s;
   }
   // GZ2 = (GY W3^T) * (1 - H2^2)
   for (int st = 0; st < 2; ++st)
@@ -225,7 +230,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(const float* __restrict__ feat
   {
     float s = 0.f;
     for (int r = 0; r < 64; ++r) s += H2f[r * 64 + lane];
-    atomicAdd(&grad[N::OFF_B2 + lane], s);
+    grad[N::OFF_B2 + lane] = s;
   }
   // dW2 = H1^T GZ2
   for (int it = 0; it < 2; ++it)
@@ -233,7 +238,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(const float* __restrict__ feat
       f32x16 acc = {};
       acc = tile_gemm(H1b + it * 32, 1, 64, GZb + ot * 32, 64, 1, 64, acc);
       for (int v = 0; v < 16; ++v)
-        atomicAdd(&grad[N::OFF_W2 + (it * 32 + tile_row(v, lane)) * 64 + ot * 32 + i], acc[v]);
+        grad[N::OFF_W2 + (it * 32 + tile_row(v, lane)) * 64 + ot * 32 + i] = acc[v];
     }
   // GZ1 = (GZ2 W2^T) * (1 - H1^2)
   f32x16 gh[2][2];
@@ -257,7 +262,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(const float* __restrict__ feat
     // db1 = sum over samples of GZ1
     float s = 0.f;
     for (int r = 0; r < 64; ++r) s += H1f[r * 64 + lane];
-    atomicAdd(&grad[N::OFF_B1 + lane], s);
+    grad[N::OFF_B1 + lane] = s;
   }
   // dW1 = X^T GZ1 (the guess network's row 27 is its first-layer bias)
   for (int ut = 0; ut < 2; ++ut) {
@@ -265,32 +270,45 @@ __global__ __launch_bounds__(64) void k_mlp_train(const float* __restrict__ feat
     acc = tile_gemm(Xb, 1, 32, GZb + ut * 32, 64, 1, 64, acc);
     for (int v = 0; v < 16; ++v) {
       const int k = tile_row(v, lane);
-      if (k < N::W1_ROWS) atomicAdd(&grad[N::OFF_W1 + k * 64 + ut * 32 + i], acc[v]);
+      if (k < N::W1_ROWS) grad[N::OFF_W1 + k * 64 + ut * 32 + i] = acc[v];
     }
   }
   for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-  if (lane == 0) atomicAdd(&grad[N::NPARAM], lsum * inv_count);
+  if (lane == 0) grad[N::NPARAM] = lsum * inv_count;
 }
 
-// Adam on the master parameters; clears the gradient accumulators.  Thread
-// n moves the accumulated loss into loss_acc.
+// Sum of the per-block partial gradients (stride n + 1, the loss last), in
+// block order.
+__device__ __forceinline__ float part_sum(const float* __restrict__ part, int blocks, int n, int t)
+{
+  float s = 0.f;
+  for (int b = 0; b < blocks; ++b) s += part[(size_t)b * (n + 1) + t];
+  return s;
+}
+
+__global__ void k_mlp_reduce(const float* __restrict__ part, int blocks, int n, float* __restrict__ out)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t <= n) out[t] = part_sum(part, blocks, n, t);
+}
+
+// Adam on the master parameters from the step's partial gradients.  Thread n
+// adds the step's loss to loss_acc.
 __global__ void k_mlp_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
-                           float* __restrict__ g, int n, float lr, float c1, float c2,
+                           const float* __restrict__ part, int blocks, int n, float lr, float c1, float c2,
                            float* __restrict__ loss_acc)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   if (t < n) {
-    const float gi = g[t];
-    g[t] = 0.f;
+    const float gi = part_sum(part, blocks, n, t);
     const float mi = b1 * m[t] + (1.f - b1) * gi;
     const float vi = b2 * v[t] + (1.f - b2) * gi * gi;
     m[t] = mi;
     v[t] = vi;
     p[t] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
   } else if (t == n) {
-    if (loss_acc) *loss_acc += g[n];
-    g[n] = 0.f;
+    if (loss_acc) *loss_acc += part_sum(part, blocks, n, n);
   }
 }
 
@@ -333,19 +351,34 @@ __global__ void k_mlp_samples(TraceMesh M, int n_elem, int n, unsigned long long
 
 // ================================================================== host
 
+// Shapes of the network kinds (FEMRT_MLP_GUESS, FEMRT_MLP_BRACKET).
+static int net_nparam(int kind) { return kind == NET_BRACKET ? Net<NET_BRACKET>::NPARAM : Net<NET_GUESS>::NPARAM; }
+static int net_in(int kind) { return kind == NET_BRACKET ? Net<NET_BRACKET>::IN : Net<NET_GUESS>::IN; }
+static int net_out(int kind) { return kind == NET_BRACKET ? Net<NET_BRACKET>::OUT : Net<NET_GUESS>::OUT; }
 static void mlp_free(femrt_mlp* m)
 {
   if (m->d_param) (void)hipFree(m->d_param);
   if (m->d_m) (void)hipFree(m->d_m);
   if (m->d_v) (void)hipFree(m->d_v);
   if (m->d_grad) (void)hipFree(m->d_grad);
+  if (m->d_part) (void)hipFree(m->d_part);
   if (m->d_pack) (void)hipFree(m->d_pack);
 }
 
-// Shapes of the network kinds (FEMRT_MLP_GUESS, FEMRT_MLP_BRACKET).
-static int net_nparam(int kind) { return kind == NET_BRACKET ? Net<NET_BRACKET>::NPARAM : Net<NET_GUESS>::NPARAM; }
-static int net_in(int kind) { return kind == NET_BRACKET ? Net<NET_BRACKET>::IN : Net<NET_GUESS>::IN; }
-static int net_out(int kind) { return kind == NET_BRACKET ? Net<NET_BRACKET>::OUT : Net<NET_GUESS>::OUT; }
+// Per-block partial gradients for `blocks` training blocks (grown on demand).
+static float* mlp_part(femrt_mlp* m, int blocks)
+{
+  if (blocks > m->part_blocks) {
+    if (m->d_part) (void)hipFree(m->d_part);
+    m->d_part = nullptr;
+    m->part_blocks = 0;
+    if (hipMalloc(&m->d_part, (size_t)blocks * (net_nparam(m->kind) + 1) * sizeof(float)) != hipSuccess)
+      return nullptr;
+    m->part_blocks = blocks;
+  }
+  return m->d_part;
+}
+
 #define FEMRT_NET_DISPATCH(KINDV, BODY)                                     \
   if ((KINDV) == NET_BRACKET) { constexpr int K_ = NET_BRACKET; BODY; }   \
   else { constexpr int K_ = NET_GUESS; BODY; }
@@ -365,7 +398,6 @@ static int mlp_reset_state(femrt_mlp* m)
   const int np = net_nparam(m->kind);
   HIPCHK(c, hipMemsetAsync(m->d_m, 0, np * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(m->d_v, 0, np * sizeof(float), c->stream));
-  HIPCHK(c, hipMemsetAsync(m->d_grad, 0, (np + 1) * sizeof(float), c->stream));
   m->adam_t = 0;
   return mlp_pack(m);
 }
@@ -553,19 +585,23 @@ FEMRT_API int femrt_mlp_loss_grad(femrt_mlp* m, int n, const float* feats, const
   if (hipMemcpyAsync(d_in, feats, (size_t)n * IN * sizeof(float), hipMemcpyHostToDevice,
                      c->stream) != hipSuccess ||
       hipMemcpyAsync(d_t, targets, (size_t)n * OUT * sizeof(float), hipMemcpyHostToDevice,
-                     c->stream) != hipSuccess ||
-      hipMemsetAsync(m->d_grad, 0, (NP + 1) * sizeof(float), c->stream) != hipSuccess)
+                     c->stream) != hipSuccess)
     rc = fail(c, FEMRT_EHIP, "femrt_mlp_loss_grad: upload failed");
+  const int blocks = (n + 63) / 64;
+  float* part = rc == FEMRT_OK ? mlp_part(m, blocks) : nullptr;
+  if (rc == FEMRT_OK && !part) rc = fail(c, FEMRT_ENOMEM, "femrt_mlp_loss_grad: device allocation failed");
   if (rc == FEMRT_OK)
     rc = timed_launch(c, "k_mlp_train", [&] {
-      FEMRT_NET_DISPATCH(m->kind, (k_mlp_train<K_><<<(n + 63) / 64, 64, 0, c->stream>>>(
-                                       d_in, d_t, n, 0, n, 1u, 0u, 1.f / n, m->d_pack, m->d_grad)));
+      FEMRT_NET_DISPATCH(m->kind, (k_mlp_train<K_><<<blocks, 64, 0, c->stream>>>(
+                                       d_in, d_t, n, 0, n, 1u, 0u, 1.f / n, m->d_pack, part)));
+    });
+  if (rc == FEMRT_OK)
+    rc = timed_launch(c, "k_mlp_reduce", [&] {
+      k_mlp_reduce<<<(NP + 1 + 255) / 256, 256, 0, c->stream>>>(part, blocks, NP, m->d_grad);
     });
   std::vector<float> g(NP + 1);
   if (rc == FEMRT_OK && (hipMemcpyAsync(g.data(), m->d_grad, (NP + 1) * sizeof(float),
                                         hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-                         hipMemsetAsync(m->d_grad, 0, (NP + 1) * sizeof(float),
-                                        c->stream) != hipSuccess ||
                          hipStreamSynchronize(c->stream) != hipSuccess))
     rc = fail(c, FEMRT_EHIP, "femrt_mlp_loss_grad: download failed");
   (void)hipFree(d_in);
@@ -642,9 +678,15 @@ static int train_pairs_dev(femrt_mlp* m, int n, const float* d_in, const float* 
       const int base = s * p->batch;
       const int end = std::min(n, base + p->batch);
       const int cnt = end - base;
+      const int blocks = (cnt + 63) / 64;
+      float* part = mlp_part(m, blocks);
+      if (!part) {
+        rc = fail(c, FEMRT_ENOMEM, "femrt_mlp_train: device allocation failed");
+        break;
+      }
       rc = timed_launch(c, "k_mlp_train", [&] {
-        FEMRT_NET_DISPATCH(m->kind, (k_mlp_train<K_><<<(cnt + 63) / 64, 64, 0, c->stream>>>(
-                                         d_in, d_t, n, base, end, pa, pb, 1.f / cnt, m->d_pack, m->d_grad)));
+        FEMRT_NET_DISPATCH(m->kind, (k_mlp_train<K_><<<blocks, 64, 0, c->stream>>>(
+                                         d_in, d_t, n, base, end, pa, pb, 1.f / cnt, m->d_pack, part)));
       });
       if (rc) break;
       // cosine decay from lr to lr_final
@@ -655,7 +697,7 @@ static int train_pairs_dev(femrt_mlp* m, int n, const float* d_in, const float* 
       const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)m->adam_t)));
       rc = timed_launch(c, "k_mlp_adam", [&] {
         k_mlp_adam<<<(NP + 1 + 255) / 256, 256, 0, c->stream>>>(
-            m->d_param, m->d_m, m->d_v, m->d_grad, NP, lr, c1, c2, d_loss + ep);
+            m->d_param, m->d_m, m->d_v, part, blocks, NP, lr, c1, c2, d_loss + ep);
       });
       if (rc == FEMRT_OK) rc = mlp_pack(m);
     }

@@ -40,7 +40,9 @@ struct femrt_mlp {
   float* d_param = nullptr;   // MLP_NPARAM
   float* d_m = nullptr;       // Adam first moment
   float* d_v = nullptr;       // Adam second moment
-  float* d_grad = nullptr;    // MLP_NPARAM gradient accumulators + [MLP_NPARAM] loss
+  float* d_grad = nullptr;    // MLP_NPARAM gradient + [MLP_NPARAM] loss (femrt_mlp_loss_grad)
+  float* d_part = nullptr;    // per training block: its partial gradient and loss
+  int part_blocks = 0;
   void* d_pack = nullptr;     // PK_BYTES
   long long adam_t = 0;
 };

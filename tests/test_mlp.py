@@ -207,6 +207,27 @@ def test_training_reduces_loss_and_guess_error(femrt, gpu_ctx):
 
 
 @pytest.mark.gpu
+def test_training_is_reproducible(femrt, gpu_ctx):
+    """Each training block writes its partial gradient and the Adam step sums
+    them in block order (no floating-point atomics), so the same seeds give
+    the same network bit for bit, and the same gradient."""
+    hm, mesh = _small_mesh(femrt, gpu_ctx)
+    out = []
+    for _ in range(2):
+        mlp = femrt.Mlp(gpu_ctx, seed=11)
+        losses = mlp.train(mesh, n_samples=1 << 15, batch=2048, epochs=2, lr=3e-3, lr_final=1e-4, seed=5)
+        feats, tgt = femrt.mlp_samples(mesh, 1000, seed=17)
+        out.append((mlp.params(), np.asarray(losses), mlp.loss_grad(feats, tgt)))
+        mlp.close()
+    (p0, l0, (lg0, g0)), (p1, l1, (lg1, g1)) = out
+    np.testing.assert_array_equal(p0, p1)
+    np.testing.assert_array_equal(l0, l1)
+    assert lg0 == lg1
+    np.testing.assert_array_equal(g0, g1)
+    mesh.close()
+
+
+@pytest.mark.gpu
 def test_learned_guess_keeps_parity_and_what_it_gains(femrt, oracle, gpu_ctx):
     """Every walk form with network guesses gives the oracle's results.  What
     the trained network gains:
