@@ -572,7 +572,8 @@ def main():
                    "mrays_s_hit_rays": R["mrays"] * hits / n_rays,
                    "variants_mrays_s": {k: v["mrays"] for k, v in results.items()},
                    "launch_cfg": {"first_hit_block": launch_cfg[0], "walk_blocks_per_cu": launch_cfg[1],
-                                  "polish_blocks_per_cu": launch_cfg[2]}},
+                                  "polish_blocks_per_cu": launch_cfg[2], "tile": launch_cfg[3],
+                                  "walk_queue_segments_per_xcd": launch_cfg[4]}},
         "roofline": roof,
         "cpu_baseline": cpu,
         "parity_sample": check,

@@ -396,15 +396,29 @@ FEMRT_API int femrt_batch_add_paths(femrt_batch* b, femrt_mesh* mesh, femrt_fram
 FEMRT_API int femrt_batch_capture(femrt_batch* b);
 FEMRT_API int femrt_batch_launch(femrt_batch* b);
 
-/* Launch shapes of a mesh's trace kernels: cfg3 = {first-hit block size
- * (64/128/256), march workgroups per CU, polish workgroups per CU}; default
- * {256, 4, 8}.  femrt_autotune times candidate shapes on the given trace
- * (reps timed traces per candidate, one kernel at a time), keeps the fastest
- * on the mesh and returns it in cfg3 (may be NULL). */
-FEMRT_API int femrt_mesh_get_launch_cfg(const femrt_mesh* mesh, int* cfg3);
-FEMRT_API int femrt_mesh_set_launch_cfg(femrt_mesh* mesh, const int* cfg3);
+/* Launch shapes of a mesh's trace kernels, FEMRT_LAUNCH_CFG_N ints in this
+ * order:
+ *   [0] first-hit block size (64/128/256),
+ *   [1] persistent-walk workgroups per CU (1..64),
+ *   [2] polish workgroups per CU (1..64),
+ *   [3] tile edge in pixels of the first hit's ray order for camera frames
+ *       (4/8/16: square tiles, rays of a tile on neighbouring lanes, and the
+ *       hit queue the walk consumes in that order),
+ *   [4] walk hit-queue segments per XCD (1/2/4: each XCD's band of the queue
+ *       split into that many sub-bands with a counter each).
+ * Default {256, 4, 8, 8, 1}.  The shape decides only which lane traces which
+ * ray and when: every result is the same under every shape.  get/set take
+ * the first n_cfg (1..FEMRT_LAUNCH_CFG_N) values; set leaves the others as
+ * they are and rejects a value outside the lists above (nothing changed).
+ * femrt_autotune times candidate values on the given trace (reps timed
+ * traces per candidate, one value at a time over the kernels it affects),
+ * keeps the fastest on the mesh and returns the first n_cfg values in cfg
+ * (cfg may be NULL). */
+#define FEMRT_LAUNCH_CFG_N 5
+FEMRT_API int femrt_mesh_get_launch_cfg(const femrt_mesh* mesh, int* cfg, int n_cfg);
+FEMRT_API int femrt_mesh_set_launch_cfg(femrt_mesh* mesh, const int* cfg, int n_cfg);
 FEMRT_API int femrt_autotune(femrt_mesh* mesh, femrt_frame* f, const femrt_trace_params* p, int reps,
-                             int* cfg3);
+                             int* cfg, int n_cfg);
 
 /* ------------------------------------------------- learned Newton guess */
 /* No counterpart in the reference (its Newton solves start from fixed seeds,

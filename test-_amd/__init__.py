@@ -27,6 +27,7 @@ BVH_HOST_SAH, BVH_GPU_LBVH = 0, 1   # femrt_mesh_build_bvh builders
 MAX_ISO = 8
 NCOUNTERS = 33
 NOALLOC_MAX_POINTS = 65536
+LAUNCH_CFG_N = 5   # launch shape values (femrt_mesh_get_launch_cfg)
 COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",
                  "geo64_it", "field64_evals", "slab_tests", "mlp_infer", "elems_marched",
                  "exit_pred", "exit_search", "exit_march", "reentry", "redo", "brackets",
@@ -191,9 +192,9 @@ def lib():
         "femrt_batch_add_paths": (I, [P, P, P, P, I, I, I, I, P, I, C.c_ulonglong]),
         "femrt_batch_capture": (I, [P]),
         "femrt_batch_launch": (I, [P]),
-        "femrt_mesh_get_launch_cfg": (I, [P, P]),
-        "femrt_mesh_set_launch_cfg": (I, [P, P]),
-        "femrt_autotune": (I, [P, P, P, I, P]),
+        "femrt_mesh_get_launch_cfg": (I, [P, P, I]),
+        "femrt_mesh_set_launch_cfg": (I, [P, P, I]),
+        "femrt_autotune": (I, [P, P, P, I, P, I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -693,21 +694,22 @@ class Mesh:
                self.ctx.h, "set_bracket_mlp")
 
     def launch_cfg(self, cfg=None):
-        """Launch shapes (first-hit block, march and polish workgroups per CU);
-        sets them when cfg is given, returns the current ones."""
-        c3 = np.zeros(3, np.int32)
+        """Launch shape: (first-hit block, walk workgroups per CU, polish
+        workgroups per CU, tile edge in pixels, walk queue segments per XCD).
+        Sets the first len(cfg) values when cfg is given; returns all five."""
         if cfg is not None:
-            c3[:] = cfg
-            _check(lib().femrt_mesh_set_launch_cfg(self.h, _ptr(c3)), self.ctx.h, "set_launch_cfg")
-        _check(lib().femrt_mesh_get_launch_cfg(self.h, _ptr(c3)), self.ctx.h, "get_launch_cfg")
-        return tuple(int(x) for x in c3)
+            c = np.ascontiguousarray(cfg, np.int32)
+            _check(lib().femrt_mesh_set_launch_cfg(self.h, _ptr(c), int(c.size)), self.ctx.h, "set_launch_cfg")
+        c = np.zeros(LAUNCH_CFG_N, np.int32)
+        _check(lib().femrt_mesh_get_launch_cfg(self.h, _ptr(c), LAUNCH_CFG_N), self.ctx.h, "get_launch_cfg")
+        return tuple(int(x) for x in c)
 
     def autotune(self, frame, params, reps=2):
         """Times candidate launch shapes on this trace and keeps the fastest."""
-        c3 = np.zeros(3, np.int32)
-        _check(lib().femrt_autotune(self.h, frame.h, C.byref(params), reps, _ptr(c3)),
+        c = np.zeros(LAUNCH_CFG_N, np.int32)
+        _check(lib().femrt_autotune(self.h, frame.h, C.byref(params), reps, _ptr(c), LAUNCH_CFG_N),
                self.ctx.h, "autotune")
-        return tuple(int(x) for x in c3)
+        return tuple(int(x) for x in c)
 
     def close(self):
         if self.h:

@@ -233,33 +233,53 @@ FEMRT_API int femrt_batch_launch(femrt_batch* b)
 
 // ------------------------------------------------------------- autotuner
 
-FEMRT_API int femrt_mesh_get_launch_cfg(const femrt_mesh* m, int* cfg3)
+// The launch shape as FEMRT_LAUNCH_CFG_N ints, in the order of femrt.h.
+static int* cfg_slot(LaunchCfg& c, int i)
 {
-  if (!m || !cfg3) return FEMRT_EINVAL;
-  cfg3[0] = m->cfg.hit_block;
-  cfg3[1] = m->cfg.march_bpc;
-  cfg3[2] = m->cfg.polish_bpc;
+  int* slots[FEMRT_LAUNCH_CFG_N] = {&c.hit_block, &c.march_bpc, &c.polish_bpc, &c.tile, &c.qseg_per_xcd};
+  return slots[i];
+}
+
+static bool cfg_valid(const LaunchCfg& c)
+{
+  auto one_of = [](int v, std::initializer_list<int> ok) {
+    for (int x : ok)
+      if (v == x) return true;
+    return false;
+  };
+  return one_of(c.hit_block, {64, 128, 256}) && c.march_bpc >= 1 && c.march_bpc <= 64 && c.polish_bpc >= 1 &&
+         c.polish_bpc <= 64 && one_of(c.tile, {4, 8, 16}) && one_of(c.qseg_per_xcd, {1, 2, 4});
+}
+
+FEMRT_API int femrt_mesh_get_launch_cfg(const femrt_mesh* m, int* cfg, int n_cfg)
+{
+  if (!m || !cfg || n_cfg < 1 || n_cfg > FEMRT_LAUNCH_CFG_N) return FEMRT_EINVAL;
+  LaunchCfg c = m->cfg;
+  for (int i = 0; i < n_cfg; ++i) cfg[i] = *cfg_slot(c, i);
   return FEMRT_OK;
 }
 
-FEMRT_API int femrt_mesh_set_launch_cfg(femrt_mesh* m, const int* cfg3)
+FEMRT_API int femrt_mesh_set_launch_cfg(femrt_mesh* m, const int* cfg, int n_cfg)
 {
-  if (!m || !cfg3) return FEMRT_EINVAL;
-  if ((cfg3[0] != 64 && cfg3[0] != 128 && cfg3[0] != 256) || cfg3[1] < 1 || cfg3[1] > 64 || cfg3[2] < 1 ||
-      cfg3[2] > 64)
-    return fail(m->ctx, FEMRT_EINVAL, "set_launch_cfg: block 64/128/256, blocks per CU 1..64");
-  m->cfg.hit_block = cfg3[0];
-  m->cfg.march_bpc = cfg3[1];
-  m->cfg.polish_bpc = cfg3[2];
+  if (!m || !cfg) return FEMRT_EINVAL;
+  if (n_cfg < 1 || n_cfg > FEMRT_LAUNCH_CFG_N)
+    return fail(m->ctx, FEMRT_EINVAL, "set_launch_cfg: 1..5 values");
+  LaunchCfg c = m->cfg;
+  for (int i = 0; i < n_cfg; ++i) *cfg_slot(c, i) = cfg[i];
+  if (!cfg_valid(c))
+    return fail(m->ctx, FEMRT_EINVAL,
+                "set_launch_cfg: block 64/128/256, blocks per CU 1..64, tile 4/8/16, queue segments per XCD 1/2/4");
+  m->cfg = c;
   return FEMRT_OK;
 }
 
 // Times each trace kernel under candidate launch shapes on the given job
 // (one dimension at a time, the others held) and keeps the fastest on the
 // mesh.
-FEMRT_API int femrt_autotune(femrt_mesh* m, femrt_frame* f, const femrt_trace_params* p, int reps, int* cfg3)
+FEMRT_API int femrt_autotune(femrt_mesh* m, femrt_frame* f, const femrt_trace_params* p, int reps, int* cfg,
+                             int n_cfg)
 {
-  if (!m || !f || !p || reps < 1) return FEMRT_EINVAL;
+  if (!m || !f || !p || reps < 1 || (cfg && (n_cfg < 1 || n_cfg > FEMRT_LAUNCH_CFG_N))) return FEMRT_EINVAL;
   femrt_ctx* c = m->ctx;
   if (c->capturing) return fail(c, FEMRT_ESTATE, "autotune: not while a batch records");
   HIPCHK(c, hipSetDevice(c->device));
@@ -269,8 +289,9 @@ FEMRT_API int femrt_autotune(femrt_mesh* m, femrt_frame* f, const femrt_trace_pa
   const bool was_counting = f->counters;
   const auto saved_stats = c->stats;
   f->counters = false;
-  // total event time of `kernel` over reps traces, after one untimed trace
-  auto measure = [&](const char* kernel, double& ms, double* iso_ms) -> int {
+  // total event time of the given kernels over reps traces, after one
+  // untimed trace (a kernel that did not run counts 0; none ran: 1e30)
+  auto measure = [&](std::initializer_list<const char*> kernels, double& ms) -> int {
     c->timing = false;
     int rc = femrt_trace(m, f, p);
     if (rc) return rc;
@@ -280,15 +301,22 @@ FEMRT_API int femrt_autotune(femrt_mesh* m, femrt_frame* f, const femrt_trace_pa
     HIPCHK(c, hipStreamSynchronize(c->stream));
     collect_timing(c);
     if (rc) return rc;
-    ms = c->stats.count(kernel) ? c->stats[kernel].first : 1e30;
-    if (iso_ms) *iso_ms = c->stats.count("k_polish_iso") ? c->stats["k_polish_iso"].first : 0.0;
+    bool any = false;
+    ms = 0.0;
+    for (const char* k : kernels)
+      if (c->stats.count(k)) {
+        ms += c->stats[k].first;
+        any = true;
+      }
+    if (!any) ms = 1e30;
     return FEMRT_OK;
   };
   // one dimension at a time, the current shape measured first; another
   // shape replaces it only when at least 3% faster (timing noise)
   LaunchCfg best = m->cfg;
   int rc = FEMRT_OK;
-  auto tune = [&](int LaunchCfg::*field, std::initializer_list<int> cands, const char* kernel, bool with_iso) {
+  auto tune = [&](int LaunchCfg::*field, std::initializer_list<int> cands,
+                  std::initializer_list<const char*> kernels) {
     double bt = 0.0;
     std::vector<int> order{best.*field};
     for (int v : cands)
@@ -296,9 +324,8 @@ FEMRT_API int femrt_autotune(femrt_mesh* m, femrt_frame* f, const femrt_trace_pa
     for (size_t i = 0; i < order.size() && rc == FEMRT_OK; ++i) {
       m->cfg = best;
       m->cfg.*field = order[i];
-      double ms = 0, ms_iso = 0;
-      if ((rc = measure(kernel, ms, with_iso ? &ms_iso : nullptr))) break;
-      ms += ms_iso;
+      double ms = 0;
+      if ((rc = measure(kernels, ms))) break;
       if (i == 0) {
         bt = ms;
       } else if (ms < 0.97 * bt) {
@@ -307,19 +334,25 @@ FEMRT_API int femrt_autotune(femrt_mesh* m, femrt_frame* f, const femrt_trace_pa
       }
     }
   };
-  tune(&LaunchCfg::hit_block, {64, 128, 256}, "k_first_hit", false);
-  if (rc == FEMRT_OK && p->iso_enable && p->n_iso > 0) tune(&LaunchCfg::march_bpc, {2, 4, 8, 16}, "k_march", false);
-  if (rc == FEMRT_OK) tune(&LaunchCfg::polish_bpc, {4, 8, 16}, "k_polish_hit", p->iso_enable && p->n_iso > 0);
+  const bool iso = p->iso_enable && p->n_iso > 0;
+  const bool image = f->img_w > 0 && (long long)f->img_w * f->img_h * f->img_spp == f->n;
+  tune(&LaunchCfg::hit_block, {64, 128, 256}, {"k_first_hit"});
+  // the tile decides the order in which the first hit, and through the hit
+  // queue the walk, visit the rays (rays without an image layout: no tiles)
+  if (rc == FEMRT_OK && image)
+    tune(&LaunchCfg::tile, {4, 8, 16}, {"k_first_hit", "k_walk_order", "k_march", "k_polish_iso"});
+  if (rc == FEMRT_OK && iso) tune(&LaunchCfg::march_bpc, {2, 4, 8, 16}, {"k_march"});
+  if (rc == FEMRT_OK && iso) tune(&LaunchCfg::qseg_per_xcd, {1, 2, 4}, {"k_walk_order", "k_march"});
+  if (rc == FEMRT_OK)
+    tune(&LaunchCfg::polish_bpc, {4, 8, 16}, iso ? std::initializer_list<const char*>{"k_polish_hit", "k_polish_iso"}
+                                                : std::initializer_list<const char*>{"k_polish_hit"});
   m->cfg = best;
   c->stats = saved_stats;
   c->timing = was_timing;
   f->counters = was_counting;
   if (rc) return rc;
-  if (cfg3) {
-    cfg3[0] = best.hit_block;
-    cfg3[1] = best.march_bpc;
-    cfg3[2] = best.polish_bpc;
-  }
+  if (cfg)
+    for (int i = 0; i < n_cfg; ++i) cfg[i] = *cfg_slot(best, i);
   return FEMRT_OK;
 }
 

@@ -69,15 +69,19 @@ __global__ void k_raygen(float* __restrict__ rays, int W, int H, float ox, float
 }
 
 // Thread -> ray order of the first-hit pass.  With an image layout (W > 0)
-// consecutive threads cover 8x8-pixel tiles (the spp samples of a pixel
-// adjacent), so a wave traces one tile and the hit queue, which the walk
-// consumes in order, stays tile-coherent; otherwise ray order.
+// consecutive threads cover square tiles of 2^sh x 2^sh pixels (sh = 2, 3
+// or 4, a launch shape; the spp samples of a pixel adjacent), tiles in row
+// order, so the lanes of a wave trace neighbouring pixels and the hit queue,
+// which the walk consumes in order, stays tile-coherent; otherwise ray order.
+// The order only decides which lane traces which ray: every ray's result is
+// the same under any tile size.
 struct TileOrder {
-  int W, H, spp, tiles_x;
+  int W, H, spp, sh, tiles_x;
   __host__ __device__ int threads(int n) const
   {
     if (W <= 0) return n;
-    return tiles_x * ((H + 7) >> 3) * 64 * spp;
+    const int te = 1 << sh;
+    return tiles_x * ((H + te - 1) >> sh) * (te * te) * spp;
   }
   __device__ bool ray(int t, int n, int& r) const
   {
@@ -85,9 +89,10 @@ struct TileOrder {
       r = t;
       return t < n;
     }
+    const int te = 1 << sh;
     const int s = t % spp, pix = t / spp;
-    const int tile = pix >> 6, in = pix & 63;
-    const int x = (tile % tiles_x) * 8 + (in & 7), y = (tile / tiles_x) * 8 + (in >> 3);
+    const int tile = pix >> (2 * sh), in = pix & (te * te - 1);
+    const int x = (tile % tiles_x) * te + (in & (te - 1)), y = (tile / tiles_x) * te + (in >> sh);
     r = (y * W + x) * spp + s;
     return x < W && y < H;
   }
@@ -242,9 +247,9 @@ __global__ __launch_bounds__(64) void k_retrace(const float* __restrict__ rays, 
   flush_work(work, w);
 }
 
-// Walk order of the hit queue.  The queue is split into FEMRT_QSEG
-// contiguous segments (tile order: each covers one band of the image), as
-// the walk reads it; within each segment the hits are put in descending
+// Walk order of the hit queue.  The queue is split into nseg (one block
+// each) contiguous segments (tile order: each covers one band of the image),
+// as the walk reads it; within each segment the hits are put in descending
 // chord bin, tile order kept within a bin (a stable counting sort, one block
 // per segment, each thread a contiguous chunk).  The walk then starts the
 // rays that may walk longest first, and the rays still walking when the
@@ -253,13 +258,13 @@ __global__ __launch_bounds__(64) void k_retrace(const float* __restrict__ rays, 
 
 __global__ __launch_bounds__(FEMRT_ORDER_THREADS) void k_walk_order(
     const int* __restrict__ queue, const unsigned char* __restrict__ qbin,
-    const int* __restrict__ counts, int* __restrict__ out)
+    const int* __restrict__ counts, int* __restrict__ out, int nseg)
 {
   __shared__ int s_cnt[FEMRT_QBINS][FEMRT_ORDER_THREADS];
   __shared__ int s_base[FEMRT_QBINS];
   const int count = counts[0], seg = blockIdx.x, t = threadIdx.x;
-  const int lo = (int)((long long)count * seg / FEMRT_QSEG);
-  const int hi = (int)((long long)count * (seg + 1) / FEMRT_QSEG);
+  const int lo = (int)((long long)count * seg / nseg);
+  const int hi = (int)((long long)count * (seg + 1) / nseg);
   const int c0 = lo + (int)((long long)(hi - lo) * t / FEMRT_ORDER_THREADS);
   const int c1 = lo + (int)((long long)(hi - lo) * (t + 1) / FEMRT_ORDER_THREADS);
   int cnt[FEMRT_QBINS];
@@ -1132,9 +1137,11 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   if (rc) return rc;
   const int dev_cus = c->cus;
   const LaunchCfg& cfg = m->cfg;
-  TileOrder order{0, 0, 1, 0};
-  if (f->img_w > 0 && (long long)f->img_w * f->img_h * f->img_spp == n)
-    order = TileOrder{f->img_w, f->img_h, f->img_spp, (f->img_w + 7) >> 3};
+  TileOrder order{0, 0, 1, 0, 0};
+  if (f->img_w > 0 && (long long)f->img_w * f->img_h * f->img_spp == n) {
+    const int sh = cfg.tile == 4 ? 2 : (cfg.tile == 16 ? 4 : 3);
+    order = TileOrder{f->img_w, f->img_h, f->img_spp, sh, (f->img_w + (1 << sh) - 1) >> sh};
+  }
   const int pblocks = dev_cus * cfg.polish_bpc;
   const int hblock = cfg.hit_block;
   IsoParams ip;
@@ -1151,6 +1158,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   ip.tmin = p->t_min;
   ip.skip = (p->flags & FEMRT_TRACE_NO_SAMPLE_SKIP) ? 0 : 1;
   ip.endgame = learned_br ? ISO_LEARNED_BISECT : ((p->flags & FEMRT_TRACE_ISO_BISECT) ? ISO_BISECT : ISO_NEWTON);
+  ip.qseg_per_xcd = cfg.qseg_per_xcd;
   TraceMesh TM;
   TM.geom = m->d_geom;
   TM.field = m->d_field;
@@ -1193,8 +1201,8 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
       if (rc) break;
       if (n_iso > 0 && !(p->flags & FEMRT_TRACE_BINNED_WALK)) {
         rc = timed_launch(c, "k_walk_order", [&] {
-          k_walk_order<<<FEMRT_QSEG, FEMRT_ORDER_THREADS, 0, c->stream>>>(f->d_queue_hit, f->d_qbin,
-                                                                          f->d_counts, f->d_queue_walk);
+          k_walk_order<<<ip.qseg_per_xcd * FEMRT_XCDS, FEMRT_ORDER_THREADS, 0, c->stream>>>(
+              f->d_queue_hit, f->d_qbin, f->d_counts, f->d_queue_walk, ip.qseg_per_xcd * FEMRT_XCDS);
         });
         if (rc) break;
       }

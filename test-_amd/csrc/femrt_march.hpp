@@ -242,21 +242,24 @@ __global__ __launch_bounds__(256, 2) void k_march(
   float o[3] = {0.f, 0.f, 0.f}, d[3] = {1.f, 1.f, 1.f}, id[3] = {1.f, 1.f, 1.f};
   MarchState S;
   march_begin(S, HitF32());
-  // The hit queue is split into FEMRT_QSEG contiguous segments (tile order,
-  // so each covers one band of the image) with a counter each.  A block
-  // starts on the segment of its XCD (workgroups are dispatched round-robin
-  // over the XCDs) so that each XCD's L2 holds the elements of one band; a
-  // wave that finds its segment drained moves on to the next one.
-  int seg = blockIdx.x % FEMRT_QSEG, drained = 0;  // wave-uniform
+  // The hit queue is split into FEMRT_XCDS x q contiguous segments (tile
+  // order, so each covers one band of the image) with a counter each: XCD x
+  // owns segments x q .. x q + q - 1.  A block starts on a segment of its XCD
+  // (workgroups are dispatched round-robin over the XCDs, so block b runs on
+  // XCD b % FEMRT_XCDS), its XCD's blocks spread over the q sub-bands, so
+  // that each XCD's L2 holds the elements of one band; a wave that finds its
+  // segment drained moves on to the next one.
+  const int q_x = P.qseg_per_xcd, nseg = FEMRT_XCDS * q_x;
+  int seg = (blockIdx.x % FEMRT_XCDS) * q_x + (blockIdx.x / FEMRT_XCDS) % q_x, drained = 0;  // wave-uniform
   // timeline with the work counters: start (min), queue dry (min), end (max)
   const bool timeline = work != nullptr && lane == 0;
   if (timeline) atomicMin((unsigned long long*)&work[16], (unsigned long long)wall_clock64());
   bool saw_dry = false;
   for (;;) {
     bool need = !active;
-    while (drained < FEMRT_QSEG && __any(need)) {
-      const int lo = (int)((long long)count * seg / FEMRT_QSEG);
-      const int hi = (int)((long long)count * (seg + 1) / FEMRT_QSEG);
+    while (drained < nseg && __any(need)) {
+      const int lo = (int)((long long)count * seg / nseg);
+      const int hi = (int)((long long)count * (seg + 1) / nseg);
       const int q = lo + wave_append(need, &counts[FEMRT_QSEG_CTR + seg]);
       const bool got = need && q < hi;
       bool took = false;
@@ -278,12 +281,12 @@ __global__ __launch_bounds__(256, 2) void k_march(
         active = true;
       }
       if (__any(need && !got)) {
-        seg = seg + 1 == FEMRT_QSEG ? 0 : seg + 1;
+        seg = seg + 1 == nseg ? 0 : seg + 1;
         ++drained;
       }
       need = need && !took;
     }
-    if (timeline && drained == FEMRT_QSEG && !saw_dry) {
+    if (timeline && drained == nseg && !saw_dry) {
       saw_dry = true;
       atomicMin((unsigned long long*)&work[17], (unsigned long long)wall_clock64());
     }

@@ -52,6 +52,8 @@ struct LaunchCfg {
   int hit_block = 256;     // k_first_hit / k_raygen threads per block
   int march_bpc = 4;       // persistent walk: blocks of 256 per compute unit
   int polish_bpc = 8;      // polish kernels: blocks of 256 per compute unit
+  int tile = 8;            // first-hit ray order: tiles of tile x tile pixels (4, 8, 16)
+  int qseg_per_xcd = 1;    // walk hit queue: segments per XCD (1, 2, 4)
 };
 
 struct femrt_mesh {
@@ -242,13 +244,16 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter)
 // Frame queue counters: [0] hits, [2..3] binned-walk lists, [4] rays to
 // trace again (FEMRT_RETRACE_CTR), [5] binned-walk start list,
 // [FEMRT_QSEG_CTR + s] progress in segment s of the hit queue (persistent walk).
+// The walk's hit queue has FEMRT_XCDS x (1, 2 or 4) segments (a launch
+// shape): each XCD's band of the queue is cut into that many sub-bands.
 #define FEMRT_RETRACE_CTR 4
-#define FEMRT_QSEG 8
+#define FEMRT_XCDS 8
+#define FEMRT_MAX_QSEG 32
 #define FEMRT_QSEG_CTR 16
-#define FEMRT_NCOUNTS 32
+#define FEMRT_NCOUNTS (FEMRT_QSEG_CTR + FEMRT_MAX_QSEG)
 
 // Grid-stride loop over [0, count) split by XCD.  The workgroups of a
-// launch are dispatched round-robin over the FEMRT_QSEG (8) XCDs, so block b
+// launch are dispatched round-robin over the FEMRT_XCDS (8) XCDs, so block b
 // runs on XCD b % 8; the blocks of XCD x stride over the x-th of 8
 // contiguous slices of the range.  Over a queue in tile order each slice is
 // one band of the image, so each XCD's L2 caches the elements of its own
@@ -259,16 +264,16 @@ struct XcdStride {
   __device__ XcdStride(int count)
   {
     const int b = blockIdx.x, nb = gridDim.x, t = threadIdx.x, bs = blockDim.x;
-    if (nb < FEMRT_QSEG) {
+    if (nb < FEMRT_XCDS) {
       q = b * bs + t;
       end = count;
       step = nb * bs;
       return;
     }
-    const int x = b % FEMRT_QSEG, rank = b / FEMRT_QSEG;
-    const int nx = (nb - x + FEMRT_QSEG - 1) / FEMRT_QSEG;  // blocks on XCD x
-    q = (int)((long long)count * x / FEMRT_QSEG) + rank * bs + t;
-    end = (int)((long long)count * (x + 1) / FEMRT_QSEG);
+    const int x = b % FEMRT_XCDS, rank = b / FEMRT_XCDS;
+    const int nx = (nb - x + FEMRT_XCDS - 1) / FEMRT_XCDS;  // blocks on XCD x
+    q = (int)((long long)count * x / FEMRT_XCDS) + rank * bs + t;
+    end = (int)((long long)count * (x + 1) / FEMRT_XCDS);
     step = nx * bs;
   }
 };

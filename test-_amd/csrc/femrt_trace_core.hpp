@@ -42,6 +42,7 @@ struct IsoParams {
   int mlp_lead;   // leading interior samples of a segment guessed by the network
   int skip;       // certified sample skipping on (see march_samples)
   int endgame;    // FP64 iso endgame: ISO_NEWTON, ISO_BISECT or ISO_LEARNED_BISECT
+  int qseg_per_xcd;  // persistent walk: hit-queue segments per XCD (1, 2 or 4)
 };
 
 // FP64 iso endgames (IsoParams::endgame)

@@ -165,17 +165,24 @@ def test_autotune_keeps_results_and_picks_a_candidate(femrt, gpu_ctx):
     f = femrt.Frame(ctx, W * H)
     f.camera_rays(_camera(femrt, hm), W, H)
     p = femrt.trace_params([-0.1, 0.0, 0.1], guess=femrt.GUESS_WARM, flags=1)
-    assert m.launch_cfg() == (256, 4, 8)
+    assert m.launch_cfg() == (256, 4, 8, 8, 1)
     f.trace(m, p)
     before = _snapshot(f, 3)
     cfg = m.autotune(f, p, reps=2)
     assert cfg[0] in (64, 128, 256) and cfg[1] in (2, 4, 8, 16) and cfg[2] in (4, 8, 16)
+    assert cfg[3] in (4, 8, 16) and cfg[4] in (1, 2, 4)
     assert m.launch_cfg() == cfg
-    for shape in [cfg, (64, 2, 4), (128, 16, 16)]:
+    # every shape, tile and queue split included, gives the same results bit for bit
+    for shape in [cfg, (64, 2, 4, 4, 2), (128, 16, 16, 16, 4), (256, 1, 8, 4, 4), (64, 4, 8, 16, 2)]:
         m.launch_cfg(shape)
         f.trace(m, p)
         _same(_snapshot(f, 3), before)
-    with pytest.raises(femrt.FemrtError):
-        m.launch_cfg((96, 4, 8))
+    # the first values alone set only those
+    m.launch_cfg((256, 4, 8))
+    assert m.launch_cfg() == (256, 4, 8, 16, 2)
+    for bad in [(96, 4, 8), (256, 4, 8, 12), (256, 4, 8, 8, 3), (256, 4, 8, 8, 1, 1)]:
+        with pytest.raises(femrt.FemrtError):
+            m.launch_cfg(bad)
+    assert m.launch_cfg() == (256, 4, 8, 16, 2)
     f.close()
     m.close()
