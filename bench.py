@@ -363,10 +363,14 @@ def main():
     # network's predicted sub-bracket (config 5's comparison)
     # learned_nested / warm_nested: each lane's first-hit face searches run to
     # completion (no flattened Newton loop), same hits
+    # learned_nohermite / warm_nohermite: a segment's first sample starts from
+    # the network's guess or the entry point even where the Hermite
+    # interpolant of rst(t) is at hand
     names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED,
              "learned_every": femrt.GUESS_LEARNED, "warm_coop": femrt.GUESS_WARM,
              "bisect": femrt.GUESS_LEARNED, "learned_bracket": femrt.GUESS_LEARNED,
-             "learned_nested": femrt.GUESS_LEARNED, "warm_nested": femrt.GUESS_WARM}
+             "learned_nested": femrt.GUESS_LEARNED, "warm_nested": femrt.GUESS_WARM,
+             "learned_nohermite": femrt.GUESS_LEARNED, "warm_nohermite": femrt.GUESS_WARM}
     variants = [v for v in args.variants.split(",") if v]
     results = {}
     mlp_info = None
@@ -453,6 +457,8 @@ def main():
             flags |= femrt.TRACE_ISO_BISECT
         if v == "learned_bracket":
             flags |= femrt.TRACE_ISO_LEARNED_BRACKET
+        if v.endswith("_nohermite"):
+            flags |= femrt.TRACE_NO_HERMITE_START
         if v.endswith("_nested"):
             flags |= femrt.TRACE_NESTED_FIRST_HIT
         wall, kt, cnt = run_variant(femrt, lanes, names[v], args.steps, args.warmup, flags)
@@ -589,8 +595,9 @@ def main():
         line["config"]["note"] = (
             "value is the learned-guess variant; the network gains nothing over the engineered warm "
             "starts (speedup_vs_warm ~1): the speedup over plain Newton comes from non-learned "
-            "engineering shared by both -- predicted exits, sub-patch face seeds, extrapolated sample "
-            "starts, certified sample skipping")
+            "engineering shared by both -- predicted exits, sub-patch face seeds, Hermite and "
+            "extrapolated sample starts, certified sample skipping.  The walk asks the network only "
+            "for segments whose exit search left no tangents (network_inferences_per_frame)")
     if single:
         line["config"]["single_frame"] = single
     if grad:

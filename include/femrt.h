@@ -270,6 +270,18 @@ typedef struct {
                                           loop that runs one Newton iteration on
                                           all the wave's lanes at a time.  Same
                                           hits either way                         */
+#define FEMRT_TRACE_NO_HERMITE_START 512 /* WARM / LEARNED: start the first
+                                          interior sample of a segment from
+                                          the entry point (warm) or the
+                                          network (learned) even where the
+                                          predicted exit left the tangents
+                                          drst/dt at both ends; by default
+                                          the cubic Hermite interpolant of
+                                          rst(t) between entry and exit is
+                                          the start there, and the network
+                                          guesses only segments without
+                                          tangents.  Same results (the FP64
+                                          endgame), other FP32 iterates     */
 #define FEMRT_TRACE_SEG_INFO 128       /* record each bracket's ends (rst and the
                                           FP32 g at ta and tb), the bracket
                                           network's inputs, for

@@ -296,10 +296,13 @@ __global__ __launch_bounds__(256, 2) void k_march(
     if constexpr (COOP) coop_redo = coop_scan_round<NG, NF>(M, P, o, d, S, active, *coop, sink, w);
     if constexpr (LEARNED && !COOP) {
       // every lane of the wave is here: the network runs on the whole wave
-      const bool seg = active && S.ok;
+      // (a segment whose first sample starts from the Hermite interpolant
+      // needs no guess there; guesses of later samples still do)
+      const bool seg = active && S.ok && (P.mlp_lead > 1 || !(S.have_m && P.hermite));
       const float* bx = M.boxes + (size_t)(seg ? S.e : 0) * FEMRT_BOX_STRIDE;
       const float* vt = M.verts + (size_t)(seg ? S.e : 0) * 24;
-      const int jmax = min(P.mlp_lead, P.samples - 1);
+      // (wave-uniform: the forward pass is a wave-wide matrix product)
+      const int jmax = __any(seg) ? min(P.mlp_lead, P.samples - 1) : 0;
       for (int j = 1; j <= jmax; ++j) {
         float f[MLP_IN];
         if (seg) {

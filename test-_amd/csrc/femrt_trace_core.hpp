@@ -43,6 +43,7 @@ struct IsoParams {
   int skip;       // certified sample skipping on (see march_samples)
   int endgame;    // FP64 iso endgame: ISO_NEWTON, ISO_BISECT or ISO_LEARNED_BISECT
   int qseg_per_xcd;  // persistent walk: hit-queue segments per XCD (1, 2 or 4)
+  int hermite;       // first interior sample from the Hermite interpolant where tangents exist
 };
 
 // FP64 iso endgames (IsoParams::endgame)
@@ -1251,7 +1252,20 @@ FEMRT_HD bool march_samples(const TraceMesh& M, const IsoParams& P, const float*
   bool zero_retry = false, cert_tried = false;
   auto start_sample = [&]() {
     zero_retry = false;
-    if (guess.get(j, rst)) {
+    if (n_prev < 2 && S.have_m && P.hermite && P.guess != FEMRT_GUESS_PLAIN) {
+      // no two samples to extrapolate from, but the predicted exit left
+      // drst/dt at both ends: the cubic Hermite interpolant of rst(t)
+      // between entry and exit, exact at both ends with their tangents
+      const float L = S.t_out - S.t_in;
+      const float q = (march_sample_t(S, j, Ms) - S.t_in) / L, q2 = q * q, q3 = q2 * q;
+      const float h00 = 2.f * q3 - 3.f * q2 + 1.f, h10 = q3 - 2.f * q2 + q, h01 = 3.f * q2 - 2.f * q3,
+                  h11 = q3 - q2;
+      float r_out[3];
+      face_to_rst(S.f_out, S.u_out, S.v_out, r_out[0], r_out[1], r_out[2]);
+      for (int c = 0; c < 3; ++c)
+        rst[c] = fminf(fmaxf(h00 * rst_in[c] + h01 * r_out[c] + L * (h10 * S.m_in[c] + h11 * S.m_out[c]), -1.f),
+                       1.f);
+    } else if (guess.get(j, rst)) {
       zero_retry = true;  // plain Newton from 0 is the fallback of a learned guess
     } else if (P.guess == FEMRT_GUESS_PLAIN) {
       rst[0] = rst[1] = rst[2] = 0.f;

@@ -237,11 +237,17 @@ def test_learned_guess_keeps_parity_and_what_it_gains(femrt, oracle, gpu_ctx):
       it guesses each segment's first sample of the sequential scan;
     * over the engineered warm starts of the same walk: nothing measurable.
       In the sequential scan the network's first-sample guess (median error
-      about 0.05 in reference units) is no better than the extrapolated warm
-      start, and the cooperative scan (warm guesses only) starts every sample
-      from the cubic Hermite interpolant of rst(t) (error O(dt^4)).  The test
-      asserts parity and that the network costs no iterations over warm
-      starts in the sequential scan, and prints the ratios.
+      about 0.05 in reference units) is no better than the warm start from
+      the entry point, and the cooperative scan (warm guesses only) starts
+      every sample from the cubic Hermite interpolant of rst(t) (error
+      O(dt^4)).  The test asserts parity and that the network costs no
+      iterations over warm starts in the sequential scan, and prints the
+      ratios.
+    These comparisons switch off the default Hermite start of each
+    segment's first sample (FEMRT_TRACE_NO_HERMITE_START), which takes the
+    network's place wherever the predicted exit left the tangents; the
+    default walk (form "hermite") keeps parity and needs no more iterations
+    than the same walk without it.
     Trained on one seeded mesh, evaluated on another."""
     import oracle.parity as parity
     train_hm = femrt.gen_structured_hex(6, 4, 6, pert=2, amp=0.08, seed=1)
@@ -261,14 +267,18 @@ def test_learned_guess_keeps_parity_and_what_it_gains(femrt, oracle, gpu_ctx):
     fr.camera_rays(cam, W, H)
     isos = [-0.1, 0.0, 0.1]
     og, oiso = oracle.Oracle.from_mesh(hm).trace(fr.get_rays(), isos)
-    forms = {"every": 1 | femrt.TRACE_LEARNED_EVERY_SAMPLE, "seq": 1, "default": 1 | femrt.TRACE_COOP_SCAN}
+    nh = femrt.TRACE_NO_HERMITE_START
+    forms = {"every": 1 | femrt.TRACE_LEARNED_EVERY_SAMPLE | nh, "seq": 1 | nh, "hermite": 1,
+             "default": 1 | femrt.TRACE_COOP_SCAN}
     work = {}
     for form, flags in forms.items():
         # the cooperative scan (samples spread over the lanes, Hermite starts)
         # takes warm guesses only
         runs = [("warm", femrt.GUESS_WARM, mlp)]
-        if form != "default":
+        if form in ("every", "seq"):
             runs += [("trained", femrt.GUESS_LEARNED, mlp), ("zero", femrt.GUESS_LEARNED, zero)]
+        elif form == "hermite":
+            runs += [("trained", femrt.GUESS_LEARNED, mlp)]
         for name, guess, net in runs:
             mesh.set_guess_mlp(net)
             fr.counters(True)
@@ -286,6 +296,9 @@ def test_learned_guess_keeps_parity_and_what_it_gains(femrt, oracle, gpu_ctx):
     assert it["trained", "seq"] < it["zero", "seq"]
     assert work["trained", "seq"]["mlp_infer"] < 0.25 * work["trained", "every"]["mlp_infer"]
     assert it["trained", "seq"] <= 1.05 * it["warm", "seq"]
+    print(json.dumps({"hermite_over_entry_start_iterations": it["warm", "hermite"] / it["warm", "seq"]}))
+    assert it["warm", "hermite"] <= it["warm", "seq"]
+    assert it["trained", "hermite"] <= it["trained", "seq"]
     with pytest.raises(femrt.FemrtError):
         mesh.set_guess_mlp(None)
         fr.trace(mesh, femrt.trace_params(isos, guess=femrt.GUESS_LEARNED))
