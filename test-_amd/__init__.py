@@ -23,10 +23,11 @@ TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE, TRACE_NO_SAMPLE_SKIP 
 TRACE_COOP_SCAN = 16
 TRACE_ISO_BISECT, TRACE_ISO_LEARNED_BRACKET, TRACE_SEG_INFO, TRACE_NESTED_FIRST_HIT = 32, 64, 128, 256
 TRACE_NO_HERMITE_START = 512
+TRACE_SKIP_JUMP = 1024
 GRAD_MFMA, GRAD_VALU = 0, 1   # femrt_frame_iso_gradients engines
 BVH_HOST_SAH, BVH_GPU_LBVH = 0, 1   # femrt_mesh_build_bvh builders
 MAX_ISO = 8
-NCOUNTERS = 33
+NCOUNTERS = 34
 NOALLOC_MAX_POINTS = 65536
 LAUNCH_CFG_N = 5   # launch shape values (femrt_mesh_get_launch_cfg)
 COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals",
@@ -35,7 +36,7 @@ COUNTER_NAMES = ["rays", "hit_rays", "face_newton_it", "invert_it", "field_evals
                  "walk_t_start", "walk_t_drained", "walk_t_end",
                  "len_1", "len_2_3", "len_4_7", "len_8_15", "len_16_31", "len_32_63", "len_64_127",
                  "len_128_up", "scan_skipped", "samples_skipped", "walks_stalled",
-                 "first_hit_face_it", "first_hit_slab_tests", "walk_skips"]
+                 "first_hit_face_it", "first_hit_slab_tests", "walk_skips", "walk_jumps"]
 
 
 class FemrtError(RuntimeError):

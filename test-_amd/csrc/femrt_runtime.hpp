@@ -224,6 +224,7 @@ __device__ __forceinline__ void flush_work(long long* work, const Work& w, bool 
   add_work(work, 28, w.samples_skipped);
   add_work(work, 29, w.stalled);
   add_work(work, 32, w.skips);
+  add_work(work, 33, w.jumps);
 }
 
 // Append `pred` lanes to a queue: one atomic per wave, ballot + prefix count

@@ -44,7 +44,15 @@ struct IsoParams {
   int endgame;    // FP64 iso endgame: ISO_NEWTON, ISO_BISECT or ISO_LEARNED_BISECT
   int qseg_per_xcd;  // persistent walk: hit-queue segments per XCD (1, 2 or 4)
   int hermite;       // first interior sample from the Hermite interpolant where tangents exist
+  int jump;          // after this many segments in a row that cannot cross, the walk jumps
+                     // over the rest of such a run by a BVH query (march_advance); 0: never
+                     // (the default; FEMRT_TRACE_SKIP_JUMP)
 };
+
+// Segments in a row whose field bounds rule out every open isovalue before
+// the walk jumps over the rest of the run (a shorter run is walked: a jump's
+// BVH query holds its wave's other lanes longer than a few exit searches)
+#define FEMRT_JUMP_AFTER 2
 
 // FP64 iso endgames (IsoParams::endgame)
 enum { ISO_NEWTON = 0, ISO_BISECT = 1, ISO_LEARNED_BISECT = 2 };
@@ -62,6 +70,7 @@ struct Work {
   unsigned samples_skipped = 0;  // scan samples whose signs the bounds certified
   unsigned stalled = 0;  // walks ended stalled (FEMRT_MAX_WALK_SKIPS stalls skipped already)
   unsigned skips = 0;    // stalls skipped past (runs of FEMRT_MAX_MARCHED_EXITS marched exits)
+  unsigned jumps = 0;    // runs of elements that cannot cross passed by a BVH query
 };
 
 // The same counters compiled away: the kernels of timed runs take NoWork,
@@ -73,8 +82,21 @@ struct NoCount {
 };
 struct NoWork {
   NoCount face_it, inv_it, field, geo64, field64, slab, mlp, marched;
-  NoCount exit_pred, exit_search, exit_march, reentry, redo, brackets, skipped, samples_skipped, stalled, skips;
+  NoCount exit_pred, exit_search, exit_march, reentry, redo, brackets, skipped, samples_skipped, stalled, skips,
+      jumps;
 };
+
+// Whether element e can hold a crossing of an isovalue not yet found
+// (`found` bits): some such isovalue lies within the element's bounds of u.
+// Without bounds every element can.
+FEMRT_HD bool elem_may_cross(const TraceMesh& M, const IsoParams& P, int e, unsigned found)
+{
+  if (!M.frange) return true;
+  const float lo = M.frange[2 * (size_t)e], hi = M.frange[2 * (size_t)e + 1];
+  for (int k = 0; k < FEMRT_MAX_ISO; ++k)
+    if (k < P.n_iso && !((found >> k) & 1u) && P.iso[k] >= lo && P.iso[k] <= hi) return true;
+  return false;
+}
 
 // FP32 result of the first-hit stage.
 struct HitF32 {
@@ -392,10 +414,12 @@ struct HitExcl {
 // BVH traversal over element boxes, near child first, short stack; per leaf
 // element: element box, then per face its box and the face root.  The
 // nearest accepted root over all elements (but `exclude`) and faces is the
-// hit.
+// hit.  With `cross` set, only elements that can hold a crossing of an
+// isovalue not in `found` take part (elem_may_cross).
 template <int NG, typename W>
 FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d, int guess,
-                              float tmin, float tmax, W& w, int exclude = -1, const HitExcl* excl = nullptr)
+                              float tmin, float tmax, W& w, int exclude = -1, const HitExcl* excl = nullptr,
+                              const IsoParams* cross = nullptr, unsigned found = 0)
 {
   HitF32 h;
   float id[3];
@@ -419,6 +443,7 @@ FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d
       for (int i = 0; i < -b; ++i) {
         const int e = M.prims[a + i];
         if (e == exclude) continue;
+        if (cross && !elem_may_cross(M, *cross, e, found)) continue;
         const float* bx = M.boxes + (size_t)e * FEMRT_BOX_STRIDE;
         float e0, e1;
         ++w.slab;
@@ -1028,6 +1053,7 @@ struct MarchState {
   int step;               // segments walked (bits 0-15), exits found by marching in a row
                           // (bits 16-23), stalls skipped (bits 24-30)
   unsigned found;         // isovalues whose bracket has been recorded
+  int run;                // segments in a row whose scan the field bounds ruled out
   bool ok;
   bool redo;              // samples showed a wrong exit: search again, entry face included
   bool have_m;            // m_in, m_out hold drst/dt at the entry and exit (predicted exits)
@@ -1050,6 +1076,7 @@ FEMRT_HD void march_begin(MarchState& S, const HitF32& h)
   S.t_out = S.u_out = S.v_out = 0.f;
   S.step = 0;
   S.found = 0;
+  S.run = 0;
   S.ok = false;
   S.redo = false;
   S.have_m = false;
@@ -1469,11 +1496,7 @@ FEMRT_HD void seg_info(const TraceMesh& M, const IsoParams& P, const float* o, c
 // open lies within the element's bounds of u (else the scan is skipped).
 FEMRT_HD bool seg_may_cross(const TraceMesh& M, const IsoParams& P, const MarchState& S)
 {
-  if (!M.frange) return true;
-  const float lo = M.frange[2 * (size_t)S.e], hi = M.frange[2 * (size_t)S.e + 1];
-  for (int k = 0; k < FEMRT_MAX_ISO; ++k)
-    if (k < P.n_iso && !((S.found >> k) & 1u) && P.iso[k] >= lo && P.iso[k] <= hi) return true;
-  return false;
+  return elem_may_cross(M, P, S.e, S.found);
 }
 
 // Ray parameter of sample j of [t_in, t_out] (the sequential scan's
@@ -1707,17 +1730,27 @@ FEMRT_HD int march_next(const TraceMesh& M, const IsoParams& P, MarchState& S)
 // through a curved boundary face can re-enter the same element through it
 // further on -- so the search starts 1e-4 of the element beyond the point
 // (as the exit search does), where the exit's own root no longer counts.
+// With `jump` (march_advance), the ray has just entered S.e at S.t_in from
+// a segment that could not cross, and S.e cannot either: the search skips
+// every element that cannot hold a crossing of an open isovalue and starts
+// at t_jump (just before S.t_in), so it lands on the entry of the next
+// element along the ray that can (or finds none, and the ray has no
+// crossing left).  Elements it skips have no root to step over, and the
+// last element that could cross was left before the skipped segment began,
+// more than the step back before t_jump.
 // Returns false when there is none.
 template <int NG, typename W>
 FEMRT_HD bool march_reenter(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
-                            MarchState& S, W& w)
+                            MarchState& S, W& w, bool jump = false, float t_jump = 0.f)
 {
   const float t_from = S.ok ? S.t_out : S.t_in;
   const float* bx = M.boxes + (size_t)S.e * FEMRT_BOX_STRIDE;
   const float ext = fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
   ++w.reentry;
-  const HitF32 h2 = first_hit_ray<NG>(M, o, d, P.guess,
-                                      t_from + fmaxf(1e-4f * ext, 1e-6f * (1.f + fabsf(t_from))), 1e30f, w);
+  w.jumps += jump ? 1u : 0u;
+  const float t_lo = jump ? t_jump : t_from + fmaxf(1e-4f * ext, 1e-6f * (1.f + fabsf(t_from)));
+  const HitF32 h2 = first_hit_ray<NG>(M, o, d, P.guess, t_lo, 1e30f, w, -1, nullptr, jump ? &P : nullptr,
+                                      S.found);
   if (h2.elem < 0) return false;
   S.e = h2.elem;
   S.f_in = h2.face;
@@ -1728,18 +1761,41 @@ FEMRT_HD bool march_reenter(const TraceMesh& M, const IsoParams& P, const float*
   return true;
 }
 
-// march_next, then march_reenter when the ray left the mesh.
+// march_next, then march_reenter when the ray left the mesh.  A segment
+// whose scan the field bounds ruled out, leading into a neighbour whose
+// bounds rule it out too, starts a run of elements that cannot hold a
+// crossing: instead of walking it element by element (an exit search each),
+// march_reenter jumps to the next element along the ray that can (P.jump).
+// The elements passed hold no crossing of an open isovalue, so the brackets
+// are the walk's.  The jump comes after P.jump such segments in a row.  The
+// search starts half the exit floor (1e-4 of the element) before the
+// segment's exit, so an element entered right at that point (the neighbour
+// only clipped) is not missed; a skipped segment shorter than the floor
+// walks on instead.
 template <int NG, typename W>
 FEMRT_HD bool march_advance(const TraceMesh& M, const IsoParams& P, const float* o, const float* d,
                             MarchState& S, W& w)
 {
+  bool none_here = P.jump > 0 && M.frange && S.ok && !seg_may_cross(M, P, S);
+  S.run = none_here ? S.run + 1 : 0;
+  float t_jump = 0.f;
+  if (none_here) {
+    const float* bx = M.boxes + (size_t)S.e * FEMRT_BOX_STRIDE;
+    const float eps = 1e-4f * fmaxf(bx[3] - bx[0], fmaxf(bx[4] - bx[1], bx[5] - bx[2]));
+    none_here = S.run >= P.jump && S.t_out - S.t_in > 2.f * eps;
+    t_jump = S.t_out - 0.5f * eps;
+  }
   const int k = march_next(M, P, S);
   if (k == MARCH_DONE) {
     if (march_stalled(S) && march_skips(S) >= FEMRT_MAX_WALK_SKIPS) ++w.stalled;
     return false;
   }
-  if (k == MARCH_NEXT) return true;
-  return march_reenter<NG>(M, P, o, d, S, w);
+  bool jump = false;
+  if (k == MARCH_NEXT) {
+    if (!(none_here && !seg_may_cross(M, P, S))) return true;
+    jump = true;
+  }
+  return march_reenter<NG>(M, P, o, d, S, w, jump, t_jump);
 }
 
 struct BracketArray {

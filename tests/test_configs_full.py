@@ -150,7 +150,7 @@ def _run(femrt, oracle, ctx, hm, rays_fn, n_rays, isos, pert, amp, seed, variant
     rep, gh, giso = _sample_check(oracle, hm, fr, isos, seed, np.union1d(np.asarray(extra, np.int64), st_ids))
     print(json.dumps({"learned_vs_oracle": rep, "counters": {k: cnt[k] for k in ("hit_rays", "mlp_infer",
                                                                                      "brackets", "walks_stalled",
-                                                                                     "walk_skips")}}),
+                                                                                     "walk_skips", "walk_jumps")}}),
           flush=True)
     assert rep["ok"], rep
     assert rep["oracle_hits"] >= N_SAMPLE_HIT - 5
