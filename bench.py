@@ -366,14 +366,14 @@ def main():
     # learned_nohermite / warm_nohermite: a segment's first sample starts from
     # the network's guess or the entry point even where the Hermite
     # interpolant of rst(t) is at hand
-    # learned_jump / warm_jump: runs of elements whose field bounds exclude
-    # every open isovalue are passed by a BVH query instead of walked
+    # learned_nojump / warm_nojump: every element along the ray is walked,
+    # also runs of elements whose field bounds exclude every open isovalue
     names = {"plain": femrt.GUESS_PLAIN, "warm": femrt.GUESS_WARM, "learned": femrt.GUESS_LEARNED,
              "learned_every": femrt.GUESS_LEARNED, "warm_coop": femrt.GUESS_WARM,
              "bisect": femrt.GUESS_LEARNED, "learned_bracket": femrt.GUESS_LEARNED,
              "learned_nested": femrt.GUESS_LEARNED, "warm_nested": femrt.GUESS_WARM,
              "learned_nohermite": femrt.GUESS_LEARNED, "warm_nohermite": femrt.GUESS_WARM,
-             "learned_jump": femrt.GUESS_LEARNED, "warm_jump": femrt.GUESS_WARM}
+             "learned_nojump": femrt.GUESS_LEARNED, "warm_nojump": femrt.GUESS_WARM}
     variants = [v for v in args.variants.split(",") if v]
     results = {}
     mlp_info = None
@@ -462,8 +462,8 @@ def main():
             flags |= femrt.TRACE_ISO_LEARNED_BRACKET
         if v.endswith("_nohermite"):
             flags |= femrt.TRACE_NO_HERMITE_START
-        if v.endswith("_jump"):
-            flags |= femrt.TRACE_SKIP_JUMP
+        if v.endswith("_nojump"):
+            flags |= femrt.TRACE_NO_SKIP_JUMP
         if v.endswith("_nested"):
             flags |= femrt.TRACE_NESTED_FIRST_HIT
         wall, kt, cnt = run_variant(femrt, lanes, names[v], args.steps, args.warmup, flags)

@@ -282,18 +282,16 @@ typedef struct {
                                           guesses only segments without
                                           tangents.  Same results (the FP64
                                           endgame), other FP32 iterates     */
-#define FEMRT_TRACE_SKIP_JUMP 1024    /* after two segments in a row whose
-                                          field bounds exclude every open
-                                          isovalue, leading into a neighbour
-                                          whose bounds do too, a BVH query
-                                          that passes such elements finds the
-                                          next element that can hold a
-                                          crossing (or ends the walk); by
-                                          default the walk visits every
-                                          element.  Opt-in: on a 64^3 P=8
-                                          frame it missed one element's
-                                          crossing among 1000 rays checked
-                                          (cause not found)                   */
+#define FEMRT_TRACE_NO_SKIP_JUMP 1024 /* walk every element along the ray; by
+                                          default, after two segments in a
+                                          row whose field bounds exclude
+                                          every open isovalue, leading into a
+                                          neighbour whose bounds do too, a
+                                          BVH query that passes such
+                                          elements (entry roots only) finds
+                                          the next element that can hold a
+                                          crossing, or ends the walk.  The
+                                          elements passed hold no crossing   */
 #define FEMRT_TRACE_SEG_INFO 128       /* record each bracket's ends (rst and the
                                           FP32 g at ta and tb), the bracket
                                           network's inputs, for
@@ -344,7 +342,7 @@ FEMRT_API int femrt_frame_get_rgba(femrt_frame* f, uint32_t* rgba);
  * first-hit pass, [31] its slab tests ([7] counts the slab tests of the
  * other passes); [32] stalls skipped past (the entry moved 1e-3 of the
  * element along the ray); [33] runs of elements that cannot hold a crossing
- * passed by a BVH query (FEMRT_TRACE_SKIP_JUMP; also counted in [13]). */
+ * passed by a BVH query (FEMRT_TRACE_NO_SKIP_JUMP; also counted in [13]). */
 #define FEMRT_NCOUNTERS 34
 FEMRT_API int femrt_frame_counters(femrt_frame* f, int enable, long long* out);
 /* Ray ids of the walks of the last counted trace that stalled (skipped past

@@ -23,7 +23,7 @@ TRACE_RGBA, TRACE_BINNED_WALK, TRACE_LEARNED_EVERY_SAMPLE, TRACE_NO_SAMPLE_SKIP 
 TRACE_COOP_SCAN = 16
 TRACE_ISO_BISECT, TRACE_ISO_LEARNED_BRACKET, TRACE_SEG_INFO, TRACE_NESTED_FIRST_HIT = 32, 64, 128, 256
 TRACE_NO_HERMITE_START = 512
-TRACE_SKIP_JUMP = 1024
+TRACE_NO_SKIP_JUMP = 1024
 GRAD_MFMA, GRAD_VALU = 0, 1   # femrt_frame_iso_gradients engines
 BVH_HOST_SAH, BVH_GPU_LBVH = 0, 1   # femrt_mesh_build_bvh builders
 MAX_ISO = 8

@@ -1160,7 +1160,7 @@ FEMRT_API int femrt_trace(femrt_mesh* m, femrt_frame* f, const femrt_trace_param
   ip.endgame = learned_br ? ISO_LEARNED_BISECT : ((p->flags & FEMRT_TRACE_ISO_BISECT) ? ISO_BISECT : ISO_NEWTON);
   ip.qseg_per_xcd = cfg.qseg_per_xcd;
   ip.hermite = (p->flags & FEMRT_TRACE_NO_HERMITE_START) ? 0 : 1;
-  ip.jump = (p->flags & FEMRT_TRACE_SKIP_JUMP) ? FEMRT_JUMP_AFTER : 0;
+  ip.jump = (p->flags & FEMRT_TRACE_NO_SKIP_JUMP) ? 0 : FEMRT_JUMP_AFTER;
   TraceMesh TM;
   TM.geom = m->d_geom;
   TM.field = m->d_field;

@@ -46,7 +46,7 @@ struct IsoParams {
   int hermite;       // first interior sample from the Hermite interpolant where tangents exist
   int jump;          // after this many segments in a row that cannot cross, the walk jumps
                      // over the rest of such a run by a BVH query (march_advance); 0: never
-                     // (the default; FEMRT_TRACE_SKIP_JUMP)
+                     // (FEMRT_TRACE_NO_SKIP_JUMP)
 };
 
 // Segments in a row whose field bounds rule out every open isovalue before
@@ -415,7 +415,10 @@ struct HitExcl {
 // element: element box, then per face its box and the face root.  The
 // nearest accepted root over all elements (but `exclude`) and faces is the
 // hit.  With `cross` set, only elements that can hold a crossing of an
-// isovalue not in `found` take part (elem_may_cross).
+// isovalue not in `found` take part (elem_may_cross), and only roots where
+// the ray enters the element: the search then starts inside or between
+// elements, and an element it only clips at a corner can show its exit root
+// alone (the grazing entry root missed), which must not pass for an entry.
 template <int NG, typename W>
 FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d, int guess,
                               float tmin, float tmax, W& w, int exclude = -1, const HitExcl* excl = nullptr,
@@ -458,6 +461,17 @@ FEMRT_HD HitF32 first_hit_ray(const TraceMesh& M, const float* o, const float* d
           if (!slab(o, id, fb, fb + 3, tmin, best_t, f0, f1)) continue;
           float tt, uu, vv;
           if (face_root<NG>(gc, bx, o, d, id, f, tmin, best_t, guess, tt, uu, vv, w) && tt < best_t) {
+            if (cross) {
+              // entry: the face's fixed coordinate moves inward along the
+              // ray, sign(face) * (J^-1 d)[c] < 0 (faces 2, 3, 5 at +1)
+              float rst[3], X[3], J[9], m[3];
+              face_to_rst(f, uu, vv, rst[0], rst[1], rst[2]);
+              geom_eval<float, NG>(gc, rst[0], rst[1], rst[2], X, J);
+              ++w.face_it;
+              const int c = (f == 2 || f == 4) ? 0 : ((f == 1 || f == 3) ? 1 : 2);
+              const float sg = (f == 2 || f == 3 || f == 5) ? 1.f : -1.f;
+              if (!solve3<float>(J, d, m) || !(sg * m[c] < 0.f)) continue;
+            }
             best_t = tt;
             h.elem = e;
             h.face = f;
